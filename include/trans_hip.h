@@ -1,0 +1,206 @@
+/*
+ * trans_hip.h -- C-ABI of the MI355X (gfx950) residual-and-update hot path of the
+ * TRANS block-structured RANS solver.
+ *
+ * The reference calls parameterless Fortran subroutines on module state, one block at a
+ * time, after GLOBAL_SetPointers(iBlock) (main.F90:65-197, Module_Global.F90:274-392).
+ * This library replaces that loop body.  A host (the Fortran driver in
+ * trans_amd/fortran/ through iso_c_binding, or Python through ctypes) keeps the
+ * reference's block arrays in their own layout -- Fortran column-major, component index
+ * first, explicit lower bounds (Input/Read_BlockDimensions.F90:306-401) -- and hands
+ * them over as trans_fview descriptors.  The library transposes them once into its
+ * device layout and runs every stage on the GPU.
+ *
+ * Every function returns 0 on success, a negative value for an API/HIP error and a
+ * positive value for a numerical failure; trans_hip_last_error() gives the text.
+ * A context is not thread safe; all device work of a context runs on one stream.
+ * No pointer passed in is retained after the call returns.
+ */
+#ifndef TRANS_HIP_H
+#define TRANS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TRANS_ML 7 /* rho, rho*u, rho*v, rho*w, rho*E, rho*k, rho*omega (Module_Global.F90:13) */
+
+/* Error codes */
+#define TRANS_OK 0
+#define TRANS_EINVAL (-1)
+#define TRANS_EHIP (-2)
+#define TRANS_ENOMEM (-3)
+#define TRANS_EUNSUPPORTED (-4)
+#define TRANS_ENONFINITE 1 /* a residual norm came out NaN (Time/TIME_Update_V.F90:124-151) */
+
+/*
+ * Deck scalars the hot path reads from module Global.  Filled by the host from
+ * original.org / dimension_global.dat exactly as INPUT_ReadOrg (Input/ReadOrg.F90:98-160),
+ * ParameterSet and NonDimensionlization (Input/ParameterSet.F90:1-96) do.
+ */
+typedef struct trans_params {
+    /* switches */
+    int kind_spatial;    /* 2 = MUSCL/ALIMITER, 3 = MDCD */
+    int kind_roe;        /* 1 = Roe, 3 = AUSMPW+ */
+    int if_euler;
+    int if_turb_org;
+    int kind_model;      /* 1 = SST */
+    int kind_dual;       /* 0, 1, 2 */
+    int kchld;           /* sub-iterations per outer iteration */
+    int if_rk;           /* 0 = LU-SGS, 1 = SSP-RK3 */
+    int kitm;            /* first iteration with turbulence on */
+    int kind_hybrid;     /* 0 = RANS, 3 = DES/DDES */
+    int kind_subhy;
+    int if_comppress;    /* SST compressibility correction */
+    int kind_cc;
+    int kind_rotmod;     /* must be 0 (RC_Correction.F90:107 reads an unset index otherwise) */
+    int if_precondition; /* must be 0 */
+    int cfl_sub_step;    /* >= 1 (SpectralRadius.F90:159 divides by it) */
+    int ki_ini;
+    int implicit_fp32;   /* 1: LU-SGS operator and Rds diagonals in FP32 */
+    /* nondimensional constants */
+    double xm, rxm2, ppf;      /* M, gamma*M^2, 1/(gamma*M^2) */
+    double csthlnd, csth1;     /* Sutherland: 110/Tinf, Tinf/273.15 */
+    double ref;                /* Reynolds number per unit grid length */
+    double omega1;             /* rotation rate about x (nondimensional) */
+    double tiny;               /* 1e-20 (main.F90:12) */
+    double vxf, vyf, vzf;      /* freestream direction */
+    double dt_lusgs_main, cfl_sub_start, cfl_sub_end;
+    double gama_max, gama_min; /* MDCD dissipation bounds */
+    double prod_lmt, visratio, fsti;
+    double a1, akapa, beta_star;
+    double sigk1, sigo1, beta1, gama1;
+    double sigk2, sigo2, beta2, gama2;
+    double p0in, t0in, pouthub, ps00, ts00;
+    double rkf, rof, rmiuf;
+} trans_params;
+
+/*
+ * One BC patch: the fields of ConnectivityStruct (Module_Block.F90:4-23) the hot path
+ * reads, as PLOT3D_Read_And_Init builds them (Input/PLOT3DReadInit.F90:135-348).
+ * Indices keep the reference's 1-based point/cell numbering; target_block and
+ * target_conn are 0-based positions in the arrays given to trans_hip_create.
+ */
+typedef struct trans_patch {
+    int bctype;          /* 1 wall, 2 far field, 3 symmetry, 5 inlet, 7 outlet, 9 interface */
+    int subbctype;
+    int pertype;         /* periodic multiple (0 = plain interface) */
+    int point_start[3];
+    int point_end[3];
+    int ijkdirect;       /* 1, 2, 3 */
+    int left_or_right;   /* 1 = min face, 2 = max face */
+    int ijkadd[3];
+    int cijkadd[3], cijkadd1[3], cijkadd2[3];
+    int ceijkadd[3], ceijkadd1[3], ceijkadd2[3];
+    int target_block;
+    int target_conn;
+    int transform[3];
+    int transform_t[3];
+    int rev_matrix[3];
+    int lcross;
+    double rotate_angle; /* rotateAngle(1) */
+} trans_patch;
+
+typedef struct trans_block_desc {
+    int ni, nj, nk; /* points; cells are ni-1, nj-1, nk-1 */
+    int npatch;
+    const trans_patch* patches;
+} trans_block_desc;
+
+/*
+ * A Fortran array A(ncomp, lo0:hi0, lo1:hi1, lo2:hi2) in column-major order.
+ * Element (c, i, j, k), c 0-based, is at
+ *   p[c + ncomp*((i-lo0) + n0*((j-lo1) + n1*(k-lo2)))],  n0 = hi0-lo0+1, n1 = hi1-lo1+1.
+ * For SD(3,3,...) ncomp = 9 and c = (dir-1) + 3*(xyz-1).
+ */
+typedef struct trans_fview {
+    double* p;
+    int ncomp;
+    int lo[3];
+    int hi[3];
+} trans_fview;
+
+/* Geometry of one block (Geom/CalcDst.F90:28-719, Geom/startup.F90:396-755). */
+typedef struct trans_geom_view {
+    trans_fview sd;    /* SD(3,3,...) face area vectors */
+    trans_fview grad;  /* Grad(3,...) face areas */
+    trans_fview alagm; /* Alagm(3,...) face-to-face lengths */
+    trans_fview vol;   /* Vol */
+    trans_fview rad;   /* rad = sqrt(yc^2+zc^2) */
+    trans_fview dst;   /* wall distance */
+    trans_fview aleng; /* largest neighbour distance (DES length) */
+    trans_fview yc, zc;
+    trans_fview thtc;  /* circumferential angle atan2(zc, yc) (outlet radial equilibrium) */
+    /* MARKWALLi0(NJ1,NK1), MARKWALLim, j0(NI1,NK1), jm, k0(NI1,NJ1), km; NULL = none */
+    const int* markwall[6];
+} trans_geom_view;
+
+/* Flow state of one block. wt2 may have p == NULL. */
+typedef struct trans_state_view {
+    trans_fview v;    /* V(7,...) conservative variables */
+    trans_fview pp;   /* PP */
+    trans_fview t;    /* T */
+    trans_fview rmiu; /* eddy viscosity */
+    trans_fview wt0;  /* dual-time history (7,...) */
+    trans_fview wt1;
+    trans_fview wt2;
+} trans_state_view;
+
+/* Per-block norms stashed by TIME_Update_V (Time/TIME_Update_V.F90:892-908). */
+typedef struct trans_resid {
+    double drsa, dua;
+    int ima, jma, kma;
+    double drsk, duk;
+    int imk, jmk, kmk;
+    double drso, duo;
+    int imo, jmo, kmo;
+} trans_resid;
+
+/* Fields readable through trans_hip_get_field (parity hooks). */
+#define TRANS_FIELD_D 0      /* D(7,...) residual, then the LU-SGS increment */
+#define TRANS_FIELD_RDS 1    /* Rds(3,...) */
+#define TRANS_FIELD_DTM 2    /* Dtm */
+#define TRANS_FIELD_DQDXYZ 3 /* DQdxyz(18,...) */
+#define TRANS_FIELD_SHOCK 4  /* shock sensor */
+
+typedef struct trans_ctx trans_ctx;
+
+/* Allocate device state for all blocks.  Replaces INPUT_Read_BlockDimensions'
+ * ALLOCATEs (Input/Read_BlockDimensions.F90:306-401) and PLOT3D_Read_And_Init's
+ * connectivity (Input/PLOT3DReadInit.F90:1-349), which the host has already built. */
+int trans_hip_create(const trans_params* params, int nblk, const trans_block_desc* blocks,
+                     trans_ctx** out);
+int trans_hip_destroy(trans_ctx* ctx);
+const char* trans_hip_last_error(const trans_ctx* ctx);
+
+/* Replace the deck scalars (original.org is re-read every 10 iterations, main.F90:49-52). */
+int trans_hip_set_params(trans_ctx* ctx, const trans_params* params);
+/* KI, the outer iteration counter (Module_Global.F90:79). */
+int trans_hip_set_iteration(trans_ctx* ctx, int ki);
+
+/* Upload geometry (after GEOM_CalcDSTs / GeomTransform / GeomTransform1, main.F90:28-31). */
+int trans_hip_set_geometry(trans_ctx* ctx, int blk, const trans_geom_view* g);
+/* Upload / download the flow state including all ghost layers the views cover. */
+int trans_hip_set_state(trans_ctx* ctx, int blk, const trans_state_view* s);
+int trans_hip_get_state(trans_ctx* ctx, int blk, const trans_state_view* s);
+int trans_hip_get_field(trans_ctx* ctx, int blk, int which, const trans_fview* out);
+
+/* One call per reference routine, over all blocks of the context: */
+int trans_hip_spectral_radius(trans_ctx* ctx);     /* TIME_SpectralRadius, main.F90:66-69 */
+int trans_hip_bc(trans_ctx* ctx, int sweep);       /* BC_Boundarys(sweep), main.F90:72,84 */
+int trans_hip_space(trans_ctx* ctx);               /* main.F90:74-133 except the two BC calls */
+int trans_hip_residual(trans_ctx* ctx);            /* main.F90:66-133: SpectralRadius..convection */
+int trans_hip_lusgs_update(trans_ctx* ctx, trans_resid* per_block);
+                                                   /* TIME_LUSGS + TURBTWO_TIME_LUSGS_SST +
+                                                      TIME_Update_V, main.F90:137-151 */
+int trans_hip_update_u(trans_ctx* ctx);            /* TIME_Update_U, main.F90:192-197 */
+/* n outer iterations of main.F90:48-211 (sub-iterations, update of Wt0/Wt1, KI += 1).
+ * per_block (nblk entries, may be NULL) receives the norms of the last sub-iteration. */
+int trans_hip_iterate(trans_ctx* ctx, int n_iter, trans_resid* per_block);
+/* Wait for all queued device work of the context. */
+int trans_hip_sync(trans_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TRANS_HIP_H */

@@ -1,0 +1,149 @@
+"""ctypes driver of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module;
+it never runs on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from trans_amd import capi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libtrans_oracle.so")
+
+DP = C.POINTER(C.c_double)
+
+
+class OraBlock(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int), ("npatch", C.c_int),
+                ("patches", C.POINTER(capi.TransPatch)),
+                ("V", DP), ("PP", DP), ("T", DP), ("Rmiu", DP),
+                ("SD", DP), ("Grad", DP), ("Alagm", DP),
+                ("Vol", DP), ("rad", DP), ("Dst", DP), ("aLeng", DP), ("Yc", DP), ("Zc", DP),
+                ("thtc", DP),
+                ("mw", C.POINTER(C.c_int) * 6),
+                ("VL", DP), ("VR", DP), ("rccl", DP), ("rccr", DP), ("D", DP), ("F", DP),
+                ("Rds", DP), ("Dtm", DP), ("Wt0", DP), ("Wt1", DP), ("Wt2", DP),
+                ("DQdxyz", DP), ("shock", DP), ("Fscheme", DP), ("f_r1", DP),
+                ("res", capi.TransResid)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        PB = C.POINTER(OraBlock)
+        PP = C.POINTER(capi.TransParams)
+        L.ora_if_turb.argtypes = [PP, C.c_int]
+        L.ora_spectral_radius.argtypes = [PP, C.c_int, PB]
+        L.ora_bc.argtypes = [PP, C.c_int, PB, C.c_int, C.c_int]
+        L.ora_spaceinterp.argtypes = [PP, C.c_int, PB]
+        L.ora_convect_muscl.argtypes = [PP, PB]
+        L.ora_convect_roe.argtypes = [PP, PB]
+        L.ora_lusgs.argtypes = [PP, PB]
+        L.ora_update_v.argtypes = [PP, C.c_int, C.c_int, PB]
+        L.ora_update_u.argtypes = [PP, PB]
+        L.ora_residual.argtypes = [PP, C.c_int, PB, C.c_int]
+        L.ora_lusgs_update.argtypes = [PP, C.c_int, PB, C.c_int]
+        L.ora_iterate.argtypes = [PP, C.POINTER(C.c_int), C.c_int, PB, C.c_int]
+        L.ora_alimiter.argtypes = [C.c_double, C.c_double]
+        L.ora_alimiter.restype = C.c_double
+        L.ora_roe_common.argtypes = [PP] + [C.c_double] * 16 + [DP]
+        L.ora_lusgs_adq.argtypes = [C.c_double] * 8 + [DP, DP]
+        L.ora_ijk_start_end.argtypes = [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.c_int,
+                                                                                   C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(DP)
+
+
+class OracleCase:
+    """Oracle-side copy of a case: every block's arrays plus the oracle's work arrays."""
+
+    def __init__(self, blocks, params):
+        self.params = params
+        self.blocks = blocks
+        self.arrs = []
+        self.patch_arrs = []
+        self.ob = (OraBlock * len(blocks))()
+        from trans_amd.mesh import patch_structs
+        for n, b in enumerate(blocks):
+            shp = b.shape()
+            A = {k: np.array(v, order="F", copy=True) for k, v in b.arrays.items()}
+
+            def z(nc):
+                s = shp if nc == 1 else (nc,) + shp
+                return np.zeros(s, order="F")
+            for nm, nc in (("VL", 24), ("VR", 24), ("rccl", 3), ("rccr", 3), ("D", 7), ("F", 7),
+                           ("Rds", 3), ("Dtm", 1), ("DQdxyz", 18), ("shock", 1),
+                           ("Fscheme", 1), ("f_r1", 1)):
+                A[nm] = z(nc)
+            A["f_r1"][...] = 1.0
+            self.arrs.append(A)
+            pa = patch_structs(b)
+            self.patch_arrs.append(pa)
+            o = self.ob[n]
+            o.ni, o.nj, o.nk, o.npatch = b.ni, b.nj, b.nk, len(b.patches)
+            o.patches = C.cast(pa, C.POINTER(capi.TransPatch))
+            for f, key in (("V", "v"), ("PP", "pp"), ("T", "t"), ("Rmiu", "rmiu"), ("SD", "sd"),
+                           ("Grad", "grad"), ("Alagm", "alagm"), ("Vol", "vol"), ("rad", "rad"),
+                           ("Dst", "dst"), ("aLeng", "aleng"), ("Yc", "yc"), ("Zc", "zc"),
+                           ("thtc", "thtc"), ("Wt0", "wt0"), ("Wt1", "wt1"), ("Wt2", "wt2")):
+                setattr(o, f, _p(A[key]))
+            for f in ("VL", "VR", "rccl", "rccr", "D", "F", "Rds", "Dtm", "DQdxyz", "shock",
+                      "Fscheme", "f_r1"):
+                setattr(o, f, _p(A[f]))
+            mwa = (C.POINTER(C.c_int) * 6)()
+            self._mw = getattr(self, "_mw", [])
+            for f in range(6):
+                m = np.ascontiguousarray(b.markwall[f]).astype(np.int32, order="F")
+                self._mw.append(m)
+                mwa[f] = m.ctypes.data_as(C.POINTER(C.c_int))
+            o.mw = mwa
+
+    def __getitem__(self, n):
+        return self.arrs[n]
+
+    @property
+    def nblk(self):
+        return len(self.blocks)
+
+    def spectral_radius(self, ki):
+        for n in range(self.nblk):
+            lib().ora_spectral_radius(C.byref(self.params), ki, C.byref(self.ob[n]))
+
+    def bc(self, ki, sweep):
+        it = lib().ora_if_turb(C.byref(self.params), ki)
+        lib().ora_bc(C.byref(self.params), it, self.ob, self.nblk, sweep)
+
+    def residual(self, ki):
+        lib().ora_residual(C.byref(self.params), ki, self.ob, self.nblk)
+
+    def lusgs_update(self, ki):
+        lib().ora_lusgs_update(C.byref(self.params), ki, self.ob, self.nblk)
+        return [self.ob[n].res.as_dict() for n in range(self.nblk)]
+
+    def iterate(self, ki, n):
+        k = C.c_int(ki)
+        lib().ora_iterate(C.byref(self.params), C.byref(k), n, self.ob, self.nblk)
+        return k.value
+
+    def resid(self):
+        return [self.ob[n].res.as_dict() for n in range(self.nblk)]

@@ -1,0 +1,77 @@
+/*
+ * CPU oracle for the TRANS residual-and-update hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C FP64 restatement of the reference's
+ * Fortran routines, kept in the reference's own array layout (component index first,
+ * column-major) and loop order so that it can be read side by side with the reference.
+ * It is the checker for the HIP library and the CPU baseline of bench.py; nothing in
+ * the product (trans_amd/, include/) calls it.
+ *
+ * Layout: every array of a block spans -2:N+2 in each direction (N = points), i.e.
+ * N+5 entries, with i fastest after the component index.  Component arrays put the
+ * component first: V(L,i,j,k) is V[(L-1) + 7*IX(i,j,k)].
+ */
+#ifndef TRANS_ORACLE_H
+#define TRANS_ORACLE_H
+
+#include "../include/trans_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_block {
+    int ni, nj, nk; /* points */
+    int npatch;
+    const trans_patch* patches;
+    /* state */
+    double *V, *PP, *T, *Rmiu;         /* V: 7 comps */
+    /* geometry */
+    double *SD, *Grad, *Alagm;         /* 9, 3, 3 comps */
+    double *Vol, *rad, *Dst, *aLeng, *Yc, *Zc, *thtc; /* scalars */
+    const int* mw[6];                  /* MARKWALLi0, im, j0, jm, k0, km */
+    /* work */
+    double *VL, *VR;                   /* (3,8): comp (d-1)+3*(L-1) */
+    double *rccl, *rccr;               /* 3 */
+    double *D, *F;                     /* 7 */
+    double *Rds;                       /* 3 */
+    double *Dtm;
+    double *Wt0, *Wt1, *Wt2;           /* 7 */
+    double *DQdxyz;                    /* 18 */
+    double *shock, *Fscheme, *f_r1;
+    trans_resid res;
+} ora_block;
+
+/* IF_turb as main.F90:59-62 derives it for iteration ki. */
+int ora_if_turb(const trans_params* p, int ki);
+
+void ora_spectral_radius(const trans_params* p, int ki, ora_block* b);
+void ora_bc(const trans_params* p, int if_turb, ora_block* blocks, int nblk, int sweep);
+void ora_spaceinterp(const trans_params* p, int if_turb, ora_block* b);
+void ora_convect_muscl(const trans_params* p, ora_block* b);
+void ora_convect_roe(const trans_params* p, ora_block* b);
+void ora_lusgs(const trans_params* p, ora_block* b);
+void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b);
+void ora_update_u(const trans_params* p, ora_block* b);
+
+/* Composite steps over all blocks, in main.F90 order. */
+void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk);
+void ora_lusgs_update(const trans_params* p, int ki, ora_block* blocks, int nblk);
+/* n outer iterations starting at *ki; *ki is advanced. */
+void ora_iterate(const trans_params* p, int* ki, int n_iter, ora_block* blocks, int nblk);
+
+/* Reference functions exposed for unit tests. */
+double ora_alimiter(double var1, double var2);
+void ora_roe_common(const trans_params* p, double PLo, double PRo, double TLo, double TRo,
+                    double ULo, double URo, double VLo, double VRo, double WLo, double WRo,
+                    double sav1, double sav2, double sav3, double sav, double rLo, double rRo,
+                    double out[5]);
+void ora_lusgs_adq(double pVx, double pVy, double pVz, double aSDx, double aSDy,
+                   double aSDz, double pPP, double roinv, const double ain[5], double dif[5]);
+void ora_ijk_start_end(int i, int j, int k, const int tl[3], const int tt[3], int lcross,
+                       int out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

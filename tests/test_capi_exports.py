@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every function include/trans_hip.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+from trans_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "trans_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(trans_hip_\w+)\s*\(", src)))
+
+
+def test_header_matches_python_binding():
+    assert declared_functions() == sorted(capi.EXPORTS)
+
+
+def test_library_exports_all_symbols():
+    lib = ctypes.CDLL(capi.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_struct_sizes_match_header():
+    # the ctypes mirror must have the C layout: compare against a compiled probe
+    import subprocess, tempfile
+    probe = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "trans_hip.h"
+int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(trans_params), sizeof(trans_patch),
+ sizeof(trans_block_desc), sizeof(trans_fview), sizeof(trans_geom_view), sizeof(trans_state_view),
+ sizeof(trans_resid)); return 0;}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "p.c")
+        open(c, "w").write(probe)
+        exe = os.path.join(d, "p")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
+    got = [int(x) for x in out]
+    want = [ctypes.sizeof(t) for t in (capi.TransParams, capi.TransPatch, capi.TransBlockDesc,
+                                       capi.TransFView, capi.TransGeomView, capi.TransStateView,
+                                       capi.TransResid)]
+    assert got == want

@@ -1,0 +1,156 @@
+"""ctypes mirror of include/trans_hip.h and the loader of libtrans_hip.so.
+
+The structs are the C-ABI's plain-old-data types; the loader refuses to fall back to
+anything else: if the HIP library has not been built, importing the solver fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtrans_hip.so")
+
+ML = 7
+
+
+class TransParams(C.Structure):
+    _fields_ = [
+        ("kind_spatial", C.c_int), ("kind_roe", C.c_int), ("if_euler", C.c_int),
+        ("if_turb_org", C.c_int), ("kind_model", C.c_int), ("kind_dual", C.c_int),
+        ("kchld", C.c_int), ("if_rk", C.c_int), ("kitm", C.c_int), ("kind_hybrid", C.c_int),
+        ("kind_subhy", C.c_int), ("if_comppress", C.c_int), ("kind_cc", C.c_int),
+        ("kind_rotmod", C.c_int), ("if_precondition", C.c_int), ("cfl_sub_step", C.c_int),
+        ("ki_ini", C.c_int), ("implicit_fp32", C.c_int),
+        ("xm", C.c_double), ("rxm2", C.c_double), ("ppf", C.c_double),
+        ("csthlnd", C.c_double), ("csth1", C.c_double), ("ref", C.c_double),
+        ("omega1", C.c_double), ("tiny", C.c_double),
+        ("vxf", C.c_double), ("vyf", C.c_double), ("vzf", C.c_double),
+        ("dt_lusgs_main", C.c_double), ("cfl_sub_start", C.c_double), ("cfl_sub_end", C.c_double),
+        ("gama_max", C.c_double), ("gama_min", C.c_double),
+        ("prod_lmt", C.c_double), ("visratio", C.c_double), ("fsti", C.c_double),
+        ("a1", C.c_double), ("akapa", C.c_double), ("beta_star", C.c_double),
+        ("sigk1", C.c_double), ("sigo1", C.c_double), ("beta1", C.c_double), ("gama1", C.c_double),
+        ("sigk2", C.c_double), ("sigo2", C.c_double), ("beta2", C.c_double), ("gama2", C.c_double),
+        ("p0in", C.c_double), ("t0in", C.c_double), ("pouthub", C.c_double),
+        ("ps00", C.c_double), ("ts00", C.c_double),
+        ("rkf", C.c_double), ("rof", C.c_double), ("rmiuf", C.c_double),
+    ]
+
+
+I3 = C.c_int * 3
+
+
+class TransPatch(C.Structure):
+    _fields_ = [
+        ("bctype", C.c_int), ("subbctype", C.c_int), ("pertype", C.c_int),
+        ("point_start", I3), ("point_end", I3),
+        ("ijkdirect", C.c_int), ("left_or_right", C.c_int),
+        ("ijkadd", I3), ("cijkadd", I3), ("cijkadd1", I3), ("cijkadd2", I3),
+        ("ceijkadd", I3), ("ceijkadd1", I3), ("ceijkadd2", I3),
+        ("target_block", C.c_int), ("target_conn", C.c_int),
+        ("transform", I3), ("transform_t", I3), ("rev_matrix", I3), ("lcross", C.c_int),
+        ("rotate_angle", C.c_double),
+    ]
+
+
+class TransBlockDesc(C.Structure):
+    _fields_ = [("ni", C.c_int), ("nj", C.c_int), ("nk", C.c_int),
+                ("npatch", C.c_int), ("patches", C.POINTER(TransPatch))]
+
+
+class TransFView(C.Structure):
+    _fields_ = [("p", C.POINTER(C.c_double)), ("ncomp", C.c_int), ("lo", I3), ("hi", I3)]
+
+
+class TransGeomView(C.Structure):
+    _fields_ = [("sd", TransFView), ("grad", TransFView), ("alagm", TransFView),
+                ("vol", TransFView), ("rad", TransFView), ("dst", TransFView),
+                ("aleng", TransFView), ("yc", TransFView), ("zc", TransFView),
+                ("thtc", TransFView),
+                ("markwall", C.POINTER(C.c_int) * 6)]
+
+
+class TransStateView(C.Structure):
+    _fields_ = [("v", TransFView), ("pp", TransFView), ("t", TransFView), ("rmiu", TransFView),
+                ("wt0", TransFView), ("wt1", TransFView), ("wt2", TransFView)]
+
+
+class TransResid(C.Structure):
+    _fields_ = [("drsa", C.c_double), ("dua", C.c_double),
+                ("ima", C.c_int), ("jma", C.c_int), ("kma", C.c_int),
+                ("drsk", C.c_double), ("duk", C.c_double),
+                ("imk", C.c_int), ("jmk", C.c_int), ("kmk", C.c_int),
+                ("drso", C.c_double), ("duo", C.c_double),
+                ("imo", C.c_int), ("jmo", C.c_int), ("kmo", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK = 0, 1, 2, 3, 4
+
+EXPORTS = [
+    "trans_hip_create", "trans_hip_destroy", "trans_hip_last_error", "trans_hip_set_params",
+    "trans_hip_set_iteration", "trans_hip_set_geometry", "trans_hip_set_state",
+    "trans_hip_get_state", "trans_hip_get_field", "trans_hip_spectral_radius", "trans_hip_bc",
+    "trans_hip_space", "trans_hip_residual", "trans_hip_lusgs_update", "trans_hip_update_u",
+    "trans_hip_iterate", "trans_hip_sync",
+]
+
+
+def fview(arr: np.ndarray, lo=(-2, -2, -2)) -> TransFView:
+    """Describe a Fortran-ordered array A(ncomp, lo:hi, ...) (or A(lo:hi, ...))."""
+    assert arr.dtype == np.float64 and arr.flags.f_contiguous
+    if arr.ndim == 4:
+        ncomp = arr.shape[0]
+        shp = arr.shape[1:]
+    else:
+        ncomp = 1
+        shp = arr.shape
+    v = TransFView()
+    v.p = arr.ctypes.data_as(C.POINTER(C.c_double))
+    v.ncomp = ncomp
+    v.lo = I3(*lo)
+    v.hi = I3(*(lo[d] + shp[d] - 1 for d in range(3)))
+    return v
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load the HIP library; raises if it is missing (there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    lib.trans_hip_create.argtypes = [P(TransParams), C.c_int, P(TransBlockDesc), P(C.c_void_p)]
+    lib.trans_hip_destroy.argtypes = [C.c_void_p]
+    lib.trans_hip_last_error.argtypes = [C.c_void_p]
+    lib.trans_hip_last_error.restype = C.c_char_p
+    lib.trans_hip_set_params.argtypes = [C.c_void_p, P(TransParams)]
+    lib.trans_hip_set_iteration.argtypes = [C.c_void_p, C.c_int]
+    lib.trans_hip_set_geometry.argtypes = [C.c_void_p, C.c_int, P(TransGeomView)]
+    lib.trans_hip_set_state.argtypes = [C.c_void_p, C.c_int, P(TransStateView)]
+    lib.trans_hip_get_state.argtypes = [C.c_void_p, C.c_int, P(TransStateView)]
+    lib.trans_hip_get_field.argtypes = [C.c_void_p, C.c_int, C.c_int, P(TransFView)]
+    lib.trans_hip_spectral_radius.argtypes = [C.c_void_p]
+    lib.trans_hip_bc.argtypes = [C.c_void_p, C.c_int]
+    lib.trans_hip_space.argtypes = [C.c_void_p]
+    lib.trans_hip_residual.argtypes = [C.c_void_p]
+    lib.trans_hip_lusgs_update.argtypes = [C.c_void_p, P(TransResid)]
+    lib.trans_hip_update_u.argtypes = [C.c_void_p]
+    lib.trans_hip_iterate.argtypes = [C.c_void_p, C.c_int, P(TransResid)]
+    lib.trans_hip_sync.argtypes = [C.c_void_p]
+    for name in EXPORTS:
+        if name != "trans_hip_last_error":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib

@@ -1,0 +1,80 @@
+// Device-side layout and helpers shared by all kernels.
+//
+// Every field of every block lives in one device buffer per field, component-major:
+//   F[c * tot + blk.off + (i+2) + nx*((j+2) + ny*(k+2))],   i,j,k in -2..N+2 (N = points)
+// so that i is the contiguous (coalesced) index and all blocks can be addressed by one
+// kernel launch through a per-workgroup schedule table (Seg).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/trans_hip.h"
+
+struct BlkInfo {
+    int ni, nj, nk;  // points
+    int nx, ny, nz;  // padded extents (N+5)
+    long off;        // element offset of the block inside each field buffer
+    int npatch;
+    int patch0;      // first entry of this block in the device patch table
+};
+
+// One workgroup's share of an index space: block `blk`, linear indices base..base+255.
+struct Seg {
+    int blk;
+    int base;
+};
+
+struct Fields {
+    long tot;  // component stride
+    double* V;        // 7
+    double* PP;
+    double* T;
+    double* Rmiu;
+    double* SD;       // 9: (dir-1) + 3*(xyz-1)
+    double* Grad;     // 3
+    double* Alagm;    // 3
+    double* Vol;
+    double* rad;
+    double* Dst;
+    double* aLeng;
+    double* Yc;
+    double* Zc;
+    double* thtc;
+    double* D;        // 7
+    double* Rds;      // 3
+    double* Dtm;
+    double* Wt0;      // 7
+    double* Wt1;      // 7
+    double* Wt2;      // 7
+    double* DQ;       // 18
+    double* shock;
+    double* F1;       // LU-SGS diagonal
+    double* G[3];     // area-weighted face fluxes per direction, 7 comps each
+    float* Ds;        // FP32 copy of D for the FP32 implicit operator (7)
+    float* Rdsf;      // FP32 Rds (3)
+    int* mw;          // wall marks, 6 faces per block, see BlkInfo (unused for Euler)
+};
+
+__device__ __forceinline__ long cidx(const BlkInfo& b, int i, int j, int k)
+{
+    return b.off + (long)(i + 2) + (long)b.nx * ((long)(j + 2) + (long)b.ny * (long)(k + 2));
+}
+
+// Decode a linear index over an ni x nj x nk box starting at (1,1,1).
+__device__ __forceinline__ void decode3(int lin, int ni, int nj, int& i, int& j, int& k)
+{
+    i = lin % ni;
+    int r = lin / ni;
+    j = r % nj;
+    k = r / nj;
+    i += 1;
+    j += 1;
+    k += 1;
+}
+
+__device__ __forceinline__ double sutherland(const trans_params& p, double tt)
+{
+    return (1.0 + p.csthlnd) / (tt * p.csth1 + p.csthlnd) * pow(p.csth1 * tt, 1.5);
+}
+
+__device__ __forceinline__ double sqr(double x) { return x * x; }

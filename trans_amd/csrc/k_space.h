@@ -1,0 +1,236 @@
+// Residual kernels: spectral radius / local time step, reconstruction + inviscid flux,
+// and the owner-computes accumulation of face fluxes into D.
+#pragma once
+#include "dev_common.h"
+
+// ---------------------------------------------------------------------------------------
+// Time/SpectralRadius.F90:47-180 -- one thread per interior cell.
+// Rds(3) and the local pseudo time step Dtm; preconditioning off, grid velocity zero,
+// IF_Local_Main = 0 and IF_Local_Sub = 1 (Input/ReadOrg.F90:155-156).
+__global__ void __launch_bounds__(256) k_spectral_radius(trans_params p, int ki, Fields f,
+                                                         const BlkInfo* __restrict__ blks,
+                                                         const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long sx = 1, sy = b.nx, sz = (long)b.nx * b.ny;
+    const long tot = f.tot;
+    const double Den = f.V[c];
+    const double Vxi = f.V[tot + c] / Den;
+    const double Vyi = f.V[2 * tot + c] / Den;
+    const double Vzi = f.V[3 * tot + c] / Den;
+    const double AAc = sqrt(1.4 * f.PP[c] / Den);
+    const double RmiuL = sutherland(p, f.T[c]);
+    const double RmiuTL = f.Rmiu[c];
+    const double uvmu = (RmiuL + RmiuTL + 1.0e-20) * 4.0 / 3.0;
+    const double uvmupr = (RmiuL / 0.72 + RmiuTL / 0.9 + 1.0e-20) * 1.4;
+    const double pruvmu = fmax(uvmu, uvmupr) / Den;
+    const double vol = f.Vol[c];
+    double rds[3];
+    const long step[3] = {sx, sy, sz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double* S = f.SD + (long)d * tot;  // component (d, x) at S[3*x*tot]
+        double ax = 0.5 * (S[c] + S[c + step[d]]);
+        double ay = 0.5 * (S[3 * tot + c] + S[3 * tot + c + step[d]]);
+        double az = 0.5 * (S[6 * tot + c] + S[6 * tot + c + step[d]]);
+        double st = ax * ax + ay * ay + az * az;
+        double un = Vxi * ax + Vyi * ay + Vzi * az + 0.0;
+        rds[d] = fabs(un) * (1.0 - 0.0) + sqrt(st) * AAc + 2.0 * st / vol * pruvmu / p.ref;
+        f.Rds[(long)d * tot + c] = rds[d];
+    }
+    if (f.Rdsf) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) f.Rdsf[(long)d * tot + c] = (float)rds[d];
+    }
+    const double dtm_local = vol / (rds[0] + rds[1] + rds[2]);
+    double dtm = p.dt_lusgs_main;
+    double cfl = p.cfl_sub_start +
+                 (p.cfl_sub_end - p.cfl_sub_start) * (double)(ki - p.ki_ini) / (double)p.cfl_sub_step;
+    cfl = fmin(cfl, p.cfl_sub_end);
+    if (p.kind_dual != 0) dtm = cfl * dtm_local;
+    f.Dtm[c] = dtm;
+}
+
+// ---------------------------------------------------------------------------------------
+// Space/UTIL_MUSCL.F90:2-18
+__device__ __forceinline__ double alimiter(double var1, double var2)
+{
+    const double epsm = 1.0e-7;
+    return (var1 * (var2 * var2 + 2.0 * epsm * epsm) + var2 * (2.0 * var1 * var1 + epsm * epsm)) /
+           (2.0 * var1 * var1 - var1 * var2 + 2.0 * var2 * var2 + 3.0 * epsm * epsm) * 0.5 *
+           fabs(copysign(1.0, var1) + copysign(1.0, var2));
+}
+
+// Space/CONVECT_Roe_Original.F90:242-407, IF_PRECONDITION = 0.  Returns the flux per unit
+// area (droR..dreR).
+__device__ __forceinline__ void roe_common(const trans_params& p, double PLo, double PRo,
+                                           double TLo, double TRo, double ULo, double URo,
+                                           double VLo, double VRo, double WLo, double WRo,
+                                           double sav1, double sav2, double sav3, double sav,
+                                           double rLo, double rRo, double out[5])
+{
+    const double rcpcv = p.ppf;
+    const double DENLo = PLo / (rcpcv * TLo);
+    const double DENRo = PRo / (rcpcv * TRo);
+    const double ga = 1.4, ga1 = 1.4 - 1.0;
+    const double VAMLo = ULo * ULo + VLo * VLo + WLo * WLo;
+    const double VAMRo = URo * URo + VRo * VRo + WRo * WRo;
+    const double HLo = PLo * ga / (DENLo * ga1) + 0.5 * VAMLo - 0.5 * (rLo * rLo);
+    const double HRo = PRo * ga / (DENRo * ga1) + 0.5 * VAMRo - 0.5 * (rRo * rRo);
+    const double rrorl = sqrt(DENRo / DENLo);
+    const double rrorlp1 = 1.0 + rrorl;
+    const double rm = sqrt(DENRo * DENLo);
+    const double um = (ULo + URo * rrorl) / rrorlp1;
+    const double vm = (VLo + VRo * rrorl) / rrorlp1;
+    const double wm = (WLo + WRo * rrorl) / rrorlp1;
+    const double hm = (HLo + HRo * rrorl) / rrorlp1;
+    const double vm2 = um * um + vm * vm + wm * wm;
+    const double rccm = (rLo + rrorl * rRo) / rrorlp1;
+    const double am2 = ga1 * fabs(hm - 0.5 * vm2 + 0.5 * (rccm * rccm));
+    const double am = sqrt(am2);
+    const double s1 = sav1 / sav, s2 = sav2 / sav, s3 = sav3 / sav;
+    const double un = (s1 * um + s2 * vm + s3 * wm);
+    double eig1 = fabs(un);
+    double eig2 = fabs(un + am);
+    double eig3 = fabs(un - am);
+    if (eig1 < 0.1) eig1 = (eig1 * eig1 + 0.1 * 0.1) / 0.2;
+    if (eig2 < 0.1) eig2 = (eig2 * eig2 + 0.1 * 0.1) / 0.2;
+    if (eig3 < 0.1) eig3 = (eig3 * eig3 + 0.1 * 0.1) / 0.2;
+    const double dun = (s1 * (URo - ULo) + s2 * (VRo - VLo) + s3 * (WRo - WLo));
+    const double astar = (eig2 + eig3) / 2.0;
+    const double Mstar = (eig2 - eig3) / 2.0 / am;
+    const double dW1 = DENRo - DENLo;
+    const double dU1 = URo - ULo, dU2 = VRo - VLo, dU3 = WRo - WLo;
+    const double dP1 = PRo - PLo;
+    const double dW2 = rm * dU1 + dW1 * um;
+    const double dW3 = rm * dU2 + dW1 * vm;
+    const double dW4 = rm * dU3 + dW1 * wm;
+    const double dW5 = dP1 / ga1 + 0.5 * vm2 * dW1 + rm * um * dU1 + rm * vm * dU2 + rm * wm * dU3;
+    const double dUroe = Mstar * dun + (astar - 1.0 * eig1) * (PRo - PLo) / rm / (1.0 * am2);
+    const double dProe = Mstar * (PRo - PLo) + (astar - eig1) * rm * dun;
+    const double droR1 = eig1 * dW1 + dUroe * rm;
+    const double drxR1 = eig1 * dW2 + dUroe * rm * um + dProe * s1;
+    const double dryR1 = eig1 * dW3 + dUroe * rm * vm + dProe * s2;
+    const double drzR1 = eig1 * dW4 + dUroe * rm * wm + dProe * s3;
+    const double dreR1 = eig1 * dW5 + dUroe * rm * hm + dProe * un;
+    const double uln = (s1 * ULo + s2 * VLo + s3 * WLo);
+    const double urn = (s1 * URo + s2 * VRo + s3 * WRo);
+    const double ruln = DENLo * uln;
+    const double rurn = DENRo * urn;
+    out[0] = -0.5 * (ruln + rurn - droR1);
+    out[1] = -0.5 * (ruln * ULo + rurn * URo + s1 * (PLo + PRo) - drxR1);
+    out[2] = -0.5 * (ruln * VLo + rurn * VRo + s2 * (PLo + PRo) - dryR1);
+    out[3] = -0.5 * (ruln * WLo + rurn * WRo + s3 * (PLo + PRo) - drzR1);
+    out[4] = -0.5 * (ruln * HLo + rurn * HRo - dreR1);
+}
+
+// Primitive set of a cell: u, v, w, p, T and the radius.
+struct Prim {
+    double q[6];
+};
+
+__device__ __forceinline__ Prim load_prim(const Fields& f, long c)
+{
+    Prim r;
+    const long tot = f.tot;
+    const double den = f.V[c];
+    r.q[0] = f.V[tot + c] / den;
+    r.q[1] = f.V[2 * tot + c] / den;
+    r.q[2] = f.V[3 * tot + c] / den;
+    r.q[3] = f.PP[c];
+    r.q[4] = f.T[c];
+    r.q[5] = f.rad[c];
+    return r;
+}
+
+// Face value of SPACEINTERP_GetSurfCenterVars (Space/SPACEINTERP_GetSurfCenterVars.F90:27-56):
+// Alagm-weighted average of cells g-1 (qm) and g (qp).
+__device__ __forceinline__ double face_avg(double am, double ap, double qm, double qp)
+{
+    const double alsinv = 1.0 / (ap + am);
+    return (ap * qm + am * qp) * alsinv;
+}
+
+// ---------------------------------------------------------------------------------------
+// One thread per face of direction D (1..3): SPACEINTERP face averages at faces f-1, f, f+1,
+// CONVECT_MUSCL reconstruction (Space/CONVECT_MUSCL.F90:69-320; the left state of face f
+// comes from cell f-1, the right state from cell f) and CONVECT_Roe_Original
+// (Space/CONVECT_Roe_Original.F90:59-231).  Stores flux*|S| in G[D-1].
+template <int DIR>
+__global__ void __launch_bounds__(256) k_muscl_roe(trans_params p, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
+    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
+    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= n0 * n1 * n2) return;
+    int i, j, k;
+    decode3(lin, n0, n1, i, j, k);
+    const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
+    const long c0 = cidx(b, i, j, k);  // cell f (right of face f)
+    const long tot = f.tot;
+    const double* Al = f.Alagm + (long)(DIR - 1) * tot;
+    // cells f-2, f-1, f, f+1
+    const Prim qm2 = load_prim(f, c0 - 2 * st);
+    const Prim qm1 = load_prim(f, c0 - st);
+    const Prim q0 = load_prim(f, c0);
+    const Prim qp1 = load_prim(f, c0 + st);
+    const double am2 = Al[c0 - 2 * st], am1 = Al[c0 - st], a0 = Al[c0], ap1 = Al[c0 + st];
+    double qL[6], qR[6];
+#pragma unroll
+    for (int L = 0; L < 6; L++) {
+        const double avm = face_avg(am2, am1, qm2.q[L], qm1.q[L]);  // face f-1
+        const double av0 = face_avg(am1, a0, qm1.q[L], q0.q[L]);    // face f
+        const double avp = face_avg(a0, ap1, q0.q[L], qp1.q[L]);    // face f+1
+        qL[L] = qm1.q[L] + alimiter(av0 - qm1.q[L], qm1.q[L] - avm);
+        qR[L] = q0.q[L] - alimiter(q0.q[L] - av0, avp - q0.q[L]);
+    }
+    const double* S = f.SD + (long)(DIR - 1) * tot;
+    const double sav = f.Grad[(long)(DIR - 1) * tot + c0];
+    double fl[5];
+    roe_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
+               S[3 * tot + c0], S[6 * tot + c0], sav, qL[5] * p.omega1, qR[5] * p.omega1, fl);
+    double* G = f.G[DIR - 1];
+#pragma unroll
+    for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L] * sav;
+}
+
+// ---------------------------------------------------------------------------------------
+// Owner-computes sum of the scatter-adds of CONVECT_Roe_Original in the reference's order:
+// for each direction the cell's own face (subtract) then the next face (add).
+__global__ void __launch_bounds__(256) k_accumulate(int neq, Fields f,
+                                                    const BlkInfo* __restrict__ blks,
+                                                    const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+    for (int L = 0; L < neq; L++) {
+        double d = f.D[(long)L * tot + c];
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) {
+            const double* G = f.G[dir] + (long)L * tot;
+            d = d - G[c];
+            d = d + G[c + st[dir]];
+        }
+        f.D[(long)L * tot + c] = d;
+    }
+}

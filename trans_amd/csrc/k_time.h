@@ -1,0 +1,390 @@
+// Implicit update kernels: LU-SGS right-hand side, diagonal, the two sweeps as
+// diagonal-hyperplane wavefronts, the state update with its residual norms, and the
+// dual-time history shift.
+#pragma once
+#include "dev_common.h"
+
+// ---------------------------------------------------------------------------------------
+// Time/TIME_LUSGS.F90:60-108: BDF2 right-hand side and the scalar diagonal F1.
+__global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    const double dtm = p.dt_lusgs_main;
+    const double vol = f.Vol[c];
+#pragma unroll
+    for (int L = 0; L < 5; L++) {
+        const long e = (long)L * tot + c;
+        f.D[e] = f.D[e] / vol - (3.0 * f.V[e] - 4.0 * f.Wt0[e] + f.Wt1[e]) * 0.5 / dtm;
+    }
+    const double Dt = f.Dtm[c];
+    const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
+    f.F1[c] = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * (stept / vol);
+}
+
+// Time/TIME_LUSGS.F90:696-768 (LUSGS_ADQ) -- flux-Jacobian times increment, gamma = 1.4.
+template <typename R>
+__device__ __forceinline__ void lusgs_adq(R pVx, R pVy, R pVz, R aSDx, R aSDy, R aSDz, R pPP,
+                                          R roinv, const R ain[5], R dif[5])
+{
+    const R pGam = R(1.4);
+    const R AinRo = ain[0], AinRx = ain[1], AinRy = ain[2], AinRz = ain[3], AinRe = ain[4];
+    const R VP = pVx * aSDx + pVy * aSDy + pVz * aSDz + R(0);
+    const R Ut2 = pVx * pVx + pVy * pVy + pVz * pVz;
+    const R aSDt = R(0);
+    const R Gm1 = pGam - R(1), Gm2 = pGam - R(2);
+    const R AA2 = pGam * pPP * roinv;
+    const R gUt2 = R(0.5) * Gm1 * Ut2;
+    dif[0] = AinRo * aSDt + AinRx * aSDx + AinRy * aSDy + AinRz * aSDz + AinRe * R(0);
+    dif[1] = AinRo * (aSDx * gUt2 - pVx * VP) + AinRx * (aSDt + VP - aSDx * Gm2 * pVx) +
+             AinRy * (aSDy * pVx - aSDx * Gm1 * pVy) + AinRz * (aSDz * pVx - aSDx * Gm1 * pVz) +
+             AinRe * (aSDx * Gm1);
+    dif[2] = AinRo * (aSDy * gUt2 - pVy * VP) + AinRx * (aSDx * pVy - aSDy * Gm1 * pVx) +
+             AinRy * (aSDt + VP - aSDy * Gm2 * pVy) + AinRz * (aSDz * pVy - aSDy * Gm1 * pVz) +
+             AinRe * (aSDy * Gm1);
+    dif[3] = AinRo * (aSDz * gUt2 - pVz * VP) + AinRx * (aSDx * pVz - aSDz * Gm1 * pVx) +
+             AinRy * (aSDy * pVz - aSDz * Gm1 * pVy) + AinRz * (aSDt + VP - aSDz * Gm2 * pVz) +
+             AinRe * (aSDz * Gm1);
+    const R GEt1 = R(0.5) * Gm2 * Ut2 - AA2 / Gm1;
+    const R GEt2 = R(0.5) * Ut2 + AA2 / Gm1;
+    dif[4] = AinRo * VP * GEt1 + AinRx * (aSDx * GEt2 - Gm1 * pVx * VP) +
+             AinRy * (aSDy * GEt2 - Gm1 * pVy * VP) + AinRz * (aSDz * GEt2 - Gm1 * pVz * VP) +
+             AinRe * (aSDt + pGam * VP);
+}
+
+// Neighbour contribution 0.5*(A_nb dQ_nb +/- Rds_nb dQ_nb) of the sweeps
+// (Time/TIME_LUSGS.F90:190-256 and the mirrored blocks).  `cn` is the neighbour cell and
+// `st` the stride of direction d; A uses the mean of the neighbour's two d-faces.
+template <typename R>
+__device__ __forceinline__ void lusgs_nb(const Fields& f, const R* __restrict__ Dq, int d,
+                                         long cn, long st, R sgn, R acc[5])
+{
+    const long tot = f.tot;
+    R ain[5];
+#pragma unroll
+    for (int L = 0; L < 5; L++) ain[L] = Dq[(long)L * tot + cn];
+    const double pRo = f.V[cn];
+    const double roinv = 1.0 / pRo;
+    const double pVx = f.V[tot + cn] * roinv;
+    const double pVy = f.V[2 * tot + cn] * roinv;
+    const double pVz = f.V[3 * tot + cn] * roinv;
+    const double* S = f.SD + (long)d * tot;
+    const double aSDx = 0.5 * (S[cn + st] + S[cn]);
+    const double aSDy = 0.5 * (S[3 * tot + cn + st] + S[3 * tot + cn]);
+    const double aSDz = 0.5 * (S[6 * tot + cn + st] + S[6 * tot + cn]);
+    R dif[5];
+    lusgs_adq<R>(R(pVx), R(pVy), R(pVz), R(aSDx), R(aSDy), R(aSDz), R(f.PP[cn]), R(roinv), ain, dif);
+    R rdst;
+    if constexpr (sizeof(R) == 4)
+        rdst = f.Rdsf[(long)d * tot + cn];
+    else
+        rdst = f.Rds[(long)d * tot + cn];
+#pragma unroll
+    for (int L = 0; L < 5; L++) acc[L] = R(0.5) * (dif[L] + sgn * rdst * ain[L]);
+}
+
+// One workgroup per block walks the diagonal hyperplanes s = (i-1)+(j-1)+(k-1); every cell of
+// a hyperplane depends only on the previous one, so this order performs exactly the
+// reference's lexicographic Gauss-Seidel operations (Time/TIME_LUSGS.F90:172-436 forward,
+// :442-675 backward).  R = double or float (FP32 implicit operator); Dq is D or its FP32 copy.
+template <typename R, bool FORWARD>
+__global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, R* Dq,
+                                                      const BlkInfo* __restrict__ blks)
+{
+    const BlkInfo b = blks[blockIdx.x];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int nplanes = NI1 + NJ1 + NK1 - 2;
+    const int nlines = NJ1 * NK1;
+    const long tot = f.tot;
+    const long sy = b.nx, sz = (long)b.nx * b.ny;
+    const double dtm = p.dt_lusgs_main;
+    for (int s0 = 0; s0 < nplanes; s0++) {
+        const int s = FORWARD ? s0 : nplanes - 1 - s0;
+        for (int ln = threadIdx.x; ln < nlines; ln += blockDim.x) {
+            const int j = ln % NJ1 + 1, k = ln / NJ1 + 1;
+            const int i = s - (j - 1) - (k - 1) + 1;
+            if (i < 1 || i > NI1) continue;
+            const long c = cidx(b, i, j, k);
+            const double Dt = f.Dtm[c];
+            const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
+            const double cvs = stept / f.Vol[c];
+            R dI[5] = {0, 0, 0, 0, 0}, dJ[5] = {0, 0, 0, 0, 0}, dK[5] = {0, 0, 0, 0, 0};
+            if (FORWARD) {
+                if (i != 1) lusgs_nb<R>(f, Dq, 0, c - 1, 1, R(1), dI);
+                if (j != 1) lusgs_nb<R>(f, Dq, 1, c - sy, sy, R(1), dJ);
+                if (k != 1) lusgs_nb<R>(f, Dq, 2, c - sz, sz, R(1), dK);
+                const R f1 = R(f.F1[c]);
+#pragma unroll
+                for (int L = 0; L < 5; L++) {
+                    const long e = (long)L * tot + c;
+                    Dq[e] = (Dq[e] * R(stept) + (dI[L] + dJ[L] + dK[L]) * R(cvs)) / f1;
+                }
+            } else {
+                if (i != NI1) lusgs_nb<R>(f, Dq, 0, c + 1, 1, R(-1), dI);
+                if (j != NJ1) lusgs_nb<R>(f, Dq, 1, c + sy, sy, R(-1), dJ);
+                if (k != NK1) lusgs_nb<R>(f, Dq, 2, c + sz, sz, R(-1), dK);
+                const R f1 = R(f.F1[c]);
+#pragma unroll
+                for (int L = 0; L < 5; L++) {
+                    const long e = (long)L * tot + c;
+                    Dq[e] = Dq[e] - (dI[L] + dJ[L] + dK[L]) * R(cvs) / f1;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Copy D(1:ncomp) of interior cells between the FP64 and FP32 arrays.
+template <typename A, typename B>
+__global__ void __launch_bounds__(256) k_convert(int ncomp, int c0, long tot, const A* __restrict__ src,
+                                                 B* __restrict__ dst, const BlkInfo* __restrict__ blks,
+                                                 const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    for (int L = c0; L < c0 + ncomp; L++) dst[(long)L * tot + c] = (B)src[(long)L * tot + c];
+}
+
+// ---------------------------------------------------------------------------------------
+// Time/TIME_Update_V.F90:109-146 (norms of the increment, last maximum wins) and
+// :502-690 (positivity-damped update of rho, p, T, velocities, rho*E), the SST k/omega
+// update and limiter :709-817.  Per-workgroup partial norms go to `part`.
+struct NormPart {
+    double sum[3];
+    double mx[3];
+    int arg[3];
+    int pad;
+};
+
+__device__ __forceinline__ void norm_combine(double& s, double& m, int& a, double s2, double m2, int a2)
+{
+    s += s2;
+    if (m2 > m || (m2 == m && a2 > a)) {
+        m = m2;
+        a = a2;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_update_v(trans_params p, int ki, int if_turb, Fields f,
+                                                  const BlkInfo* __restrict__ blks,
+                                                  const Seg* __restrict__ seg, NormPart* part)
+{
+    __shared__ double ssum[3][256];
+    __shared__ double smax[3][256];
+    __shared__ int sarg[3][256];
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    const long tot = f.tot;
+    double dr[3] = {0.0, 0.0, 0.0};
+    int arg = -1;
+    if (lin < NI1 * NJ1 * NK1) {
+        int i, j, k;
+        decode3(lin, NI1, NJ1, i, j, k);
+        arg = lin;
+        const long c = cidx(b, i, j, k);
+        const double v1 = f.V[c];
+        dr[0] = fabs(f.D[c] / v1);
+        dr[1] = fabs(f.D[5 * tot + c] / v1);
+        dr[2] = fabs(f.D[6 * tot + c] / v1);
+
+        const double AinRo = f.D[c], AinRx = f.D[tot + c], AinRy = f.D[2 * tot + c];
+        const double AinRz = f.D[3 * tot + c], AinRe = f.D[4 * tot + c];
+        double pRo = v1;
+        const double roinv = 1.0 / pRo;
+        double pVx = f.V[tot + c] * roinv;
+        double pVy = f.V[2 * tot + c] * roinv;
+        double pVz = f.V[3 * tot + c] * roinv;
+        const double AinX = (AinRx - pVx * AinRo) * roinv;
+        const double AinY = (AinRy - pVy * AinRo) * roinv;
+        const double AinZ = (AinRz - pVz * AinRo) * roinv;
+        const double AinP = (1.4 - 1.0) * (AinRe - 0.5 * AinRo * (pVx * pVx + pVy * pVy + pVz * pVz) -
+                                           pRo * (AinX * pVx + AinY * pVy + AinZ * pVz));
+        if (AinRo >= 0.)
+            pRo = pRo + AinRo;
+        else
+            pRo = pRo * exp(AinRo / (pRo + 1.e-10));
+        pRo = fabs(pRo);
+        double pp = f.PP[c];
+        if (AinP >= 0.)
+            pp = pp + AinP;
+        else
+            pp = pp * exp(AinP / (pp + 1.e-10));
+        pp = fabs(pp);
+        f.PP[c] = pp;
+        f.T[c] = p.rxm2 * pp / pRo;
+        pVx = pVx + AinX;
+        pVy = pVy + AinY;
+        pVz = pVz + AinZ;
+        const double rad = f.rad[c];
+        f.V[c] = pRo;
+        f.V[tot + c] = pRo * pVx;
+        f.V[2 * tot + c] = pRo * pVy;
+        f.V[3 * tot + c] = pRo * pVz;
+        const double DV = 0.5 * (pVx * pVx + pVy * pVy + pVz * pVz);
+        const double v5 = pp / (1.4 - 1.0) + DV * pRo - 0.5 * sqr(p.omega1 * rad) * pRo;
+        f.V[4 * tot + c] = v5;
+
+        if (p.if_euler == 0 && if_turb == 1 && ki >= p.kitm) {
+            const double AinRTk = f.D[5 * tot + c], AinRTO = f.D[6 * tot + c];
+            double ARTk = f.V[5 * tot + c], ARTO = f.V[6 * tot + c];
+            if (AinRTk >= 0.0)
+                ARTk = ARTk + AinRTk;
+            else
+                ARTk = ARTk * exp(AinRTk / (ARTk + 1.e-10));
+            ARTk = fmax(ARTk, 1.e-10);
+            if (AinRTO >= 0.0)
+                ARTO = ARTO + AinRTO;
+            else
+                ARTO = ARTO * exp(AinRTO / (ARTO + 1.e-10));
+            ARTO = fmax(ARTO, 1.e-10);
+            const double VV = sqr(pRo * pVx) + sqr(pRo * pVy) + sqr(pRo * pVz);
+            const double PPP = (1.4 - 1.0) * v5 - 0.5 * (1.4 - 1.0) * (VV / pRo - sqr(p.omega1 * rad) * pRo);
+            const double TT = p.rxm2 * PPP / pRo;
+            const double RmiuL = sutherland(p, TT);
+            const double DistN = f.Dst[c];
+            const double* Q = f.DQ;
+            const double dudx = Q[c], dudy = Q[tot + c], dudz = Q[2 * tot + c];
+            const double dvdx = Q[3 * tot + c], dvdy = Q[4 * tot + c], dvdz = Q[5 * tot + c];
+            const double dwdx = Q[6 * tot + c], dwdy = Q[7 * tot + c], dwdz = Q[8 * tot + c];
+            const double Div = dudx + dvdy + dwdz;
+            const double prod = (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sqr(dudy + dvdx) +
+                                 sqr(dvdz + dwdy) + sqr(dwdx + dudz) - 2.0 / 3.0 * Div * Div);
+            if (p.kind_model == 1) {
+                double arg2 = 2.0 * sqrt(ARTk / pRo) / p.beta_star / (ARTO / pRo) / DistN / p.ref;
+                arg2 = fmax(arg2, 500.0 * RmiuL / DistN / DistN / ARTO / p.ref / p.ref);
+                arg2 = tanh(arg2 * arg2);
+                const double Sii = dudx, Sjj = dvdy, Skk = dwdz;
+                const double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
+                const double Skl = sqrt(2.0 * (1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk +
+                                               2.0 * Sij * Sij + 2.0 * Sik * Sik + 2.0 * Sjk * Sjk));
+                const double Cmua = fmin(1.0, p.a1 * ARTO / pRo / ((Skl)*arg2 + 1.e-10) * p.ref);
+                ARTO = fmax(ARTO, pow(pRo * Cmua * fabs(prod) / (p.prod_lmt * p.beta_star), 0.5) / p.ref);
+                ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo / Cmua);
+                f.V[5 * tot + c] = ARTk;
+                f.V[6 * tot + c] = ARTO;
+                f.Rmiu[c] = Cmua * ARTk / ARTO * pRo;
+            }
+        }
+    }
+    // workgroup reduction of sum(dr^2) and max(dr) with the last index winning ties
+    for (int q = 0; q < 3; q++) {
+        ssum[q][threadIdx.x] = dr[q] * dr[q];
+        smax[q][threadIdx.x] = arg >= 0 ? dr[q] : -1.0;
+        sarg[q][threadIdx.x] = arg;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            for (int q = 0; q < 3; q++) {
+                double sm = ssum[q][threadIdx.x], mx = smax[q][threadIdx.x];
+                int a = sarg[q][threadIdx.x];
+                norm_combine(sm, mx, a, ssum[q][threadIdx.x + w], smax[q][threadIdx.x + w],
+                             sarg[q][threadIdx.x + w]);
+                ssum[q][threadIdx.x] = sm;
+                smax[q][threadIdx.x] = mx;
+                sarg[q][threadIdx.x] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        NormPart o;
+        for (int q = 0; q < 3; q++) {
+            o.sum[q] = ssum[q][0];
+            o.mx[q] = smax[q][0];
+            o.arg[q] = sarg[q][0];
+        }
+        o.pad = 0;
+        part[blockIdx.x] = o;
+    }
+}
+
+// Final per-block norms from the partials of the block's workgroups [w0, w1).
+__global__ void k_norm_final(const NormPart* __restrict__ part, const int* __restrict__ wrange,
+                             const BlkInfo* __restrict__ blks, trans_resid* out)
+{
+    const int blk = blockIdx.x;
+    const BlkInfo b = blks[blk];
+    const int w0 = wrange[blk], w1 = wrange[blk + 1];
+    __shared__ NormPart sp[256];
+    NormPart acc;
+    for (int q = 0; q < 3; q++) {
+        acc.sum[q] = 0.0;
+        acc.mx[q] = -1.0;
+        acc.arg[q] = -1;
+    }
+    for (int w = w0 + threadIdx.x; w < w1; w += blockDim.x)
+        for (int q = 0; q < 3; q++)
+            norm_combine(acc.sum[q], acc.mx[q], acc.arg[q], part[w].sum[q], part[w].mx[q], part[w].arg[q]);
+    sp[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 3; q++)
+                norm_combine(sp[threadIdx.x].sum[q], sp[threadIdx.x].mx[q], sp[threadIdx.x].arg[q],
+                             sp[threadIdx.x + w].sum[q], sp[threadIdx.x + w].mx[q],
+                             sp[threadIdx.x + w].arg[q]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+        trans_resid r;
+        double* rs[3] = {&r.drsa, &r.drsk, &r.drso};
+        double* rm[3] = {&r.dua, &r.duk, &r.duo};
+        int* ri[3] = {&r.ima, &r.imk, &r.imo};
+        int* rj[3] = {&r.jma, &r.jmk, &r.jmo};
+        int* rk[3] = {&r.kma, &r.kmk, &r.kmo};
+        for (int q = 0; q < 3; q++) {
+            *rs[q] = sqrt(sp[0].sum[q] / NI1 / NJ1 / NK1);
+            *rm[q] = sp[0].mx[q];
+            const int a = sp[0].arg[q];
+            *ri[q] = a % NI1 + 1;
+            *rj[q] = (a / NI1) % NJ1 + 1;
+            *rk[q] = a / (NI1 * NJ1) + 1;
+        }
+        out[blk] = r;
+    }
+}
+
+// Time/TIME_Update_U.F90:64-92
+__global__ void __launch_bounds__(256) k_update_u(int kind_dual, Fields f,
+                                                  const BlkInfo* __restrict__ blks,
+                                                  const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    for (int L = 0; L < TRANS_ML; L++) {
+        const long e = (long)L * tot + c;
+        if (kind_dual == 1) {
+            f.Wt0[e] = f.V[e];
+        } else if (kind_dual == 2) {
+            f.Wt1[e] = f.Wt0[e];
+            f.Wt0[e] = f.V[e];
+        }
+    }
+}

@@ -1,0 +1,491 @@
+// C-ABI of the MI355X residual-and-update hot path (include/trans_hip.h).
+//
+// Unity build: the kernels live in the headers included below; this file owns the
+// context, the device layout, the schedule tables and the launch sequence that replaces
+// the body of the reference's pseudo-time loop (main.F90:65-197).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dev_common.h"
+#include "k_bc.h"
+#include "k_space.h"
+#include "k_time.h"
+
+namespace {
+
+constexpr int WG = 256;  // threads per workgroup of the cell/face kernels
+
+struct HostBlock {
+    int ni, nj, nk;
+    int nx, ny, nz;
+    long off;
+    long ncell() const { return (long)(ni - 1) * (nj - 1) * (nk - 1); }
+};
+
+struct SegTable {
+    Seg* d = nullptr;
+    int n = 0;
+    std::vector<int> wrange;  // per-block first workgroup (size nblk+1)
+};
+
+}  // namespace
+
+struct trans_ctx {
+    trans_params p{};
+    int ki = 1;
+    std::vector<HostBlock> hb;
+    std::vector<trans_patch> patches;      // all blocks' patches, concatenated
+    std::vector<int> patch0;               // first patch of each block
+    long tot = 0;
+    Fields f{};
+    BlkInfo* d_blk = nullptr;
+    DPatch* d_patch = nullptr;
+    BcSeg* d_bcseg = nullptr;
+    int n_bcseg = 0;
+    SegTable seg_cell, seg_face[3];
+    int* d_wrange = nullptr;
+    NormPart* d_part = nullptr;
+    trans_resid* d_resid = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    std::string err;
+};
+
+static int fail(trans_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, x)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return fail(ctx, TRANS_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+static int dalloc(trans_ctx* c, T** out, size_t n)
+{
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) return fail(c, TRANS_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return fail(c, TRANS_EHIP, "hipMemset");
+    c->allocs.push_back(p);
+    *out = (T*)p;
+    return 0;
+}
+
+static SegTable make_segs(trans_ctx* c, int kind)  // kind: 0 cells, 1..3 faces of dir
+{
+    std::vector<Seg> segs;
+    SegTable t;
+    t.wrange.push_back(0);
+    for (int b = 0; b < (int)c->hb.size(); b++) {
+        const HostBlock& h = c->hb[b];
+        long n0 = h.ni - 1 + (kind == 1), n1 = h.nj - 1 + (kind == 2), n2 = h.nk - 1 + (kind == 3);
+        long cnt = n0 * n1 * n2;
+        for (long base = 0; base < cnt; base += WG) segs.push_back(Seg{b, (int)base});
+        t.wrange.push_back((int)segs.size());
+    }
+    t.n = (int)segs.size();
+    if (dalloc(c, &t.d, segs.size()) == 0)
+        (void)hipMemcpy(t.d, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice);
+    return t;
+}
+
+extern "C" {
+
+const char* trans_hip_last_error(const trans_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static int check_params(trans_ctx* c, const trans_params* p)
+{
+    if (p->if_precondition != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_PRECONDITION must be 0");
+    if (p->kind_rotmod != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_RotMod must be 0");
+    if (p->cfl_sub_step < 1) return fail(c, TRANS_EINVAL, "CFL_Sub_step must be >= 1");
+    if (p->if_rk != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_RK=1 is not supported by this build");
+    if (p->kind_spatial != 2 && p->kind_spatial != 3)
+        return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 2 (MUSCL) or 3 (MDCD)");
+    if (p->kind_roe != 1 && p->kind_roe != 3)
+        return fail(c, TRANS_EUNSUPPORTED, "Kind_Roe must be 1 (Roe) or 3 (AUSMPW+)");
+    return 0;
+}
+
+int trans_hip_create(const trans_params* params, int nblk, const trans_block_desc* blocks,
+                     trans_ctx** out)
+{
+    if (!params || !blocks || !out || nblk <= 0) return TRANS_EINVAL;
+    trans_ctx* c = new trans_ctx();
+    *out = c;
+    int rc = check_params(c, params);
+    if (rc) return rc;
+    c->p = *params;
+    long off = 0;
+    for (int b = 0; b < nblk; b++) {
+        const trans_block_desc& d = blocks[b];
+        if (d.ni < 4 || d.nj < 4 || d.nk < 3) return fail(c, TRANS_EINVAL, "block too small");
+        HostBlock h;
+        h.ni = d.ni; h.nj = d.nj; h.nk = d.nk;
+        h.nx = d.ni + 5; h.ny = d.nj + 5; h.nz = d.nk + 5;
+        h.off = off;
+        off += (long)h.nx * h.ny * h.nz;
+        c->hb.push_back(h);
+        c->patch0.push_back((int)c->patches.size());
+        for (int q = 0; q < d.npatch; q++) c->patches.push_back(d.patches[q]);
+    }
+    c->patch0.push_back((int)c->patches.size());
+    c->tot = off;
+    for (int b = 0; b < nblk; b++)
+        for (int q = c->patch0[b]; q < c->patch0[b + 1]; q++) {
+            const trans_patch& pt = c->patches[q];
+            if (pt.bctype == 9 && (pt.target_block < 0 || pt.target_block >= nblk ||
+                                   pt.target_conn < 0 ||
+                                   pt.target_conn >= c->patch0[pt.target_block + 1] - c->patch0[pt.target_block]))
+                return fail(c, TRANS_EINVAL, "interface target out of range");
+        }
+    HIPCHK(c, hipStreamCreate(&c->stream));
+
+    Fields& f = c->f;
+    f.tot = off;
+    const long n = off;
+    if ((rc = dalloc(c, &f.V, 7 * n)) || (rc = dalloc(c, &f.PP, n)) || (rc = dalloc(c, &f.T, n)) ||
+        (rc = dalloc(c, &f.Rmiu, n)) || (rc = dalloc(c, &f.SD, 9 * n)) || (rc = dalloc(c, &f.Grad, 3 * n)) ||
+        (rc = dalloc(c, &f.Alagm, 3 * n)) || (rc = dalloc(c, &f.Vol, n)) || (rc = dalloc(c, &f.rad, n)) ||
+        (rc = dalloc(c, &f.Dst, n)) || (rc = dalloc(c, &f.aLeng, n)) || (rc = dalloc(c, &f.Yc, n)) ||
+        (rc = dalloc(c, &f.Zc, n)) || (rc = dalloc(c, &f.thtc, n)) || (rc = dalloc(c, &f.D, 7 * n)) ||
+        (rc = dalloc(c, &f.Rds, 3 * n)) || (rc = dalloc(c, &f.Dtm, n)) || (rc = dalloc(c, &f.Wt0, 7 * n)) ||
+        (rc = dalloc(c, &f.Wt1, 7 * n)) || (rc = dalloc(c, &f.Wt2, 7 * n)) || (rc = dalloc(c, &f.DQ, 18 * n)) ||
+        (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.G[0], 7 * n)) ||
+        (rc = dalloc(c, &f.G[1], 7 * n)) || (rc = dalloc(c, &f.G[2], 7 * n)))
+        return rc;
+    if (c->p.implicit_fp32) {
+        if ((rc = dalloc(c, &f.Ds, 7 * n)) || (rc = dalloc(c, &f.Rdsf, 3 * n))) return rc;
+    }
+    if ((rc = dalloc(c, &f.mw, 1))) return rc;
+
+    std::vector<BlkInfo> bi(nblk);
+    for (int b = 0; b < nblk; b++) {
+        const HostBlock& h = c->hb[b];
+        bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b]};
+    }
+    if ((rc = dalloc(c, &c->d_blk, nblk))) return rc;
+    HIPCHK(c, hipMemcpy(c->d_blk, bi.data(), nblk * sizeof(BlkInfo), hipMemcpyHostToDevice));
+
+    // BC patch table and its schedule
+    std::vector<DPatch> dp;
+    std::vector<BcSeg> bs;
+    for (int b = 0; b < nblk; b++)
+        for (int q = c->patch0[b]; q < c->patch0[b + 1]; q++) {
+            DPatch d{};
+            d.p = c->patches[q];
+            d.blk = b;
+            d.tpatch = d.p.bctype == 9 ? c->patch0[d.p.target_block] + d.p.target_conn : 0;
+            const int n0 = d.p.point_end[0] - d.p.point_start[0] + 1;
+            const int n1 = d.p.point_end[1] - d.p.point_start[1] + 1;
+            const int n2 = d.p.point_end[2] - d.p.point_start[2] + 1;
+            d.count = n0 * n1 * n2;
+            if (d.p.bctype == 7 && d.p.ijkdirect == 2)
+                return fail(c, TRANS_EUNSUPPORTED, "outlet on a j face");
+            dp.push_back(d);
+        }
+    for (int q = 0; q < (int)dp.size(); q++)
+        for (int base = 0; base < dp[q].count; base += 128) bs.push_back(BcSeg{q, base});
+    c->n_bcseg = (int)bs.size();
+    if ((rc = dalloc(c, &c->d_patch, dp.size())) || (rc = dalloc(c, &c->d_bcseg, bs.size()))) return rc;
+    if (!dp.empty()) HIPCHK(c, hipMemcpy(c->d_patch, dp.data(), dp.size() * sizeof(DPatch), hipMemcpyHostToDevice));
+    if (!bs.empty()) HIPCHK(c, hipMemcpy(c->d_bcseg, bs.data(), bs.size() * sizeof(BcSeg), hipMemcpyHostToDevice));
+
+    c->seg_cell = make_segs(c, 0);
+    for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
+    if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
+    HIPCHK(c, hipMemcpy(c->d_wrange, c->seg_cell.wrange.data(), (nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if ((rc = dalloc(c, &c->d_part, c->seg_cell.n)) || (rc = dalloc(c, &c->d_resid, nblk))) return rc;
+    return 0;
+}
+
+int trans_hip_destroy(trans_ctx* c)
+{
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int trans_hip_set_params(trans_ctx* c, const trans_params* p)
+{
+    if (!c || !p) return TRANS_EINVAL;
+    int rc = check_params(c, p);
+    if (rc) return rc;
+    if (p->implicit_fp32 != c->p.implicit_fp32)
+        return fail(c, TRANS_EINVAL, "implicit_fp32 is fixed at context creation");
+    c->p = *p;
+    return 0;
+}
+
+int trans_hip_set_iteration(trans_ctx* c, int ki)
+{
+    if (!c) return TRANS_EINVAL;
+    c->ki = ki;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------------------
+// host <-> device transfers through Fortran-layout views
+
+static int copy_view(trans_ctx* c, int blk, const trans_fview& v, double* dbase, int ncomp, bool upload)
+{
+    if (!v.p) return 0;
+    if (v.ncomp != ncomp) return fail(c, TRANS_EINVAL, "view has the wrong component count");
+    const HostBlock& h = c->hb[blk];
+    // clip the view to the device extent -2..N+2
+    int lo[3], hi[3];
+    const int dims[3] = {h.ni, h.nj, h.nk};
+    for (int d = 0; d < 3; d++) {
+        lo[d] = std::max(v.lo[d], -2);
+        hi[d] = std::min(v.hi[d], dims[d] + 2);
+        if (hi[d] < lo[d]) return 0;
+    }
+    const long vn0 = v.hi[0] - v.lo[0] + 1, vn1 = v.hi[1] - v.lo[1] + 1;
+    const long blkn = (long)h.nx * h.ny * h.nz;
+    std::vector<double> stage((size_t)ncomp * blkn);
+    for (int cc = 0; cc < ncomp; cc++) {
+        double* dst = dbase + (long)cc * c->tot + h.off;
+        if (!upload) {
+            hipError_t e = hipMemcpy(stage.data() + (size_t)cc * blkn, dst, blkn * sizeof(double),
+                                     hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return fail(c, TRANS_EHIP, hipGetErrorString(e));
+        } else {
+            hipError_t e = hipMemcpy(stage.data() + (size_t)cc * blkn, dst, blkn * sizeof(double),
+                                     hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return fail(c, TRANS_EHIP, hipGetErrorString(e));
+        }
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    const long vi = cc + (long)ncomp * ((i - v.lo[0]) + vn0 * ((j - v.lo[1]) + vn1 * (long)(k - v.lo[2])));
+                    const long di = (long)cc * blkn + (i + 2) + (long)h.nx * ((j + 2) + (long)h.ny * (k + 2));
+                    if (upload)
+                        stage[di] = v.p[vi];
+                    else
+                        v.p[vi] = stage[di];
+                }
+        if (upload) {
+            hipError_t e = hipMemcpy(dst, stage.data() + (size_t)cc * blkn, blkn * sizeof(double),
+                                     hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(c, TRANS_EHIP, hipGetErrorString(e));
+        }
+    }
+    return 0;
+}
+
+int trans_hip_set_geometry(trans_ctx* c, int blk, const trans_geom_view* g)
+{
+    if (!c || !g || blk < 0 || blk >= (int)c->hb.size()) return TRANS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Fields& f = c->f;
+    int rc;
+    if ((rc = copy_view(c, blk, g->sd, f.SD, 9, true)) || (rc = copy_view(c, blk, g->grad, f.Grad, 3, true)) ||
+        (rc = copy_view(c, blk, g->alagm, f.Alagm, 3, true)) || (rc = copy_view(c, blk, g->vol, f.Vol, 1, true)) ||
+        (rc = copy_view(c, blk, g->rad, f.rad, 1, true)) || (rc = copy_view(c, blk, g->dst, f.Dst, 1, true)) ||
+        (rc = copy_view(c, blk, g->aleng, f.aLeng, 1, true)) || (rc = copy_view(c, blk, g->yc, f.Yc, 1, true)) ||
+        (rc = copy_view(c, blk, g->zc, f.Zc, 1, true)) || (rc = copy_view(c, blk, g->thtc, f.thtc, 1, true)))
+        return rc;
+    return 0;
+}
+
+int trans_hip_set_state(trans_ctx* c, int blk, const trans_state_view* s)
+{
+    if (!c || !s || blk < 0 || blk >= (int)c->hb.size()) return TRANS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Fields& f = c->f;
+    int rc;
+    if ((rc = copy_view(c, blk, s->v, f.V, 7, true)) || (rc = copy_view(c, blk, s->pp, f.PP, 1, true)) ||
+        (rc = copy_view(c, blk, s->t, f.T, 1, true)) || (rc = copy_view(c, blk, s->rmiu, f.Rmiu, 1, true)) ||
+        (rc = copy_view(c, blk, s->wt0, f.Wt0, 7, true)) || (rc = copy_view(c, blk, s->wt1, f.Wt1, 7, true)) ||
+        (rc = copy_view(c, blk, s->wt2, f.Wt2, 7, true)))
+        return rc;
+    return 0;
+}
+
+int trans_hip_get_state(trans_ctx* c, int blk, const trans_state_view* s)
+{
+    if (!c || !s || blk < 0 || blk >= (int)c->hb.size()) return TRANS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Fields& f = c->f;
+    int rc;
+    if ((rc = copy_view(c, blk, s->v, f.V, 7, false)) || (rc = copy_view(c, blk, s->pp, f.PP, 1, false)) ||
+        (rc = copy_view(c, blk, s->t, f.T, 1, false)) || (rc = copy_view(c, blk, s->rmiu, f.Rmiu, 1, false)) ||
+        (rc = copy_view(c, blk, s->wt0, f.Wt0, 7, false)) || (rc = copy_view(c, blk, s->wt1, f.Wt1, 7, false)) ||
+        (rc = copy_view(c, blk, s->wt2, f.Wt2, 7, false)))
+        return rc;
+    return 0;
+}
+
+int trans_hip_get_field(trans_ctx* c, int blk, int which, const trans_fview* out)
+{
+    if (!c || !out || blk < 0 || blk >= (int)c->hb.size()) return TRANS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    Fields& f = c->f;
+    switch (which) {
+    case TRANS_FIELD_D: return copy_view(c, blk, *out, f.D, 7, false);
+    case TRANS_FIELD_RDS: return copy_view(c, blk, *out, f.Rds, 3, false);
+    case TRANS_FIELD_DTM: return copy_view(c, blk, *out, f.Dtm, 1, false);
+    case TRANS_FIELD_DQDXYZ: return copy_view(c, blk, *out, f.DQ, 18, false);
+    case TRANS_FIELD_SHOCK: return copy_view(c, blk, *out, f.shock, 1, false);
+    default: return fail(c, TRANS_EINVAL, "unknown field");
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// stages
+
+static int if_turb_now(const trans_ctx* c)
+{
+    const trans_params& p = c->p;
+    return (p.if_euler == 0 && p.if_turb_org == 1 && (c->ki >= p.kitm || p.kind_hybrid > 0)) ? 1 : 0;
+}
+
+static int launch_check(trans_ctx* c, const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(c, TRANS_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+    return 0;
+}
+
+int trans_hip_spectral_radius(trans_ctx* c)
+{
+    if (!c) return TRANS_EINVAL;
+    hipLaunchKernelGGL(k_spectral_radius, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p, c->ki, c->f,
+                       c->d_blk, c->seg_cell.d);
+    return launch_check(c, "k_spectral_radius");
+}
+
+int trans_hip_bc(trans_ctx* c, int sweep)
+{
+    if (!c || (sweep != -1 && sweep != 0)) return TRANS_EINVAL;
+    if (c->n_bcseg == 0) return 0;
+    hipLaunchKernelGGL(k_bc, dim3(c->n_bcseg), dim3(128), 0, c->stream, c->p, if_turb_now(c), sweep, c->f,
+                       c->d_blk, c->d_patch, c->d_bcseg);
+    return launch_check(c, "k_bc");
+}
+
+int trans_hip_space(trans_ctx* c)
+{
+    if (!c) return TRANS_EINVAL;
+    const int if_turb = if_turb_now(c);
+    if (c->p.if_euler == 0) return fail(c, TRANS_EUNSUPPORTED, "viscous path not built yet");
+    (void)if_turb;
+    // D = 0 (main.F90:94)
+    HIPCHK(c, hipMemsetAsync(c->f.D, 0, sizeof(double) * 7 * c->tot, c->stream));
+    hipLaunchKernelGGL(k_muscl_roe<1>, dim3(c->seg_face[0].n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+                       c->seg_face[0].d);
+    hipLaunchKernelGGL(k_muscl_roe<2>, dim3(c->seg_face[1].n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+                       c->seg_face[1].d);
+    hipLaunchKernelGGL(k_muscl_roe<3>, dim3(c->seg_face[2].n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+                       c->seg_face[2].d);
+    hipLaunchKernelGGL(k_accumulate, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, 5, c->f, c->d_blk,
+                       c->seg_cell.d);
+    return launch_check(c, "space");
+}
+
+int trans_hip_residual(trans_ctx* c)
+{
+    int rc;
+    if ((rc = trans_hip_spectral_radius(c))) return rc;
+    if ((rc = trans_hip_bc(c, -1))) return rc;
+    if ((rc = trans_hip_space(c))) return rc;
+    return 0;
+}
+
+static int lusgs_update_async(trans_ctx* c)
+{
+    const int nblk = (int)c->hb.size();
+    hipLaunchKernelGGL(k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+                       c->seg_cell.d);
+    if (c->p.implicit_fp32) {
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), 0, c->stream, 5, 0, c->tot,
+                           (const double*)c->f.D, c->f.Ds, c->d_blk, c->seg_cell.d);
+        hipLaunchKernelGGL((k_lusgs_sweep<float, true>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
+                           c->f.Ds, c->d_blk);
+        hipLaunchKernelGGL((k_lusgs_sweep<float, false>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
+                           c->f.Ds, c->d_blk);
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(c->seg_cell.n), dim3(WG), 0, c->stream, 5, 0, c->tot,
+                           (const float*)c->f.Ds, c->f.D, c->d_blk, c->seg_cell.d);
+    } else {
+        hipLaunchKernelGGL((k_lusgs_sweep<double, true>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
+                           c->f.D, c->d_blk);
+        hipLaunchKernelGGL((k_lusgs_sweep<double, false>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
+                           c->f.D, c->d_blk);
+    }
+    hipLaunchKernelGGL(k_update_v, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p, c->ki, if_turb_now(c), c->f,
+                       c->d_blk, c->seg_cell.d, c->d_part);
+    hipLaunchKernelGGL(k_norm_final, dim3(nblk), dim3(256), 0, c->stream, c->d_part, c->d_wrange, c->d_blk,
+                       c->d_resid);
+    return launch_check(c, "lusgs_update");
+}
+
+int trans_hip_lusgs_update(trans_ctx* c, trans_resid* per_block)
+{
+    if (!c) return TRANS_EINVAL;
+    int rc = lusgs_update_async(c);
+    if (rc) return rc;
+    if (per_block) {
+        HIPCHK(c, hipMemcpyAsync(per_block, c->d_resid, c->hb.size() * sizeof(trans_resid),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t b = 0; b < c->hb.size(); b++)
+            if (per_block[b].drsa != per_block[b].drsa)
+                return fail(c, TRANS_ENONFINITE, "residual of block " + std::to_string(b + 1) + " is NaN");
+    }
+    return 0;
+}
+
+int trans_hip_update_u(trans_ctx* c)
+{
+    if (!c) return TRANS_EINVAL;
+    hipLaunchKernelGGL(k_update_u, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p.kind_dual, c->f, c->d_blk,
+                       c->seg_cell.d);
+    return launch_check(c, "k_update_u");
+}
+
+int trans_hip_iterate(trans_ctx* c, int n_iter, trans_resid* per_block)
+{
+    if (!c || n_iter < 0) return TRANS_EINVAL;
+    int rc;
+    for (int it = 0; it < n_iter; it++) {
+        const int nsub = c->p.kind_dual != 0 ? c->p.kchld : 1;
+        for (int s = 0; s < nsub; s++) {
+            if ((rc = trans_hip_residual(c))) return rc;
+            if ((rc = lusgs_update_async(c))) return rc;
+        }
+        c->ki += 1;
+        if (c->p.kind_dual != 0)
+            if ((rc = trans_hip_update_u(c))) return rc;
+    }
+    if (per_block) {
+        HIPCHK(c, hipMemcpyAsync(per_block, c->d_resid, c->hb.size() * sizeof(trans_resid),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (size_t b = 0; b < c->hb.size(); b++)
+            if (per_block[b].drsa != per_block[b].drsa)
+                return fail(c, TRANS_ENONFINITE, "residual of block " + std::to_string(b + 1) + " is NaN");
+    }
+    return 0;
+}
+
+int trans_hip_sync(trans_ctx* c)
+{
+    if (!c) return TRANS_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,124 @@
+"""Python host of the C-ABI: the calls the reference's driver makes per iteration
+(main.F90:48-211), issued to libtrans_hip.so through ctypes."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi, mesh
+
+
+class TransError(RuntimeError):
+    def __init__(self, rc, msg):
+        super().__init__(f"trans_hip error {rc}: {msg}")
+        self.rc = rc
+
+
+class Solver:
+    def __init__(self, blocks, params: capi.TransParams, ki: int = 1):
+        self.lib = capi.load_library()
+        self.blocks = blocks
+        self.params = params
+        n = len(blocks)
+        self._patches = [mesh.patch_structs(b) for b in blocks]
+        descs = (capi.TransBlockDesc * n)()
+        for i, b in enumerate(blocks):
+            descs[i].ni, descs[i].nj, descs[i].nk = b.ni, b.nj, b.nk
+            descs[i].npatch = len(b.patches)
+            descs[i].patches = C.cast(self._patches[i], C.POINTER(capi.TransPatch))
+        self.ctx = C.c_void_p()
+        rc = self.lib.trans_hip_create(C.byref(params), n, descs, C.byref(self.ctx))
+        self._check(rc)
+        for i, b in enumerate(blocks):
+            self._check(self.lib.trans_hip_set_geometry(self.ctx, i, C.byref(mesh.geom_view(b))))
+            self._check(self.lib.trans_hip_set_state(self.ctx, i, C.byref(mesh.state_view(b))))
+        self.set_iteration(ki)
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.lib.trans_hip_last_error(self.ctx).decode() if self.ctx else ""
+            raise TransError(rc, msg)
+
+    def close(self):
+        if self.ctx:
+            self.lib.trans_hip_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- per-routine drop-ins
+    def set_iteration(self, ki):
+        self.ki = ki
+        self._check(self.lib.trans_hip_set_iteration(self.ctx, ki))
+
+    def set_params(self, params):
+        self.params = params
+        self._check(self.lib.trans_hip_set_params(self.ctx, C.byref(params)))
+
+    def spectral_radius(self):
+        self._check(self.lib.trans_hip_spectral_radius(self.ctx))
+
+    def bc(self, sweep):
+        self._check(self.lib.trans_hip_bc(self.ctx, sweep))
+
+    def space(self):
+        self._check(self.lib.trans_hip_space(self.ctx))
+
+    def residual(self):
+        self._check(self.lib.trans_hip_residual(self.ctx))
+
+    def lusgs_update(self):
+        res = (capi.TransResid * len(self.blocks))()
+        self._check(self.lib.trans_hip_lusgs_update(self.ctx, res))
+        return [r.as_dict() for r in res]
+
+    def update_u(self):
+        self._check(self.lib.trans_hip_update_u(self.ctx))
+
+    def iterate(self, n, want_resid=True):
+        res = (capi.TransResid * len(self.blocks))() if want_resid else None
+        self._check(self.lib.trans_hip_iterate(self.ctx, n, res))
+        self.ki += n
+        return [r.as_dict() for r in res] if want_resid else None
+
+    def sync(self):
+        self._check(self.lib.trans_hip_sync(self.ctx))
+
+    # ---- data out
+    def get_state(self, blk):
+        b = self.blocks[blk]
+        out = {}
+        shp = b.shape()
+        for nm, nc in (("v", 7), ("pp", 1), ("t", 1), ("rmiu", 1), ("wt0", 7), ("wt1", 7), ("wt2", 7)):
+            out[nm] = np.zeros(shp if nc == 1 else (nc,) + shp, order="F")
+        sv = capi.TransStateView()
+        for nm in out:
+            setattr(sv, nm, capi.fview(out[nm]))
+        self._check(self.lib.trans_hip_get_state(self.ctx, blk, C.byref(sv)))
+        return out
+
+    def get_field(self, blk, which):
+        b = self.blocks[blk]
+        nc = {capi.FIELD_D: 7, capi.FIELD_RDS: 3, capi.FIELD_DTM: 1, capi.FIELD_DQDXYZ: 18,
+              capi.FIELD_SHOCK: 1}[which]
+        shp = b.shape()
+        a = np.zeros(shp if nc == 1 else (nc,) + shp, order="F")
+        v = capi.fview(a)
+        self._check(self.lib.trans_hip_get_field(self.ctx, blk, which, C.byref(v)))
+        return a
+
+
+def prepare_case(blocks, org, glob, implicit_fp32=0):
+    """Connectivity, geometry and initial field for a generated case."""
+    from .params import make_params
+    prm = make_params(org, glob, implicit_fp32=implicit_fp32)
+    mesh.build_connectivity(blocks, nblade=glob.nblade, npassage=glob.Npassage)
+    mesh.attach_wall_panels(blocks)
+    mesh.compute_geometry(blocks)
+    mesh.initial_state(blocks, prm, q_dst=org.Q_Dst, if_ns_uniform=org.IF_NS_Uniform)
+    return prm
