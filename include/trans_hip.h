@@ -200,6 +200,15 @@ int trans_hip_iterate(trans_ctx* ctx, int n_iter, trans_resid* per_block);
 /* Wait for all queued device work of the context. */
 int trans_hip_sync(trans_ctx* ctx);
 
+/* Kernel timing (no counterpart in the reference, which has no timers): when on, every
+ * kernel launch of the context is bracketed by HIP events on the context's stream.
+ * trans_hip_profile(ctx, 1) also clears the totals.  profile_get returns the summed
+ * device time and the launch count of the idx-th kernel name. */
+int trans_hip_profile(trans_ctx* ctx, int on);
+int trans_hip_profile_count(trans_ctx* ctx);
+int trans_hip_profile_get(trans_ctx* ctx, int idx, char* name, int name_len, double* total_ms,
+                          long* launches);
+
 #ifdef __cplusplus
 }
 #endif

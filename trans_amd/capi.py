@@ -97,7 +97,8 @@ EXPORTS = [
     "trans_hip_set_iteration", "trans_hip_set_geometry", "trans_hip_set_state",
     "trans_hip_get_state", "trans_hip_get_field", "trans_hip_spectral_radius", "trans_hip_bc",
     "trans_hip_space", "trans_hip_residual", "trans_hip_lusgs_update", "trans_hip_update_u",
-    "trans_hip_iterate", "trans_hip_sync",
+    "trans_hip_iterate", "trans_hip_sync", "trans_hip_profile", "trans_hip_profile_count",
+    "trans_hip_profile_get",
 ]
 
 
@@ -149,6 +150,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_update_u.argtypes = [C.c_void_p]
     lib.trans_hip_iterate.argtypes = [C.c_void_p, C.c_int, P(TransResid)]
     lib.trans_hip_sync.argtypes = [C.c_void_p]
+    lib.trans_hip_profile.argtypes = [C.c_void_p, C.c_int]
+    lib.trans_hip_profile_count.argtypes = [C.c_void_p]
+    lib.trans_hip_profile_get.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_long)]
     for name in EXPORTS:
         if name != "trans_hip_last_error":
             getattr(lib, name).restype = C.c_int

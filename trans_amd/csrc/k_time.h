@@ -144,6 +144,145 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Line-march wavefront of the same sweeps.  The block is cut into lines along a march axis m
+// (j when the (i,k) cross-section fits one workgroup, so that consecutive lanes hold
+// consecutive i and every load is coalesced).  At step t the thread of line (a,b) updates the
+// cell of its line on hyperplane s = (i-1)+(j-1)+(k-1).  After updating its increment a cell
+// evaluates the three terms its downstream neighbours need, 0.5*(A_d dQ +/- Rds_d dQ) (exactly
+// the expressions of Time/TIME_LUSGS.F90:190-256 the neighbour would evaluate), keeps the
+// march-axis term in registers for its own next cell and hands the two others to the
+// neighbouring lines through ping-pong LDS.  Incoming terms are summed in i, j, k order as
+// the reference does.  One LDS-only barrier per step; the next cell's inputs are loaded
+// while the current one is computed.
+struct LineRaw {
+    double d[5];
+    double dtm, vol, f1;
+    double v[4], pp;
+    double sdm[3][3];  // face mean of SD in each direction
+    double rds[3];
+};
+
+template <typename R>
+__device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__ Dq, const long st[3],
+                                          long c, LineRaw& x)
+{
+    const long tot = f.tot;
+#pragma unroll
+    for (int L = 0; L < 5; L++) x.d[L] = (double)Dq[(long)L * tot + c];
+    x.dtm = f.Dtm[c];
+    x.vol = f.Vol[c];
+    x.f1 = f.F1[c];
+#pragma unroll
+    for (int L = 0; L < 4; L++) x.v[L] = f.V[(long)L * tot + c];
+    x.pp = f.PP[c];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const double* S = f.SD + (long)d * tot;
+#pragma unroll
+        for (int q = 0; q < 3; q++) x.sdm[d][q] = 0.5 * (S[(long)q * 3 * tot + c + st[d]] + S[(long)q * 3 * tot + c]);
+        if constexpr (sizeof(R) == 4)
+            x.rds[d] = (double)f.Rdsf[(long)d * tot + c];
+        else
+            x.rds[d] = f.Rds[(long)d * tot + c];
+    }
+}
+
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// march: 0 = i, 1 = j, 2 = k.  Lines are indexed a-fastest over the two other axes (a < b).
+template <typename R, bool FORWARD>
+__global__ void __launch_bounds__(1024) k_lusgs_line(trans_params p, Fields f, R* Dq,
+                                                     const BlkInfo* __restrict__ blks, int march)
+{
+    extern __shared__ unsigned char lds_raw[];
+    const BlkInfo b = blks[blockIdx.x];
+    const int N[3] = {b.ni - 1, b.nj - 1, b.nk - 1};
+    const int ax = march == 0 ? 1 : 0;
+    const int bx = march == 2 ? 1 : 2;
+    const int Na = N[ax], Nb = N[bx], Nm = N[march];
+    const int nlines = Na * Nb;
+    const int nplanes = N[0] + N[1] + N[2] - 2;
+    R* buf = reinterpret_cast<R*>(lds_raw);  // [2 parity][2 dir][5][nlines]
+    const long tot = f.tot;
+    const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+    const int ln = threadIdx.x;
+    const bool active = ln < nlines;
+    const int ia = active ? ln % Na + 1 : 1, ib = active ? ln / Na + 1 : 1;
+    const double dtm_main = p.dt_lusgs_main;
+    const R sgn = FORWARD ? R(1) : R(-1);
+    const int ln_a = FORWARD ? (ia < Na ? ln + 1 : -1) : (ia > 1 ? ln - 1 : -1);
+    const int ln_b = FORWARD ? (ib < Nb ? ln + Na : -1) : (ib > 1 ? ln - Na : -1);
+    const bool in_a = FORWARD ? (ia > 1) : (ia < Na);
+    const bool in_b = FORWARD ? (ib > 1) : (ib < Nb);
+    const int start = FORWARD ? (ia - 1) + (ib - 1) : (Na - ia) + (Nb - ib);
+    int pos[3];
+    pos[ax] = ia;
+    pos[bx] = ib;
+    pos[march] = FORWARD ? 1 : Nm;
+    const long cstep = FORWARD ? st[march] : -st[march];
+    long c = active ? cidx(b, pos[0], pos[1], pos[2]) : 0;
+    R tm[5] = {0, 0, 0, 0, 0};  // term from the previous cell of this line
+    LineRaw nxt;
+    if (active) line_load<R>(f, Dq, st, c, nxt);
+    for (int t = 0; t < nplanes; t++) {
+        const int par = t & 1;
+        const int rel = t - start;
+        if (active && rel >= 0 && rel < Nm) {
+            const LineRaw cur = nxt;
+            if (rel + 1 < Nm) line_load<R>(f, Dq, st, c + cstep, nxt);
+            R tin[3][5];
+#pragma unroll
+            for (int L = 0; L < 5; L++) {
+                tin[march][L] = tm[L];
+                tin[ax][L] = in_a ? buf[((par * 2 + 0) * 5 + L) * nlines + ln] : R(0);
+                tin[bx][L] = in_b ? buf[((par * 2 + 1) * 5 + L) * nlines + ln] : R(0);
+            }
+            const double stept = 2.0 * cur.dtm / (3.0 * cur.dtm / dtm_main + 2.0);
+            const double cvs = stept / cur.vol;
+            const R f1 = R(cur.f1);
+            R dq[5];
+#pragma unroll
+            for (int L = 0; L < 5; L++) {
+                const R sum = tin[0][L] + tin[1][L] + tin[2][L];
+                if (FORWARD)
+                    dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
+                else
+                    dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
+                Dq[(long)L * tot + c] = dq[L];
+            }
+            const double roinv = 1.0 / cur.v[0];
+            const R pVx = R(cur.v[1] * roinv), pVy = R(cur.v[2] * roinv), pVz = R(cur.v[3] * roinv);
+            const int npar = par ^ 1;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                R dif[5];
+                lusgs_adq<R>(pVx, pVy, pVz, R(cur.sdm[d][0]), R(cur.sdm[d][1]), R(cur.sdm[d][2]), R(cur.pp),
+                             R(roinv), dq, dif);
+                const R rd = R(cur.rds[d]);
+#pragma unroll
+                for (int L = 0; L < 5; L++) {
+                    const R o = R(0.5) * (dif[L] + sgn * rd * dq[L]);
+                    if (d == march)
+                        tm[L] = o;
+                    else if (d == ax) {
+                        if (ln_a >= 0) buf[((npar * 2 + 0) * 5 + L) * nlines + ln_a] = o;
+                    } else {
+                        if (ln_b >= 0) buf[((npar * 2 + 1) * 5 + L) * nlines + ln_b] = o;
+                    }
+                }
+            }
+            c += cstep;
+        }
+        lds_barrier();
+    }
+}
+
 // Copy D(1:ncomp) of interior cells between the FP64 and FP32 arrays.
 template <typename A, typename B>
 __global__ void __launch_bounds__(256) k_convert(int ncomp, int c0, long tot, const A* __restrict__ src,

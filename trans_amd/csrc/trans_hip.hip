@@ -35,8 +35,20 @@ struct SegTable {
 
 }  // namespace
 
+struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+};
+
 struct trans_ctx {
     trans_params p{};
+    // kernel profiler (HIP events on the context's stream)
+    bool prof = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> prof_pool;
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<long> prof_n;
     int ki = 1;
     std::vector<HostBlock> hb;
     std::vector<trans_patch> patches;      // all blocks' patches, concatenated
@@ -52,9 +64,79 @@ struct trans_ctx {
     NormPart* d_part = nullptr;
     trans_resid* d_resid = nullptr;
     hipStream_t stream = nullptr;
+    int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
+    int max_lines = 0;     // largest line count over the blocks for that axis
+    int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
     std::vector<void*> allocs;
     std::string err;
 };
+
+static hipEvent_t prof_event(trans_ctx* c)
+{
+    if (!c->prof_pool.empty()) {
+        hipEvent_t e = c->prof_pool.back();
+        c->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+static void prof_flush(trans_ctx* c)
+{
+    if (c->prof_pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (ProfRec& r : c->prof_pending) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        size_t q = 0;
+        while (q < c->prof_names.size() && c->prof_names[q] != r.name) q++;
+        if (q == c->prof_names.size()) {
+            c->prof_names.push_back(r.name);
+            c->prof_ms.push_back(0.0);
+            c->prof_n.push_back(0);
+        }
+        c->prof_ms[q] += ms;
+        c->prof_n[q] += 1;
+        c->prof_pool.push_back(r.a);
+        c->prof_pool.push_back(r.b);
+    }
+    c->prof_pending.clear();
+}
+
+// Records HIP events around one kernel launch when profiling is on.
+struct ProfScope {
+    trans_ctx* c;
+    ProfRec r{};
+    ProfScope(trans_ctx* cc, const char* name) : c(cc)
+    {
+        if (!c->prof) return;
+        r.name = name;
+        r.a = prof_event(c);
+        r.b = prof_event(c);
+        (void)hipEventRecord(r.a, c->stream);
+    }
+    ~ProfScope()
+    {
+        if (!c->prof) return;
+        (void)hipEventRecord(r.b, c->stream);
+        c->prof_pending.push_back(r);
+        if (c->prof_pending.size() >= 4096) prof_flush(c);
+    }
+};
+
+#define KLAUNCH_LDS(c, name, kern, grid, blk, lds, ...)                    \
+    do {                                                                   \
+        ProfScope ps_(c, name);                                            \
+        hipLaunchKernelGGL(kern, grid, blk, lds, (c)->stream, __VA_ARGS__); \
+    } while (0)
+
+#define KLAUNCH(c, name, kern, grid, blk, ...)                          \
+    do {                                                                 \
+        ProfScope ps_(c, name);                                          \
+        hipLaunchKernelGGL(kern, grid, blk, 0, (c)->stream, __VA_ARGS__); \
+    } while (0)
 
 static int fail(trans_ctx* c, int code, const std::string& msg)
 {
@@ -201,6 +283,24 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
     if (!dp.empty()) HIPCHK(c, hipMemcpy(c->d_patch, dp.data(), dp.size() * sizeof(DPatch), hipMemcpyHostToDevice));
     if (!bs.empty()) HIPCHK(c, hipMemcpy(c->d_bcseg, bs.data(), bs.size() * sizeof(BcSeg), hipMemcpyHostToDevice));
 
+    {
+        // march along j (lanes on i: coalesced) unless the (i,k) sections do not fit a workgroup
+        const int order[3] = {1, 2, 0};
+        const size_t esz = c->p.implicit_fp32 ? sizeof(float) : sizeof(double);
+        for (int q = 0; q < 3 && !c->line_threads; q++) {
+            const int m = order[q];
+            int mx = 0;
+            for (int b = 0; b < nblk; b++) {
+                const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
+                mx = std::max(mx, n[(m + 1) % 3] * n[(m + 2) % 3]);
+            }
+            if (mx <= 1024 && (size_t)20 * mx * esz <= 128 * 1024) {
+                c->march = m;
+                c->max_lines = mx;
+                c->line_threads = (mx + 63) / 64 * 64;
+            }
+        }
+    }
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
@@ -213,6 +313,11 @@ int trans_hip_destroy(trans_ctx* c)
 {
     if (!c) return 0;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (ProfRec& r : c->prof_pending) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -363,7 +468,7 @@ static int launch_check(trans_ctx* c, const char* what)
 int trans_hip_spectral_radius(trans_ctx* c)
 {
     if (!c) return TRANS_EINVAL;
-    hipLaunchKernelGGL(k_spectral_radius, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p, c->ki, c->f,
+    KLAUNCH(c, "k_spectral_radius", k_spectral_radius, dim3(c->seg_cell.n), dim3(WG), c->p, c->ki, c->f,
                        c->d_blk, c->seg_cell.d);
     return launch_check(c, "k_spectral_radius");
 }
@@ -372,7 +477,7 @@ int trans_hip_bc(trans_ctx* c, int sweep)
 {
     if (!c || (sweep != -1 && sweep != 0)) return TRANS_EINVAL;
     if (c->n_bcseg == 0) return 0;
-    hipLaunchKernelGGL(k_bc, dim3(c->n_bcseg), dim3(128), 0, c->stream, c->p, if_turb_now(c), sweep, c->f,
+    KLAUNCH(c, "k_bc", k_bc, dim3(c->n_bcseg), dim3(128), c->p, if_turb_now(c), sweep, c->f,
                        c->d_blk, c->d_patch, c->d_bcseg);
     return launch_check(c, "k_bc");
 }
@@ -385,13 +490,13 @@ int trans_hip_space(trans_ctx* c)
     (void)if_turb;
     // D = 0 (main.F90:94)
     HIPCHK(c, hipMemsetAsync(c->f.D, 0, sizeof(double) * 7 * c->tot, c->stream));
-    hipLaunchKernelGGL(k_muscl_roe<1>, dim3(c->seg_face[0].n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+    KLAUNCH(c, "k_muscl_roe<1>", k_muscl_roe<1>, dim3(c->seg_face[0].n), dim3(WG), c->p, c->f, c->d_blk,
                        c->seg_face[0].d);
-    hipLaunchKernelGGL(k_muscl_roe<2>, dim3(c->seg_face[1].n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+    KLAUNCH(c, "k_muscl_roe<2>", k_muscl_roe<2>, dim3(c->seg_face[1].n), dim3(WG), c->p, c->f, c->d_blk,
                        c->seg_face[1].d);
-    hipLaunchKernelGGL(k_muscl_roe<3>, dim3(c->seg_face[2].n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+    KLAUNCH(c, "k_muscl_roe<3>", k_muscl_roe<3>, dim3(c->seg_face[2].n), dim3(WG), c->p, c->f, c->d_blk,
                        c->seg_face[2].d);
-    hipLaunchKernelGGL(k_accumulate, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, 5, c->f, c->d_blk,
+    KLAUNCH(c, "k_accumulate", k_accumulate, dim3(c->seg_cell.n), dim3(WG), 5, c->f, c->d_blk,
                        c->seg_cell.d);
     return launch_check(c, "space");
 }
@@ -408,26 +513,42 @@ int trans_hip_residual(trans_ctx* c)
 static int lusgs_update_async(trans_ctx* c)
 {
     const int nblk = (int)c->hb.size();
-    hipLaunchKernelGGL(k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p, c->f, c->d_blk,
+    KLAUNCH(c, "k_lusgs_rhs", k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk,
                        c->seg_cell.d);
     if (c->p.implicit_fp32) {
-        hipLaunchKernelGGL((k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), 0, c->stream, 5, 0, c->tot,
+        KLAUNCH(c, "k_convert<double,float>", (k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), 5, 0, c->tot,
                            (const double*)c->f.D, c->f.Ds, c->d_blk, c->seg_cell.d);
-        hipLaunchKernelGGL((k_lusgs_sweep<float, true>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
-                           c->f.Ds, c->d_blk);
-        hipLaunchKernelGGL((k_lusgs_sweep<float, false>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
-                           c->f.Ds, c->d_blk);
-        hipLaunchKernelGGL((k_convert<float, double>), dim3(c->seg_cell.n), dim3(WG), 0, c->stream, 5, 0, c->tot,
+        if (c->line_threads) {
+            const size_t lds = (size_t)20 * c->max_lines * sizeof(float);
+            KLAUNCH_LDS(c, "k_lusgs_line<float,true>", (k_lusgs_line<float, true>), dim3(nblk), dim3(c->line_threads),
+                        lds, c->p, c->f, c->f.Ds, c->d_blk, c->march);
+            KLAUNCH_LDS(c, "k_lusgs_line<float,false>", (k_lusgs_line<float, false>), dim3(nblk),
+                        dim3(c->line_threads), lds, c->p, c->f, c->f.Ds, c->d_blk, c->march);
+        } else {
+            KLAUNCH(c, "k_lusgs_sweep<float,true>", (k_lusgs_sweep<float, true>), dim3(nblk), dim3(1024), c->p, c->f,
+                    c->f.Ds, c->d_blk);
+            KLAUNCH(c, "k_lusgs_sweep<float,false>", (k_lusgs_sweep<float, false>), dim3(nblk), dim3(1024), c->p,
+                    c->f, c->f.Ds, c->d_blk);
+        }
+        KLAUNCH(c, "k_convert<float,double>", (k_convert<float, double>), dim3(c->seg_cell.n), dim3(WG), 5, 0, c->tot,
                            (const float*)c->f.Ds, c->f.D, c->d_blk, c->seg_cell.d);
     } else {
-        hipLaunchKernelGGL((k_lusgs_sweep<double, true>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
-                           c->f.D, c->d_blk);
-        hipLaunchKernelGGL((k_lusgs_sweep<double, false>), dim3(nblk), dim3(1024), 0, c->stream, c->p, c->f,
-                           c->f.D, c->d_blk);
+        if (c->line_threads) {
+            const size_t lds = (size_t)20 * c->max_lines * sizeof(double);
+            KLAUNCH_LDS(c, "k_lusgs_line<double,true>", (k_lusgs_line<double, true>), dim3(nblk),
+                        dim3(c->line_threads), lds, c->p, c->f, c->f.D, c->d_blk, c->march);
+            KLAUNCH_LDS(c, "k_lusgs_line<double,false>", (k_lusgs_line<double, false>), dim3(nblk),
+                        dim3(c->line_threads), lds, c->p, c->f, c->f.D, c->d_blk, c->march);
+        } else {
+            KLAUNCH(c, "k_lusgs_sweep<double,true>", (k_lusgs_sweep<double, true>), dim3(nblk), dim3(1024), c->p,
+                    c->f, c->f.D, c->d_blk);
+            KLAUNCH(c, "k_lusgs_sweep<double,false>", (k_lusgs_sweep<double, false>), dim3(nblk), dim3(1024), c->p,
+                    c->f, c->f.D, c->d_blk);
+        }
     }
-    hipLaunchKernelGGL(k_update_v, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p, c->ki, if_turb_now(c), c->f,
+    KLAUNCH(c, "k_update_v", k_update_v, dim3(c->seg_cell.n), dim3(WG), c->p, c->ki, if_turb_now(c), c->f,
                        c->d_blk, c->seg_cell.d, c->d_part);
-    hipLaunchKernelGGL(k_norm_final, dim3(nblk), dim3(256), 0, c->stream, c->d_part, c->d_wrange, c->d_blk,
+    KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk,
                        c->d_resid);
     return launch_check(c, "lusgs_update");
 }
@@ -451,7 +572,7 @@ int trans_hip_lusgs_update(trans_ctx* c, trans_resid* per_block)
 int trans_hip_update_u(trans_ctx* c)
 {
     if (!c) return TRANS_EINVAL;
-    hipLaunchKernelGGL(k_update_u, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->p.kind_dual, c->f, c->d_blk,
+    KLAUNCH(c, "k_update_u", k_update_u, dim3(c->seg_cell.n), dim3(WG), c->p.kind_dual, c->f, c->d_blk,
                        c->seg_cell.d);
     return launch_check(c, "k_update_u");
 }
@@ -478,6 +599,40 @@ int trans_hip_iterate(trans_ctx* c, int n_iter, trans_resid* per_block)
             if (per_block[b].drsa != per_block[b].drsa)
                 return fail(c, TRANS_ENONFINITE, "residual of block " + std::to_string(b + 1) + " is NaN");
     }
+    return 0;
+}
+
+int trans_hip_profile(trans_ctx* c, int on)
+{
+    if (!c) return TRANS_EINVAL;
+    prof_flush(c);
+    c->prof = on != 0;
+    if (on) {
+        c->prof_names.clear();
+        c->prof_ms.clear();
+        c->prof_n.clear();
+    }
+    return 0;
+}
+
+int trans_hip_profile_count(trans_ctx* c)
+{
+    if (!c) return TRANS_EINVAL;
+    prof_flush(c);
+    return (int)c->prof_names.size();
+}
+
+int trans_hip_profile_get(trans_ctx* c, int idx, char* name, int name_len, double* total_ms, long* launches)
+{
+    if (!c || idx < 0) return TRANS_EINVAL;
+    prof_flush(c);
+    if (idx >= (int)c->prof_names.size()) return TRANS_EINVAL;
+    if (name && name_len > 0) {
+        std::strncpy(name, c->prof_names[idx].c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    if (total_ms) *total_ms = c->prof_ms[idx];
+    if (launches) *launches = c->prof_n[idx];
     return 0;
 }
 

@@ -86,6 +86,22 @@ class Solver:
         self.ki += n
         return [r.as_dict() for r in res] if want_resid else None
 
+    def profile(self, on=True):
+        self._check(self.lib.trans_hip_profile(self.ctx, 1 if on else 0))
+
+    def profile_stats(self):
+        """{kernel name: (total device ms, launches)} since profiling was switched on."""
+        n = self.lib.trans_hip_profile_count(self.ctx)
+        if n < 0:
+            self._check(n)
+        out = {}
+        buf = C.create_string_buffer(128)
+        for i in range(n):
+            ms, cnt = C.c_double(), C.c_long()
+            self._check(self.lib.trans_hip_profile_get(self.ctx, i, buf, 128, C.byref(ms), C.byref(cnt)))
+            out[buf.value.decode()] = (ms.value, cnt.value)
+        return out
+
     def sync(self):
         self._check(self.lib.trans_hip_sync(self.ctx))
 
