@@ -34,7 +34,7 @@ class OraBlock(C.Structure):
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
 
 
 _lib = None

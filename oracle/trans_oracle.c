@@ -11,38 +11,11 @@
 #include <stdlib.h>
 #include <string.h>
 
-/* Index of (i,j,k) in a -2:N+2 array of block b (N = points). */
-#define SX ((long)b->ni + 5)
-#define SY ((long)b->nj + 5)
-#define IX(i, j, k) ((long)((i) + 2) + SX * ((long)((j) + 2) + SY * (long)((k) + 2)))
+#include "oracle_idx.h"
 
-#define V(L, i, j, k) b->V[(L)-1 + 7 * IX(i, j, k)]
-#define D(L, i, j, k) b->D[(L)-1 + 7 * IX(i, j, k)]
-#define F(L, i, j, k) b->F[(L)-1 + 7 * IX(i, j, k)]
-#define WT0(L, i, j, k) b->Wt0[(L)-1 + 7 * IX(i, j, k)]
-#define WT1(L, i, j, k) b->Wt1[(L)-1 + 7 * IX(i, j, k)]
-#define PP(i, j, k) b->PP[IX(i, j, k)]
-#define T(i, j, k) b->T[IX(i, j, k)]
-#define RMIU(i, j, k) b->Rmiu[IX(i, j, k)]
-#define SD(d, x, i, j, k) b->SD[(d)-1 + 3 * ((x)-1) + 9 * IX(i, j, k)]
-#define GRAD(d, i, j, k) b->Grad[(d)-1 + 3 * IX(i, j, k)]
-#define ALAGM(d, i, j, k) b->Alagm[(d)-1 + 3 * IX(i, j, k)]
-#define VOL(i, j, k) b->Vol[IX(i, j, k)]
-#define RAD(i, j, k) b->rad[IX(i, j, k)]
-#define DST(i, j, k) b->Dst[IX(i, j, k)]
-#define YC(i, j, k) b->Yc[IX(i, j, k)]
-#define ZC(i, j, k) b->Zc[IX(i, j, k)]
-#define THTC(i, j, k) b->thtc[IX(i, j, k)]
-#define VL(d, L, i, j, k) b->VL[(d)-1 + 3 * ((L)-1) + 24 * IX(i, j, k)]
-#define VR(d, L, i, j, k) b->VR[(d)-1 + 3 * ((L)-1) + 24 * IX(i, j, k)]
-#define RCCL(d, i, j, k) b->rccl[(d)-1 + 3 * IX(i, j, k)]
-#define RCCR(d, i, j, k) b->rccr[(d)-1 + 3 * IX(i, j, k)]
-#define RDS(d, i, j, k) b->Rds[(d)-1 + 3 * IX(i, j, k)]
-#define DTM(i, j, k) b->Dtm[IX(i, j, k)]
-#define DQ(L, i, j, k) b->DQdxyz[(L)-1 + 18 * IX(i, j, k)]
-#define SHOCK(i, j, k) b->shock[IX(i, j, k)]
-
-static double sqr(double x) { return x * x; }
+double ora_sqr(double x) { return x * x; }
+#define sqr ora_sqr
+#define sutherland ora_sutherland
 
 /* main.F90:59-62 */
 int ora_if_turb(const trans_params* p, int ki)
@@ -51,7 +24,7 @@ int ora_if_turb(const trans_params* p, int ki)
 }
 
 /* Sutherland law as used by the mean flow (Time/SpectralRadius.F90:59). */
-static double sutherland(const trans_params* p, double tt)
+double ora_sutherland(const trans_params* p, double tt)
 {
     return (1.0 + p->csthlnd) / (tt * p->csth1 + p->csthlnd) * pow(p->csth1 * tt, 1.5);
 }
@@ -1047,18 +1020,26 @@ void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk)
     for (int ib = 0; ib < nblk; ib++) ora_spectral_radius(p, ki, &blocks[ib]);
     ora_bc(p, if_turb, blocks, nblk, -1);
     for (int ib = 0; ib < nblk; ib++) ora_spaceinterp(p, if_turb, &blocks[ib]);
-    /* DIFF_DqDxyz (viscous only) and BC(0) come with the viscous path */
+    if (p->if_euler == 0)
+        for (int ib = 0; ib < nblk; ib++) ora_diff_dqdxyz(p, if_turb, &blocks[ib]);
     ora_bc(p, if_turb, blocks, nblk, 0);
+    /* RCCorrection: Kind_RotMod = 0 leaves f_r1 = 1 */
     for (int ib = 0; ib < nblk; ib++) {
         ora_block* b = &blocks[ib];
         long n = 7L * (b->ni + 5) * (b->nj + 5) * (b->nk + 5);
         memset(b->D, 0, sizeof(double) * (size_t)n);
+        if (p->if_euler == 0) ora_diffusion(p, b);
+        if (if_turb) ora_turb_diffusion(p, b);
     }
     for (int ib = 0; ib < nblk; ib++) {
         if (p->kind_spatial == 2) ora_convect_muscl(p, &blocks[ib]);
+        if (p->kind_spatial == 3) ora_convect_mdcd(p, &blocks[ib]);
+        if (if_turb) ora_turb_convect_muscl(p, &blocks[ib]);
     }
     for (int ib = 0; ib < nblk; ib++) {
         if (p->kind_roe == 1) ora_convect_roe(p, &blocks[ib]);
+        if (p->kind_roe == 3) ora_convect_ausmpw(p, &blocks[ib]);
+        if (if_turb) ora_turb_convect_upwind(p, &blocks[ib]);
     }
 }
 
@@ -1068,6 +1049,7 @@ void ora_lusgs_update(const trans_params* p, int ki, ora_block* blocks, int nblk
     const int if_turb = ora_if_turb(p, ki);
     for (int ib = 0; ib < nblk; ib++) {
         ora_lusgs(p, &blocks[ib]);
+        if (if_turb) ora_turb_lusgs_sst(p, &blocks[ib]);
         ora_update_v(p, ki, if_turb, &blocks[ib]);
     }
 }

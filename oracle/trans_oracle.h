@@ -54,6 +54,19 @@ void ora_lusgs(const trans_params* p, ora_block* b);
 void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b);
 void ora_update_u(const trans_params* p, ora_block* b);
 
+void ora_diff_dqdxyz(const trans_params* p, int if_turb, ora_block* b);
+void ora_diffusion(const trans_params* p, ora_block* b);
+void ora_turb_diffusion(const trans_params* p, ora_block* b);
+void ora_convect_mdcd(const trans_params* p, ora_block* b);
+void ora_turb_convect_muscl(const trans_params* p, ora_block* b);
+void ora_convect_ausmpw(const trans_params* p, ora_block* b);
+void ora_turb_convect_upwind(const trans_params* p, ora_block* b);
+void ora_turb_lusgs_sst(const trans_params* p, ora_block* b);
+double ora_mdcd_rec(double gama, const double s[6], int right);
+void ora_ausmpw_common(const trans_params* p, double PLo, double PRo, double TLo, double TRo,
+                       double ULo, double URo, double VLo, double VRo, double WLo, double WRo,
+                       double sav1, double sav2, double sav3, double sav, double out[5]);
+
 /* Composite steps over all blocks, in main.F90 order. */
 void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk);
 void ora_lusgs_update(const trans_params* p, int ki, ora_block* blocks, int nblk);

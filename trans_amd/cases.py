@@ -62,18 +62,19 @@ def split_sizes(total_cells: int, nblk: int, lo: int, hi: int, seed: int):
 
 
 def flat_plate(nblk=8, ni_total=257, nj=129, nk=5, seed=20261202, re_l=5.0e6, mach=0.3,
-               ratio=1.1, yplus=1.0):
+               ratio=1.1, yplus=1.0, lo=17, hi=65):
     """Turbulent flat plate split into `nblk` blocks of uneven i-extent.
 
     The plate starts at x = 0 and ends at x = L (the block row covers a short
     symmetry run-in of 1/8 of the length); wall-normal geometric stretching at `ratio`
-    with the first cell height from y+ ~ 1 at the plate end (Cf ~ 0.0027)."""
+    with the first cell height from y+ ~ 1 at the plate end (Cf ~ 0.0027).  Block point
+    extents are drawn from [lo, hi]."""
     from .params import make_params
-    sizes = split_sizes(ni_total - 1, nblk, 17, 65, seed)
+    sizes = split_sizes(ni_total - 1, nblk, lo, hi, seed)
     org = OrgDeck(XM=mach, Tinf=288.0, Kind_Spatial=3, Kind_Roe=3, IF_Euler=0, IF_Turb_Org=1,
                   Kind_Model=1, IF_RK=0, Kind_Dual=2, Kchld=1, DT_LUSGS_Main=1.0e10,
                   CFL_Sub_start=5.0, CFL_Sub_end=5.0, CFL_Sub_step=1, KItm=1,
-                  gama_max=0.5, gama_min=0.01, Prod_Lmt=20.0, VisRatio=1.0, FSTI=0.01,
+                  gama_max=0.012, gama_min=0.001, Prod_Lmt=20.0, VisRatio=1.0, FSTI=0.01,
                   IF_NS_Uniform=1, MA=20000)
     glob = GlobalInfo(nblocks=nblk, T0in=288.0)
     ref = make_params(org, glob).ref
