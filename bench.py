@@ -21,42 +21,62 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-
-# Algorithmic bytes per unit of work of each kernel (FP64 unless stated): every input the
-# kernel needs read once and every output written once ("Roofline").
-ALG_BYTES = {
-    # per interior cell: D(5) r/w, V(1:4)+PP of the cell, 3 faces x SD(3), Rds(3), Dtm, Vol, F1
-    "k_lusgs_sweep<double,true>": ("cell", 8 * (5 + 5 + 5 + 9 + 3 + 3)),
-    "k_lusgs_sweep<double,false>": ("cell", 8 * (5 + 5 + 5 + 9 + 3 + 3)),
-    "k_lusgs_sweep<float,true>": ("cell", 4 * (5 + 5 + 3) + 8 * (5 + 9 + 3)),
-    "k_lusgs_sweep<float,false>": ("cell", 4 * (5 + 5 + 3) + 8 * (5 + 9 + 3)),
-    # per face: cell V(1:4), PP, T, rad, Alagm (each cell once), SD(3)+Grad, flux out (5)
-    "k_muscl_roe<1>": ("face1", 8 * (8 + 4 + 5)),
-    "k_muscl_roe<2>": ("face2", 8 * (8 + 4 + 5)),
-    "k_muscl_roe<3>": ("face3", 8 * (8 + 4 + 5)),
-    # per cell: V(1:4), PP, T, Rmiu, Vol, 3 faces SD(3) in; Rds(3), Dtm out
-    "k_spectral_radius": ("cell", 8 * (7 + 1 + 9 + 4)),
-    # per cell: D(5) in, 3 directions x 2 faces x flux(5) (faces shared -> 3 x 5), D(5) out
-    "k_accumulate": ("cell", 8 * (5 + 15 + 5)),
-    # per cell: D(5), V(5), Wt0(5), Wt1(5), Rds(3), Dtm, Vol in; D(5), F1 out
-    "k_lusgs_rhs": ("cell", 8 * (5 + 5 + 5 + 5 + 3 + 1 + 1 + 5 + 1)),
-    # per cell: D(7), V(7), PP, rad, (DQ 9, Dst, Rmiu when turbulent) in; V(7), PP, T, Rmiu out
-    "k_update_v": ("cell", 8 * (7 + 7 + 1 + 1 + 7 + 1 + 1)),
-    "k_update_u": ("cell", 8 * (7 + 7 + 7 + 7)),
-}
+METRIC = "Mcell\u00b7iter/s (RANS-SST, FP64 residual); solves/hr + p50/p99 to 1e-8 drop"
+UNIT = "Mcell\u00b7iter/s"
+# compulsory HBM bytes per cell of one fused iteration, FP64: 8*(N_in + N_geom + N_out) + 2*8*N_eq
+ITER_BYTES_RANS = 8 * (10 + 20 + 10) + 2 * 8 * 7    # 432
+ITER_BYTES_EULER = 8 * (7 + 17 + 7) + 2 * 8 * 5    # 328
 
 
-def build_workload(name):
+def alg_bytes(name, neq, fp32):
+    """Algorithmic bytes per unit of work of a kernel: every input it needs read once and
+    every output written once, in the precision it is stored in.  Returns (unit, bytes)."""
+    d = 8
+    r = 4 if fp32 else 8
+    turb = neq == 7
+    table = {
+        # V(1:4), PP, T, Rmiu, Vol, SD (9 face values per cell) in; Rds(3), Dtm out
+        "k_spectral_radius": ("cell", d * (4 + 1 + 1 + 1 + 1 + 9 + 3 + 1) + (4 * 3 if fp32 else 0)),
+        # V(1:4[,6:7]), T, Alagm(3), SD(9), Vol in; DQ(12[,18]), shock out
+        "k_gradients": ("cell", d * ((6 if turb else 4) + 1 + 3 + 9 + 1 + (18 if turb else 12) + 1)),
+        # per face of direction d: cell V(1:4,6:7), T, Rmiu, Dst, Alagm, Vol, DQ(18) once; SD(3); GV(6) out
+        "k_visc_face<1>": ("face1", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),
+        "k_visc_face<2>": ("face2", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),
+        "k_visc_face<3>": ("face3", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),
+        # per face: cell V(1:4[,6:7]), PP, T, rad, Alagm, shock once; SD(3), Grad; G(5[,7]) out
+        "k_conv_face<1>": ("face1", d * ((6 if turb else 4) + 5 + 4 + (7 if turb else 5))),
+        "k_conv_face<2>": ("face2", d * ((6 if turb else 4) + 5 + 4 + (7 if turb else 5))),
+        "k_conv_face<3>": ("face3", d * ((6 if turb else 4) + 5 + 4 + (7 if turb else 5))),
+        # per cell: G 3 x 7, GV 3 x 6 (viscous), V(1,3,4), Yc, Zc, Vol; D(7) out
+        "k_accumulate_visc": ("cell", d * (21 + 18 + 6 + 7)),
+        # D(5), V(5), Wt0(5), Wt1(5), Rds(3), Dtm, Vol in; D(5), F1 out
+        "k_lusgs_rhs": ("cell", d * (5 + 5 + 5 + 5 + 3 + 1 + 1 + 5 + 1)),
+        # D(6:7), V(1,6,7), Wt0, Wt1 (2 each), DQ(15), Rmiu, T, Dst, Dtm, Rds(3), Vol in; D(6:7), F1t out
+        "k_sst_rhs": ("cell", d * (2 + 3 + 2 + 2 + 15 + 1 + 1 + 1 + 1 + 3 + 1 + 2 + 1)),
+        # D(neq) in/out, Dtm, Vol, F1(,F1t), V(1:4), PP, SD(9), Rds(3)
+        "k_lusgs_line<fwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
+        "k_lusgs_line<bwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
+        "k_lusgs_sweep<fwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
+        "k_lusgs_sweep<bwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
+        # D(7), V(7), PP, rad, DQ(9), Dst in; V(7), PP, T, Rmiu out
+        "k_update_v": ("cell", d * (7 + 7 + 1 + 1 + 9 + 1 + 7 + 1 + 1 + 1)),
+        # V(7), Wt0(7) in; Wt0(7), Wt1(7) out
+        "k_update_u": ("cell", d * 28),
+    }
+    return table.get(name, ("cell", None))
+
+
+def build_workload(name, implicit_fp32=0):
     from trans_amd import cases, solver
     if name == "bump":
         blocks, org, glob = cases.bump_channel()
-        desc = "bump channel 129x65x9, Euler, Roe + MUSCL, LU-SGS CFL 5, FP64"
+        desc = "bump channel 129x65x9, Euler, Roe + MUSCL, LU-SGS CFL 5"
     elif name == "flatplate":
         blocks, org, glob = cases.flat_plate()
-        desc = "flat plate 257x129x5 in 8 uneven blocks, RANS-SST, AUSMPW+ + MDCD, LU-SGS, FP64"
+        desc = "flat plate 257x129x5 in 8 uneven blocks, RANS-SST, AUSMPW+ + MDCD, LU-SGS CFL 5"
     else:
         raise SystemExit(f"unknown workload {name}")
-    prm = solver.prepare_case(blocks, org, glob)
+    prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32)
     return blocks, prm, desc
 
 
@@ -84,9 +104,59 @@ def cpu_baseline(blocks, prm, budget_s=12.0):
             break
     dt = time.perf_counter() - t0
     cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
-    return {"value": cells * n / dt / 1e6, "unit": "Mcell*iter/s", "cores": 1, "kind": "port",
+    return {"value": cells * n / dt / 1e6, "unit": UNIT, "cores": 1, "kind": "port",
             "sample": f"{n} iterations of the same workload after 1 warm-up iteration, "
                       f"oracle/trans_oracle.c single-threaded, {dt:.1f} s"}
+
+
+def timed_run(blocks, prm, steps, warmup, prof_steps):
+    """Warm up, time `steps` iterations without instrumentation, then profile `prof_steps`
+    more with HIP events on the library's stream."""
+    from trans_amd.solver import Solver
+    sv = Solver(blocks, prm, ki=1)
+    sv.iterate(warmup, want_resid=False)
+    sv.sync()
+    t0 = time.perf_counter()
+    res = sv.iterate(steps, want_resid=True)
+    sv.sync()
+    dt = time.perf_counter() - t0
+    stats = {}
+    if prof_steps > 0:
+        sv.profile(True)
+        sv.iterate(prof_steps, want_resid=False)
+        sv.sync()
+        stats = sv.profile_stats()
+        sv.profile(False)
+    sv.close()
+    return dt, res, stats
+
+
+def global_drsa(res, blocks):
+    """sqrt(sum_b N_b DRSa_b^2 / sum_b N_b) over the blocks."""
+    n = [(b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks]
+    return (sum(nb * r["drsa"] ** 2 for nb, r in zip(n, res)) / sum(n)) ** 0.5
+
+
+def converge(blocks, prm, desc, max_iter, drop=1e-8, chunk=100):
+    """Iterate to `drop` x the first iteration's global DRSa (time-to-convergence)."""
+    from trans_amd.solver import Solver
+    cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
+    sv = Solver(blocks, prm, ki=1)
+    t0 = time.perf_counter()
+    r0 = global_drsa(sv.iterate(1), blocks)
+    it, r = 1, r0
+    print(f"[converge] {desc}: iteration 1 DRSa {r0:.4e}", flush=True)
+    while it < max_iter and r > drop * r0:
+        n = min(chunk, max_iter - it)
+        r = global_drsa(sv.iterate(n), blocks)
+        it += n
+        print(f"[converge] iteration {it} DRSa {r:.4e} drop {r / r0:.3e} "
+              f"{time.perf_counter() - t0:.1f} s", flush=True)
+    dt = time.perf_counter() - t0
+    sv.close()
+    print(json.dumps({"converge": {"iterations": it, "seconds": round(dt, 3), "drop": r / r0,
+                                   "reached": r <= drop * r0,
+                                   "Mcell_iter_per_s": round(cells * it / dt / 1e6, 3)}}))
 
 
 def main():
@@ -94,39 +164,39 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=os.environ.get("TRANS_BENCH_WORKLOAD", "bump"))
+    ap.add_argument("--workload", default=os.environ.get("TRANS_BENCH_WORKLOAD", "flatplate"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-implicit leg")
     ap.add_argument("--profile-only", action="store_true",
-                    help="run warmup+steps without the CPU leg and print nothing else")
+                    help="run warmup+steps of the FP64 path only and print the kernel table")
+    ap.add_argument("--converge", type=int, default=0,
+                    help="instead of the timed run: iterate up to N times to a 1e-8 drop of the "
+                         "global DRSa and print the history")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                                   "only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU sharding of the "
+                                   "blocks is a design only", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    from trans_amd.solver import Solver
     blocks, prm, desc = build_workload(args.workload)
     cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
-    sv = Solver(blocks, prm, ki=1)
-    print(f"[bench] {desc}: {cells} cells, warmup {args.warmup}", flush=True)
-    sv.iterate(args.warmup, want_resid=False)
-    sv.sync()
-    sv.profile(True)
-    sv.sync()
-    t0 = time.perf_counter()
-    res = sv.iterate(args.steps, want_resid=True)
-    sv.sync()
-    dt = time.perf_counter() - t0
-    stats = sv.profile_stats()
-    sv.profile(False)
-    print(f"[bench] {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)
+    if args.converge:
+        converge(blocks, prm, desc, args.converge)
+        return
+    neq = 7 if (prm.if_euler == 0 and prm.if_turb_org == 1) else 5
+    print(f"[bench] {desc}: {cells} cells, warmup {args.warmup}, steps {args.steps}", flush=True)
+    prof_steps = min(args.steps, 50)
+    dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps)
+    print(f"[bench] FP64: {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)
     if args.profile_only:
+        for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+            print(f"[bench] {k:28s} {v[0] / v[1] * 1e3:9.2f} us x {v[1]}")
         return
     u = units(blocks)
     # dominant kernel: largest total device time
     name, (ms, n) = max(stats.items(), key=lambda kv: kv[1][0])
     avg_s = ms / 1e3 / n
-    unit, per = ALG_BYTES.get(name, ("cell", None))
+    unit, per = alg_bytes(name, neq, False)
     roof = None
     if per is not None:
         alg = per * u[unit]
@@ -135,16 +205,35 @@ def main():
                 "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
                 "avg_launch_us": round(avg_s * 1e6, 3), "alg_bytes_per_launch": alg}
     value = cells * args.steps / dt / 1e6
+    kern = {}
+    for k, (kms, kn) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+        un, pb = alg_bytes(k, neq, False)
+        avg = kms / kn / 1e3
+        kern[k] = {"us": round(avg * 1e6, 2), "per_step": round(kn / prof_steps, 3),
+                   "GBs": round(pb * u[un] / avg / 1e9, 1) if pb else None}
+    fp32 = None
+    if not args.no_fp32:
+        b32, p32, _ = build_workload(args.workload, implicit_fp32=1)
+        dt32, res32, _ = timed_run(b32, p32, args.steps, args.warmup, 0)
+        fp32 = {"value": round(cells * args.steps / dt32 / 1e6, 3),
+                "ms_per_step": round(dt32 / args.steps * 1e3, 4),
+                "drsa_block1": res32[0]["drsa"], "drsa_block1_fp64": res[0]["drsa"]}
+        print(f"[bench] FP32 implicit: {args.steps} steps in {dt32:.3f} s", flush=True)
     cpu = None if args.no_cpu else cpu_baseline(blocks, prm)
+    # whole-iteration figure: compulsory bytes of one fused iteration per cell (state in and
+    # out once, geometry once, one round trip of the increment between the two sweeps)
+    b_it = ITER_BYTES_RANS if neq == 7 else ITER_BYTES_EULER
+    it_roof = {"alg_bytes_per_cell": b_it, "achieved": round(value * 1e6 * b_it / 1e9, 2),
+               "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(value * 1e6 * b_it / 1e9 / HBM_PEAK_GBS, 5)}
     out = {
-        "metric": "Mcell*iter/s (residual + LU-SGS update, FP64 residual)",
-        "value": round(value, 3), "unit": "Mcell*iter/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": round(value, 3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (generated grid, uniform start)",
-        "config": {"workload": desc, "cells": cells, "blocks": len(blocks)},
-        "roofline": roof, "cpu_baseline": cpu,
-        "kernels_ms": {k: round(v[0] / v[1], 4) for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0])},
+        "config": {"workload": desc, "cells": cells, "blocks": len(blocks), "implicit": "fp64"},
+        "roofline": roof, "iteration_roofline": it_roof, "cpu_baseline": cpu, "fp32_implicit": fp32,
+        "kernels": kern,
     }
     print(json.dumps(out))
 

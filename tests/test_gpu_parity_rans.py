@@ -1,0 +1,195 @@
+"""GPU parity of the RANS-SST path against the CPU oracle.
+
+Case: a small turbulent flat plate of three blocks of uneven i-extent (block interfaces
+through BC_P2P, total-pressure inlet, static-pressure outlet, a wall with a symmetry run-in,
+far field at jmax, symmetry side planes), with MDCD reconstruction + AUSMPW+ and the k-omega
+SST model, perturbed by a seeded +-1 % field (k, omega and mu_t included) so that every
+limiter and switch is exercised.
+
+Bars: ghost layers within 1e-14 per component (last-bit pow/sin/cos differences); gradients,
+shock sensor and residual D(1:7) from identical inputs within 1e-12 relative (max-norm per
+component; the device evaluates pow/tanh/exp in its own math library, whose last-bit
+differences from glibc stay below 1e-14 relative in one evaluation); one LU-SGS + update
+within 1e-12; ten iterations within 1e-6 (the same last-bit differences amplified through
+the limiters of the update, as for the inviscid path).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleCase
+from trans_amd import capi, cases, solver
+from trans_amd.solver import Solver
+
+from helpers import interior, perturb, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _plate(spatial=3, roe=3, seed=20261202, **kw):
+    args = dict(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
+    args.update(kw)
+    blocks, org, glob = cases.flat_plate(**args)
+    org.Kind_Spatial = spatial
+    org.Kind_Roe = roe
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=seed, turb=True)
+    return blocks, prm
+
+
+def _percomp(g, o):
+    """max |g - o| / max |o| per component (or overall for a scalar field)."""
+    if g.ndim == 4:
+        return np.array([np.max(np.abs(g[c] - o[c])) / max(np.max(np.abs(o[c])), 1e-300)
+                         for c in range(g.shape[0])])
+    return np.max(np.abs(g - o)) / max(np.max(np.abs(o)), 1e-300)
+
+
+def test_bc_interfaces_and_ghosts():
+    """All ghost layers after BC(-1), per component.  Interface copies, walls, symmetry and
+    the outlet are plain data movement and arithmetic; the far-field and inlet states call
+    pow/sin/cos, whose device results may differ from glibc in the last bit, hence 1e-14 of
+    each component's magnitude."""
+    print("case: flat plate 3 blocks 48x32x3 cells, BC(-1) all ghosts", flush=True)
+    blocks, prm = _plate()
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    oc.spectral_radius(1)
+    sv.spectral_radius()
+    oc.bc(1, -1)
+    sv.bc(-1)
+    for n in range(len(blocks)):
+        st = sv.get_state(n)
+        for nm in ("v", "pp", "t", "rmiu"):
+            e = _percomp(st[nm], oc[n][nm])
+            assert np.all(e <= 1e-14), (n, nm, e)
+    sv.close()
+
+
+def _post_bc_case(spatial, roe):
+    """Oracle spectral radius + BC(-1) on the perturbed case; the blocks then carry the
+    oracle's ghost layers, so that device and oracle residuals start from identical bits."""
+    blocks, prm = _plate(spatial, roe)
+    oc = OracleCase(blocks, prm)
+    oc.spectral_radius(1)
+    oc.bc(1, -1)
+    for n, b in enumerate(blocks):
+        for nm in ("v", "pp", "t", "rmiu"):
+            b[nm][...] = oc[n][nm]
+    return blocks, prm, oc
+
+
+@pytest.mark.parametrize("spatial,roe", [(3, 3), (2, 1), (3, 1), (2, 3)])
+def test_single_residual(spatial, roe):
+    """One residual evaluation (gradients, shock sensor, BC(0), viscous and k-omega diffusion,
+    reconstruction, inviscid and k-omega fluxes) from identical post-BC states.
+
+    Identical inputs matter for MUSCL: ALIMITER switches off when its arguments change sign
+    (factor |sgn a + sgn b| / 2), so a last-bit difference in a ghost value from a far-field
+    pow() can move a face state by 2/3 of the other argument.  The BC test above bounds
+    those ghost differences separately."""
+    print(f"case: flat plate, residual, Kind_Spatial={spatial} Kind_Roe={roe}", flush=True)
+    blocks, prm, oc = _post_bc_case(spatial, roe)
+    sv = Solver(blocks, prm, ki=1)
+    sv.spectral_radius()
+    sv.space()
+    oc.residual(1)  # BC(-1) is idempotent on an already filled halo
+    for n, b in enumerate(blocks):
+        dq = sv.get_field(n, capi.FIELD_DQDXYZ)
+        e = relerr(interior(dq, b), interior(oc[n]["DQdxyz"], b))
+        assert np.all(e < TOL), (n, "DQdxyz", e)
+        # ghost layers written by BC(0)
+        e = _percomp(dq, oc[n]["DQdxyz"])
+        assert np.all(e < TOL), (n, "DQdxyz ghosts", e)
+        sh = sv.get_field(n, capi.FIELD_SHOCK)
+        assert relerr(interior(sh, b), interior(oc[n]["shock"], b)) < TOL
+        d = sv.get_field(n, capi.FIELD_D)
+        e = relerr(interior(d, b), interior(oc[n]["D"], b))
+        print(n, "D relerr", e)
+        assert np.all(e < TOL), (n, "D", e)
+    sv.close()
+
+
+def test_composite_residual_mdcd():
+    """BC(-1) + residual on the device against the oracle for the configured scheme
+    (MDCD + AUSMPW+), whose smooth nonlinear weights keep last-bit ghost differences at the
+    last-bit level."""
+    print("case: flat plate, composite residual MDCD + AUSMPW+", flush=True)
+    blocks, prm = _plate(3, 3)
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    oc.residual(1)
+    sv.residual()
+    for n, b in enumerate(blocks):
+        d = sv.get_field(n, capi.FIELD_D)
+        e = relerr(interior(d, b), interior(oc[n]["D"], b))
+        print(n, "D relerr", e)
+        assert np.all(e < TOL), (n, "D", e)
+    sv.close()
+
+
+def test_lusgs_update_and_norms():
+    print("case: flat plate, residual + LU-SGS (mean + k-omega) + update", flush=True)
+    blocks, prm = _plate()
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    oc.residual(1)
+    sv.residual()
+    ores = oc.lusgs_update(1)
+    gres = sv.lusgs_update()
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        for nm in ("v", "pp", "t", "rmiu"):
+            e = relerr(interior(st[nm], b), interior(oc[n][nm], b))
+            print(n, nm, e)
+            assert np.all(np.asarray(e) < TOL), (n, nm)
+        for key in ("drsa", "dua", "drsk", "duk", "drso", "duo"):
+            assert abs(gres[n][key] - ores[n][key]) <= TOL * abs(ores[n][key]), (n, key)
+        for key in ("ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo"):
+            assert gres[n][key] == ores[n][key], (n, key)
+    sv.close()
+
+
+def test_ten_iterations():
+    print("case: flat plate, 10 iterations", flush=True)
+    blocks, prm = _plate()
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    oc.iterate(1, 10)
+    gres = sv.iterate(10)
+    ores = oc.resid()
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        e = relerr(interior(st["v"], b), interior(oc[n]["v"], b))
+        print(n, "state relerr after 10 iterations", e)
+        assert np.all(e < 1e-6)
+        for key in ("drsa", "drsk", "drso"):
+            assert abs(gres[n][key] - ores[n][key]) <= 1e-6 * ores[n][key], (n, key)
+    sv.close()
+
+
+def test_fp32_implicit_operator():
+    """FP32 LU-SGS sweeps (residual and update stay FP64): after one update the state
+    differs from the FP64 oracle by no more than 1e-5 of each component's increment scale
+    -- the sweep's rounding is ~1e-7 relative per operation and does not compound beyond
+    the ~100 steps of one wavefront."""
+    print("case: flat plate, FP32 implicit operator", flush=True)
+    blocks, prm = _plate()
+    oc = OracleCase(blocks, prm)
+    v0 = [b["v"].copy() for b in blocks]
+    p32 = capi.TransParams.from_buffer_copy(prm)
+    p32.implicit_fp32 = 1
+    sv = Solver(blocks, p32, ki=1)
+    oc.residual(1)
+    sv.residual()
+    oc.lusgs_update(1)
+    sv.lusgs_update()
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        dv_o = interior(oc[n]["v"], b) - interior(v0[n], b)
+        dv_g = interior(st["v"], b) - interior(v0[n], b)
+        e = relerr(dv_g, dv_o)
+        print(n, "FP32 increment relerr", e)
+        assert np.all(e < 1e-5)
+    sv.close()

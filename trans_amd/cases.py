@@ -65,6 +65,12 @@ def flat_plate(nblk=8, ni_total=257, nj=129, nk=5, seed=20261202, re_l=5.0e6, ma
                ratio=1.1, yplus=1.0, lo=17, hi=65):
     """Turbulent flat plate split into `nblk` blocks of uneven i-extent.
 
+    Inflow through the total-pressure inlet (BC 5) at the first block's imin, static-pressure
+    outlet (BC 7) at the last block's imax, far field (BC 2) at jmax.  The far-field BC is
+    kept off the min faces: its outward normal there is -2 times the unit normal
+    (BC/BC_far.F90:38), which doubles the reflection of the normal velocity and makes an
+    imin far-field inflow diverge after ~100 iterations.
+
     The plate starts at x = 0 and ends at x = L (the block row covers a short
     symmetry run-in of 1/8 of the length); wall-normal geometric stretching at `ratio`
     with the first cell height from y+ ~ 1 at the plate end (Cf ~ 0.0027).  Block point
@@ -76,7 +82,10 @@ def flat_plate(nblk=8, ni_total=257, nj=129, nk=5, seed=20261202, re_l=5.0e6, ma
                   CFL_Sub_start=5.0, CFL_Sub_end=5.0, CFL_Sub_step=1, KItm=1,
                   gama_max=0.012, gama_min=0.001, Prod_Lmt=20.0, VisRatio=1.0, FSTI=0.01,
                   IF_NS_Uniform=1, MA=20000)
-    glob = GlobalInfo(nblocks=nblk, T0in=288.0)
+    # freestream static state of the nondimensionalisation: rho = 1.2 kg/m^3, T = Tinf
+    p_inf = 1.2 * 287.0 * org.Tinf
+    stag = 1.0 + 0.2 * mach * mach
+    glob = GlobalInfo(nblocks=nblk, P0in=p_inf * stag ** 3.5, T0in=org.Tinf * stag, Pouthub=p_inf)
     ref = make_params(org, glob).ref
     L = re_l / ref  # grid length of the plate so that Re_L = re_l
     cf = 0.0027
@@ -100,14 +109,14 @@ def flat_plate(nblk=8, ni_total=257, nj=129, nk=5, seed=20261202, re_l=5.0e6, ma
         Y = np.broadcast_to(y1[None, :, None], (n, nj, nk))
         Z = np.broadcast_to(z1[None, None, :], (n, nj, nk))
         bcs = []
-        # imin
+        # imin: total-pressure inlet on the first block
         if ib == 0:
-            bcs.append(BCRow(1, 0, 0, 0, 0, 2))
+            bcs.append(BCRow(1, 0, 0, 0, 0, 5))
         else:
             bcs.append(BCRow(1, 0, 0, 0, 0, 9, 0, ib, 2))  # to previous block's imax (row 2)
-        # imax
+        # imax: static-pressure outlet on the last block
         if ib == nblk - 1:
-            bcs.append(BCRow(2, 0, 0, 0, 0, 2))
+            bcs.append(BCRow(2, 0, 0, 0, 0, 7))
         else:
             bcs.append(BCRow(2, 0, 0, 0, 0, 9, 0, ib + 2, 1))
         # jmin: symmetry ahead of the leading edge, wall on the plate

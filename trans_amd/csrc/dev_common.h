@@ -16,6 +16,7 @@ struct BlkInfo {
     long off;        // element offset of the block inside each field buffer
     int npatch;
     int patch0;      // first entry of this block in the device patch table
+    long mwoff;      // offset of the block's wall marks in Fields::mw
 };
 
 // One workgroup's share of an index space: block `blk`, linear indices base..base+255.
@@ -49,10 +50,12 @@ struct Fields {
     double* DQ;       // 18
     double* shock;
     double* F1;       // LU-SGS diagonal
-    double* G[3];     // area-weighted face fluxes per direction, 7 comps each
+    double* F1t;      // k-omega LU-SGS diagonal
+    double* G[3];     // area-weighted convective face fluxes per direction, 7 comps each
+    double* GV[3];    // viscous face fluxes per direction: x, y, z momentum, energy, k, omega
     float* Ds;        // FP32 copy of D for the FP32 implicit operator (7)
     float* Rdsf;      // FP32 Rds (3)
-    int* mw;          // wall marks, 6 faces per block, see BlkInfo (unused for Euler)
+    int* mw;          // wall marks per block: i0 (NJ1*NK1), im, j0 (NI1*NK1), jm, k0 (NI1*NJ1), km
 };
 
 __device__ __forceinline__ long cidx(const BlkInfo& b, int i, int j, int k)

@@ -157,80 +157,3 @@ __device__ __forceinline__ double face_avg(double am, double ap, double qm, doub
     const double alsinv = 1.0 / (ap + am);
     return (ap * qm + am * qp) * alsinv;
 }
-
-// ---------------------------------------------------------------------------------------
-// One thread per face of direction D (1..3): SPACEINTERP face averages at faces f-1, f, f+1,
-// CONVECT_MUSCL reconstruction (Space/CONVECT_MUSCL.F90:69-320; the left state of face f
-// comes from cell f-1, the right state from cell f) and CONVECT_Roe_Original
-// (Space/CONVECT_Roe_Original.F90:59-231).  Stores flux*|S| in G[D-1].
-template <int DIR>
-__global__ void __launch_bounds__(256) k_muscl_roe(trans_params p, Fields f,
-                                                   const BlkInfo* __restrict__ blks,
-                                                   const Seg* __restrict__ seg)
-{
-    const Seg s = seg[blockIdx.x];
-    const BlkInfo b = blks[s.blk];
-    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
-    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
-    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
-    const int lin = s.base + threadIdx.x;
-    if (lin >= n0 * n1 * n2) return;
-    int i, j, k;
-    decode3(lin, n0, n1, i, j, k);
-    const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
-    const long c0 = cidx(b, i, j, k);  // cell f (right of face f)
-    const long tot = f.tot;
-    const double* Al = f.Alagm + (long)(DIR - 1) * tot;
-    // cells f-2, f-1, f, f+1
-    const Prim qm2 = load_prim(f, c0 - 2 * st);
-    const Prim qm1 = load_prim(f, c0 - st);
-    const Prim q0 = load_prim(f, c0);
-    const Prim qp1 = load_prim(f, c0 + st);
-    const double am2 = Al[c0 - 2 * st], am1 = Al[c0 - st], a0 = Al[c0], ap1 = Al[c0 + st];
-    double qL[6], qR[6];
-#pragma unroll
-    for (int L = 0; L < 6; L++) {
-        const double avm = face_avg(am2, am1, qm2.q[L], qm1.q[L]);  // face f-1
-        const double av0 = face_avg(am1, a0, qm1.q[L], q0.q[L]);    // face f
-        const double avp = face_avg(a0, ap1, q0.q[L], qp1.q[L]);    // face f+1
-        qL[L] = qm1.q[L] + alimiter(av0 - qm1.q[L], qm1.q[L] - avm);
-        qR[L] = q0.q[L] - alimiter(q0.q[L] - av0, avp - q0.q[L]);
-    }
-    const double* S = f.SD + (long)(DIR - 1) * tot;
-    const double sav = f.Grad[(long)(DIR - 1) * tot + c0];
-    double fl[5];
-    roe_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
-               S[3 * tot + c0], S[6 * tot + c0], sav, qL[5] * p.omega1, qR[5] * p.omega1, fl);
-    double* G = f.G[DIR - 1];
-#pragma unroll
-    for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L] * sav;
-}
-
-// ---------------------------------------------------------------------------------------
-// Owner-computes sum of the scatter-adds of CONVECT_Roe_Original in the reference's order:
-// for each direction the cell's own face (subtract) then the next face (add).
-__global__ void __launch_bounds__(256) k_accumulate(int neq, Fields f,
-                                                    const BlkInfo* __restrict__ blks,
-                                                    const Seg* __restrict__ seg)
-{
-    const Seg s = seg[blockIdx.x];
-    const BlkInfo b = blks[s.blk];
-    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
-    const int lin = s.base + threadIdx.x;
-    if (lin >= NI1 * NJ1 * NK1) return;
-    int i, j, k;
-    decode3(lin, NI1, NJ1, i, j, k);
-    const long c = cidx(b, i, j, k);
-    const long tot = f.tot;
-    const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
-    for (int L = 0; L < neq; L++) {
-        double d = f.D[(long)L * tot + c];
-#pragma unroll
-        for (int dir = 0; dir < 3; dir++) {
-            const double* G = f.G[dir] + (long)L * tot;
-            d = d - G[c];
-            d = d + G[c + st[dir]];
-        }
-        f.D[(long)L * tot + c] = d;
-    }
-}

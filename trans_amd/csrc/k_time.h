@@ -92,11 +92,37 @@ __device__ __forceinline__ void lusgs_nb(const Fields& f, const R* __restrict__ 
     for (int L = 0; L < 5; L++) acc[L] = R(0.5) * (dif[L] + sgn * rdst * ain[L]);
 }
 
+// k-omega neighbour term 0.5*(V.S dQ +/- Rds dQ) (Time/TURBTWO_TIME_LUSGS_SST.F90:578-620).
+template <typename R>
+__device__ __forceinline__ void turb_nb(const Fields& f, const R* __restrict__ Dq, int d, long cn, long st,
+                                        R sgn, R acc[2])
+{
+    const long tot = f.tot;
+    const double roinv = 1.0 / f.V[cn];
+    const R pVx = R(f.V[tot + cn] * roinv), pVy = R(f.V[2 * tot + cn] * roinv), pVz = R(f.V[3 * tot + cn] * roinv);
+    const double* S = f.SD + (long)d * tot;
+    const R aSDx = R(0.5 * (S[cn + st] + S[cn]));
+    const R aSDy = R(0.5 * (S[3 * tot + cn + st] + S[3 * tot + cn]));
+    const R aSDz = R(0.5 * (S[6 * tot + cn + st] + S[6 * tot + cn]));
+    const R VP = pVx * aSDx + pVy * aSDy + pVz * aSDz;
+    R rdst;
+    if constexpr (sizeof(R) == 4)
+        rdst = f.Rdsf[(long)d * tot + cn];
+    else
+        rdst = f.Rds[(long)d * tot + cn];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const R dd = Dq[(long)(5 + q) * tot + cn];
+        acc[q] = R(0.5) * (dd * VP + sgn * rdst * dd);
+    }
+}
+
 // One workgroup per block walks the diagonal hyperplanes s = (i-1)+(j-1)+(k-1); every cell of
 // a hyperplane depends only on the previous one, so this order performs exactly the
 // reference's lexicographic Gauss-Seidel operations (Time/TIME_LUSGS.F90:172-436 forward,
-// :442-675 backward).  R = double or float (FP32 implicit operator); Dq is D or its FP32 copy.
-template <typename R, bool FORWARD>
+// :442-675 backward; NEQ = 7 adds Time/TURBTWO_TIME_LUSGS_SST.F90:550-766).
+// R = double or float (FP32 implicit operator); Dq is D or its FP32 copy.
+template <typename R, bool FORWARD, int NEQ>
 __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, R* Dq,
                                                       const BlkInfo* __restrict__ blks)
 {
@@ -105,7 +131,7 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
     const int nplanes = NI1 + NJ1 + NK1 - 2;
     const int nlines = NJ1 * NK1;
     const long tot = f.tot;
-    const long sy = b.nx, sz = (long)b.nx * b.ny;
+    const long st3[3] = {1, (long)b.nx, (long)b.nx * b.ny};
     const double dtm = p.dt_lusgs_main;
     for (int s0 = 0; s0 < nplanes; s0++) {
         const int s = FORWARD ? s0 : nplanes - 1 - s0;
@@ -117,26 +143,36 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
             const double Dt = f.Dtm[c];
             const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
             const double cvs = stept / f.Vol[c];
-            R dI[5] = {0, 0, 0, 0, 0}, dJ[5] = {0, 0, 0, 0, 0}, dK[5] = {0, 0, 0, 0, 0};
-            if (FORWARD) {
-                if (i != 1) lusgs_nb<R>(f, Dq, 0, c - 1, 1, R(1), dI);
-                if (j != 1) lusgs_nb<R>(f, Dq, 1, c - sy, sy, R(1), dJ);
-                if (k != 1) lusgs_nb<R>(f, Dq, 2, c - sz, sz, R(1), dK);
-                const R f1 = R(f.F1[c]);
+            R dn[3][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+            R tn[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+            const int pos[3] = {i, j, k};
+            const int nmax[3] = {NI1, NJ1, NK1};
 #pragma unroll
-                for (int L = 0; L < 5; L++) {
-                    const long e = (long)L * tot + c;
-                    Dq[e] = (Dq[e] * R(stept) + (dI[L] + dJ[L] + dK[L]) * R(cvs)) / f1;
-                }
-            } else {
-                if (i != NI1) lusgs_nb<R>(f, Dq, 0, c + 1, 1, R(-1), dI);
-                if (j != NJ1) lusgs_nb<R>(f, Dq, 1, c + sy, sy, R(-1), dJ);
-                if (k != NK1) lusgs_nb<R>(f, Dq, 2, c + sz, sz, R(-1), dK);
-                const R f1 = R(f.F1[c]);
+            for (int d = 0; d < 3; d++) {
+                const bool edge = FORWARD ? pos[d] == 1 : pos[d] == nmax[d];
+                if (edge) continue;
+                const long cn = FORWARD ? c - st3[d] : c + st3[d];
+                lusgs_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), dn[d]);
+                if (NEQ > 5) turb_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), tn[d]);
+            }
+            const R f1 = R(f.F1[c]);
 #pragma unroll
-                for (int L = 0; L < 5; L++) {
-                    const long e = (long)L * tot + c;
-                    Dq[e] = Dq[e] - (dI[L] + dJ[L] + dK[L]) * R(cvs) / f1;
+            for (int L = 0; L < 5; L++) {
+                const long e = (long)L * tot + c;
+                if (FORWARD)
+                    Dq[e] = (Dq[e] * R(stept) + (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs)) / f1;
+                else
+                    Dq[e] = Dq[e] - (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs) / f1;
+            }
+            if (NEQ > 5) {
+                const R f1t = R(f.F1t[c]);
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const long e = (long)(5 + q) * tot + c;
+                    if (FORWARD)
+                        Dq[e] = (Dq[e] * R(stept) + (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs)) / f1t;
+                    else
+                        Dq[e] = Dq[e] - (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs) / f1t;
                 }
             }
         }
@@ -155,24 +191,31 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
 // neighbouring lines through ping-pong LDS.  Incoming terms are summed in i, j, k order as
 // the reference does.  One LDS-only barrier per step; the next cell's inputs are loaded
 // while the current one is computed.
+//
+// NEQ = 7 adds the k-omega sweeps of Time/TURBTWO_TIME_LUSGS_SST.F90:550-766 to the same
+// wavefront: their flux Jacobian is the normal velocity V.S of the neighbour (no grid
+// velocity term) and their diagonal is F1t.  They read only their own increments, so running
+// them in the mean-flow wavefront performs the reference's operations in its order.
+template <int NEQ>
 struct LineRaw {
-    double d[5];
-    double dtm, vol, f1;
+    double d[NEQ];
+    double dtm, vol, f1, f1t;
     double v[4], pp;
     double sdm[3][3];  // face mean of SD in each direction
     double rds[3];
 };
 
-template <typename R>
+template <typename R, int NEQ>
 __device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__ Dq, const long st[3],
-                                          long c, LineRaw& x)
+                                          long c, LineRaw<NEQ>& x)
 {
     const long tot = f.tot;
 #pragma unroll
-    for (int L = 0; L < 5; L++) x.d[L] = (double)Dq[(long)L * tot + c];
+    for (int L = 0; L < NEQ; L++) x.d[L] = (double)Dq[(long)L * tot + c];
     x.dtm = f.Dtm[c];
     x.vol = f.Vol[c];
     x.f1 = f.F1[c];
+    x.f1t = NEQ > 5 ? f.F1t[c] : 1.0;
 #pragma unroll
     for (int L = 0; L < 4; L++) x.v[L] = f.V[(long)L * tot + c];
     x.pp = f.PP[c];
@@ -196,7 +239,8 @@ __device__ __forceinline__ void lds_barrier()
 }
 
 // march: 0 = i, 1 = j, 2 = k.  Lines are indexed a-fastest over the two other axes (a < b).
-template <typename R, bool FORWARD>
+// Dynamic LDS: 2 parities x 2 directions x NEQ x nlines values of R.
+template <typename R, bool FORWARD, int NEQ>
 __global__ void __launch_bounds__(1024) k_lusgs_line(trans_params p, Fields f, R* Dq,
                                                      const BlkInfo* __restrict__ blks, int march)
 {
@@ -208,7 +252,7 @@ __global__ void __launch_bounds__(1024) k_lusgs_line(trans_params p, Fields f, R
     const int Na = N[ax], Nb = N[bx], Nm = N[march];
     const int nlines = Na * Nb;
     const int nplanes = N[0] + N[1] + N[2] - 2;
-    R* buf = reinterpret_cast<R*>(lds_raw);  // [2 parity][2 dir][5][nlines]
+    R* buf = reinterpret_cast<R*>(lds_raw);  // [2 parity][2 dir][NEQ][nlines]
     const long tot = f.tot;
     const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
     const int ln = threadIdx.x;
@@ -227,28 +271,30 @@ __global__ void __launch_bounds__(1024) k_lusgs_line(trans_params p, Fields f, R
     pos[march] = FORWARD ? 1 : Nm;
     const long cstep = FORWARD ? st[march] : -st[march];
     long c = active ? cidx(b, pos[0], pos[1], pos[2]) : 0;
-    R tm[5] = {0, 0, 0, 0, 0};  // term from the previous cell of this line
-    LineRaw nxt;
-    if (active) line_load<R>(f, Dq, st, c, nxt);
+    R tm[NEQ];  // term from the previous cell of this line
+#pragma unroll
+    for (int L = 0; L < NEQ; L++) tm[L] = R(0);
+    LineRaw<NEQ> nxt;
+    if (active) line_load<R, NEQ>(f, Dq, st, c, nxt);
     for (int t = 0; t < nplanes; t++) {
         const int par = t & 1;
         const int rel = t - start;
         if (active && rel >= 0 && rel < Nm) {
-            const LineRaw cur = nxt;
-            if (rel + 1 < Nm) line_load<R>(f, Dq, st, c + cstep, nxt);
-            R tin[3][5];
+            const LineRaw<NEQ> cur = nxt;
+            if (rel + 1 < Nm) line_load<R, NEQ>(f, Dq, st, c + cstep, nxt);
+            R tin[3][NEQ];
 #pragma unroll
-            for (int L = 0; L < 5; L++) {
+            for (int L = 0; L < NEQ; L++) {
                 tin[march][L] = tm[L];
-                tin[ax][L] = in_a ? buf[((par * 2 + 0) * 5 + L) * nlines + ln] : R(0);
-                tin[bx][L] = in_b ? buf[((par * 2 + 1) * 5 + L) * nlines + ln] : R(0);
+                tin[ax][L] = in_a ? buf[((par * 2 + 0) * NEQ + L) * nlines + ln] : R(0);
+                tin[bx][L] = in_b ? buf[((par * 2 + 1) * NEQ + L) * nlines + ln] : R(0);
             }
             const double stept = 2.0 * cur.dtm / (3.0 * cur.dtm / dtm_main + 2.0);
             const double cvs = stept / cur.vol;
-            const R f1 = R(cur.f1);
-            R dq[5];
+            R dq[NEQ];
 #pragma unroll
-            for (int L = 0; L < 5; L++) {
+            for (int L = 0; L < NEQ; L++) {
+                const R f1 = L < 5 ? R(cur.f1) : R(cur.f1t);
                 const R sum = tin[0][L] + tin[1][L] + tin[2][L];
                 if (FORWARD)
                     dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
@@ -261,19 +307,26 @@ __global__ void __launch_bounds__(1024) k_lusgs_line(trans_params p, Fields f, R
             const int npar = par ^ 1;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
+                R o[NEQ];
                 R dif[5];
-                lusgs_adq<R>(pVx, pVy, pVz, R(cur.sdm[d][0]), R(cur.sdm[d][1]), R(cur.sdm[d][2]), R(cur.pp),
-                             R(roinv), dq, dif);
+                const R sx = R(cur.sdm[d][0]), sy = R(cur.sdm[d][1]), sz = R(cur.sdm[d][2]);
+                lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, R(cur.pp), R(roinv), dq, dif);
                 const R rd = R(cur.rds[d]);
 #pragma unroll
-                for (int L = 0; L < 5; L++) {
-                    const R o = R(0.5) * (dif[L] + sgn * rd * dq[L]);
+                for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
+                if (NEQ > 5) {
+                    const R VP = pVx * sx + pVy * sy + pVz * sz;
+#pragma unroll
+                    for (int L = 5; L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
+                }
+#pragma unroll
+                for (int L = 0; L < NEQ; L++) {
                     if (d == march)
-                        tm[L] = o;
+                        tm[L] = o[L];
                     else if (d == ax) {
-                        if (ln_a >= 0) buf[((npar * 2 + 0) * 5 + L) * nlines + ln_a] = o;
+                        if (ln_a >= 0) buf[((npar * 2 + 0) * NEQ + L) * nlines + ln_a] = o[L];
                     } else {
-                        if (ln_b >= 0) buf[((npar * 2 + 1) * 5 + L) * nlines + ln_b] = o;
+                        if (ln_b >= 0) buf[((npar * 2 + 1) * NEQ + L) * nlines + ln_b] = o[L];
                     }
                 }
             }

@@ -1,0 +1,622 @@
+// Viscous / RANS-SST residual kernels: Green-Gauss gradients with the shock sensor,
+// viscous and k-omega diffusion face fluxes, MDCD / MUSCL reconstruction with the AUSMPW+
+// or Roe flux and the k-omega upwind flux, and the ordered owner-computes accumulation.
+// Face averages of SPACEINTERP_GetSurfCenterVars are recomputed where they are needed
+// instead of being stored (the reference keeps 48 doubles per cell of VL/VR).
+#pragma once
+#include "dev_common.h"
+#include "k_space.h"
+
+// Cell value set used by face averages: u, v, w, T, k/rho, omega/rho.
+struct CellQ {
+    double u, v, w, t, k, o;
+};
+
+__device__ __forceinline__ CellQ load_q(const Fields& f, long c, bool turb)
+{
+    const long tot = f.tot;
+    CellQ q;
+    const double den = f.V[c];
+    q.u = f.V[tot + c] / den;
+    q.v = f.V[2 * tot + c] / den;
+    q.w = f.V[3 * tot + c] / den;
+    q.t = f.T[c];
+    if (turb) {
+        q.k = f.V[5 * tot + c] / den;
+        q.o = f.V[6 * tot + c] / den;
+    } else {
+        q.k = 0.0;
+        q.o = 0.0;
+    }
+    return q;
+}
+
+// face value between cell m (below) and p (above) in direction d (SPACEINTERP)
+__device__ __forceinline__ CellQ face_q(const Fields& f, int d, long cm, long cp, const CellQ& qm,
+                                        const CellQ& qp, bool turb)
+{
+    const double* Al = f.Alagm + (long)d * f.tot;
+    const double am = Al[cm], ap = Al[cp];
+    CellQ r;
+    r.u = face_avg(am, ap, qm.u, qp.u);
+    r.v = face_avg(am, ap, qm.v, qp.v);
+    r.w = face_avg(am, ap, qm.w, qp.w);
+    r.t = face_avg(am, ap, qm.t, qp.t);
+    if (turb) {
+        r.k = face_avg(am, ap, qm.k, qp.k);
+        r.o = face_avg(am, ap, qm.o, qp.o);
+    } else {
+        r.k = 0.0;
+        r.o = 0.0;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// Space/DIFF_DqDxyz.F90:25-78, :232-265 (interior cells) and the shock sensor :300-384.
+// The ghost-layer gradients of the reference are overwritten by BC(0) and are not formed.
+__global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    const bool turb = if_turb != 0;
+    const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+    const CellQ q0 = load_q(f, c, turb);
+    // face values on the low (m) and high (p) face of each direction
+    CellQ fm[3], fp[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const CellQ qm = load_q(f, c - st[d], turb);
+        const CellQ qp = load_q(f, c + st[d], turb);
+        fm[d] = face_q(f, d, c - st[d], c, qm, q0, turb);
+        fp[d] = face_q(f, d, c, c + st[d], q0, qp, turb);
+    }
+    const double volinv = 1.0 / f.Vol[c];
+    double g[18];
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        const double s1p = f.SD[(long)(3 * x + 0) * tot + c + st[0]], s1m = f.SD[(long)(3 * x + 0) * tot + c];
+        const double s2p = f.SD[(long)(3 * x + 1) * tot + c + st[1]], s2m = f.SD[(long)(3 * x + 1) * tot + c];
+        const double s3p = f.SD[(long)(3 * x + 2) * tot + c + st[2]], s3m = f.SD[(long)(3 * x + 2) * tot + c];
+#define GG(F) (fp[0].F * s1p - fm[0].F * s1m + fp[1].F * s2p - fm[1].F * s2m + fp[2].F * s3p - fm[2].F * s3m)
+        g[0 + x] = volinv * GG(u);
+        g[3 + x] = volinv * GG(v);
+        g[6 + x] = volinv * GG(w);
+        g[9 + x] = volinv * GG(t);
+        if (turb) {
+            g[12 + x] = volinv * GG(k);
+            g[15 + x] = volinv * GG(o);
+        }
+#undef GG
+    }
+    const int ng = turb ? 18 : 12;
+    for (int q = 0; q < ng; q++) f.DQ[(long)q * tot + c] = g[q];
+    const double DuDx = g[0], DuDy = g[1], DuDz = g[2], DvDx = g[3], DvDy = g[4], DvDz = g[5];
+    const double DwDx = g[6], DwDy = g[7], DwDz = g[8];
+    const double W12 = (DuDy - DvDx) / 2., W13 = (DuDz - DwDx) / 2., W23 = (DvDz - DwDy) / 2.;
+    const double Vort = sqrt(2. * (2. * W12 * W12 + 2. * W13 * W13 + 2. * W23 * W23));
+    const double DivU = sqrt(DuDx * DuDx + DvDy * DvDy + DwDz * DwDz);
+    f.shock[c] = DivU * DivU / fmax(DivU * DivU + Vort * Vort, 1.0);
+}
+
+// Wall mark of face (i,j,k) in direction DIR (MARKWALL arrays, Input/Read_BlockDimensions.F90:146-283)
+__device__ __forceinline__ bool wall_face(const int* __restrict__ m, const BlkInfo& b, int DIR, int i, int j,
+                                          int k)
+{
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    if (DIR == 1) {
+        if (i == 1) return m[(j - 1) + NJ1 * (k - 1)] == 1;
+        if (i == b.ni) return m[NJ1 * NK1 + (j - 1) + NJ1 * (k - 1)] == 1;
+    } else if (DIR == 2) {
+        const int o = 2 * NJ1 * NK1;
+        if (j == 1) return m[o + (i - 1) + NI1 * (k - 1)] == 1;
+        if (j == b.nj) return m[o + NI1 * NK1 + (i - 1) + NI1 * (k - 1)] == 1;
+    } else {
+        const int o = 2 * NJ1 * NK1 + 2 * NI1 * NK1;
+        if (k == 1) return m[o + (i - 1) + NI1 * (j - 1)] == 1;
+        if (k == b.nk) return m[o + NI1 * NJ1 + (i - 1) + NI1 * (j - 1)] == 1;
+    }
+    return false;
+}
+
+// face gradient: volume-weighted cell gradients + face-centred correction
+// (Space/DIFFUSION_Centered.F90:72-105 and its siblings)
+__device__ __forceinline__ void face_grad(const Fields& f, const double* Sd, long cf, long st, int q0,
+                                          long cm, long c0, double fpp, double fnn, double qp, double qn,
+                                          double all, double alr, double alsinv, double out[3])
+{
+    const long tot = f.tot;
+#pragma unroll
+    for (int x = 0; x < 3; x++) {
+        const double g1 = f.DQ[(long)(q0 + x) * tot + cm], g0 = f.DQ[(long)(q0 + x) * tot + c0];
+        const double gl = all * g1 + alr * g0;
+        const double* S = Sd + (long)x * 3 * tot;
+        out[x] = alsinv * (-S[cf + st] * (fpp - qp) - S[cf - st] * (qn - fnn) + S[cf] * (qp - qn)) + gl;
+    }
+}
+
+// One thread per face of direction DIR: viscous flux (Space/DIFFUSION_Centered.F90) and,
+// when turbulent, the k-omega diffusion flux (Space/TURBTWO_DIFFUSION_Centered.F90, SST).
+// GV[DIR-1] receives Fx, Fy, Fz, Fe, Fk, Fo.
+template <int DIR>
+__global__ void __launch_bounds__(256) k_visc_face(trans_params p, int if_turb, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
+    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
+    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= n0 * n1 * n2) return;
+    int i, j, k;
+    decode3(lin, n0, n1, i, j, k);
+    const bool turb = if_turb != 0;
+    const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
+    const long c0 = cidx(b, i, j, k);  // cell on the high side of face f
+    const long cm = c0 - st;
+    const long tot = f.tot;
+    const bool wall = wall_face(f.mw + b.mwoff, b, DIR, i, j, k);
+    // cells f-2, f-1, f, f+1 and the face values at faces f-1, f, f+1
+    const CellQ qm2 = load_q(f, c0 - 2 * st, turb);
+    const CellQ qm1 = load_q(f, cm, turb);
+    const CellQ q0 = load_q(f, c0, turb);
+    const CellQ qp1 = load_q(f, c0 + st, turb);
+    const int d = DIR - 1;
+    const CellQ fnn = face_q(f, d, c0 - 2 * st, cm, qm2, qm1, turb);
+    const CellQ f0 = face_q(f, d, cm, c0, qm1, q0, turb);
+    const CellQ fpp = face_q(f, d, c0, c0 + st, q0, qp1, turb);
+    const double* Sd = f.SD + (long)d * tot;  // component (d, x) at Sd[3*x*tot]
+
+    double alsinv = 1.0 / (f.Vol[c0] + f.Vol[cm]);
+    double all = f.Vol[cm] * alsinv;
+    double alr = f.Vol[c0] * alsinv;
+    if (wall) alsinv = 0.0;
+    double gu[3], gv[3], gw[3], gt[3];
+    face_grad(f, Sd, c0, st, 0, cm, c0, fpp.u, fnn.u, q0.u, qm1.u, all, alr, alsinv, gu);
+    face_grad(f, Sd, c0, st, 3, cm, c0, fpp.v, fnn.v, q0.v, qm1.v, all, alr, alsinv, gv);
+    face_grad(f, Sd, c0, st, 6, cm, c0, fpp.w, fnn.w, q0.w, qm1.w, all, alr, alsinv, gw);
+    face_grad(f, Sd, c0, st, 9, cm, c0, fpp.t, fnn.t, q0.t, qm1.t, all, alr, alsinv, gt);
+    double gk[3], go[3];
+    if (turb) {
+        face_grad(f, Sd, c0, st, 12, cm, c0, fpp.k, fnn.k, q0.k, qm1.k, all, alr, alsinv, gk);
+        face_grad(f, Sd, c0, st, 15, cm, c0, fpp.o, fnn.o, q0.o, qm1.o, all, alr, alsinv, go);
+    }
+    const double* Al = f.Alagm + (long)d * tot;
+    alsinv = 1.0 / (Al[c0] + Al[cm]);
+    all = Al[c0] * alsinv;
+    alr = Al[cm] * alsinv;
+    const double RmiuL = sutherland(p, f0.t);
+    double RmiuTL = all * f.Rmiu[cm] + alr * f.Rmiu[c0];
+    if (wall) RmiuTL = 0.0;
+    const double NMD = -2. / 3.;
+    const double DuDxL = gu[0], DuDyL = gu[1], DuDzL = gu[2];
+    const double DvDxL = gv[0], DvDyL = gv[1], DvDzL = gv[2];
+    const double DwDxL = gw[0], DwDyL = gw[1], DwDzL = gw[2];
+    const double TxxLam = RmiuL * (NMD * (DuDxL + DvDyL + DwDzL) + 2.0 * DuDxL);
+    const double TyyLam = RmiuL * (NMD * (DuDxL + DvDyL + DwDzL) + 2.0 * DvDyL);
+    const double TzzLam = RmiuL * (NMD * (DuDxL + DvDyL + DwDzL) + 2.0 * DwDzL);
+    const double TxyLam = RmiuL * (DuDyL + DvDxL);
+    const double TyzLam = RmiuL * (DvDzL + DwDyL);
+    const double TzxLam = RmiuL * (DuDzL + DwDxL);
+    const double TxxTur = RmiuTL * (NMD * (DuDxL + DvDyL + DwDzL) + 2.0 * DuDxL);
+    const double TyyTur = RmiuTL * (NMD * (DuDxL + DvDyL + DwDzL) + 2.0 * DvDyL);
+    const double TzzTur = RmiuTL * (NMD * (DuDxL + DvDyL + DwDzL) + 2.0 * DwDzL);
+    const double TxyTur = RmiuTL * (DuDyL + DvDxL);
+    const double TyzTur = RmiuTL * (DvDzL + DwDyL);
+    const double TzxTur = RmiuTL * (DuDzL + DwDxL);
+    const double Txx = TxxLam + TxxTur, Tyy = TyyLam + TyyTur, Tzz = TzzLam + TzzTur;
+    const double Txy = TxyLam + TxyTur, Tyz = TyzLam + TyzTur, Tzx = TzxLam + TzxTur;
+    const double prl = 0.72, prte = 0.9;
+    const double cp = 1.4 * p.ppf / (1.4 - 1.0);
+    const double akmu = cp * (RmiuL / prl + RmiuTL / prte);
+    const double ucc = f0.u, vcc = f0.v, wcc = f0.w;
+    const double Phix = akmu * gt[0] + ucc * Txx + vcc * Txy + wcc * Tzx;
+    const double Phiy = akmu * gt[1] + ucc * Txy + vcc * Tyy + wcc * Tyz;
+    const double Phiz = akmu * gt[2] + ucc * Tzx + vcc * Tyz + wcc * Tzz;
+    const double Dx = Sd[c0], Dy = Sd[3 * tot + c0], Dz = Sd[6 * tot + c0];
+    double* G = f.GV[d];
+    G[c0] = (Txx * Dx + Txy * Dy + Tzx * Dz) / p.ref;
+    G[tot + c0] = (Txy * Dx + Tyy * Dy + Tyz * Dz) / p.ref;
+    G[2 * tot + c0] = (Tzx * Dx + Tyz * Dy + Tzz * Dz) / p.ref;
+    G[3 * tot + c0] = (Phix * Dx + Phiy * Dy + Phiz * Dz) / p.ref;
+    if (!turb) return;
+    // k-omega diffusion (Space/TURBTWO_DIFFUSION_Centered.F90:40-214), same Alagm weights
+    const double TTL = all * f.T[cm] + alr * f.T[c0];
+    const double RmiuLt = (1.0 + p.csthlnd) / (TTL + p.csthlnd) * pow(TTL, 1.5);
+    const double DEN0 = f.V[c0];
+    const double TKK0 = f.V[5 * tot + c0] / DEN0, TOO0 = f.V[6 * tot + c0] / DEN0;
+    const double DEN1 = f.V[cm];
+    const double TKK1 = f.V[5 * tot + cm] / DEN1, TOO1 = f.V[6 * tot + cm] / DEN1;
+    const double Den = all * DEN1 + alr * DEN0;
+    const double TKK = all * TKK1 + alr * TKK0;
+    const double TOO = all * TOO1 + alr * TOO0;
+    const double dTKO = gk[0] * go[0] + gk[1] * go[1] + gk[2] * go[2];
+    const double CDKO = fmax(2.0 * Den * p.sigo2 / TOO * (dTKO), 1.0e-10);
+    const double distfc = alr * f.Dst[c0] + all * f.Dst[cm];
+    double arg1 = fmax(sqrt(TKK) / p.beta_star / TOO / distfc / p.ref,
+                       500.0 * RmiuLt / Den / distfc / distfc / TOO / p.ref / p.ref);
+    arg1 = fmin(arg1, 4.0 * Den * p.sigo2 * TKK / CDKO / distfc / distfc);
+    const double a2 = arg1 * arg1;
+    const double fc1c = tanh(a2 * a2);
+    const double SigkAB = fc1c * p.sigk1 + (1. - fc1c) * p.sigk2;
+    const double SigoAB = fc1c * p.sigo1 + (1. - fc1c) * p.sigo2;
+    const double vmute = RmiuLt + RmiuTL * SigkAB;
+    const double vmuoq = RmiuLt + RmiuTL * SigoAB;
+    G[4 * tot + c0] = (vmute * (gk[0] * Dx + gk[1] * Dy + gk[2] * Dz)) / p.ref;
+    G[5 * tot + c0] = (vmuoq * (go[0] * Dx + go[1] * Dy + go[2] * Dz)) / p.ref;
+}
+
+// ---------------------------------------------------------------------------------------
+// Space/CONVECT_MDCD.F90:41-88 -- one reconstruction from the six values a_2..a3.
+__device__ __forceinline__ double mdcd_rec(double gama, double v_2, double v_1, double v0, double v1,
+                                           double v2, double v3)
+{
+    const double beta = sqrt(13.0 / 12.0), epsm = 1.0e-6;
+    const double aw0 = 1.5 * (0.0463783 + gama);
+    const double aw1 = 0.5 - 1.5 * 0.0463783 + 4.5 * gama;
+    const double aw2 = 0.5 - 1.5 * 0.0463783 - 4.5 * gama;
+    const double aw3 = 1.5 * (0.0463783 - gama);
+    const double I0 = sqr(0.5 * v_2 - 2.0 * v_1 + 1.5 * v0) + sqr(beta * v_2 - 2.0 * beta * v_1 + beta * v0);
+    const double I1 = sqr(-0.5 * v_1 + 0.0 * v0 + 0.5 * v1) + sqr(beta * v_1 - 2.0 * beta * v0 + beta * v1);
+    const double I2 = sqr(-1.5 * v0 + 2.0 * v1 - 0.5 * v2) + sqr(beta * v0 - 2.0 * beta * v1 + beta * v2);
+    double I3 = sqr(2.5 * v1 - 4.0 * v2 + 1.5 * v3) + sqr(beta * v1 - 2.0 * beta * v2 + beta * v3);
+    I3 = fmax(fmax(fmax(I0, I1), I2), I3);
+    const double ak0 = aw0 / sqr(epsm + I0);
+    const double ak1 = aw1 / sqr(epsm + I1);
+    const double ak2 = aw2 / sqr(epsm + I2);
+    const double ak3 = aw3 / sqr(epsm + I3);
+    const double w0 = ak0 / (ak0 + ak1 + ak2 + ak3);
+    const double w1 = ak1 / (ak0 + ak1 + ak2 + ak3);
+    const double w2 = ak2 / (ak0 + ak1 + ak2 + ak3);
+    const double w3 = ak3 / (ak0 + ak1 + ak2 + ak3);
+    const double qq0 = 2. / 6. * v_2 - 7. / 6. * v_1 + 11. / 6. * v0;
+    const double qq1 = -1. / 6. * v_1 + 5. / 6. * v0 + 2. / 6. * v1;
+    const double qq2 = 2. / 6. * v0 + 5. / 6. * v1 - 1. / 6. * v2;
+    const double qq3 = 11. / 6. * v1 - 7. / 6. * v2 + 1. / 3. * v3;
+    return w0 * qq0 + w1 * qq1 + w2 * qq2 + w3 * qq3;
+}
+
+// Space/CONVECT_AUSMPW_Plus.F90:204-317 (the 3/16 coefficient is an integer division = 0)
+__device__ __forceinline__ void ausmpw_common(const trans_params& p, double PLo, double PRo, double TLo,
+                                              double TRo, double ULo, double URo, double VLo, double VRo,
+                                              double WLo, double WRo, double sav1, double sav2, double sav3,
+                                              double sav, double out[5])
+{
+    const double rcpcv = p.ppf, tiny = p.tiny;
+    const double DENLo = PLo / (rcpcv * TLo);
+    const double DENRo = PRo / (rcpcv * TRo);
+    const double s1 = sav1 / sav, s2 = sav2 / sav, s3 = sav3 / sav;
+    const double ga = 1.4, ga1 = 1.4 - 1.0;
+    const double VAMLo = ULo * ULo + VLo * VLo + WLo * WLo;
+    const double VAMRo = URo * URo + VRo * VRo + WRo * WRo;
+    const double HLo = PLo * ga / (DENLo * ga1) + 0.5 * VAMLo;
+    const double HRo = PRo * ga / (DENRo * ga1) + 0.5 * VAMRo;
+    const double ULn = (s1 * ULo + s2 * VLo + s3 * WLo);
+    const double URn = (s1 * URo + s2 * VRo + s3 * WRo);
+    const double astar_L = sqrt(2 * ga1 * HLo / (ga + 1));
+    const double astar_R = sqrt(2 * ga1 * HRo / (ga + 1));
+    const double aas_L = sqr(astar_L) / fmax(sqrt(VAMLo), astar_L);
+    const double aas_R = sqr(astar_R) / fmax(sqrt(VAMRo), astar_R);
+    const double Acc = fmin(aas_L, aas_R);
+    const double MaL = ULn / Acc, MaR = URn / Acc;
+    const double c316 = 0.0;
+    double MaL_plus, pL_plus, MaR_neg, pR_neg;
+    if (fabs(MaL) > 1.0) {
+        MaL_plus = 0.5 * (MaL + fabs(MaL));
+        pL_plus = 0.5 * (MaL + fabs(MaL)) / (MaL + tiny);
+    } else {
+        MaL_plus = 0.25 * sqr(MaL + 1);
+        pL_plus = 0.25 * sqr(MaL + 1) * (2 - MaL) + c316 * MaL * sqr(MaL * MaL - 1);
+    }
+    if (fabs(MaR) > 1.0) {
+        MaR_neg = 0.5 * (MaR - fabs(MaR));
+        pR_neg = 0.5 * (MaR - fabs(MaR)) / (MaR + tiny);
+    } else {
+        MaR_neg = (-1) * 0.25 * sqr(MaR - 1);
+        pR_neg = 0.25 * sqr(MaR - 1) * (2 + MaR) - c316 * MaR * sqr(MaR * MaR - 1);
+    }
+    const double pS = pL_plus * PLo + pR_neg * PRo;
+    const double Mas = MaL_plus + MaR_neg;
+    const double mn = fmin(PLo / PRo, PRo / PLo);
+    const double wP = 1.0 - (mn * mn) * mn;
+    const double fL = fabs(MaL) < 1 ? PLo / (pS + tiny) - 1 : 0;
+    const double fR = fabs(MaR) < 1 ? PRo / (pS + tiny) - 1 : 0;
+    double MMaL_plus, MMaR_neg;
+    if (Mas >= 0) {
+        MMaL_plus = MaL_plus + MaR_neg * ((1 - wP) * (1 + fR) - fL);
+        MMaR_neg = MaR_neg * wP * (1 + fR);
+    } else {
+        MMaL_plus = MaL_plus * wP * (1 + fL);
+        MMaR_neg = MaR_neg + MaL_plus * ((1 - wP) * (1 + fL) - fR);
+    }
+    out[0] = (-1) * MMaL_plus * Acc * DENLo - MMaR_neg * Acc * DENRo;
+    out[1] = (-1) * MMaL_plus * Acc * DENLo * ULo - MMaR_neg * Acc * DENRo * URo - s1 * (pL_plus * PLo + pR_neg * PRo);
+    out[2] = (-1) * MMaL_plus * Acc * DENLo * VLo - MMaR_neg * Acc * DENRo * VRo - s2 * (pL_plus * PLo + pR_neg * PRo);
+    out[3] = (-1) * MMaL_plus * Acc * DENLo * WLo - MMaR_neg * Acc * DENRo * WRo - s3 * (pL_plus * PLo + pR_neg * PRo);
+    out[4] = (-1) * MMaL_plus * Acc * DENLo * HLo - MMaR_neg * Acc * DENRo * HRo;
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// One thread per face of direction DIR: reconstruction (SPATIAL 2 = CONVECT_MUSCL,
+// 3 = CONVECT_MDCD), inviscid flux (ROE 1 = CONVECT_Roe_Original, 3 = CONVECT_AUSMPW_Plus)
+// and, when turbulent, TURBTWO_CONVECT_MUSCL + TURBTWO_CONVECT_Upwind.  G[DIR-1] receives
+// flux*|S| (5 comps) and the area-weighted k, omega fluxes (comps 5, 6).
+template <int DIR, int SPATIAL, int ROE>
+__global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
+    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
+    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= n0 * n1 * n2) return;
+    int i, j, k;
+    decode3(lin, n0, n1, i, j, k);
+    const bool turb = if_turb != 0;
+    const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
+    const long c0 = cidx(b, i, j, k);
+    const long tot = f.tot;
+    const int d = DIR - 1;
+    const double* Al = f.Alagm + (long)d * tot;
+    double qL[6], qR[6];
+    if (SPATIAL == 3) {
+        // six cells f-3..f+2; the shock sensor's ghost values are those of the nearest
+        // interior cell (Space/DIFF_DqDxyz.F90:388-397)
+        double sv[6][6];
+        double sensor = 0.0;
+        const int pos[3] = {i, j, k};
+        const int nmax[3] = {b.ni - 1, b.nj - 1, b.nk - 1};
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            const long c = c0 + (long)(n - 3) * st;
+            const double den = f.V[c];
+            sv[0][n] = f.V[tot + c] / den;
+            sv[1][n] = f.V[2 * tot + c] / den;
+            sv[2][n] = f.V[3 * tot + c] / den;
+            sv[3][n] = f.PP[c];
+            sv[4][n] = f.T[c];
+            sv[5][n] = f.rad[c];
+            int cp[3] = {pos[0], pos[1], pos[2]};
+            cp[d] += n - 3;
+            const long cs = cidx(b, clampi(cp[0], 1, nmax[0]), clampi(cp[1], 1, nmax[1]), clampi(cp[2], 1, nmax[2]));
+            const double sh = f.shock[cs];
+            sensor = n == 0 ? sh : fmax(sensor, sh);
+        }
+        if (DIR == 3) sensor = fmax(sensor, 0.0);
+        const double gama = p.gama_min + sensor * (p.gama_max - p.gama_min);
+#pragma unroll
+        for (int L = 0; L < 5; L++) {
+            qL[L] = mdcd_rec(gama, sv[L][0], sv[L][1], sv[L][2], sv[L][3], sv[L][4], sv[L][5]);
+            qR[L] = mdcd_rec(gama, sv[L][5], sv[L][4], sv[L][3], sv[L][2], sv[L][1], sv[L][0]);
+        }
+        if (DIR != 3) {
+            qL[5] = mdcd_rec(gama, sv[5][0], sv[5][1], sv[5][2], sv[5][3], sv[5][4], sv[5][5]);
+            qR[5] = mdcd_rec(gama, sv[5][5], sv[5][4], sv[5][3], sv[5][2], sv[5][1], sv[5][0]);
+        } else {
+            // k-face radii keep the face average (CONVECT_MDCD.F90:450-521 stands outside its loop)
+            qL[5] = face_avg(Al[c0 - st], Al[c0], sv[5][2], sv[5][3]);
+            qR[5] = qL[5];
+        }
+    } else {
+        const Prim qm2 = load_prim(f, c0 - 2 * st);
+        const Prim qm1 = load_prim(f, c0 - st);
+        const Prim q0 = load_prim(f, c0);
+        const Prim qp1 = load_prim(f, c0 + st);
+        const double am2 = Al[c0 - 2 * st], am1 = Al[c0 - st], a0 = Al[c0], ap1 = Al[c0 + st];
+#pragma unroll
+        for (int L = 0; L < 6; L++) {
+            const double avm = face_avg(am2, am1, qm2.q[L], qm1.q[L]);
+            const double av0 = face_avg(am1, a0, qm1.q[L], q0.q[L]);
+            const double avp = face_avg(a0, ap1, q0.q[L], qp1.q[L]);
+            qL[L] = qm1.q[L] + alimiter(av0 - qm1.q[L], qm1.q[L] - avm);
+            qR[L] = q0.q[L] - alimiter(q0.q[L] - av0, avp - q0.q[L]);
+        }
+    }
+    const double* S = f.SD + (long)d * tot;
+    const double sav = f.Grad[(long)d * tot + c0];
+    double fl[5];
+    if (ROE == 1)
+        roe_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0], S[3 * tot + c0],
+                   S[6 * tot + c0], sav, qL[5] * p.omega1, qR[5] * p.omega1, fl);
+    else
+        ausmpw_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
+                      S[3 * tot + c0], S[6 * tot + c0], sav, fl);
+    double* G = f.G[d];
+#pragma unroll
+    for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L] * sav;
+    if (!turb) return;
+    // k/rho, omega/rho: face averages + ALIMITER reconstruction (Space/TURBTWO_CONVECT_MUSCL.F90)
+    double tL[2], tR[2];
+    {
+        const long cm2 = c0 - 2 * st, cm1 = c0 - st, cp1 = c0 + st;
+        const double am2 = Al[cm2], am1 = Al[cm1], a0 = Al[c0], ap1 = Al[cp1];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const long off = (long)(5 + q) * tot;
+            const double vm2 = f.V[off + cm2] / f.V[cm2];
+            const double vm1 = f.V[off + cm1] / f.V[cm1];
+            const double v0 = f.V[off + c0] / f.V[c0];
+            const double vp1 = f.V[off + cp1] / f.V[cp1];
+            const double avm = face_avg(am2, am1, vm2, vm1);
+            const double av0 = face_avg(am1, a0, vm1, v0);
+            const double avp = face_avg(a0, ap1, v0, vp1);
+            tL[q] = vm1 + alimiter(av0 - vm1, vm1 - avm);
+            tR[q] = v0 - alimiter(v0 - av0, avp - v0);
+        }
+    }
+    // Space/TURBTWO_CONVECT_Upwind.F90:58-115
+    const double rcpcv = p.ppf;
+    const double DENL = qL[3] / (rcpcv * qL[4]), DENR = qR[3] / (rcpcv * qR[4]);
+    const double sav1 = S[c0], sav2 = S[3 * tot + c0], sav3 = S[6 * tot + c0];
+    const double ulnormal = (sav1 * qL[0] + sav2 * qL[1] + sav3 * qL[2]);
+    const double urnormal = (sav1 * qR[0] + sav2 * qR[1] + sav3 * qR[2]);
+    const double rulnormal = DENL * ulnormal, rurnormal = DENR * urnormal;
+    const double small = 1.e-10;
+    const double eigTKO = 0.5 * fmax(fabs(ulnormal), fabs(urnormal));
+    const double TKL = fmax(tL[0], small), TKR = fmax(tR[0], small);
+    const double dTK = -0.5 * (rulnormal * TKL + rurnormal * TKR - eigTKO * (DENR * TKR - DENL * TKL));
+    const double TOL = fmax(tL[1], small), TOR = fmax(tR[1], small);
+    const double dTO = -0.5 * (rulnormal * TOL + rurnormal * TOR - eigTKO * (DENR * TOR - DENL * TOL));
+    G[5 * tot + c0] = dTK;
+    G[6 * tot + c0] = dTO;
+}
+
+// ---------------------------------------------------------------------------------------
+// D in the reference's accumulation order: D = 0; diffusion i, j, k (own face subtracted,
+// next face added); rotating-frame source; convection i, j, k.  D(6:7): k-omega diffusion
+// i, j, k then k-omega upwind i, j, k (main.F90:92-133).
+__global__ void __launch_bounds__(256) k_accumulate_visc(trans_params p, int if_euler, int if_turb, Fields f,
+                                                         const BlkInfo* __restrict__ blks,
+                                                         const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+    double dd[7];
+    dd[0] = 0.0;
+#pragma unroll
+    for (int L = 1; L < 5; L++) {
+        double x = 0.0;
+        if (!if_euler)
+#pragma unroll
+            for (int dir = 0; dir < 3; dir++) {
+                const double* G = f.GV[dir] + (long)(L - 1) * tot;
+                x = x - G[c];
+                x = x + G[c + st[dir]];
+            }
+        dd[L] = x;
+    }
+    if (!if_euler) {
+        const double RoW = p.omega1;
+        const double v1 = f.V[c];
+        const double ss3 = 2.0 * RoW * f.V[3 * tot + c] + RoW * f.Yc[c] * p.omega1 * v1;
+        const double ss4 = -2.0 * RoW * f.V[2 * tot + c] + RoW * f.Zc[c] * p.omega1 * v1;
+        dd[2] = dd[2] + ss3 * f.Vol[c];
+        dd[3] = dd[3] + ss4 * f.Vol[c];
+    }
+#pragma unroll
+    for (int L = 0; L < 5; L++) {
+        double x = dd[L];
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) {
+            const double* G = f.G[dir] + (long)L * tot;
+            x = x - G[c];
+            x = x + G[c + st[dir]];
+        }
+        f.D[(long)L * tot + c] = x;
+    }
+    if (!if_turb) {
+        f.D[5 * tot + c] = 0.0;
+        f.D[6 * tot + c] = 0.0;
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        double x = 0.0;
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) {
+            const double* G = f.GV[dir] + (long)(4 + q) * tot;
+            x = x - G[c];
+            x = x + G[c + st[dir]];
+        }
+#pragma unroll
+        for (int dir = 0; dir < 3; dir++) {
+            const double* G = f.G[dir] + (long)(5 + q) * tot;
+            x = x - G[c];
+            x = x + G[c + st[dir]];
+        }
+        f.D[(long)(5 + q) * tot + c] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Time/TURBTWO_TIME_LUSGS_SST.F90:60-415: k-omega right-hand side, SST sources (RANS,
+// IF_Comppress = 0, f_r1 = 1) and the diagonal F1t = 1 + sum(Rds)*stept/Vol + (Cneg)*stept.
+__global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
+                                                 const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    const double dtm = p.dt_lusgs_main, Ref = p.ref;
+    const double vol = f.Vol[c];
+    double d6 = f.D[5 * tot + c] / vol - (3.0 * f.V[5 * tot + c] - 4.0 * f.Wt0[5 * tot + c] + f.Wt1[5 * tot + c]) * 0.5 / dtm;
+    double d7 = f.D[6 * tot + c] / vol - (3.0 * f.V[6 * tot + c] - 4.0 * f.Wt0[6 * tot + c] + f.Wt1[6 * tot + c]) * 0.5 / dtm;
+    const double* Q = f.DQ;
+    const double dudx = Q[c], dudy = Q[tot + c], dudz = Q[2 * tot + c];
+    const double dvdx = Q[3 * tot + c], dvdy = Q[4 * tot + c], dvdz = Q[5 * tot + c];
+    const double dwdx = Q[6 * tot + c], dwdy = Q[7 * tot + c], dwdz = Q[8 * tot + c];
+    const double dtkdx = Q[12 * tot + c], dtkdy = Q[13 * tot + c], dtkdz = Q[14 * tot + c];
+    const double dtodx = Q[15 * tot + c], dtody = Q[16 * tot + c], dtodz = Q[17 * tot + c];
+    const double Div = dudx + dvdy + dwdz;
+    const double Votg = sqr(dudy - dvdx) + sqr(dvdz - dwdy) + sqr(dwdx - dudz);
+    const double prod = (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sqr(dudy + dvdx) + sqr(dvdz + dwdy) +
+                         sqr(dwdx + dudz) - 2.0 / 3.0 * Div * Div);
+    const double cdif = dtkdx * dtodx + dtkdy * dtody + dtkdz * dtodz;
+    double tprod = fmin(prod, Votg) * f.Rmiu[c] / Ref;
+    const double ARo = f.V[c], ARTk = f.V[5 * tot + c], ARTO = f.V[6 * tot + c];
+    const double tprodl1 = p.prod_lmt * p.beta_star * ARTk * ARTO / ARo * Ref;
+    tprod = fmin(tprod, tprodl1);
+    const double RmiuL = sutherland(p, f.T[c]);
+    const double Fdes = 1.0;
+    const double DistN = f.Dst[c];
+    const double CDKO = fmax(2.0 * ARo * p.sigo2 / (ARTO / ARo) * cdif, 1.0e-10);
+    double arg1 = fmax(sqrt(ARTk / ARo) / p.beta_star / (ARTO / ARo) / DistN / Ref,
+                       500.0 * RmiuL / DistN / DistN / ARTO / Ref / Ref);
+    arg1 = fmin(arg1, 4.0 * p.sigo2 * ARTk / CDKO / DistN / DistN);
+    {
+        const double a2 = arg1 * arg1;
+        arg1 = tanh(a2 * a2);
+    }
+    double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+    SourceP = SourceP * 1.0;  // f_r1 = 1 (Kind_RotMod = 0)
+    double SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p.beta_star * ARTO / ARo * Ref * Fdes;
+    double Source = SourceP + SourceN1 * ARTk;
+    const double CnegTk = fabs(SourceN1);
+    d6 = d6 + Source;
+    tprod = fmin(prod, Votg) / Ref;
+    const double GamaAB = p.gama1 * arg1 + p.gama2 * (1.0 - arg1);
+    const double BetaAB = p.beta1 * arg1 + p.beta2 * (1.0 - arg1);
+    SourceP = GamaAB * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
+    SourceP = SourceP * 1.0;
+    SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) - BetaAB * ARTO / ARo * Ref;
+    Source = SourceP + SourceN1 * ARTO;
+    double SourceCross = 2.0 * (1 - arg1) * ARo * p.sigo2 / (ARTO / ARo) * cdif / Ref;
+    SourceCross = copysign(1.0, SourceCross) * fmin(fabs(SourceCross), fabs(SourceP) * 0.5);
+    Source = Source + SourceCross;
+    d7 = d7 + Source;
+    const double CnegTO = 2.0 * fabs(SourceN1);
+    const double Dt = f.Dtm[c];
+    const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
+    f.F1t[c] = 1. + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * (stept / vol) + (CnegTk + CnegTO) * stept;
+    f.D[5 * tot + c] = d6;
+    f.D[6 * tot + c] = d7;
+}

@@ -15,6 +15,7 @@
 #include "k_bc.h"
 #include "k_space.h"
 #include "k_time.h"
+#include "k_visc.h"
 
 namespace {
 
@@ -67,6 +68,7 @@ struct trans_ctx {
     int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
     int max_lines = 0;     // largest line count over the blocks for that axis
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
+    std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
     std::vector<void*> allocs;
     std::string err;
 };
@@ -196,6 +198,11 @@ static int check_params(trans_ctx* c, const trans_params* p)
         return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 2 (MUSCL) or 3 (MDCD)");
     if (p->kind_roe != 1 && p->kind_roe != 3)
         return fail(c, TRANS_EUNSUPPORTED, "Kind_Roe must be 1 (Roe) or 3 (AUSMPW+)");
+    if (p->if_euler == 0 && p->if_turb_org == 1) {
+        if (p->kind_model != 1) return fail(c, TRANS_EUNSUPPORTED, "Kind_Model must be 1 (SST)");
+        if (p->kind_hybrid != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_Hybrid must be 0 (RANS)");
+        if (p->if_comppress != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_Comppress must be 0");
+    }
     return 0;
 }
 
@@ -243,18 +250,31 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
         (rc = dalloc(c, &f.Zc, n)) || (rc = dalloc(c, &f.thtc, n)) || (rc = dalloc(c, &f.D, 7 * n)) ||
         (rc = dalloc(c, &f.Rds, 3 * n)) || (rc = dalloc(c, &f.Dtm, n)) || (rc = dalloc(c, &f.Wt0, 7 * n)) ||
         (rc = dalloc(c, &f.Wt1, 7 * n)) || (rc = dalloc(c, &f.Wt2, 7 * n)) || (rc = dalloc(c, &f.DQ, 18 * n)) ||
-        (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.G[0], 7 * n)) ||
-        (rc = dalloc(c, &f.G[1], 7 * n)) || (rc = dalloc(c, &f.G[2], 7 * n)))
+        (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.F1t, n)) ||
+        (rc = dalloc(c, &f.G[0], 7 * n)) || (rc = dalloc(c, &f.G[1], 7 * n)) || (rc = dalloc(c, &f.G[2], 7 * n)))
         return rc;
+    if (c->p.if_euler == 0) {
+        if ((rc = dalloc(c, &f.GV[0], 6 * n)) || (rc = dalloc(c, &f.GV[1], 6 * n)) ||
+            (rc = dalloc(c, &f.GV[2], 6 * n)))
+            return rc;
+    }
     if (c->p.implicit_fp32) {
         if ((rc = dalloc(c, &f.Ds, 7 * n)) || (rc = dalloc(c, &f.Rdsf, 3 * n))) return rc;
     }
-    if ((rc = dalloc(c, &f.mw, 1))) return rc;
+    c->mwoff.push_back(0);
+    for (int b = 0; b < nblk; b++) {
+        const HostBlock& h = c->hb[b];
+        const long a = (long)(h.nj - 1) * (h.nk - 1), bb = (long)(h.ni - 1) * (h.nk - 1),
+                   cc = (long)(h.ni - 1) * (h.nj - 1);
+        c->mwoff.push_back(c->mwoff.back() + 2 * (a + bb + cc));
+    }
+    if ((rc = dalloc(c, &f.mw, c->mwoff.back()))) return rc;
 
     std::vector<BlkInfo> bi(nblk);
     for (int b = 0; b < nblk; b++) {
         const HostBlock& h = c->hb[b];
-        bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b]};
+        bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b],
+                        c->mwoff[b]};
     }
     if ((rc = dalloc(c, &c->d_blk, nblk))) return rc;
     HIPCHK(c, hipMemcpy(c->d_blk, bi.data(), nblk * sizeof(BlkInfo), hipMemcpyHostToDevice));
@@ -287,6 +307,7 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
         // march along j (lanes on i: coalesced) unless the (i,k) sections do not fit a workgroup
         const int order[3] = {1, 2, 0};
         const size_t esz = c->p.implicit_fp32 ? sizeof(float) : sizeof(double);
+        const size_t neq = (c->p.if_euler == 0 && c->p.if_turb_org == 1) ? 7 : 5;
         for (int q = 0; q < 3 && !c->line_threads; q++) {
             const int m = order[q];
             int mx = 0;
@@ -294,7 +315,7 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
                 const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
                 mx = std::max(mx, n[(m + 1) % 3] * n[(m + 2) % 3]);
             }
-            if (mx <= 1024 && (size_t)20 * mx * esz <= 128 * 1024) {
+            if (mx <= 1024 && 4 * neq * mx * esz <= 128 * 1024) {
                 c->march = m;
                 c->max_lines = mx;
                 c->line_threads = (mx + 63) / 64 * 64;
@@ -331,6 +352,8 @@ int trans_hip_set_params(trans_ctx* c, const trans_params* p)
     if (rc) return rc;
     if (p->implicit_fp32 != c->p.implicit_fp32)
         return fail(c, TRANS_EINVAL, "implicit_fp32 is fixed at context creation");
+    if (p->if_euler != c->p.if_euler || p->if_turb_org != c->p.if_turb_org)
+        return fail(c, TRANS_EINVAL, "IF_Euler and IF_Turb_Org are fixed at context creation");
     c->p = *p;
     return 0;
 }
@@ -403,6 +426,19 @@ int trans_hip_set_geometry(trans_ctx* c, int blk, const trans_geom_view* g)
         (rc = copy_view(c, blk, g->aleng, f.aLeng, 1, true)) || (rc = copy_view(c, blk, g->yc, f.Yc, 1, true)) ||
         (rc = copy_view(c, blk, g->zc, f.Zc, 1, true)) || (rc = copy_view(c, blk, g->thtc, f.thtc, 1, true)))
         return rc;
+    // wall marks (Input/Read_BlockDimensions.F90:146-283): faces i0, im, j0, jm, k0, km
+    const HostBlock& h = c->hb[blk];
+    const long fsz[6] = {(long)(h.nj - 1) * (h.nk - 1), (long)(h.nj - 1) * (h.nk - 1),
+                         (long)(h.ni - 1) * (h.nk - 1), (long)(h.ni - 1) * (h.nk - 1),
+                         (long)(h.ni - 1) * (h.nj - 1), (long)(h.ni - 1) * (h.nj - 1)};
+    std::vector<int> mw;
+    for (int q = 0; q < 6; q++) {
+        if (g->markwall[q])
+            mw.insert(mw.end(), g->markwall[q], g->markwall[q] + fsz[q]);
+        else
+            mw.insert(mw.end(), (size_t)fsz[q], 0);
+    }
+    HIPCHK(c, hipMemcpy(f.mw + c->mwoff[blk], mw.data(), mw.size() * sizeof(int), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -482,22 +518,54 @@ int trans_hip_bc(trans_ctx* c, int sweep)
     return launch_check(c, "k_bc");
 }
 
+}  // extern "C"
+
+template <int SPATIAL, int ROE>
+static void launch_conv(trans_ctx* c, int if_turb)
+{
+    KLAUNCH(c, "k_conv_face<1>", (k_conv_face<1, SPATIAL, ROE>), dim3(c->seg_face[0].n), dim3(WG), c->p, if_turb,
+            c->f, c->d_blk, c->seg_face[0].d);
+    KLAUNCH(c, "k_conv_face<2>", (k_conv_face<2, SPATIAL, ROE>), dim3(c->seg_face[1].n), dim3(WG), c->p, if_turb,
+            c->f, c->d_blk, c->seg_face[1].d);
+    KLAUNCH(c, "k_conv_face<3>", (k_conv_face<3, SPATIAL, ROE>), dim3(c->seg_face[2].n), dim3(WG), c->p, if_turb,
+            c->f, c->d_blk, c->seg_face[2].d);
+}
+
+extern "C" {
+
+// Residual after BC(-1): gradients and BC(0) when viscous, face fluxes, accumulation into D
+// (main.F90:86-133).
 int trans_hip_space(trans_ctx* c)
 {
     if (!c) return TRANS_EINVAL;
     const int if_turb = if_turb_now(c);
-    if (c->p.if_euler == 0) return fail(c, TRANS_EUNSUPPORTED, "viscous path not built yet");
-    (void)if_turb;
-    // D = 0 (main.F90:94)
-    HIPCHK(c, hipMemsetAsync(c->f.D, 0, sizeof(double) * 7 * c->tot, c->stream));
-    KLAUNCH(c, "k_muscl_roe<1>", k_muscl_roe<1>, dim3(c->seg_face[0].n), dim3(WG), c->p, c->f, c->d_blk,
-                       c->seg_face[0].d);
-    KLAUNCH(c, "k_muscl_roe<2>", k_muscl_roe<2>, dim3(c->seg_face[1].n), dim3(WG), c->p, c->f, c->d_blk,
-                       c->seg_face[1].d);
-    KLAUNCH(c, "k_muscl_roe<3>", k_muscl_roe<3>, dim3(c->seg_face[2].n), dim3(WG), c->p, c->f, c->d_blk,
-                       c->seg_face[2].d);
-    KLAUNCH(c, "k_accumulate", k_accumulate, dim3(c->seg_cell.n), dim3(WG), 5, c->f, c->d_blk,
-                       c->seg_cell.d);
+    const bool visc = c->p.if_euler == 0;
+    int rc;
+    if (visc) {
+        KLAUNCH(c, "k_gradients", k_gradients, dim3(c->seg_cell.n), dim3(WG), c->p, if_turb, c->f, c->d_blk,
+                c->seg_cell.d);
+        if ((rc = launch_check(c, "k_gradients"))) return rc;
+        if ((rc = trans_hip_bc(c, 0))) return rc;
+        KLAUNCH(c, "k_visc_face<1>", k_visc_face<1>, dim3(c->seg_face[0].n), dim3(WG), c->p, if_turb, c->f,
+                c->d_blk, c->seg_face[0].d);
+        KLAUNCH(c, "k_visc_face<2>", k_visc_face<2>, dim3(c->seg_face[1].n), dim3(WG), c->p, if_turb, c->f,
+                c->d_blk, c->seg_face[1].d);
+        KLAUNCH(c, "k_visc_face<3>", k_visc_face<3>, dim3(c->seg_face[2].n), dim3(WG), c->p, if_turb, c->f,
+                c->d_blk, c->seg_face[2].d);
+    }
+    if (c->p.kind_spatial == 2) {
+        if (c->p.kind_roe == 1)
+            launch_conv<2, 1>(c, if_turb);
+        else
+            launch_conv<2, 3>(c, if_turb);
+    } else {
+        if (c->p.kind_roe == 1)
+            launch_conv<3, 1>(c, if_turb);
+        else
+            launch_conv<3, 3>(c, if_turb);
+    }
+    KLAUNCH(c, "k_accumulate_visc", k_accumulate_visc, dim3(c->seg_cell.n), dim3(WG), c->p, c->p.if_euler,
+            if_turb, c->f, c->d_blk, c->seg_cell.d);
     return launch_check(c, "space");
 }
 
@@ -510,46 +578,55 @@ int trans_hip_residual(trans_ctx* c)
     return 0;
 }
 
+}  // extern "C"
+
+template <typename R, int NEQ>
+static void launch_sweeps(trans_ctx* c, R* dq)
+{
+    const int nblk = (int)c->hb.size();
+    if (c->line_threads) {
+        const size_t lds = (size_t)4 * NEQ * c->max_lines * sizeof(R);
+        KLAUNCH_LDS(c, "k_lusgs_line<fwd>", (k_lusgs_line<R, true, NEQ>), dim3(nblk), dim3(c->line_threads), lds,
+                    c->p, c->f, dq, c->d_blk, c->march);
+        KLAUNCH_LDS(c, "k_lusgs_line<bwd>", (k_lusgs_line<R, false, NEQ>), dim3(nblk), dim3(c->line_threads), lds,
+                    c->p, c->f, dq, c->d_blk, c->march);
+    } else {
+        KLAUNCH(c, "k_lusgs_sweep<fwd>", (k_lusgs_sweep<R, true, NEQ>), dim3(nblk), dim3(1024), c->p, c->f, dq,
+                c->d_blk);
+        KLAUNCH(c, "k_lusgs_sweep<bwd>", (k_lusgs_sweep<R, false, NEQ>), dim3(nblk), dim3(1024), c->p, c->f, dq,
+                c->d_blk);
+    }
+}
+
+extern "C" {
+
+// LU-SGS (Time/TIME_LUSGS.F90, Time/TURBTWO_TIME_LUSGS_SST.F90) and Update_V (main.F90:135-150)
 static int lusgs_update_async(trans_ctx* c)
 {
     const int nblk = (int)c->hb.size();
-    KLAUNCH(c, "k_lusgs_rhs", k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk,
-                       c->seg_cell.d);
+    const int if_turb = if_turb_now(c);
+    KLAUNCH(c, "k_lusgs_rhs", k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+    if (if_turb)
+        KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+    const int neq = if_turb ? 7 : 5;
     if (c->p.implicit_fp32) {
-        KLAUNCH(c, "k_convert<double,float>", (k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), 5, 0, c->tot,
-                           (const double*)c->f.D, c->f.Ds, c->d_blk, c->seg_cell.d);
-        if (c->line_threads) {
-            const size_t lds = (size_t)20 * c->max_lines * sizeof(float);
-            KLAUNCH_LDS(c, "k_lusgs_line<float,true>", (k_lusgs_line<float, true>), dim3(nblk), dim3(c->line_threads),
-                        lds, c->p, c->f, c->f.Ds, c->d_blk, c->march);
-            KLAUNCH_LDS(c, "k_lusgs_line<float,false>", (k_lusgs_line<float, false>), dim3(nblk),
-                        dim3(c->line_threads), lds, c->p, c->f, c->f.Ds, c->d_blk, c->march);
-        } else {
-            KLAUNCH(c, "k_lusgs_sweep<float,true>", (k_lusgs_sweep<float, true>), dim3(nblk), dim3(1024), c->p, c->f,
-                    c->f.Ds, c->d_blk);
-            KLAUNCH(c, "k_lusgs_sweep<float,false>", (k_lusgs_sweep<float, false>), dim3(nblk), dim3(1024), c->p,
-                    c->f, c->f.Ds, c->d_blk);
-        }
-        KLAUNCH(c, "k_convert<float,double>", (k_convert<float, double>), dim3(c->seg_cell.n), dim3(WG), 5, 0, c->tot,
-                           (const float*)c->f.Ds, c->f.D, c->d_blk, c->seg_cell.d);
+        KLAUNCH(c, "k_convert<double,float>", (k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), neq, 0,
+                c->tot, (const double*)c->f.D, c->f.Ds, c->d_blk, c->seg_cell.d);
+        if (if_turb)
+            launch_sweeps<float, 7>(c, c->f.Ds);
+        else
+            launch_sweeps<float, 5>(c, c->f.Ds);
+        KLAUNCH(c, "k_convert<float,double>", (k_convert<float, double>), dim3(c->seg_cell.n), dim3(WG), neq, 0,
+                c->tot, (const float*)c->f.Ds, c->f.D, c->d_blk, c->seg_cell.d);
     } else {
-        if (c->line_threads) {
-            const size_t lds = (size_t)20 * c->max_lines * sizeof(double);
-            KLAUNCH_LDS(c, "k_lusgs_line<double,true>", (k_lusgs_line<double, true>), dim3(nblk),
-                        dim3(c->line_threads), lds, c->p, c->f, c->f.D, c->d_blk, c->march);
-            KLAUNCH_LDS(c, "k_lusgs_line<double,false>", (k_lusgs_line<double, false>), dim3(nblk),
-                        dim3(c->line_threads), lds, c->p, c->f, c->f.D, c->d_blk, c->march);
-        } else {
-            KLAUNCH(c, "k_lusgs_sweep<double,true>", (k_lusgs_sweep<double, true>), dim3(nblk), dim3(1024), c->p,
-                    c->f, c->f.D, c->d_blk);
-            KLAUNCH(c, "k_lusgs_sweep<double,false>", (k_lusgs_sweep<double, false>), dim3(nblk), dim3(1024), c->p,
-                    c->f, c->f.D, c->d_blk);
-        }
+        if (if_turb)
+            launch_sweeps<double, 7>(c, c->f.D);
+        else
+            launch_sweeps<double, 5>(c, c->f.D);
     }
-    KLAUNCH(c, "k_update_v", k_update_v, dim3(c->seg_cell.n), dim3(WG), c->p, c->ki, if_turb_now(c), c->f,
-                       c->d_blk, c->seg_cell.d, c->d_part);
-    KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk,
-                       c->d_resid);
+    KLAUNCH(c, "k_update_v", k_update_v, dim3(c->seg_cell.n), dim3(WG), c->p, c->ki, if_turb, c->f, c->d_blk,
+            c->seg_cell.d, c->d_part);
+    KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk, c->d_resid);
     return launch_check(c, "lusgs_update");
 }
 
