@@ -60,6 +60,15 @@ def lib():
         L.ora_residual.argtypes = [PP, C.c_int, PB, C.c_int]
         L.ora_lusgs_update.argtypes = [PP, C.c_int, PB, C.c_int]
         L.ora_iterate.argtypes = [PP, C.POINTER(C.c_int), C.c_int, PB, C.c_int]
+        L.ora_diff_dqdxyz.argtypes = [PP, C.c_int, PB]
+        L.ora_diffusion.argtypes = [PP, PB]
+        L.ora_turb_diffusion.argtypes = [PP, PB]
+        L.ora_convect_mdcd.argtypes = [PP, PB]
+        L.ora_convect_ausmpw.argtypes = [PP, PB]
+        L.ora_turb_lusgs_sst.argtypes = [PP, PB]
+        L.ora_ausmpw_common.argtypes = [PP] + [C.c_double] * 14 + [DP]
+        L.ora_mdcd_rec.argtypes = [C.c_double, DP, C.c_int]
+        L.ora_mdcd_rec.restype = C.c_double
         L.ora_alimiter.argtypes = [C.c_double, C.c_double]
         L.ora_alimiter.restype = C.c_double
         L.ora_roe_common.argtypes = [PP] + [C.c_double] * 16 + [DP]
@@ -144,6 +153,10 @@ class OracleCase:
         k = C.c_int(ki)
         lib().ora_iterate(C.byref(self.params), C.byref(k), n, self.ob, self.nblk)
         return k.value
+
+    def call(self, name, n, *args):
+        """Run one oracle routine `name`(params, *args, block n)."""
+        getattr(lib(), name)(C.byref(self.params), *args, C.byref(self.ob[n]))
 
     def resid(self):
         return [self.ob[n].res.as_dict() for n in range(self.nblk)]

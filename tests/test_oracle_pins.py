@@ -1,0 +1,645 @@
+"""Pins of the CPU oracle that do not depend on the oracle's own formulas.
+
+The reference ships no tests or golden vectors and cannot be compiled here, so every
+oracle routine is checked against what the mathematics fixes: closed forms (Euler flux
+consistency, viscous dissipation of a linear shear, BC mirror images), exactness on linear
+data (Green-Gauss, MDCD, ALIMITER on equal slopes), finite-difference Jacobians (LUSGS_ADQ),
+a dense linear solve of the same LU-SGS factorisation, freestream preservation on a
+curvilinear grid, and block-interface halos against the global grid.  Each check is
+chosen so that a plausible slip (a sign, an index, a swapped operand) fails it.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleCase, lib
+from trans_amd import cases, solver
+from trans_amd.mesh import O, BCRow, Block, sl
+from trans_amd.params import GlobalInfo, OrgDeck, make_params
+
+from helpers import interior
+
+EPS = 1.0e-7  # ALIMITER's epsilon (Space/UTIL_MUSCL.F90)
+
+
+def _prm(**kw):
+    return make_params(OrgDeck(**kw), GlobalInfo())
+
+
+def _euler_flux(rho, u, v, w, p, S):
+    """Physical Euler flux per unit area through a face of area vector S (no rotation)."""
+    n = S / np.sqrt((S ** 2).sum())
+    un = u * n[0] + v * n[1] + w * n[2]
+    h = 1.4 * p / 0.4 / rho + 0.5 * (u * u + v * v + w * w)
+    return np.array([rho * un, rho * u * un + p * n[0], rho * v * un + p * n[1],
+                     rho * w * un + p * n[2], rho * h * un])
+
+
+# ---------------------------------------------------------------------------------------
+# ALIMITER (Space/UTIL_MUSCL.F90:2-18)
+
+@pytest.mark.parametrize("a", [1.0, -0.3, 2.5e-3, -4e-9, 1e-12])
+def test_alimiter_equal_slopes(a):
+    assert lib().ora_alimiter(a, a) == pytest.approx(a, rel=1e-14)
+
+
+@pytest.mark.parametrize("a,b", [(1.0, -1e-3), (-2.0, 0.5), (3e-6, -3e-6)])
+def test_alimiter_opposite_signs_is_zero(a, b):
+    assert lib().ora_alimiter(a, b) == 0.0
+    assert lib().ora_alimiter(b, a) == 0.0
+
+
+def test_alimiter_limits_to_smaller_slope_and_is_odd():
+    L = lib()
+    a, b = 1.0, 1e-3  # |a| >> |b| >> eps: van Albada-type limiting to ~b
+    r = L.ora_alimiter(a, b)
+    assert abs(r - b) < 2e-3 * b
+    r2 = L.ora_alimiter(b, a)  # small first argument: b a^2 / (2 a^2) -> b / 2
+    assert abs(r2 - 0.5 * b) < 2e-3 * b
+    for x, y in ((0.3, 0.1), (2e-8, 5e-8), (1.0, 1.0)):
+        assert L.ora_alimiter(-x, -y) == -L.ora_alimiter(x, y)
+    # |a|, |b| << eps: the limiter is the linear blend (2a + b)/3
+    a, b = 3e-12, 1e-12
+    assert L.ora_alimiter(a, b) == pytest.approx((2 * a + b) / 3, rel=1e-6)
+
+
+# ---------------------------------------------------------------------------------------
+# inviscid fluxes: consistency F(U, U) = physical flux, upwinding
+
+STATES = [(1.1, 0.8, 0.1, -0.05, 1 / (1.4 * 0.25)), (0.7, -0.2, 0.4, 0.3, 2.0),
+          (1.0, 0.01, 0.0, 0.0, 1 / (1.4 * 0.09))]
+FACES = [np.array([0.3, 0.1, 0.05]), np.array([-0.02, 0.5, 0.0]), np.array([1e-3, 2e-3, -4e-3])]
+
+
+@pytest.mark.parametrize("st", STATES)
+@pytest.mark.parametrize("S", FACES)
+def test_roe_consistency(st, S):
+    prm = _prm(XM=0.5)
+    rho, u, v, w, p = st
+    T = p / (prm.ppf * rho)
+    out = (C.c_double * 5)()
+    sav = float(np.sqrt((S ** 2).sum()))
+    lib().ora_roe_common(C.byref(prm), p, p, T, T, u, u, v, v, w, w, *S, sav, 0.0, 0.0, out)
+    np.testing.assert_allclose(np.array(out[:]), -_euler_flux(rho, u, v, w, p, S), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("st", STATES)
+@pytest.mark.parametrize("S", FACES)
+def test_ausmpw_consistency(st, S):
+    prm = _prm(XM=0.5)
+    rho, u, v, w, p = st
+    T = p / (prm.ppf * rho)
+    out = (C.c_double * 5)()
+    sav = float(np.sqrt((S ** 2).sum()))
+    lib().ora_ausmpw_common(C.byref(prm), p, p, T, T, u, u, v, v, w, w, *S, sav, out)
+    np.testing.assert_allclose(np.array(out[:]), -_euler_flux(rho, u, v, w, p, S), rtol=1e-13, atol=1e-13)
+
+
+def test_ausmpw_supersonic_upwinding():
+    """Both sides supersonic in the +n direction: the flux is the left state's flux."""
+    prm = _prm(XM=2.0)
+    S = np.array([0.2, 0.0, 0.0])
+    L_, R_ = (1.0, 3.0, 0.1, 0.0, 1.0), (0.8, 2.9, -0.1, 0.05, 0.9)
+    TL, TR = L_[4] / (prm.ppf * L_[0]), R_[4] / (prm.ppf * R_[0])
+    out = (C.c_double * 5)()
+    lib().ora_ausmpw_common(C.byref(prm), L_[4], R_[4], TL, TR, L_[1], R_[1], L_[2], R_[2], L_[3], R_[3],
+                            *S, 0.2, out)
+    np.testing.assert_allclose(np.array(out[:]), -_euler_flux(*L_, S), rtol=1e-12)
+
+
+# ---------------------------------------------------------------------------------------
+# MDCD reconstruction (Space/CONVECT_MDCD.F90:41-88): exact on linear data
+
+@pytest.mark.parametrize("gama", [0.001, 0.006, 0.012, 0.04])
+@pytest.mark.parametrize("a,b", [(1.0, 0.25), (-3.0, 1e-4), (7.9, -0.6)])
+def test_mdcd_linear_exact(gama, a, b):
+    k = np.arange(-3, 3)  # cells f-3 .. f+2; the face f sits at k = -0.5
+    s = (C.c_double * 6)(*(a + b * k))
+    left = lib().ora_mdcd_rec(gama, s, 0)
+    right = lib().ora_mdcd_rec(gama, s, 1)
+    assert left == pytest.approx(a - 0.5 * b, rel=1e-13, abs=1e-13)
+    assert right == pytest.approx(a - 0.5 * b, rel=1e-13, abs=1e-13)
+
+
+def test_mdcd_constant_and_mirror():
+    s = (C.c_double * 6)(*([2.5] * 6))
+    assert lib().ora_mdcd_rec(0.01, s, 0) == pytest.approx(2.5, rel=1e-15)
+    data = [1.0, 1.2, 1.1, 3.0, 2.9, 3.1]
+    fwd = (C.c_double * 6)(*data)
+    rev = (C.c_double * 6)(*data[::-1])
+    assert lib().ora_mdcd_rec(0.01, fwd, 1) == lib().ora_mdcd_rec(0.01, rev, 0)
+
+
+# ---------------------------------------------------------------------------------------
+# LUSGS_ADQ (Time/TIME_LUSGS.F90:696-768) is the Jacobian of F.S
+
+def _fs(Q, S):
+    rho, mx, my, mz, E = Q
+    u, v, w = mx / rho, my / rho, mz / rho
+    p = 0.4 * (E - 0.5 * rho * (u * u + v * v + w * w))
+    un = u * S[0] + v * S[1] + w * S[2]
+    return np.array([rho * un, mx * un + p * S[0], my * un + p * S[1], mz * un + p * S[2], (E + p) * un])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_lusgs_adq_is_flux_jacobian(seed):
+    rng = np.random.Generator(np.random.MT19937(seed))
+    rho = 1.0 + 0.2 * rng.uniform(-1, 1)
+    u, v, w = rng.uniform(-1, 1, 3)
+    p = 2.0 + rng.uniform(-0.5, 0.5)
+    E = p / 0.4 + 0.5 * rho * (u * u + v * v + w * w)
+    Q = np.array([rho, rho * u, rho * v, rho * w, E])
+    S = rng.uniform(-1, 1, 3)
+    dq = rng.uniform(-1, 1, 5)
+    h = 1e-6
+    fd = (_fs(Q + h * dq, S) - _fs(Q - h * dq, S)) / (2 * h)
+    ain = (C.c_double * 5)(*dq)
+    dif = (C.c_double * 5)()
+    lib().ora_lusgs_adq(u, v, w, *S, p, 1.0 / rho, ain, dif)
+    np.testing.assert_allclose(np.array(dif[:]), fd, rtol=1e-8, atol=1e-8)
+
+
+# ---------------------------------------------------------------------------------------
+# whole-block pins on generated grids
+
+def _box(ni=9, nj=8, nk=5, ratio=1.15, bcs=None, **deck):
+    """Rectilinear box (uniform in x and z, geometric in y), far field in i, symmetry in j, k."""
+    x1 = np.linspace(0.0, 1.0, ni)
+    y1 = np.concatenate([[0.0], np.cumsum(0.05 * ratio ** np.arange(nj - 1))])
+    z1 = np.linspace(0.0, 0.3, nk)
+    if bcs is None:
+        bcs = [BCRow(1, 0, 0, 0, 0, 2), BCRow(2, 0, 0, 0, 0, 2), BCRow(3, 0, 0, 0, 0, 3),
+               BCRow(4, 0, 0, 0, 0, 3), BCRow(5, 0, 0, 0, 0, 3), BCRow(6, 0, 0, 0, 0, 3)]
+    b = Block(ni, nj, nk, bcs)
+    X, Y, Z = np.meshgrid(x1, y1, z1, indexing="ij")
+    cases._points(b, X, Y, Z)
+    org = OrgDeck(**{**dict(XM=0.5, Tinf=288.0, Kind_Spatial=2, Kind_Roe=1, IF_Euler=1), **deck})
+    glob = GlobalInfo(nblocks=1)
+    prm = solver.prepare_case([b], org, glob)
+    return b, prm, (x1, y1, z1)
+
+
+def _centres(edges):
+    """Cell centres 1..N-1 and the mirror images of cells 1..3 / N-3..N-1 as ghosts -2..0 / N..N+2."""
+    c = 0.5 * (edges[:-1] + edges[1:])
+    lo = [2 * edges[0] - c[m] for m in (2, 1, 0)]
+    hi = [2 * edges[-1] - c[-1 - m] for m in (0, 1, 2)]
+    return np.concatenate([lo, c, hi])  # reference indices -2 .. N+2
+
+
+def _set_linear(b, prm, coef_u, coef_t, edges):
+    """rho = 1, p uniform, u and T linear in (x, y, z) at the cell centres incl. mirrored ghosts."""
+    xc, yc, zc = (_centres(e) for e in edges)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    u = coef_u[0] + coef_u[1] * X + coef_u[2] * Y + coef_u[3] * Z
+    T = coef_t[0] + coef_t[1] * X + coef_t[2] * Y + coef_t[3] * Z
+    V = b["v"]
+    V[0] = 1.0
+    V[1] = u
+    V[2] = 0.0
+    V[3] = 0.0
+    b["t"][...] = T
+    b["pp"][...] = T / prm.rxm2
+    V[4] = b["pp"] / 0.4 + 0.5 * u * u
+    b["rmiu"][...] = 0.0
+    return u, T
+
+
+def test_green_gauss_exact_on_linear_fields():
+    b, prm, edges = _box()
+    cu, ct = (0.3, 0.7, -1.1, 0.4), (1.0, 0.05, 0.2, -0.3)
+    _set_linear(b, prm, cu, ct, edges)
+    oc = OracleCase([b], prm)
+    oc.call("ora_spaceinterp", 0, 0)
+    oc.call("ora_diff_dqdxyz", 0, 0)
+    dq = interior(oc[0]["DQdxyz"], b)
+    for x in range(3):
+        np.testing.assert_allclose(dq[x], cu[1 + x], rtol=0, atol=1e-12)      # grad u
+        np.testing.assert_allclose(dq[3 + x], 0.0, rtol=0, atol=1e-12)        # grad v
+        np.testing.assert_allclose(dq[9 + x], ct[1 + x], rtol=0, atol=1e-12)  # grad T
+
+
+def test_shock_sensor_closed_form():
+    """Pure shear u(y): div = 0 -> sensor 0; pure expansion u(x): sensor = div^2 / max(div^2, 1)."""
+    b, prm, edges = _box()
+    _set_linear(b, prm, (0.3, 0.0, 0.5, 0.0), (1.0, 0, 0, 0), edges)
+    oc = OracleCase([b], prm)
+    oc.call("ora_spaceinterp", 0, 0)
+    oc.call("ora_diff_dqdxyz", 0, 0)
+    assert np.max(np.abs(interior(oc[0]["shock"], b))) < 1e-20
+    b, prm, edges = _box()
+    _set_linear(b, prm, (0.3, 2.0, 0.0, 0.0), (1.0, 0, 0, 0), edges)
+    oc = OracleCase([b], prm)
+    oc.call("ora_spaceinterp", 0, 0)
+    oc.call("ora_diff_dqdxyz", 0, 0)
+    np.testing.assert_allclose(interior(oc[0]["shock"], b), 1.0, rtol=1e-12)
+
+
+def test_couette_dissipation_closed_form():
+    """u = a*y, uniform p and T, laminar: the momentum residual vanishes and the energy
+    residual is the viscous dissipation mu a^2 Vol / Re of each cell (interior cells whose
+    stencil stays off the symmetry ghosts)."""
+    b, prm, edges = _box(ni=9, nj=10, nk=5, ratio=1.1, IF_Euler=0)
+    a = 0.8
+    _set_linear(b, prm, (0.4, 0.0, a, 0.0), (1.0, 0.0, 0.0, 0.0), edges)
+    oc = OracleCase([b], prm)
+    oc.call("ora_spaceinterp", 0, 0)
+    oc.call("ora_diff_dqdxyz", 0, 0)
+    D = oc[0]["D"]
+    D[...] = 0.0
+    oc.call("ora_diffusion", 0)
+    vol = b["vol"]
+    mu = (1.0 + prm.csthlnd) / (1.0 * prm.csth1 + prm.csthlnd) * (prm.csth1 * 1.0) ** 1.5
+    I, J, K = sl(2, b.ni - 2), sl(3, b.nj - 3), sl(1, b.nk - 1)
+    scale = mu * a / prm.ref * 0.1
+    for L in range(4):
+        assert np.max(np.abs(D[L][I, J, K])) < 1e-12 * scale
+    np.testing.assert_allclose(D[4][I, J, K], mu * a * a * vol[I, J, K] / prm.ref, rtol=1e-10)
+
+
+def test_freestream_preservation_curvilinear():
+    """Uniform flow on the bump channel: away from the boundaries the residual of every
+    scheme vanishes, which pins the face vectors, the accumulation signs and the flux
+    consistency together.  (Near the top wall the symmetry BC takes its normal from the last
+    interior face, BC/BC_Symm.F90:47-52, which is tilted above the bump, so the stencils of
+    the top four rows see a turned ghost state.)"""
+    for spatial, roe, euler in ((2, 1, 1), (3, 3, 0), (2, 3, 0), (3, 1, 1)):
+        blocks, org, glob = cases.bump_channel(33, 17, 5)
+        org.Kind_Spatial, org.Kind_Roe, org.IF_Euler = spatial, roe, euler
+        prm = solver.prepare_case(blocks, org, glob)
+        oc = OracleCase(blocks, prm)
+        oc.residual(1)
+        b = blocks[0]
+        D = oc[0]["D"]
+        flux = (1 / (1.4 * 0.25)) * np.max(b["grad"])
+        region = D[:5, sl(1, b.ni - 1), sl(8, b.nj - 5), sl(1, b.nk - 1)]
+        assert np.max(np.abs(region)) < 1e-12 * flux, (spatial, roe, euler)
+
+
+# ---------------------------------------------------------------------------------------
+# boundary conditions (BC/BC_Symm.F90, BC/BC_Wall.F90, BC/BC_far.F90, BC/BC_Outlet.F90)
+
+def test_symmetry_mirror_and_wall_no_slip():
+    b, prm, edges = _box(bcs=[BCRow(1, 0, 0, 0, 0, 2), BCRow(2, 0, 0, 0, 0, 2), BCRow(3, 0, 0, 0, 0, 1),
+                              BCRow(4, 0, 0, 0, 0, 3), BCRow(5, 0, 0, 0, 0, 3), BCRow(6, 0, 0, 0, 0, 3)],
+                         IF_Euler=0)
+    rng = np.random.Generator(np.random.MT19937(7))
+    V = b["v"]
+    u, v, w = (rng.uniform(-0.5, 0.5, V[0].shape) for _ in range(3))
+    V[0], V[1], V[2], V[3] = 1.0, u, v, w
+    oc = OracleCase([b], prm)
+    oc.bc(1, -1)
+    Vo, P, T = oc[0]["v"], oc[0]["pp"], oc[0]["t"]
+    I, K = sl(1, b.ni - 1), sl(1, b.nk - 1)
+    # jmax symmetry (normal +-y): ghost layer L holds cell NJ-L with the first cell's normal
+    # momentum subtracted twice (BC/BC_Symm.F90:57-110), so layer 1 is the mirror image
+    vn1 = Vo[2][I, O + b.nj - 1, K]
+    for L in (1, 2, 3):
+        g, s = O + b.nj - 1 + L, O + b.nj - L
+        np.testing.assert_allclose(Vo[2][I, g, K], Vo[2][I, s, K] - 2.0 * vn1 * Vo[0][I, s, K], rtol=1e-13,
+                                   atol=1e-15)
+        np.testing.assert_array_equal(Vo[1][I, g, K], Vo[1][I, s, K])
+        np.testing.assert_array_equal(Vo[3][I, g, K], Vo[3][I, s, K])
+        np.testing.assert_array_equal(P[I, g, K], P[I, s, K])
+    np.testing.assert_allclose(Vo[2][I, O + b.nj, K], -Vo[2][I, O + b.nj - 1, K], rtol=1e-13)
+    # jmin wall (SubBCType 0, BC/BC_Wall.F90:41-90): ghosts take p and T of the mirror cell,
+    # rho = p/T * gamma M^2, and the wall velocity (zero)
+    for L in (1, 2, 3):
+        g, s = O + 1 - L, O + L
+        for c in (1, 2, 3):
+            assert np.all(Vo[c][I, g, K] == 0.0)
+        np.testing.assert_array_equal(P[I, g, K], P[I, s, K])
+        np.testing.assert_array_equal(T[I, g, K], T[I, s, K])
+        np.testing.assert_allclose(Vo[0][I, g, K], P[I, s, K] / T[I, s, K] * prm.rxm2, rtol=1e-14)
+
+
+def test_far_field_and_outlet_keep_uniform_states():
+    """A uniform freestream interior gives freestream ghosts at far-field min and max faces
+    (despite the -2 / 0 normal scaling); the outlet without swirl sets the hub pressure on
+    every ghost of the line."""
+    b, prm, edges = _box(bcs=[BCRow(1, 0, 0, 0, 0, 2), BCRow(2, 0, 0, 0, 0, 7), BCRow(3, 0, 0, 0, 0, 3),
+                              BCRow(4, 0, 0, 0, 0, 2), BCRow(5, 0, 0, 0, 0, 3), BCRow(6, 0, 0, 0, 0, 3)])
+    oc = OracleCase([b], prm)
+    oc.bc(1, -1)
+    Vo, P = oc[0]["v"], oc[0]["pp"]
+    J, K, I = sl(1, b.nj - 1), sl(1, b.nk - 1), sl(1, b.ni - 1)
+    p_inf = 1 / (1.4 * prm.xm ** 2)
+    for g in (O + 0, O - 1, O - 2):  # imin far field
+        np.testing.assert_allclose(Vo[0][g, J, K], 1.0, rtol=1e-13)
+        np.testing.assert_allclose(Vo[1][g, J, K], 1.0, rtol=1e-13)
+        np.testing.assert_allclose(P[g, J, K], p_inf, rtol=1e-13)
+    for g in (O + b.nj, O + b.nj + 1):  # jmax far field
+        np.testing.assert_allclose(Vo[0][I, g, K], 1.0, rtol=1e-13)
+        np.testing.assert_allclose(P[I, g, K], p_inf, rtol=1e-13)
+    for g in (O + b.ni, O + b.ni + 1, O + b.ni + 2):  # imax outlet
+        np.testing.assert_allclose(P[g, J, K], prm.pouthub, rtol=1e-14)
+
+
+def test_interface_halo_matches_global_grid():
+    """Block-interface ghosts (BC_P2P with IJKStartEnd) carry the neighbour's cells: with the
+    cell centres stored as the state, every interface ghost holds the centre of the global
+    cell it overlaps."""
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=17, nk=4, lo=9, hi=25)
+    prm = solver.prepare_case(blocks, org, glob)
+    # global x of cell centres along i
+    xs = np.concatenate([blocks[0].xx[O + 1:O + 1 + blocks[0].ni, O + 1, O + 1]] +
+                        [b.xx[O + 2:O + 1 + b.ni, O + 1, O + 1] for b in blocks[1:]])
+    xg = 0.5 * (xs[:-1] + xs[1:])
+    for b in blocks:
+        b["pp"][...] = b["xc"]
+        b["t"][...] = b["yc"]
+        b["rmiu"][...] = b["zc"]
+    oc = OracleCase(blocks, prm)
+    oc.bc(1, -1)
+    i0 = 0
+    for n, b in enumerate(blocks):
+        J, K = sl(1, b.nj - 1), sl(1, b.nk - 1)
+        for L in (1, 2, 3):
+            if n > 0:
+                # the donor is cell NI1+1-L of the previous block, bit for bit ...
+                bp = blocks[n - 1]
+                np.testing.assert_array_equal(oc[n]["pp"][O + 1 - L, J, K], bp["xc"][O + bp.ni - L, J, K])
+                # ... which is where the global grid puts it
+                np.testing.assert_allclose(oc[n]["pp"][O + 1 - L, J, K], xg[i0 - L], rtol=1e-14)
+                np.testing.assert_array_equal(oc[n]["t"][O + 1 - L, J, K], b["yc"][O + 1, J, K])
+                np.testing.assert_array_equal(oc[n]["rmiu"][O + 1 - L, J, K], b["zc"][O + 1, J, K])
+            if n < len(blocks) - 1:
+                bn = blocks[n + 1]
+                np.testing.assert_array_equal(oc[n]["pp"][O + b.ni - 1 + L, J, K], bn["xc"][O + L, J, K])
+                np.testing.assert_allclose(oc[n]["pp"][O + b.ni - 1 + L, J, K], xg[i0 + b.ni - 2 + L],
+                                           rtol=1e-14)
+        i0 += b.ni - 1
+
+
+# ---------------------------------------------------------------------------------------
+# LU-SGS (Time/TIME_LUSGS.F90) against a dense solve of the same factorisation
+
+def test_lusgs_equals_dense_factorised_solve():
+    b, prm, edges = _box(ni=5, nj=4, nk=4, ratio=1.3)
+    rng = np.random.Generator(np.random.MT19937(11))
+    shp = b["pp"].shape
+    V = b["v"]
+    rho = 1.0 + 0.05 * rng.uniform(-1, 1, shp)
+    u, v, w = 1.0 + 0.1 * rng.uniform(-1, 1, shp), 0.1 * rng.uniform(-1, 1, shp), 0.1 * rng.uniform(-1, 1, shp)
+    p = (1 / (1.4 * 0.25)) * (1 + 0.05 * rng.uniform(-1, 1, shp))
+    V[0], V[1], V[2], V[3] = rho, rho * u, rho * v, rho * w
+    V[4] = p / 0.4 + 0.5 * rho * (u * u + v * v + w * w)
+    b["pp"][...] = p
+    b["t"][...] = prm.rxm2 * p / rho
+    b["wt0"][...] = V
+    b["wt1"][...] = V
+    oc = OracleCase([b], prm)
+    oc.spectral_radius(1)
+    D = oc[0]["D"]
+    D[...] = 0.0
+    rhs = rng.uniform(-1e-3, 1e-3, (5, b.ni - 1, b.nj - 1, b.nk - 1))
+    vol = interior(b["vol"], b)
+    D[:5, sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)] = rhs * vol  # ora_lusgs divides by Vol
+    oc.call("ora_lusgs", 0)
+    got = interior(oc[0]["D"], b, slice(0, 5))
+
+    # dense matrices: forward (F1 I - cvs L) x* = stept b, backward (I + cvs/F1 U) x = x*
+    n1, n2, n3 = b.ni - 1, b.nj - 1, b.nk - 1
+    N = n1 * n2 * n3
+    idx = lambda i, j, k: (i - 1) + n1 * ((j - 1) + n2 * (k - 1))
+    Rds, Dtm, SD, Vol = oc[0]["Rds"], oc[0]["Dtm"], b["sd"], b["vol"]
+    dtm = prm.dt_lusgs_main
+    A = np.zeros((5 * N, 5 * N))
+    B = np.zeros((5 * N, 5 * N))
+    st = np.zeros(N)
+    for k in range(1, n3 + 1):
+        for j in range(1, n2 + 1):
+            for i in range(1, n1 + 1):
+                r = idx(i, j, k)
+                dt = Dtm[O + i, O + j, O + k]
+                stept = 2.0 * dt / (3.0 * dt / dtm + 2.0)
+                cvs = stept / Vol[O + i, O + j, O + k]
+                f1 = 1.0 + (Rds[0][O + i, O + j, O + k] + Rds[1][O + i, O + j, O + k] +
+                            Rds[2][O + i, O + j, O + k]) * cvs
+                st[r] = stept
+                A[5 * r:5 * r + 5, 5 * r:5 * r + 5] = f1 * np.eye(5)
+                B[5 * r:5 * r + 5, 5 * r:5 * r + 5] = np.eye(5)
+                for d, (di, dj, dk) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+                    for sgn, M in ((-1, A), (1, B)):
+                        ii, jj, kk = i + sgn * di, j + sgn * dj, k + sgn * dk
+                        if not (1 <= ii <= n1 and 1 <= jj <= n2 and 1 <= kk <= n3):
+                            continue
+                        c = (O + ii, O + jj, O + kk)
+                        rho_n = V[0][c]
+                        vel = [V[q][c] / rho_n for q in (1, 2, 3)]
+                        sm = [0.5 * (SD[d + 3 * x][O + ii + di, O + jj + dj, O + kk + dk] + SD[d + 3 * x][c])
+                              for x in range(3)]
+                        blk = np.zeros((5, 5))
+                        for q in range(5):
+                            e = (C.c_double * 5)(*np.eye(5)[q])
+                            o = (C.c_double * 5)()
+                            lib().ora_lusgs_adq(*vel, *sm, b["pp"][c], 1.0 / rho_n, e, o)
+                            blk[:, q] = 0.5 * (np.array(o[:]) - sgn * Rds[d][c] * np.eye(5)[q])
+                        rn = idx(ii, jj, kk)
+                        if sgn < 0:   # lower neighbour: -cvs * 0.5 (A + Rds)
+                            M[5 * r:5 * r + 5, 5 * rn:5 * rn + 5] -= cvs * blk
+                        else:         # upper neighbour: +cvs/F1 * 0.5 (A - Rds)
+                            M[5 * r:5 * r + 5, 5 * rn:5 * rn + 5] += cvs / f1 * blk
+    bvec = np.zeros(5 * N)
+    for k in range(1, n3 + 1):
+        for j in range(1, n2 + 1):
+            for i in range(1, n1 + 1):
+                r = idx(i, j, k)
+                bvec[5 * r:5 * r + 5] = st[r] * (rhs[:, i - 1, j - 1, k - 1] - (3 * V[:5, O + i, O + j, O + k] -
+                                                 4 * b["wt0"][:5, O + i, O + j, O + k] +
+                                                 b["wt1"][:5, O + i, O + j, O + k]) * 0.5 / dtm)
+    x = np.linalg.solve(B, np.linalg.solve(A, bvec))
+    want = np.zeros_like(got)
+    for k in range(1, n3 + 1):
+        for j in range(1, n2 + 1):
+            for i in range(1, n1 + 1):
+                want[:, i - 1, j - 1, k - 1] = x[5 * idx(i, j, k):5 * idx(i, j, k) + 5]
+    np.testing.assert_allclose(got, want, rtol=1e-9, atol=1e-12 * np.max(np.abs(want)))
+
+
+# ---------------------------------------------------------------------------------------
+# TIME_Update_V (Time/TIME_Update_V.F90:502-690): thermodynamic consistency of the update
+
+def test_update_v_consistency_and_norms():
+    b, prm, edges = _box()
+    rng = np.random.Generator(np.random.MT19937(5))
+    oc = OracleCase([b], prm)
+    D = oc[0]["D"]
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    v0 = oc[0]["v"].copy()
+    p0 = oc[0]["pp"].copy()
+    D[:5, I, J, K] = 1e-4 * rng.uniform(-1, 1, D[:5, I, J, K].shape)
+    d = D.copy()
+    oc.call("ora_update_v", 0, 1, 0)
+    V, P, T = oc[0]["v"], oc[0]["pp"], oc[0]["t"]
+    rho = V[0][I, J, K]
+    np.testing.assert_allclose(T[I, J, K], prm.rxm2 * P[I, J, K] / rho, rtol=1e-14)
+    ke = 0.5 * (V[1] ** 2 + V[2] ** 2 + V[3] ** 2)[I, J, K] / rho
+    np.testing.assert_allclose(V[4][I, J, K], P[I, J, K] / 0.4 + ke, rtol=1e-13)
+    # density: +d when d >= 0, rho * exp(d / rho) otherwise
+    r0, dr = v0[0][I, J, K], d[0][I, J, K]
+    want = np.where(dr >= 0, r0 + dr, r0 * np.exp(dr / (r0 + 1e-10)))
+    np.testing.assert_allclose(rho, want, rtol=1e-14)
+    # pressure follows the primitive increment to first order
+    dp_lin = 0.4 * (d[4] - (v0[1] * d[1] + v0[2] * d[2] + v0[3] * d[3]) / v0[0] +
+                    0.5 * d[0] * (v0[1] ** 2 + v0[2] ** 2 + v0[3] ** 2) / v0[0] ** 2)[I, J, K]
+    np.testing.assert_allclose(P[I, J, K] - p0[I, J, K], dp_lin, rtol=0, atol=1e-7)
+    res = oc.ob[0].res
+    n = (b.ni - 1) * (b.nj - 1) * (b.nk - 1)
+    rel = np.abs(dr / r0)
+    assert res.drsa == pytest.approx(np.sqrt(np.sum(rel ** 2) / n), rel=1e-12)
+    assert res.dua == pytest.approx(np.max(rel), rel=1e-15)
+
+
+# ---------------------------------------------------------------------------------------
+# further pins: inlet, reconstructions on linear data, Roe upwinding, turbulence, Rds
+
+def test_inlet_freestream_state():
+    """BC/BC_Inlet.F90:40-96: an interior at the freestream static pressure gives the
+    freestream ghost (isentropic expansion from P0, T0 of the same Mach number)."""
+    b, prm, edges = _box(bcs=[BCRow(1, 0, 0, 0, 0, 5), BCRow(2, 0, 0, 0, 0, 7), BCRow(3, 0, 0, 0, 0, 3),
+                              BCRow(4, 0, 0, 0, 0, 3), BCRow(5, 0, 0, 0, 0, 3), BCRow(6, 0, 0, 0, 0, 3)])
+    p_inf = 1.2 * 287.0 * 288.0
+    stag = 1.0 + 0.2 * 0.25
+    glob = GlobalInfo(nblocks=1, P0in=p_inf * stag ** 3.5, T0in=288.0 * stag, Pouthub=p_inf)
+    prm = make_params(OrgDeck(XM=0.5, Tinf=288.0), glob)
+    oc = OracleCase([b], prm)
+    oc.bc(1, -1)
+    J, K = sl(1, b.nj - 1), sl(1, b.nk - 1)
+    for g in (O + 0, O - 1, O - 2):
+        np.testing.assert_allclose(oc[0]["v"][0][g, J, K], 1.0, rtol=1e-12)
+        np.testing.assert_allclose(oc[0]["v"][1][g, J, K], 1.0, rtol=1e-12)
+        np.testing.assert_allclose(oc[0]["t"][g, J, K], 1.0, rtol=1e-12)
+        np.testing.assert_allclose(oc[0]["pp"][g, J, K], 1 / (1.4 * 0.25), rtol=1e-12)
+
+
+@pytest.mark.parametrize("routine,spatial", [("ora_convect_muscl", 2), ("ora_convect_mdcd", 3)])
+def test_reconstruction_exact_on_linear_data(routine, spatial):
+    """On a uniform grid, u, p and T linear in (x, y, z): both reconstructed states of every
+    interior face equal the exact face value (ALIMITER of equal slopes is the slope; MDCD is
+    exact on linear data)."""
+    b, prm, edges = _box(ni=12, nj=10, nk=9, ratio=1.0)
+    cu, ct = (0.5, 0.3, -0.2, 0.1), (1.0, 0.1, 0.05, -0.02)
+    _set_linear(b, prm, cu, ct, edges)
+    oc = OracleCase([b], prm)
+    oc.call("ora_spaceinterp", 0, 0)
+    if spatial == 3:
+        oc.call("ora_diff_dqdxyz", 0, 0)
+    oc.call(routine, 0)
+    x1, y1, z1 = edges
+    xc, yc, zc = (0.5 * (e[:-1] + e[1:]) for e in edges)
+    VL, VR = oc[0]["VL"], oc[0]["VR"]
+    for d, (X, Y, Z) in enumerate(((x1, yc, zc), (xc, y1, zc), (xc, yc, z1))):
+        Xf, Yf, Zf = np.meshgrid(X, Y, Z, indexing="ij")
+        u = cu[0] + cu[1] * Xf + cu[2] * Yf + cu[3] * Zf
+        # interior faces two cells away from every boundary
+        cut = [slice(3, -3)] * 3
+        sel = tuple(slice(O + 1 + 3, O + 1 + Xf.shape[q] - 3) for q in range(3))
+        for A in (VL, VR):
+            np.testing.assert_allclose(A[d + 3 * 0][sel], u[tuple(cut)], rtol=0, atol=1e-12)
+
+
+def test_roe_supersonic_upwinding():
+    prm = _prm(XM=2.0)
+    S = np.array([0.2, 0.0, 0.0])
+    L_, R_ = (1.0, 3.0, 0.1, 0.0, 1.0), (0.8, 2.9, -0.1, 0.05, 0.9)
+    TL, TR = L_[4] / (prm.ppf * L_[0]), R_[4] / (prm.ppf * R_[0])
+    out = (C.c_double * 5)()
+    lib().ora_roe_common(C.byref(prm), L_[4], R_[4], TL, TR, L_[1], R_[1], L_[2], R_[2], L_[3], R_[3],
+                         *S, 0.2, 0.0, 0.0, out)
+    np.testing.assert_allclose(np.array(out[:]), -_euler_flux(*L_, S), rtol=1e-12)
+
+
+def test_turbulent_freestream_preservation():
+    """Uniform flow with uniform k, omega and mu_t on the bump channel: the k-omega
+    diffusion and upwind fluxes, the viscous fluxes and the inviscid fluxes all cancel in
+    the interior (D(1:7) = 0 there)."""
+    blocks, org, glob = cases.bump_channel(33, 17, 5)
+    org.Kind_Spatial, org.Kind_Roe, org.IF_Euler, org.IF_Turb_Org = 3, 3, 0, 1
+    prm = solver.prepare_case(blocks, org, glob)
+    b = blocks[0]
+    oc = OracleCase(blocks, prm)
+    oc.residual(1)
+    D = oc[0]["D"]
+    flux = (1 / (1.4 * 0.25)) * np.max(b["grad"])
+    # (the far field imposes mu_t = VisRatio * mu on inflow ghosts, BC/BC_far.F90:92-104, so
+    # the cells next to the i boundaries are not in uniform flow)
+    region = D[:, sl(5, b.ni - 5), sl(8, b.nj - 5), sl(1, b.nk - 1)]
+    assert np.max(np.abs(region[:5])) < 1e-12 * flux
+    tflux = np.max(np.abs(b["v"][5:7])) * np.max(b["grad"])
+    assert np.max(np.abs(region[5:7])) < 1e-12 * tflux
+
+
+def test_sst_sources_at_zero_strain():
+    """Time/TURBTWO_TIME_LUSGS_SST.F90:139-415 with zero gradients, no walls (F1 -> 0) and no
+    neighbour coupling: the k and omega increments are the SST destruction terms
+    -beta* rho k omega Re and -beta_2 rho omega^2 Re over the implicit diagonal."""
+    b, prm, edges = _box(ni=5, nj=5, nk=5, IF_Euler=0, IF_Turb_Org=1)
+    oc = OracleCase([b], prm)
+    V = oc[0]["v"]
+    V[1:4] = 0.0  # at rest: no convection, no strain
+    oc[0]["wt0"][...] = V
+    oc[0]["wt1"][...] = V
+    oc[0]["Rds"][...] = 0.0
+    oc[0]["Dtm"][...] = 0.7
+    oc[0]["D"][...] = 0.0
+    oc[0]["DQdxyz"][...] = 0.0
+    oc.call("ora_turb_lusgs_sst", 0)
+    rho, rk, rw = V[0], V[5], V[6]
+    om = rw / rho
+    Re = prm.ref
+    stept = 2.0 * 0.7 / (3.0 * 0.7 / prm.dt_lusgs_main + 2.0)
+    f = 1.0 + (prm.beta_star * om * Re + 2.0 * prm.beta2 * om * Re) * stept
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    np.testing.assert_allclose(oc[0]["D"][5][I, J, K], (-prm.beta_star * om * Re * rk * stept / f)[I, J, K], rtol=1e-12)
+    np.testing.assert_allclose(oc[0]["D"][6][I, J, K], (-prm.beta2 * om * Re * rw * stept / f)[I, J, K], rtol=1e-12)
+
+
+def test_spectral_radius_closed_form():
+    """Time/SpectralRadius.F90:59-137 on a rectilinear box: Rds_d = |u.S_d| + |S_d| a +
+    2 |S_d|^2 / Vol * max(4/3 (mu + mu_t), 1.4 (mu/0.72 + mu_t/0.9)) / rho / Re."""
+    b, prm, edges = _box(IF_Euler=0, CFL_Sub_start=3.0, CFL_Sub_end=3.0)
+    rng = np.random.Generator(np.random.MT19937(3))
+    V = b["v"]
+    V[1] = 1.0 + 0.1 * rng.uniform(-1, 1, V[0].shape)
+    V[2] = 0.2 * rng.uniform(-1, 1, V[0].shape)
+    b["rmiu"][...] = 1e-3 * rng.uniform(0, 1, V[0].shape)
+    oc = OracleCase([b], prm)
+    oc.spectral_radius(1)
+    x1, y1, z1 = edges
+    dx = np.diff(x1)[:, None, None]
+    dy = np.diff(y1)[None, :, None]
+    dz = np.diff(z1)[None, None, :]
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    rho, u, v = V[0][I, J, K], V[1][I, J, K], V[2][I, J, K]
+    a = np.sqrt(1.4 * b["pp"][I, J, K] / rho)
+    mu = (1 + prm.csthlnd) / (b["t"][I, J, K] * prm.csth1 + prm.csthlnd) * (prm.csth1 * b["t"][I, J, K]) ** 1.5
+    mt = b["rmiu"][I, J, K]
+    nu = np.maximum(4.0 / 3.0 * (mu + mt), 1.4 * (mu / 0.72 + mt / 0.9)) / rho
+    vol = dx * dy * dz
+    for d, (S, un) in enumerate(((dy * dz, u), (dx * dz, v), (dx * dy, 0.0 * u))):
+        S = np.broadcast_to(S, rho.shape)
+        want = np.abs(un * S) + S * a + 2 * S * S / vol * nu / prm.ref
+        np.testing.assert_allclose(oc[0]["Rds"][d][I, J, K], want, rtol=1e-12)
+    rsum = sum(oc[0]["Rds"][d][I, J, K] for d in range(3))
+    np.testing.assert_allclose(oc[0]["Dtm"][I, J, K], 3.0 * vol / rsum, rtol=1e-13)
+
+
+def test_update_u_shifts_history():
+    b, prm, edges = _box()
+    oc = OracleCase([b], prm)
+    oc[0]["v"][...] = 3.0
+    oc[0]["wt0"][...] = 2.0
+    oc[0]["wt1"][...] = 1.0
+    lib().ora_update_u(C.byref(prm), C.byref(oc.ob[0]))  # Kind_Dual = 2 (BDF2)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    assert np.all(oc[0]["wt1"][:, I, J, K] == 2.0) and np.all(oc[0]["wt0"][:, I, J, K] == 3.0)
+
+
+def test_if_turb_switch():
+    prm = _prm(IF_Euler=0, IF_Turb_Org=1, KItm=5)
+    assert lib().ora_if_turb(C.byref(prm), 4) == 0
+    assert lib().ora_if_turb(C.byref(prm), 5) == 1
+    prm = _prm(IF_Euler=1, IF_Turb_Org=1, KItm=1)
+    assert lib().ora_if_turb(C.byref(prm), 5) == 0

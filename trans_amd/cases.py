@@ -62,14 +62,15 @@ def split_sizes(total_cells: int, nblk: int, lo: int, hi: int, seed: int):
 
 
 def flat_plate(nblk=8, ni_total=257, nj=129, nk=5, seed=20261202, re_l=5.0e6, mach=0.3,
-               ratio=1.1, yplus=1.0, lo=17, hi=65):
+               ratio=1.1, yplus=1.0, lo=17, hi=65, top=2):
     """Turbulent flat plate split into `nblk` blocks of uneven i-extent.
 
     Inflow through the total-pressure inlet (BC 5) at the first block's imin, static-pressure
     outlet (BC 7) at the last block's imax, far field (BC 2) at jmax.  The far-field BC is
     kept off the min faces: its outward normal there is -2 times the unit normal
     (BC/BC_far.F90:38), which doubles the reflection of the normal velocity and makes an
-    imin far-field inflow diverge after ~100 iterations.
+    imin far-field inflow diverge after ~100 iterations.  `top` is the BC kind at jmax
+    (2 far field, 3 symmetry).
 
     The plate starts at x = 0 and ends at x = L (the block row covers a short
     symmetry run-in of 1/8 of the length); wall-normal geometric stretching at `ratio`
@@ -130,7 +131,7 @@ def flat_plate(nblk=8, ni_total=257, nj=129, nk=5, seed=20261202, re_l=5.0e6, ma
             first = int(wall_cells[0]) + 1
             bcs.append(BCRow(3, 1, first - 1, 0, 0, 3))
             bcs.append(BCRow(3, first, n - 1, 0, 0, 1))
-        bcs.append(BCRow(4, 0, 0, 0, 0, 2))
+        bcs.append(BCRow(4, 0, 0, 0, 0, top))
         bcs.append(BCRow(5, 0, 0, 0, 0, 3))
         bcs.append(BCRow(6, 0, 0, 0, 0, 3))
         b = Block(n, nj, nk, bcs)

@@ -32,6 +32,7 @@ struct Fields {
     double* T;
     double* Rmiu;
     double* SD;       // 9: (dir-1) + 3*(xyz-1)
+    double* SDm;      // 9: mean of a cell's two faces of direction dir, same component order
     double* Grad;     // 3
     double* Alagm;    // 3
     double* Vol;
@@ -50,6 +51,7 @@ struct Fields {
     double* DQ;       // 18
     double* shock;
     double* F1;       // LU-SGS diagonal
+    double* Stp;      // 2: dual-time pseudo step stept and stept/Vol of the LU-SGS
     double* F1t;      // k-omega LU-SGS diagonal
     double* G[3];     // area-weighted convective face fluxes per direction, 7 comps each
     double* GV[3];    // viscous face fluxes per direction: x, y, z momentum, energy, k, omega

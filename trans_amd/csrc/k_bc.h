@@ -10,7 +10,7 @@ struct DPatch {
     trans_patch p;
     int blk;    // owning block
     int tpatch; // global index of the target patch (interfaces)
-    int count;  // work items: face cells, or (i,k) lines for the outlet march
+    int count;  // face cells of the patch
     int pad;
 };
 
@@ -110,21 +110,13 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
     const int n0 = c.point_end[0] - c.point_start[0] + 1;
     const int n1 = c.point_end[1] - c.point_start[1] + 1;
     const int n2 = c.point_end[2] - c.point_start[2] + 1;
-    const bool march = (c.bctype == 7 && sweep == -1);
-    if (lin >= (march ? n0 * n2 : n0 * n1 * n2)) return;
+    if (c.bctype == 7 && sweep == -1) return;  // outlet ghosts: k_bc_outlet
+    if (lin >= n0 * n1 * n2) return;
     const BlkInfo b = blks[dp.blk];
     const long tot = f.tot;
-    int i, j, k;
-    if (march) {
-        // outlet: one work item per (i,k), marching j (radial equilibrium)
-        i = c.point_start[0] + lin % n0;
-        k = c.point_start[2] + lin / n0;
-        j = c.point_start[1];
-    } else {
-        i = c.point_start[0] + lin % n0;
-        j = c.point_start[1] + (lin / n0) % n1;
-        k = c.point_start[2] + lin / (n0 * n1);
-    }
+    const int i = c.point_start[0] + lin % n0;
+    const int j = c.point_start[1] + (lin / n0) % n1;
+    const int k = c.point_start[2] + lin / (n0 * n1);
     const int* ad = c.cijkadd;
     const int* e = c.ceijkadd;
 
@@ -361,56 +353,8 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
         }
         break;
     }
-    case 7: {  // BC/BC_Outlet.F90:49-107, j-march from the hub
-        const int Jbgn = c.point_start[1];
-        for (j = c.point_start[1]; j <= c.point_end[1]; j++) {
-            const long is = gidx(b, i + ad[0], j + ad[1], k + ad[2]);
-            const long il = gidx(b, i + e[0], j + e[1], k + e[2]);
-            f.T[il] = f.T[is];
-            if (j == Jbgn) {
-                f.PP[il] = p.pouthub;
-            } else {
-                const long im = gidx(b, i + e[0], j - 1 + e[1], k + e[2]);
-                const double rcs1 = f.rad[il];
-                const double rcs2 = f.rad[im];
-                const double sinr = sin(f.thtc[im]);
-                const double cosr = cos(f.thtc[im]);
-                const double den = f.V[im];
-                const double RoVr2 = sqr(f.V[3 * tot + im] / den * cosr - sinr * f.V[2 * tot + im] / den +
-                                         p.omega1 * rcs2);
-                f.PP[il] = f.PP[im] + fabs((rcs1 - rcs2) * den * RoVr2) / (rcs2 + p.tiny);
-            }
-            const double v1 = f.PP[il] / f.T[il] * p.rxm2;
-            f.V[il] = v1;
-            const double den_ratio = v1 / f.V[is];
-            f.V[tot + il] = f.V[tot + is] * den_ratio;
-            f.V[2 * tot + il] = f.V[2 * tot + is] * den_ratio;
-            f.V[3 * tot + il] = f.V[3 * tot + is] * den_ratio;
-            for (int d = 0; d < 3; d++) f.Rds[(long)d * tot + il] = f.Rds[(long)d * tot + is];
-            if (f.Rdsf)
-                for (int d = 0; d < 3; d++) f.Rdsf[(long)d * tot + il] = f.Rdsf[(long)d * tot + is];
-            for (int lay = 1; lay < 3; lay++) {
-                const int* ea = lay_dst(c, lay);
-                const long d2 = gidx(b, i + ea[0], j + ea[1], k + ea[2]);
-                for (int L = 0; L < 4; L++) f.V[(long)L * tot + d2] = f.V[(long)L * tot + il];
-                f.PP[d2] = f.PP[il];
-                f.T[d2] = f.T[il];
-            }
-            if (if_turb) {
-                f.V[5 * tot + il] = f.V[5 * tot + is];
-                f.V[6 * tot + il] = f.V[6 * tot + is];
-                f.Rmiu[il] = f.Rmiu[is];
-                for (int lay = 1; lay < 3; lay++) {
-                    const int* ea = lay_dst(c, lay);
-                    const long d2 = gidx(b, i + ea[0], j + ea[1], k + ea[2]);
-                    f.V[5 * tot + d2] = f.V[5 * tot + il];
-                    f.V[6 * tot + d2] = f.V[6 * tot + il];
-                    f.Rmiu[d2] = f.Rmiu[il];
-                }
-            }
-        }
+    case 7:  // outlets: k_bc_outlet
         break;
-    }
     case 9: {  // BC/BC_P2P.F90:76-139
         const DPatch& tp = pt[dp.tpatch];
         const BlkInfo br = blks[tp.blk];
@@ -461,5 +405,99 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
     }
     default:
         break;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// BC/BC_Outlet.F90:49-107: static-pressure outlet with the radial-equilibrium march along j
+// from the hub.  One workgroup per (i,k) line of an outlet patch: the march inputs are staged
+// in LDS by all lanes, lane 0 runs the pressure recurrence, and all lanes write the ghosts.
+constexpr int OUTLET_MAXJ = 512;
+
+struct OutLine {
+    int patch;  // index into the device patch table
+    int i, k;
+};
+
+__global__ void __launch_bounds__(256) k_bc_outlet(trans_params p, int if_turb, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const DPatch* __restrict__ pt,
+                                                   const OutLine* __restrict__ lines)
+{
+    __shared__ double s_pp[OUTLET_MAXJ], s_t[OUTLET_MAXJ], s_v1[OUTLET_MAXJ], s_v3[OUTLET_MAXJ],
+        s_v4[OUTLET_MAXJ], s_rad[OUTLET_MAXJ], s_sin[OUTLET_MAXJ], s_cos[OUTLET_MAXJ];
+    const OutLine ln = lines[blockIdx.x];
+    const DPatch& dp = pt[ln.patch];
+    const trans_patch& c = dp.p;
+    const BlkInfo b = blks[dp.blk];
+    const long tot = f.tot;
+    const int* ad = c.cijkadd;
+    const int* e = c.ceijkadd;
+    const int j0 = c.point_start[1];
+    const int nj = c.point_end[1] - j0 + 1;
+    const int i = ln.i, k = ln.k;
+    for (int jj = threadIdx.x; jj < nj; jj += blockDim.x) {
+        const int j = j0 + jj;
+        const long is = gidx(b, i + ad[0], j + ad[1], k + ad[2]);
+        const long il = gidx(b, i + e[0], j + e[1], k + e[2]);
+        s_t[jj] = f.T[is];
+        s_v1[jj] = f.V[is];
+        s_v3[jj] = f.V[2 * tot + is];
+        s_v4[jj] = f.V[3 * tot + is];
+        s_rad[jj] = f.rad[il];
+        s_sin[jj] = sin(f.thtc[il]);
+        s_cos[jj] = cos(f.thtc[il]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double pp = p.pouthub;
+        s_pp[0] = pp;
+        for (int jj = 1; jj < nj; jj++) {
+            // ghost state of the previous j (density, y/z momentum) as the reference has it
+            const double den = pp / s_t[jj - 1] * p.rxm2;
+            const double ratio = den / s_v1[jj - 1];
+            const double gv3 = s_v3[jj - 1] * ratio, gv4 = s_v4[jj - 1] * ratio;
+            const double rcs1 = s_rad[jj], rcs2 = s_rad[jj - 1];
+            const double RoVr2 = sqr(gv4 / den * s_cos[jj - 1] - s_sin[jj - 1] * gv3 / den + p.omega1 * rcs2);
+            pp = pp + fabs((rcs1 - rcs2) * den * RoVr2) / (rcs2 + p.tiny);
+            s_pp[jj] = pp;
+        }
+    }
+    __syncthreads();
+    for (int jj = threadIdx.x; jj < nj; jj += blockDim.x) {
+        const int j = j0 + jj;
+        const long is = gidx(b, i + ad[0], j + ad[1], k + ad[2]);
+        const long il = gidx(b, i + e[0], j + e[1], k + e[2]);
+        const double tt = s_t[jj], pp = s_pp[jj];
+        f.T[il] = tt;
+        f.PP[il] = pp;
+        const double v1 = pp / tt * p.rxm2;
+        f.V[il] = v1;
+        const double den_ratio = v1 / s_v1[jj];
+        f.V[tot + il] = f.V[tot + is] * den_ratio;
+        f.V[2 * tot + il] = s_v3[jj] * den_ratio;
+        f.V[3 * tot + il] = s_v4[jj] * den_ratio;
+        for (int d = 0; d < 3; d++) f.Rds[(long)d * tot + il] = f.Rds[(long)d * tot + is];
+        if (f.Rdsf)
+            for (int d = 0; d < 3; d++) f.Rdsf[(long)d * tot + il] = f.Rdsf[(long)d * tot + is];
+        for (int lay = 1; lay < 3; lay++) {
+            const int* ea = lay_dst(c, lay);
+            const long d2 = gidx(b, i + ea[0], j + ea[1], k + ea[2]);
+            for (int L = 0; L < 4; L++) f.V[(long)L * tot + d2] = f.V[(long)L * tot + il];
+            f.PP[d2] = pp;
+            f.T[d2] = tt;
+        }
+        if (if_turb) {
+            f.V[5 * tot + il] = f.V[5 * tot + is];
+            f.V[6 * tot + il] = f.V[6 * tot + is];
+            f.Rmiu[il] = f.Rmiu[is];
+            for (int lay = 1; lay < 3; lay++) {
+                const int* ea = lay_dst(c, lay);
+                const long d2 = gidx(b, i + ea[0], j + ea[1], k + ea[2]);
+                f.V[5 * tot + d2] = f.V[5 * tot + il];
+                f.V[6 * tot + d2] = f.V[6 * tot + il];
+                f.Rmiu[d2] = f.Rmiu[il];
+            }
+        }
     }
 }

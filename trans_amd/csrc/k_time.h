@@ -28,7 +28,34 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
     }
     const double Dt = f.Dtm[c];
     const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
-    f.F1[c] = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * (stept / vol);
+    const double cvs = stept / vol;
+    f.F1[c] = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * cvs;
+    f.Stp[c] = stept;
+    f.Stp[tot + c] = cvs;
+}
+
+// Face-mean vectors 0.5*(S_d(f+1) + S_d(f)) of every interior cell (geometry; the LU-SGS
+// flux Jacobians of Time/TIME_LUSGS.F90:190-256 use them for the neighbour cell).
+__global__ void __launch_bounds__(256) k_face_means(Fields f, const BlkInfo* __restrict__ blks,
+                                                    const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const long o = (long)(d + 3 * q) * tot;
+            f.SDm[o + c] = 0.5 * (f.SD[o + c + st[d]] + f.SD[o + c]);
+        }
 }
 
 // Time/TIME_LUSGS.F90:696-768 (LUSGS_ADQ) -- flux-Jacobian times increment, gamma = 1.4.
@@ -199,21 +226,20 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
 template <int NEQ>
 struct LineRaw {
     double d[NEQ];
-    double dtm, vol, f1, f1t;
+    double stept, cvs, f1, f1t;
     double v[4], pp;
     double sdm[3][3];  // face mean of SD in each direction
     double rds[3];
 };
 
 template <typename R, int NEQ>
-__device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__ Dq, const long st[3],
-                                          long c, LineRaw<NEQ>& x)
+__device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__ Dq, long c, LineRaw<NEQ>& x)
 {
     const long tot = f.tot;
 #pragma unroll
     for (int L = 0; L < NEQ; L++) x.d[L] = (double)Dq[(long)L * tot + c];
-    x.dtm = f.Dtm[c];
-    x.vol = f.Vol[c];
+    x.stept = f.Stp[c];
+    x.cvs = f.Stp[tot + c];
     x.f1 = f.F1[c];
     x.f1t = NEQ > 5 ? f.F1t[c] : 1.0;
 #pragma unroll
@@ -221,9 +247,8 @@ __device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__
     x.pp = f.PP[c];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        const double* S = f.SD + (long)d * tot;
 #pragma unroll
-        for (int q = 0; q < 3; q++) x.sdm[d][q] = 0.5 * (S[(long)q * 3 * tot + c + st[d]] + S[(long)q * 3 * tot + c]);
+        for (int q = 0; q < 3; q++) x.sdm[d][q] = f.SDm[(long)(d + 3 * q) * tot + c];
         if constexpr (sizeof(R) == 4)
             x.rds[d] = (double)f.Rdsf[(long)d * tot + c];
         else
@@ -239,100 +264,117 @@ __device__ __forceinline__ void lds_barrier()
 }
 
 // march: 0 = i, 1 = j, 2 = k.  Lines are indexed a-fastest over the two other axes (a < b).
-// Dynamic LDS: 2 parities x 2 directions x NEQ x nlines values of R.
-template <typename R, bool FORWARD, int NEQ>
-__global__ void __launch_bounds__(1024) k_lusgs_line(trans_params p, Fields f, R* Dq,
-                                                     const BlkInfo* __restrict__ blks, int march)
+// Dynamic LDS: 2 parities x 2 directions x NEQ x (nlines + 1) values of R; slot nlines of each
+// row is a sink for the lanes that have nothing to hand on.
+//
+// The step loop is free of wave-level branches: lanes outside their line's window compute on
+// a clamped cell and send their results to sinks (`sink` in global memory, slot nlines in
+// LDS).  Every step then issues the same memory operations, which keeps the compiler's
+// memory-counter waits precise across the loop (the next cell's loads stay in flight while
+// the current cell is computed).
+template <typename R, bool FORWARD, int NEQ, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R* Dq,
+                                                     const BlkInfo* __restrict__ blks, int march,
+                                                     R* __restrict__ sink)
 {
     extern __shared__ unsigned char lds_raw[];
     const BlkInfo b = blks[blockIdx.x];
-    const int N[3] = {b.ni - 1, b.nj - 1, b.nk - 1};
-    const int ax = march == 0 ? 1 : 0;
-    const int bx = march == 2 ? 1 : 2;
-    const int Na = N[ax], Nb = N[bx], Nm = N[march];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    // axes: march m, the two others a < b; all selects, so that nothing is indexed at run time
+    const int Nm = march == 0 ? NI1 : (march == 1 ? NJ1 : NK1);
+    const int Na = march == 0 ? NJ1 : NI1;
+    const int Nb = march == 2 ? NJ1 : NK1;
     const int nlines = Na * Nb;
-    const int nplanes = N[0] + N[1] + N[2] - 2;
-    R* buf = reinterpret_cast<R*>(lds_raw);  // [2 parity][2 dir][NEQ][nlines]
+    const int row = nlines + 1;
+    const int nplanes = NI1 + NJ1 + NK1 - 2;
+    R* buf = reinterpret_cast<R*>(lds_raw);  // [2 parity][2 dir][NEQ][nlines + 1]
     const long tot = f.tot;
-    const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+    const long stm = march == 0 ? 1 : (march == 1 ? (long)b.nx : (long)b.nx * b.ny);
     const int ln = threadIdx.x;
     const bool active = ln < nlines;
     const int ia = active ? ln % Na + 1 : 1, ib = active ? ln / Na + 1 : 1;
-    const double dtm_main = p.dt_lusgs_main;
     const R sgn = FORWARD ? R(1) : R(-1);
-    const int ln_a = FORWARD ? (ia < Na ? ln + 1 : -1) : (ia > 1 ? ln - 1 : -1);
-    const int ln_b = FORWARD ? (ib < Nb ? ln + Na : -1) : (ib > 1 ? ln - Na : -1);
+    // LDS slots this lane writes (the sink when there is no downstream line)
+    const int wa = active && (FORWARD ? ia < Na : ia > 1) ? (FORWARD ? ln + 1 : ln - 1) : nlines;
+    const int wb = active && (FORWARD ? ib < Nb : ib > 1) ? (FORWARD ? ln + Na : ln - Na) : nlines;
+    const int rd_slot = active ? ln : nlines;  // padding lanes read the sink slot
     const bool in_a = FORWARD ? (ia > 1) : (ia < Na);
     const bool in_b = FORWARD ? (ib > 1) : (ib < Nb);
     const int start = FORWARD ? (ia - 1) + (ib - 1) : (Na - ia) + (Nb - ib);
-    int pos[3];
-    pos[ax] = ia;
-    pos[bx] = ib;
-    pos[march] = FORWARD ? 1 : Nm;
-    const long cstep = FORWARD ? st[march] : -st[march];
-    long c = active ? cidx(b, pos[0], pos[1], pos[2]) : 0;
+    const int im0 = FORWARD ? 1 : Nm;
+    const int pi = march == 0 ? im0 : ia;
+    const int pj = march == 1 ? im0 : (march == 0 ? ia : ib);
+    const int pk = march == 2 ? im0 : ib;
+    const long cstep = FORWARD ? stm : -stm;
+    const long c0 = cidx(b, pi, pj, pk);
+    R* my_sink = sink + (long)blockIdx.x * NEQ * MAXT + ln;
     R tm[NEQ];  // term from the previous cell of this line
 #pragma unroll
     for (int L = 0; L < NEQ; L++) tm[L] = R(0);
-    LineRaw<NEQ> nxt;
-    if (active) line_load<R, NEQ>(f, Dq, st, c, nxt);
-    for (int t = 0; t < nplanes; t++) {
+    // one step of the wavefront on cell `cur`, loading the next cell into `nxt`; the loop
+    // below alternates two buffers so that loads stay in flight across a whole step
+    auto step = [&](const int t, const LineRaw<NEQ>& cur, LineRaw<NEQ>& nxt) {
         const int par = t & 1;
         const int rel = t - start;
-        if (active && rel >= 0 && rel < Nm) {
-            const LineRaw<NEQ> cur = nxt;
-            if (rel + 1 < Nm) line_load<R, NEQ>(f, Dq, st, c + cstep, nxt);
-            R tin[3][NEQ];
+        const bool act = active && rel >= 0 && rel < Nm;
+        const long c = c0 + (long)min(max(rel, 0), Nm - 1) * cstep;
+        line_load<R, NEQ>(f, Dq, c0 + (long)min(max(rel + 1, 0), Nm - 1) * cstep, nxt);
+        const double stept = cur.stept;
+        const double cvs = cur.cvs;
+        R dq[NEQ];
 #pragma unroll
-            for (int L = 0; L < NEQ; L++) {
-                tin[march][L] = tm[L];
-                tin[ax][L] = in_a ? buf[((par * 2 + 0) * NEQ + L) * nlines + ln] : R(0);
-                tin[bx][L] = in_b ? buf[((par * 2 + 1) * NEQ + L) * nlines + ln] : R(0);
+        for (int L = 0; L < NEQ; L++) {
+            const R sa = in_a ? buf[((par * 2 + 0) * NEQ + L) * row + rd_slot] : R(0);
+            const R sb = in_b ? buf[((par * 2 + 1) * NEQ + L) * row + rd_slot] : R(0);
+            const R sm = rel == 0 ? R(0) : tm[L];
+            // incoming terms summed in i, j, k order
+            const R x0 = march == 0 ? sm : sa;
+            const R x1 = march == 0 ? sa : (march == 1 ? sm : sb);
+            const R x2 = march == 2 ? sm : sb;
+            const R sum = x0 + x1 + x2;
+            const R f1 = L < 5 ? R(cur.f1) : R(cur.f1t);
+            if (FORWARD)
+                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
+            else
+                dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
+            R* dst = act ? Dq + (long)L * tot + c : my_sink + (long)L * MAXT;
+            *dst = dq[L];
+        }
+        const double roinv = 1.0 / cur.v[0];
+        const R pVx = R(cur.v[1] * roinv), pVy = R(cur.v[2] * roinv), pVz = R(cur.v[3] * roinv);
+        const int npar = par ^ 1;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            R o[NEQ];
+            R dif[5];
+            const R sx = R(cur.sdm[d][0]), sy = R(cur.sdm[d][1]), sz = R(cur.sdm[d][2]);
+            lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, R(cur.pp), R(roinv), dq, dif);
+            const R rd = R(cur.rds[d]);
+#pragma unroll
+            for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
+            if (NEQ > 5) {
+                const R VP = pVx * sx + pVy * sy + pVz * sz;
+#pragma unroll
+                for (int L = 5; L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
             }
-            const double stept = 2.0 * cur.dtm / (3.0 * cur.dtm / dtm_main + 2.0);
-            const double cvs = stept / cur.vol;
-            R dq[NEQ];
+            const bool to_a = d == (march == 0 ? 1 : 0);
+            if (d == march) {
 #pragma unroll
-            for (int L = 0; L < NEQ; L++) {
-                const R f1 = L < 5 ? R(cur.f1) : R(cur.f1t);
-                const R sum = tin[0][L] + tin[1][L] + tin[2][L];
-                if (FORWARD)
-                    dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
-                else
-                    dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
-                Dq[(long)L * tot + c] = dq[L];
+                for (int L = 0; L < NEQ; L++) tm[L] = o[L];
+            } else {
+                const int slot = to_a ? wa : wb;
+                const int dir = to_a ? 0 : 1;
+#pragma unroll
+                for (int L = 0; L < NEQ; L++) buf[((npar * 2 + dir) * NEQ + L) * row + slot] = o[L];
             }
-            const double roinv = 1.0 / cur.v[0];
-            const R pVx = R(cur.v[1] * roinv), pVy = R(cur.v[2] * roinv), pVz = R(cur.v[3] * roinv);
-            const int npar = par ^ 1;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                R o[NEQ];
-                R dif[5];
-                const R sx = R(cur.sdm[d][0]), sy = R(cur.sdm[d][1]), sz = R(cur.sdm[d][2]);
-                lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, R(cur.pp), R(roinv), dq, dif);
-                const R rd = R(cur.rds[d]);
-#pragma unroll
-                for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
-                if (NEQ > 5) {
-                    const R VP = pVx * sx + pVy * sy + pVz * sz;
-#pragma unroll
-                    for (int L = 5; L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
-                }
-#pragma unroll
-                for (int L = 0; L < NEQ; L++) {
-                    if (d == march)
-                        tm[L] = o[L];
-                    else if (d == ax) {
-                        if (ln_a >= 0) buf[((npar * 2 + 0) * NEQ + L) * nlines + ln_a] = o[L];
-                    } else {
-                        if (ln_b >= 0) buf[((npar * 2 + 1) * NEQ + L) * nlines + ln_b] = o[L];
-                    }
-                }
-            }
-            c += cstep;
         }
         lds_barrier();
+    };
+    LineRaw<NEQ> bufA, bufB;
+    line_load<R, NEQ>(f, Dq, c0 + (long)min(max(-start, 0), Nm - 1) * cstep, bufA);
+    for (int t = 0; t < nplanes; t += 2) {
+        step(t, bufA, bufB);
+        if (t + 1 < nplanes) step(t + 1, bufB, bufA);
     }
 }
 

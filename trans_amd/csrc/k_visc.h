@@ -98,8 +98,11 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
         }
 #undef GG
     }
-    const int ng = turb ? 18 : 12;
-    for (int q = 0; q < ng; q++) f.DQ[(long)q * tot + c] = g[q];
+#pragma unroll
+    for (int q = 0; q < 12; q++) f.DQ[(long)q * tot + c] = g[q];
+    if (turb)
+#pragma unroll
+        for (int q = 12; q < 18; q++) f.DQ[(long)q * tot + c] = g[q];
     const double DuDx = g[0], DuDy = g[1], DuDz = g[2], DvDx = g[3], DvDy = g[4], DvDz = g[5];
     const double DwDx = g[6], DwDy = g[7], DwDz = g[8];
     const double W12 = (DuDy - DvDx) / 2., W13 = (DuDz - DwDx) / 2., W23 = (DvDz - DwDy) / 2.;

@@ -60,6 +60,8 @@ struct trans_ctx {
     DPatch* d_patch = nullptr;
     BcSeg* d_bcseg = nullptr;
     int n_bcseg = 0;
+    OutLine* d_outl = nullptr;  // (patch, i, k) lines of the outlet patches
+    int n_outl = 0;
     SegTable seg_cell, seg_face[3];
     int* d_wrange = nullptr;
     NormPart* d_part = nullptr;
@@ -68,6 +70,7 @@ struct trans_ctx {
     int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
     int max_lines = 0;     // largest line count over the blocks for that axis
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
+    void* line_sink = nullptr;  // write sink of the idle lanes of the line-march sweep
     std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
     std::vector<void*> allocs;
     std::string err;
@@ -251,6 +254,7 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
         (rc = dalloc(c, &f.Rds, 3 * n)) || (rc = dalloc(c, &f.Dtm, n)) || (rc = dalloc(c, &f.Wt0, 7 * n)) ||
         (rc = dalloc(c, &f.Wt1, 7 * n)) || (rc = dalloc(c, &f.Wt2, 7 * n)) || (rc = dalloc(c, &f.DQ, 18 * n)) ||
         (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.F1t, n)) ||
+        (rc = dalloc(c, &f.SDm, 9 * n)) || (rc = dalloc(c, &f.Stp, 2 * n)) ||
         (rc = dalloc(c, &f.G[0], 7 * n)) || (rc = dalloc(c, &f.G[1], 7 * n)) || (rc = dalloc(c, &f.G[2], 7 * n)))
         return rc;
     if (c->p.if_euler == 0) {
@@ -298,6 +302,20 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
         }
     for (int q = 0; q < (int)dp.size(); q++)
         for (int base = 0; base < dp[q].count; base += 128) bs.push_back(BcSeg{q, base});
+    std::vector<OutLine> ol;
+    for (int q = 0; q < (int)dp.size(); q++) {
+        const trans_patch& pt = dp[q].p;
+        if (pt.bctype != 7) continue;
+        if (pt.point_end[1] - pt.point_start[1] + 1 > OUTLET_MAXJ)
+            return fail(c, TRANS_EUNSUPPORTED, "outlet patch longer than " + std::to_string(OUTLET_MAXJ) + " in j");
+        for (int k = pt.point_start[2]; k <= pt.point_end[2]; k++)
+            for (int i = pt.point_start[0]; i <= pt.point_end[0]; i++) ol.push_back(OutLine{q, i, k});
+    }
+    c->n_outl = (int)ol.size();
+    if (c->n_outl) {
+        if ((rc = dalloc(c, &c->d_outl, ol.size()))) return rc;
+        HIPCHK(c, hipMemcpy(c->d_outl, ol.data(), ol.size() * sizeof(OutLine), hipMemcpyHostToDevice));
+    }
     c->n_bcseg = (int)bs.size();
     if ((rc = dalloc(c, &c->d_patch, dp.size())) || (rc = dalloc(c, &c->d_bcseg, bs.size()))) return rc;
     if (!dp.empty()) HIPCHK(c, hipMemcpy(c->d_patch, dp.data(), dp.size() * sizeof(DPatch), hipMemcpyHostToDevice));
@@ -315,12 +333,17 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
                 const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
                 mx = std::max(mx, n[(m + 1) % 3] * n[(m + 2) % 3]);
             }
-            if (mx <= 1024 && 4 * neq * mx * esz <= 128 * 1024) {
+            if (mx <= 1024 && 4 * neq * (mx + 1) * esz <= 128 * 1024) {
                 c->march = m;
                 c->max_lines = mx;
                 c->line_threads = (mx + 63) / 64 * 64;
             }
         }
+    }
+    if (c->line_threads) {
+        double* sk = nullptr;
+        if ((rc = dalloc(c, &sk, (size_t)nblk * 7 * 1024))) return rc;
+        c->line_sink = sk;
     }
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
@@ -439,6 +462,9 @@ int trans_hip_set_geometry(trans_ctx* c, int blk, const trans_geom_view* g)
             mw.insert(mw.end(), (size_t)fsz[q], 0);
     }
     HIPCHK(c, hipMemcpy(f.mw + c->mwoff[blk], mw.data(), mw.size() * sizeof(int), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_face_means, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->f, c->d_blk, c->seg_cell.d);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
@@ -515,6 +541,9 @@ int trans_hip_bc(trans_ctx* c, int sweep)
     if (c->n_bcseg == 0) return 0;
     KLAUNCH(c, "k_bc", k_bc, dim3(c->n_bcseg), dim3(128), c->p, if_turb_now(c), sweep, c->f,
                        c->d_blk, c->d_patch, c->d_bcseg);
+    if (sweep == -1 && c->n_outl)
+        KLAUNCH(c, "k_bc_outlet", k_bc_outlet, dim3(c->n_outl), dim3(256), c->p, if_turb_now(c), c->f, c->d_blk,
+                c->d_patch, c->d_outl);
     return launch_check(c, "k_bc");
 }
 
@@ -580,16 +609,30 @@ int trans_hip_residual(trans_ctx* c)
 
 }  // extern "C"
 
+template <typename R, int NEQ, int MAXT>
+static void launch_lines(trans_ctx* c, R* dq)
+{
+    const int nblk = (int)c->hb.size();
+    const size_t lds = (size_t)4 * NEQ * (c->max_lines + 1) * sizeof(R);
+    R* sink = reinterpret_cast<R*>(c->line_sink);
+    KLAUNCH_LDS(c, "k_lusgs_line<fwd>", (k_lusgs_line<R, true, NEQ, MAXT>), dim3(nblk), dim3(c->line_threads), lds,
+                c->p, c->f, dq, c->d_blk, c->march, sink);
+    KLAUNCH_LDS(c, "k_lusgs_line<bwd>", (k_lusgs_line<R, false, NEQ, MAXT>), dim3(nblk), dim3(c->line_threads), lds,
+                c->p, c->f, dq, c->d_blk, c->march, sink);
+}
+
 template <typename R, int NEQ>
 static void launch_sweeps(trans_ctx* c, R* dq)
 {
     const int nblk = (int)c->hb.size();
     if (c->line_threads) {
-        const size_t lds = (size_t)4 * NEQ * c->max_lines * sizeof(R);
-        KLAUNCH_LDS(c, "k_lusgs_line<fwd>", (k_lusgs_line<R, true, NEQ>), dim3(nblk), dim3(c->line_threads), lds,
-                    c->p, c->f, dq, c->d_blk, c->march);
-        KLAUNCH_LDS(c, "k_lusgs_line<bwd>", (k_lusgs_line<R, false, NEQ>), dim3(nblk), dim3(c->line_threads), lds,
-                    c->p, c->f, dq, c->d_blk, c->march);
+        // the register budget per lane follows the workgroup size
+        if (c->line_threads <= 256)
+            launch_lines<R, NEQ, 256>(c, dq);
+        else if (c->line_threads <= 512)
+            launch_lines<R, NEQ, 512>(c, dq);
+        else
+            launch_lines<R, NEQ, 1024>(c, dq);
     } else {
         KLAUNCH(c, "k_lusgs_sweep<fwd>", (k_lusgs_sweep<R, true, NEQ>), dim3(nblk), dim3(1024), c->p, c->f, dq,
                 c->d_blk);
