@@ -92,21 +92,31 @@ def units(blocks):
 
 
 def cpu_baseline(blocks, prm, budget_s=12.0):
-    """The CPU oracle (single thread) on the same workload, a bounded number of iterations."""
-    from oracle.oracle import OracleCase
-    oc = OracleCase(blocks, prm)
-    ki = oc.iterate(1, 1)  # warm
-    n, t0 = 0, time.perf_counter()
-    while True:
-        ki = oc.iterate(ki, 1)
-        n += 1
-        if time.perf_counter() - t0 > budget_s or n >= 200:
-            break
-    dt = time.perf_counter() - t0
+    """The CPU oracle on the same workload for a bounded number of iterations: OpenMP over
+    blocks with as many threads as blocks (capped by OMP_NUM_THREADS), then one thread."""
+    from oracle.oracle import OracleCase, lib
     cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
-    return {"value": cells * n / dt / 1e6, "unit": UNIT, "cores": 1, "kind": "port",
+
+    def run(threads, budget):
+        lib().ora_set_threads(threads)
+        oc = OracleCase(blocks, prm)
+        ki = oc.iterate(1, 1)  # warm
+        n, t0 = 0, time.perf_counter()
+        while True:
+            ki = oc.iterate(ki, 1)
+            n += 1
+            if time.perf_counter() - t0 > budget or n >= 200:
+                break
+        dt = time.perf_counter() - t0
+        return cells * n / dt / 1e6, n, dt
+
+    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16")), len(blocks), os.cpu_count() or 1))
+    v, n, dt = run(threads, budget_s)
+    v1, n1, dt1 = run(1, budget_s / 2)
+    return {"value": round(v, 4), "unit": UNIT, "cores": threads, "kind": "port",
             "sample": f"{n} iterations of the same workload after 1 warm-up iteration, "
-                      f"oracle/trans_oracle.c single-threaded, {dt:.1f} s"}
+                      f"oracle/trans_oracle*.c with OpenMP over blocks on {threads} threads, "
+                      f"{dt:.1f} s; single thread: {v1:.4f} {UNIT} ({n1} iterations, {dt1:.1f} s)"}
 
 
 def timed_run(blocks, prm, steps, warmup, prof_steps):

@@ -69,6 +69,8 @@ def lib():
         L.ora_ausmpw_common.argtypes = [PP] + [C.c_double] * 14 + [DP]
         L.ora_mdcd_rec.argtypes = [C.c_double, DP, C.c_int]
         L.ora_mdcd_rec.restype = C.c_double
+        L.ora_set_threads.argtypes = [C.c_int]
+        L.ora_num_threads.restype = C.c_int
         L.ora_alimiter.argtypes = [C.c_double, C.c_double]
         L.ora_alimiter.restype = C.c_double
         L.ora_roe_common.argtypes = [PP] + [C.c_double] * 16 + [DP]

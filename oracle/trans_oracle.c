@@ -8,6 +8,9 @@
 #include "trans_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -1016,30 +1019,32 @@ void ora_update_u(const trans_params* p, ora_block* b)
 /* main.F90:66-133 */
 void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk)
 {
+    /* Blocks are independent between the two BC sweeps; OpenMP over blocks (the BC
+     * sweeps read only interior cells of other blocks and write only own ghosts). */
     const int if_turb = ora_if_turb(p, ki);
+#pragma omp parallel for schedule(dynamic, 1)
     for (int ib = 0; ib < nblk; ib++) ora_spectral_radius(p, ki, &blocks[ib]);
     ora_bc(p, if_turb, blocks, nblk, -1);
-    for (int ib = 0; ib < nblk; ib++) ora_spaceinterp(p, if_turb, &blocks[ib]);
-    if (p->if_euler == 0)
-        for (int ib = 0; ib < nblk; ib++) ora_diff_dqdxyz(p, if_turb, &blocks[ib]);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ib = 0; ib < nblk; ib++) {
+        ora_spaceinterp(p, if_turb, &blocks[ib]);
+        if (p->if_euler == 0) ora_diff_dqdxyz(p, if_turb, &blocks[ib]);
+    }
     ora_bc(p, if_turb, blocks, nblk, 0);
     /* RCCorrection: Kind_RotMod = 0 leaves f_r1 = 1 */
+#pragma omp parallel for schedule(dynamic, 1)
     for (int ib = 0; ib < nblk; ib++) {
         ora_block* b = &blocks[ib];
         long n = 7L * (b->ni + 5) * (b->nj + 5) * (b->nk + 5);
         memset(b->D, 0, sizeof(double) * (size_t)n);
         if (p->if_euler == 0) ora_diffusion(p, b);
         if (if_turb) ora_turb_diffusion(p, b);
-    }
-    for (int ib = 0; ib < nblk; ib++) {
-        if (p->kind_spatial == 2) ora_convect_muscl(p, &blocks[ib]);
-        if (p->kind_spatial == 3) ora_convect_mdcd(p, &blocks[ib]);
-        if (if_turb) ora_turb_convect_muscl(p, &blocks[ib]);
-    }
-    for (int ib = 0; ib < nblk; ib++) {
-        if (p->kind_roe == 1) ora_convect_roe(p, &blocks[ib]);
-        if (p->kind_roe == 3) ora_convect_ausmpw(p, &blocks[ib]);
-        if (if_turb) ora_turb_convect_upwind(p, &blocks[ib]);
+        if (p->kind_spatial == 2) ora_convect_muscl(p, b);
+        if (p->kind_spatial == 3) ora_convect_mdcd(p, b);
+        if (if_turb) ora_turb_convect_muscl(p, b);
+        if (p->kind_roe == 1) ora_convect_roe(p, b);
+        if (p->kind_roe == 3) ora_convect_ausmpw(p, b);
+        if (if_turb) ora_turb_convect_upwind(p, b);
     }
 }
 
@@ -1047,11 +1052,31 @@ void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk)
 void ora_lusgs_update(const trans_params* p, int ki, ora_block* blocks, int nblk)
 {
     const int if_turb = ora_if_turb(p, ki);
+#pragma omp parallel for schedule(dynamic, 1)
     for (int ib = 0; ib < nblk; ib++) {
         ora_lusgs(p, &blocks[ib]);
         if (if_turb) ora_turb_lusgs_sst(p, &blocks[ib]);
         ora_update_v(p, ki, if_turb, &blocks[ib]);
     }
+}
+
+/* Threads an OpenMP region of the oracle uses. */
+int ora_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ora_set_threads(int n)
+{
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
 }
 
 /* main.F90:48-211 (no output) */

@@ -70,6 +70,9 @@ void ora_ausmpw_common(const trans_params* p, double PLo, double PRo, double TLo
 /* Composite steps over all blocks, in main.F90 order. */
 void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk);
 void ora_lusgs_update(const trans_params* p, int ki, ora_block* blocks, int nblk);
+/* Threads an OpenMP region of the oracle uses (blocks run in parallel). */
+int ora_num_threads(void);
+void ora_set_threads(int n);
 /* n outer iterations starting at *ki; *ki is advanced. */
 void ora_iterate(const trans_params* p, int* ki, int n_iter, ora_block* blocks, int nblk);
 
