@@ -119,6 +119,20 @@ def cpu_baseline(blocks, prm, budget_s=12.0):
                       f"{dt:.1f} s; single thread: {v1:.4f} {UNIT} ({n1} iterations, {dt1:.1f} s)"}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (profiles/hbm_traffic.json, counters scaled by a plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        tab = json.load(fh).get(workload, {})
+    for name, rec in tab.items():
+        if name.startswith(kernel.split("<")[0]) and (("true" in name) == ("fwd" in kernel)):
+            return rec["bytes"]
+    return None
+
+
 def timed_run(blocks, prm, steps, warmup, prof_steps):
     """Warm up, time `steps` iterations without instrumentation, then profile `prof_steps`
     more with HIP events on the library's stream."""
@@ -224,8 +238,8 @@ def main():
         alg = per * u[unit]
         ach = alg / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
-                "avg_launch_us": round(avg_s * 1e6, 3), "alg_bytes_per_launch": alg}
+                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, name),
+                "kernel": name, "avg_launch_us": round(avg_s * 1e6, 3), "alg_bytes_per_launch": alg}
     value = cells * args.steps / dt / 1e6
     kern = {}
     for k, (kms, kn) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
