@@ -17,6 +17,7 @@ struct BlkInfo {
     int npatch;
     int patch0;      // first entry of this block in the device patch table
     long mwoff;      // offset of the block's wall marks in Fields::mw
+    long woff;       // offset of the block in the hyperplane-major sweep buffer Fields::W
 };
 
 // One workgroup's share of an index space: block `blk`, linear indices base..base+255.
@@ -58,7 +59,25 @@ struct Fields {
     float* Ds;        // FP32 copy of D for the FP32 implicit operator (7)
     float* Rdsf;      // FP32 Rds (3)
     int* mw;          // wall marks per block: i0 (NJ1*NK1), im, j0 (NI1*NK1), jm, k0 (NI1*NJ1), km
+    // Inputs of the line-march LU-SGS in hyperplane-major order (see widx): component
+    // stride wtot, components W_* below.  Null when the hyperplane fallback sweep is used.
+    double* W;
+    float* Wf;        // FP32 increments of the FP32 implicit operator (7 components)
+    long wtot;
+    int march;        // march axis of the line sweep (0 i, 1 j, 2 k)
 };
+
+// components of Fields::W
+constexpr int W_D = 0;      // 7: right-hand side, then the increment (FP64 operator)
+constexpr int W_STEPT = 7;  // stept
+constexpr int W_CVS = 8;    // stept / Vol
+constexpr int W_F1 = 9;     // mean-flow diagonal
+constexpr int W_F1T = 10;   // k-omega diagonal
+constexpr int W_V = 11;     // 4: rho, rho*u, rho*v, rho*w
+constexpr int W_PP = 15;    // pressure
+constexpr int W_RDS = 16;   // 3: spectral radii
+constexpr int W_SDM = 19;   // 9: face-mean vectors (geometry)
+constexpr int W_NCOMP = 28;
 
 __device__ __forceinline__ long cidx(const BlkInfo& b, int i, int j, int k)
 {
@@ -75,6 +94,22 @@ __device__ __forceinline__ void decode3(int lin, int ni, int nj, int& i, int& j,
     i += 1;
     j += 1;
     k += 1;
+}
+
+// Hyperplane-major index of interior cell (i,j,k) for the line-march sweeps.  Lines run
+// along the march axis m; the two other axes a < b number the lines a-fastest.  Cells of one
+// hyperplane s = (i-1)+(j-1)+(k-1) are stored contiguously by line, so the lanes of a wave,
+// which hold consecutive lines on the same hyperplane at every step, read consecutive
+// addresses (in the natural layout they would be nx-1 elements apart).
+__device__ __forceinline__ long widx(const BlkInfo& b, int march, int i, int j, int k)
+{
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1;
+    const int Na = march == 0 ? NJ1 : NI1;
+    const int Nb = march == 2 ? NJ1 : b.nk - 1;
+    const int a = march == 0 ? j : i;
+    const int bb = march == 2 ? j : k;
+    const long s = (long)(i - 1) + (j - 1) + (k - 1);
+    return b.woff + s * (long)Na * Nb + (a - 1) + (long)Na * (bb - 1);
 }
 
 __device__ __forceinline__ double sutherland(const trans_params& p, double tt)

@@ -5,7 +5,9 @@
 #include "dev_common.h"
 
 // ---------------------------------------------------------------------------------------
-// Time/TIME_LUSGS.F90:60-108: BDF2 right-hand side and the scalar diagonal F1.
+// Time/TIME_LUSGS.F90:60-108: BDF2 right-hand side and the scalar diagonal F1.  With the
+// line-march sweep the results, and the state the sweep needs, go to the hyperplane-major
+// buffer W (FP32 increments to Wf for the FP32 operator); otherwise to D, F1 and Stp.
 __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
                                                    const BlkInfo* __restrict__ blks,
                                                    const Seg* __restrict__ seg)
@@ -21,21 +23,46 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
     const long tot = f.tot;
     const double dtm = p.dt_lusgs_main;
     const double vol = f.Vol[c];
+    double rhs[5];
 #pragma unroll
     for (int L = 0; L < 5; L++) {
         const long e = (long)L * tot + c;
-        f.D[e] = f.D[e] / vol - (3.0 * f.V[e] - 4.0 * f.Wt0[e] + f.Wt1[e]) * 0.5 / dtm;
+        rhs[L] = f.D[e] / vol - (3.0 * f.V[e] - 4.0 * f.Wt0[e] + f.Wt1[e]) * 0.5 / dtm;
     }
     const double Dt = f.Dtm[c];
     const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
     const double cvs = stept / vol;
-    f.F1[c] = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * cvs;
-    f.Stp[c] = stept;
-    f.Stp[tot + c] = cvs;
+    const double f1 = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * cvs;
+    if (f.W) {
+        const long w = widx(b, f.march, i, j, k);
+        const long wt = f.wtot;
+#pragma unroll
+        for (int L = 0; L < 5; L++) {
+            if (f.Wf)
+                f.Wf[(long)L * wt + w] = (float)rhs[L];
+            else
+                f.W[(long)(W_D + L) * wt + w] = rhs[L];
+        }
+        f.W[W_STEPT * wt + w] = stept;
+        f.W[W_CVS * wt + w] = cvs;
+        f.W[W_F1 * wt + w] = f1;
+#pragma unroll
+        for (int L = 0; L < 4; L++) f.W[(long)(W_V + L) * wt + w] = f.V[(long)L * tot + c];
+        f.W[W_PP * wt + w] = f.PP[c];
+#pragma unroll
+        for (int d = 0; d < 3; d++) f.W[(long)(W_RDS + d) * wt + w] = f.Rds[(long)d * tot + c];
+    } else {
+#pragma unroll
+        for (int L = 0; L < 5; L++) f.D[(long)L * tot + c] = rhs[L];
+        f.F1[c] = f1;
+        f.Stp[c] = stept;
+        f.Stp[tot + c] = cvs;
+    }
 }
 
 // Face-mean vectors 0.5*(S_d(f+1) + S_d(f)) of every interior cell (geometry; the LU-SGS
-// flux Jacobians of Time/TIME_LUSGS.F90:190-256 use them for the neighbour cell).
+// flux Jacobians of Time/TIME_LUSGS.F90:190-256 use them for the neighbour cell), also in
+// hyperplane-major order when the line-march sweep is used.
 __global__ void __launch_bounds__(256) k_face_means(Fields f, const BlkInfo* __restrict__ blks,
                                                     const Seg* __restrict__ seg)
 {
@@ -49,13 +76,34 @@ __global__ void __launch_bounds__(256) k_face_means(Fields f, const BlkInfo* __r
     const long c = cidx(b, i, j, k);
     const long tot = f.tot;
     const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+    const long w = f.W ? widx(b, f.march, i, j, k) : 0;
 #pragma unroll
     for (int d = 0; d < 3; d++)
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             const long o = (long)(d + 3 * q) * tot;
-            f.SDm[o + c] = 0.5 * (f.SD[o + c + st[d]] + f.SD[o + c]);
+            const double m = 0.5 * (f.SD[o + c + st[d]] + f.SD[o + c]);
+            f.SDm[o + c] = m;
+            if (f.W) f.W[(long)(W_SDM + d + 3 * q) * f.wtot + w] = m;
         }
+}
+
+// Increments of the line-march sweep back to D in the natural layout (TIME_Update_V reads D).
+template <typename R>
+__global__ void __launch_bounds__(256) k_unskew(int neq, Fields f, const R* __restrict__ Wd,
+                                                const BlkInfo* __restrict__ blks,
+                                                const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long w = widx(b, f.march, i, j, k);
+    for (int L = 0; L < neq; L++) f.D[(long)L * f.tot + c] = (double)Wd[(long)L * f.wtot + w];
 }
 
 // Time/TIME_LUSGS.F90:696-768 (LUSGS_ADQ) -- flux-Jacobian times increment, gamma = 1.4.
@@ -232,27 +280,27 @@ struct LineRaw {
     double rds[3];
 };
 
+// Inputs of one cell of the sweep from the hyperplane-major buffer (slot w).  For the FP32
+// operator R(rds) rounds the spectral radius exactly as the FP32 copy Rdsf does.
 template <typename R, int NEQ>
-__device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__ Dq, long c, LineRaw<NEQ>& x)
+__device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__ Dq, long w, LineRaw<NEQ>& x)
 {
-    const long tot = f.tot;
+    const long wt = f.wtot;
+    const double* W = f.W;
 #pragma unroll
-    for (int L = 0; L < NEQ; L++) x.d[L] = (double)Dq[(long)L * tot + c];
-    x.stept = f.Stp[c];
-    x.cvs = f.Stp[tot + c];
-    x.f1 = f.F1[c];
-    x.f1t = NEQ > 5 ? f.F1t[c] : 1.0;
+    for (int L = 0; L < NEQ; L++) x.d[L] = (double)Dq[(long)L * wt + w];
+    x.stept = W[W_STEPT * wt + w];
+    x.cvs = W[W_CVS * wt + w];
+    x.f1 = W[W_F1 * wt + w];
+    x.f1t = NEQ > 5 ? W[W_F1T * wt + w] : 1.0;
 #pragma unroll
-    for (int L = 0; L < 4; L++) x.v[L] = f.V[(long)L * tot + c];
-    x.pp = f.PP[c];
+    for (int L = 0; L < 4; L++) x.v[L] = W[(long)(W_V + L) * wt + w];
+    x.pp = W[W_PP * wt + w];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) x.sdm[d][q] = f.SDm[(long)(d + 3 * q) * tot + c];
-        if constexpr (sizeof(R) == 4)
-            x.rds[d] = (double)f.Rdsf[(long)d * tot + c];
-        else
-            x.rds[d] = f.Rds[(long)d * tot + c];
+        for (int q = 0; q < 3; q++) x.sdm[d][q] = W[(long)(W_SDM + d + 3 * q) * wt + w];
+        x.rds[d] = W[(long)(W_RDS + d) * wt + w];
     }
 }
 
@@ -265,11 +313,12 @@ __device__ __forceinline__ void lds_barrier()
 
 // march: 0 = i, 1 = j, 2 = k.  Lines are indexed a-fastest over the two other axes (a < b).
 // Dynamic LDS: 2 parities x 2 directions x NEQ x (nlines + 1) values of R; slot nlines of each
-// row is a sink for the lanes that have nothing to hand on.
+// row is a sink for the lanes that have nothing to hand on.  Dq is W (FP64 operator) or Wf.
 //
 // The step loop is free of wave-level branches: lanes outside their line's window compute on
-// a clamped cell and send their results to sinks (`sink` in global memory, slot nlines in
-// LDS).  Every step then issues the same memory operations, which keeps the compiler's
+// whatever their slot holds and send their results to sinks (`sink` in global memory, slot
+// nlines in LDS).  Inputs and increments live in the hyperplane-major buffer (widx), so the
+// lanes of a wave read and write consecutive addresses at every step.  Every step then issues the same memory operations, which keeps the compiler's
 // memory-counter waits precise across the loop (the next cell's loads stay in flight while
 // the current cell is computed).
 template <typename R, bool FORWARD, int NEQ, int MAXT>
@@ -288,8 +337,6 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
     const int row = nlines + 1;
     const int nplanes = NI1 + NJ1 + NK1 - 2;
     R* buf = reinterpret_cast<R*>(lds_raw);  // [2 parity][2 dir][NEQ][nlines + 1]
-    const long tot = f.tot;
-    const long stm = march == 0 ? 1 : (march == 1 ? (long)b.nx : (long)b.nx * b.ny);
     const int ln = threadIdx.x;
     const bool active = ln < nlines;
     const int ia = active ? ln % Na + 1 : 1, ib = active ? ln / Na + 1 : 1;
@@ -301,12 +348,9 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
     const bool in_a = FORWARD ? (ia > 1) : (ia < Na);
     const bool in_b = FORWARD ? (ib > 1) : (ib < Nb);
     const int start = FORWARD ? (ia - 1) + (ib - 1) : (Na - ia) + (Nb - ib);
-    const int im0 = FORWARD ? 1 : Nm;
-    const int pi = march == 0 ? im0 : ia;
-    const int pj = march == 1 ? im0 : (march == 0 ? ia : ib);
-    const int pk = march == 2 ? im0 : ib;
-    const long cstep = FORWARD ? stm : -stm;
-    const long c0 = cidx(b, pi, pj, pk);
+    // hyperplane-major slots: the lane's cell at step t is on hyperplane t (forward) or
+    // nplanes-1-t (backward); padding lanes read the last line's slots
+    const long wline = b.woff + (active ? ln : nlines - 1);
     R* my_sink = sink + (long)blockIdx.x * NEQ * MAXT + ln;
     R tm[NEQ];  // term from the previous cell of this line
 #pragma unroll
@@ -317,8 +361,9 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
         const int par = t & 1;
         const int rel = t - start;
         const bool act = active && rel >= 0 && rel < Nm;
-        const long c = c0 + (long)min(max(rel, 0), Nm - 1) * cstep;
-        line_load<R, NEQ>(f, Dq, c0 + (long)min(max(rel + 1, 0), Nm - 1) * cstep, nxt);
+        const long w = wline + (long)(FORWARD ? t : nplanes - 1 - t) * nlines;
+        line_load<R, NEQ>(f, Dq, wline + (long)(FORWARD ? min(t + 1, nplanes - 1) : max(nplanes - 2 - t, 0)) * nlines,
+                          nxt);
         const double stept = cur.stept;
         const double cvs = cur.cvs;
         R dq[NEQ];
@@ -337,7 +382,7 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
                 dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
             else
                 dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
-            R* dst = act ? Dq + (long)L * tot + c : my_sink + (long)L * MAXT;
+            R* dst = act ? Dq + (long)L * f.wtot + w : my_sink + (long)L * MAXT;
             *dst = dq[L];
         }
         const double roinv = 1.0 / cur.v[0];
@@ -371,7 +416,7 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
         lds_barrier();
     };
     LineRaw<NEQ> bufA, bufB;
-    line_load<R, NEQ>(f, Dq, c0 + (long)min(max(-start, 0), Nm - 1) * cstep, bufA);
+    line_load<R, NEQ>(f, Dq, wline + (long)(FORWARD ? 0 : nplanes - 1) * nlines, bufA);
     for (int t = 0; t < nplanes; t += 2) {
         step(t, bufA, bufB);
         if (t + 1 < nplanes) step(t + 1, bufB, bufA);

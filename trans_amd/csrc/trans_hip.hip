@@ -274,14 +274,6 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
     }
     if ((rc = dalloc(c, &f.mw, c->mwoff.back()))) return rc;
 
-    std::vector<BlkInfo> bi(nblk);
-    for (int b = 0; b < nblk; b++) {
-        const HostBlock& h = c->hb[b];
-        bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b],
-                        c->mwoff[b]};
-    }
-    if ((rc = dalloc(c, &c->d_blk, nblk))) return rc;
-    HIPCHK(c, hipMemcpy(c->d_blk, bi.data(), nblk * sizeof(BlkInfo), hipMemcpyHostToDevice));
 
     // BC patch table and its schedule
     std::vector<DPatch> dp;
@@ -340,11 +332,31 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
             }
         }
     }
+    std::vector<long> woff(nblk, 0);
     if (c->line_threads) {
         double* sk = nullptr;
         if ((rc = dalloc(c, &sk, (size_t)nblk * 7 * 1024))) return rc;
         c->line_sink = sk;
+        // hyperplane-major sweep buffer: nplanes x nlines slots per block
+        long wt = 0;
+        for (int b = 0; b < nblk; b++) {
+            const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
+            woff[b] = wt;
+            wt += (long)(n[0] + n[1] + n[2] - 2) * n[(c->march + 1) % 3] * n[(c->march + 2) % 3];
+        }
+        f.wtot = wt;
+        f.march = c->march;
+        if ((rc = dalloc(c, &f.W, (size_t)W_NCOMP * wt))) return rc;
+        if (c->p.implicit_fp32 && (rc = dalloc(c, &f.Wf, (size_t)7 * wt))) return rc;
     }
+    std::vector<BlkInfo> bi(nblk);
+    for (int b = 0; b < nblk; b++) {
+        const HostBlock& h = c->hb[b];
+        bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b],
+                        c->mwoff[b], woff[b]};
+    }
+    if ((rc = dalloc(c, &c->d_blk, nblk))) return rc;
+    HIPCHK(c, hipMemcpy(c->d_blk, bi.data(), nblk * sizeof(BlkInfo), hipMemcpyHostToDevice));
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
@@ -652,7 +664,24 @@ static int lusgs_update_async(trans_ctx* c)
     if (if_turb)
         KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
     const int neq = if_turb ? 7 : 5;
-    if (c->p.implicit_fp32) {
+    if (c->f.W) {
+        // line-march sweeps on the hyperplane-major buffer, increments back to D after
+        if (c->p.implicit_fp32) {
+            if (if_turb)
+                launch_sweeps<float, 7>(c, c->f.Wf);
+            else
+                launch_sweeps<float, 5>(c, c->f.Wf);
+            KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), neq, c->f, (const float*)c->f.Wf,
+                    c->d_blk, c->seg_cell.d);
+        } else {
+            if (if_turb)
+                launch_sweeps<double, 7>(c, c->f.W);
+            else
+                launch_sweeps<double, 5>(c, c->f.W);
+            KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), neq, c->f,
+                    (const double*)c->f.W, c->d_blk, c->seg_cell.d);
+        }
+    } else if (c->p.implicit_fp32) {
         KLAUNCH(c, "k_convert<double,float>", (k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), neq, 0,
                 c->tot, (const double*)c->f.D, c->f.Ds, c->d_blk, c->seg_cell.d);
         if (if_turb)
