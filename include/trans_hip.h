@@ -170,6 +170,9 @@ typedef struct trans_ctx trans_ctx;
  * connectivity (Input/PLOT3DReadInit.F90:1-349), which the host has already built. */
 int trans_hip_create(const trans_params* params, int nblk, const trans_block_desc* blocks,
                      trans_ctx** out);
+/* (The LU-SGS sweeps run as a line march when every block's cross-section fits one workgroup,
+ * else as a hyperplane wavefront; TRANS_HIP_LINE_SWEEP=0 in the environment at create time
+ * selects the hyperplane wavefront for all blocks.  Both give the same results.) */
 int trans_hip_destroy(trans_ctx* ctx);
 const char* trans_hip_last_error(const trans_ctx* ctx);
 
@@ -193,8 +196,14 @@ int trans_hip_residual(trans_ctx* ctx);            /* main.F90:66-133: SpectralR
 int trans_hip_lusgs_update(trans_ctx* ctx, trans_resid* per_block);
                                                    /* TIME_LUSGS + TURBTWO_TIME_LUSGS_SST +
                                                       TIME_Update_V, main.F90:137-151 */
+/* Stage itr_chld (1, 2 or 3) of the Runge-Kutta update (IF_RK = 1): TIME_RK_Save + TIME_RK +
+ * TURBTWO_TIME_LUSGS_SST + TIME_RK_V_Update_New, main.F90:158-170.  per_block (may be NULL)
+ * receives DRSa from the density residual before the stage and DRSk/DRSo from the k-omega
+ * increments. */
+int trans_hip_rk_update(trans_ctx* ctx, int itr_chld, trans_resid* per_block);
 int trans_hip_update_u(trans_ctx* ctx);            /* TIME_Update_U, main.F90:192-197 */
-/* n outer iterations of main.F90:48-211 (sub-iterations, update of Wt0/Wt1, KI += 1).
+/* n outer iterations of main.F90:48-211 (Kchld sub-iterations, or the three Runge-Kutta stages
+ * when IF_RK = 1; update of Wt0/Wt1, KI += 1).
  * per_block (nblk entries, may be NULL) receives the norms of the last sub-iteration. */
 int trans_hip_iterate(trans_ctx* ctx, int n_iter, trans_resid* per_block);
 /* Wait for all queued device work of the context. */

@@ -60,6 +60,10 @@ def lib():
         L.ora_residual.argtypes = [PP, C.c_int, PB, C.c_int]
         L.ora_lusgs_update.argtypes = [PP, C.c_int, PB, C.c_int]
         L.ora_iterate.argtypes = [PP, C.POINTER(C.c_int), C.c_int, PB, C.c_int]
+        L.ora_rk_save.argtypes = [C.c_int, PB]
+        L.ora_rk.argtypes = [C.c_int, PB]
+        L.ora_rk_update_v.argtypes = [PP, C.c_int, C.c_int, PB]
+        L.ora_rk_update.argtypes = [PP, C.c_int, C.c_int, PB, C.c_int]
         L.ora_diff_dqdxyz.argtypes = [PP, C.c_int, PB]
         L.ora_diffusion.argtypes = [PP, PB]
         L.ora_turb_diffusion.argtypes = [PP, PB]
@@ -149,6 +153,10 @@ class OracleCase:
 
     def lusgs_update(self, ki):
         lib().ora_lusgs_update(C.byref(self.params), ki, self.ob, self.nblk)
+        return [self.ob[n].res.as_dict() for n in range(self.nblk)]
+
+    def rk_update(self, ki, itr_chld):
+        lib().ora_rk_update(C.byref(self.params), ki, itr_chld, self.ob, self.nblk)
         return [self.ob[n].res.as_dict() for n in range(self.nblk)]
 
     def iterate(self, ki, n):

@@ -10,6 +10,7 @@
 #define F(L, i, j, k) b->F[(L)-1 + 7 * IX(i, j, k)]
 #define WT0(L, i, j, k) b->Wt0[(L)-1 + 7 * IX(i, j, k)]
 #define WT1(L, i, j, k) b->Wt1[(L)-1 + 7 * IX(i, j, k)]
+#define WT2(L, i, j, k) b->Wt2[(L)-1 + 7 * IX(i, j, k)]
 #define PP(i, j, k) b->PP[IX(i, j, k)]
 #define T(i, j, k) b->T[IX(i, j, k)]
 #define RMIU(i, j, k) b->Rmiu[IX(i, j, k)]

@@ -1060,6 +1060,141 @@ void ora_lusgs_update(const trans_params* p, int ki, ora_block* blocks, int nblk
     }
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* Time/TIME_RK_Save.F90:14-27: the state at the start of the Runge-Kutta step. */
+void ora_rk_save(int itr_chld, ora_block* b)
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    if (itr_chld != 1) return;
+    for (int L = 1; L <= TRANS_ML; L++)
+        for (int J = 1; J <= NJ1; J++)
+            for (int K = 1; K <= NK1; K++)
+                for (int I = 1; I <= NI1; I++) WT2(L, I, J, K) = V(L, I, J, K);
+}
+
+/* Time/TIME_RK.F90:24-91: norms of the density residual D(1)/rho before the stage (last
+ * maximum wins), then the stage of the three-stage TVD Runge-Kutta scheme on rho..rho*E. */
+void ora_rk(int itr_chld, ora_block* b)
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    double DRSa = 0.0, DUa = 0.0;
+    int Ima = 0, Jma = 0, Kma = 0;
+    for (int K = 1; K <= NK1; K++)
+        for (int J = 1; J <= NJ1; J++)
+            for (int I = 1; I <= NI1; I++) {
+                double DR = fabs(D(1, I, J, K) / V(1, I, J, K));
+                if (DUa <= DR) { DUa = DR; Ima = I; Jma = J; Kma = K; }
+                DRSa = DRSa + DR * DR;
+            }
+    DRSa = sqrt(DRSa / NI1 / NJ1 / NK1);
+    for (int K = 1; K <= NK1; K++)
+        for (int J = 1; J <= NJ1; J++)
+            for (int I = 1; I <= NI1; I++) {
+                if (itr_chld == 1)
+                    for (int L = 1; L <= 5; L++)
+                        V(L, I, J, K) = V(L, I, J, K) + DTM(I, J, K) * D(L, I, J, K) / VOL(I, J, K);
+                if (itr_chld == 2)
+                    for (int L = 1; L <= 5; L++)
+                        V(L, I, J, K) = 0.75 * WT2(L, I, J, K) + 0.25 * V(L, I, J, K) +
+                                        0.25 * DTM(I, J, K) * D(L, I, J, K) / VOL(I, J, K);
+                if (itr_chld == 3)
+                    for (int L = 1; L <= 5; L++)
+                        V(L, I, J, K) = 1.0 / 3.0 * WT2(L, I, J, K) + 2.0 / 3.0 * V(L, I, J, K) +
+                                        2.0 / 3.0 * DTM(I, J, K) * D(L, I, J, K) / VOL(I, J, K);
+            }
+    b->res.drsa = DRSa; b->res.dua = DUa; b->res.ima = Ima; b->res.jma = Jma; b->res.kma = Kma;
+}
+
+/* Time/TIME_RK_V_Update_New.F90:75-260: pressure and temperature from the new conserved
+ * variables (no rotation term), the positivity-damped k/omega update with the SST limiter
+ * (its own constants: p = 0.4*rho*E - 0.2*|rho u|^2/rho and the k-omega Sutherland form
+ * without Csth1), then :265-355 the k and omega norms of the LU-SGS increments. */
+void ora_rk_update_v(const trans_params* p, int ki, int if_turb, ora_block* b)
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    for (int K = 1; K <= NK1; K++)
+        for (int J = 1; J <= NJ1; J++)
+            for (int I = 1; I <= NI1; I++) {
+                double VV = V(2, I, J, K) * V(2, I, J, K) + V(3, I, J, K) * V(3, I, J, K) +
+                            V(4, I, J, K) * V(4, I, J, K);
+                PP(I, J, K) = (1.4 - 1.0) * V(5, I, J, K) - 0.5 * (1.4 - 1.0) * VV / V(1, I, J, K);
+                T(I, J, K) = p->rxm2 * PP(I, J, K) / V(1, I, J, K);
+                if (!(p->if_euler == 0 && if_turb == 1 && ki >= p->kitm)) continue;
+                double AinRTk = D(6, I, J, K), AinRTO = D(7, I, J, K);
+                double ARTk = V(6, I, J, K), ARTO = V(7, I, J, K);
+                if (AinRTk >= 0.0)
+                    ARTk = ARTk + AinRTk;
+                else
+                    ARTk = ARTk * exp(AinRTk / (ARTk + 1.e-10));
+                ARTk = fmax(ARTk, 1.e-10);
+                if (AinRTO >= 0.0)
+                    ARTO = ARTO + AinRTO;
+                else
+                    ARTO = ARTO * exp(AinRTO / (ARTO + 1.e-10));
+                ARTO = fmax(ARTO, 1.e-10);
+                double pRo = V(1, I, J, K);
+                VV = sqr(V(2, I, J, K)) + sqr(V(3, I, J, K)) + sqr(V(4, I, J, K));
+                double PPP = 0.4 * V(5, I, J, K) - 0.2 * VV / pRo;
+                double TT = p->rxm2 * PPP / pRo;
+                double RmiuL = (1.0 + p->csthlnd) / (TT + p->csthlnd) * pow(TT, 1.5);
+                double DistN = DST(I, J, K);
+                double dudx = DQ(1, I, J, K), dudy = DQ(2, I, J, K), dudz = DQ(3, I, J, K);
+                double dvdx = DQ(4, I, J, K), dvdy = DQ(5, I, J, K), dvdz = DQ(6, I, J, K);
+                double dwdx = DQ(7, I, J, K), dwdy = DQ(8, I, J, K), dwdz = DQ(9, I, J, K);
+                double Div = dudx + dvdy + dwdz;
+                double prod = (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sqr(dudy + dvdx) +
+                               sqr(dvdz + dwdy) + sqr(dwdx + dudz) - 2.0 / 3.0 * Div * Div);
+                if (p->kind_model == 1) {
+                    double arg2 = 2.0 * sqrt(ARTk / pRo) / p->beta_star / (ARTO / pRo) / DistN / p->ref;
+                    arg2 = fmax(arg2, 500.0 * RmiuL / DistN / DistN / ARTO / p->ref / p->ref);
+                    arg2 = tanh(arg2 * arg2);
+                    double Sii = dudx, Sjj = dvdy, Skk = dwdz;
+                    double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
+                    double Skl = sqrt(2.0 * (1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk +
+                                             2.0 * Sij * Sij + 2.0 * Sik * Sik + 2.0 * Sjk * Sjk));
+                    double Cmua = fmin(1.0, p->a1 * ARTO / pRo / ((Skl)*arg2 + 1.e-10) * p->ref);
+                    ARTO = fmax(ARTO, pow(pRo * Cmua * fabs(prod) / (p->prod_lmt * p->beta_star), 0.5) / p->ref);
+                    ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo / Cmua);
+                    V(6, I, J, K) = ARTk;
+                    V(7, I, J, K) = ARTO;
+                    RMIU(I, J, K) = Cmua * ARTk / ARTO * pRo;
+                }
+            }
+    double DRSk = 0.0, DUk = 0.0, DRSo = 0.0, DUo = 0.0;
+    int Imk = 0, Jmk = 0, Kmk = 0, Imo = 0, Jmo = 0, Kmo = 0;
+    for (int K = 1; K <= NK1; K++)
+        for (int J = 1; J <= NJ1; J++)
+            for (int I = 1; I <= NI1; I++) {
+                double DRk = fabs(D(6, I, J, K) / V(1, I, J, K));
+                if (DUk <= DRk) { DUk = DRk; Imk = I; Jmk = J; Kmk = K; }
+                DRSk = DRSk + DRk * DRk;
+            }
+    DRSk = sqrt(DRSk / NI1 / NJ1 / NK1);
+    for (int K = 1; K <= NK1; K++)
+        for (int J = 1; J <= NJ1; J++)
+            for (int I = 1; I <= NI1; I++) {
+                double DRo = fabs(D(7, I, J, K) / V(1, I, J, K));
+                if (DUo <= DRo) { DUo = DRo; Imo = I; Jmo = J; Kmo = K; }
+                DRSo = DRSo + DRo * DRo;
+            }
+    DRSo = sqrt(DRSo / NI1 / NJ1 / NK1);
+    b->res.drsk = DRSk; b->res.duk = DUk; b->res.imk = Imk; b->res.jmk = Jmk; b->res.kmk = Kmk;
+    b->res.drso = DRSo; b->res.duo = DUo; b->res.imo = Imo; b->res.jmo = Jmo; b->res.kmo = Kmo;
+}
+
+/* main.F90:158-170 */
+void ora_rk_update(const trans_params* p, int ki, int itr_chld, ora_block* blocks, int nblk)
+{
+    const int if_turb = ora_if_turb(p, ki);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int ib = 0; ib < nblk; ib++) {
+        ora_rk_save(itr_chld, &blocks[ib]);
+        ora_rk(itr_chld, &blocks[ib]);
+        if (if_turb) ora_turb_lusgs_sst(p, &blocks[ib]);
+        ora_rk_update_v(p, ki, if_turb, &blocks[ib]);
+    }
+}
+
 /* Threads an OpenMP region of the oracle uses. */
 int ora_num_threads(void)
 {
@@ -1084,9 +1219,13 @@ void ora_iterate(const trans_params* p, int* ki, int n_iter, ora_block* blocks, 
 {
     for (int it = 0; it < n_iter; it++) {
         int nsub = p->kind_dual != 0 ? p->kchld : 1;
+        if (p->if_rk == 1) nsub = 3;
         for (int s = 0; s < nsub; s++) {
             ora_residual(p, *ki, blocks, nblk);
-            ora_lusgs_update(p, *ki, blocks, nblk);
+            if (p->if_rk == 1)
+                ora_rk_update(p, *ki, s + 1, blocks, nblk);
+            else
+                ora_lusgs_update(p, *ki, blocks, nblk);
         }
         *ki += 1;
         if (p->kind_dual != 0)

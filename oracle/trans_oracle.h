@@ -70,6 +70,11 @@ void ora_ausmpw_common(const trans_params* p, double PLo, double PRo, double TLo
 /* Composite steps over all blocks, in main.F90 order. */
 void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk);
 void ora_lusgs_update(const trans_params* p, int ki, ora_block* blocks, int nblk);
+/* Third-order Runge-Kutta stage itr_chld (1..3) with the k-omega LU-SGS (IF_RK = 1). */
+void ora_rk_save(int itr_chld, ora_block* b);
+void ora_rk(int itr_chld, ora_block* b);
+void ora_rk_update_v(const trans_params* p, int ki, int if_turb, ora_block* b);
+void ora_rk_update(const trans_params* p, int ki, int itr_chld, ora_block* blocks, int nblk);
 /* Threads an OpenMP region of the oracle uses (blocks run in parallel). */
 int ora_num_threads(void);
 void ora_set_threads(int n);

@@ -643,3 +643,153 @@ def test_if_turb_switch():
     assert lib().ora_if_turb(C.byref(prm), 5) == 1
     prm = _prm(IF_Euler=1, IF_Turb_Org=1, KItm=1)
     assert lib().ora_if_turb(C.byref(prm), 5) == 0
+
+
+# ---------------------------------------------------------------------------------------
+# Runge-Kutta update (Time/TIME_RK_Save.F90, Time/TIME_RK.F90, Time/TIME_RK_V_Update_New.F90)
+
+def _rk_stages(oc, z_of_v, h):
+    """Three stages with the residual set each stage to D = lambda * V * Vol (Dtm = h), i.e.
+    the linear ODE dV/dt = lambda V."""
+    b = oc.blocks[0]
+    A = oc[0]
+    A["Dtm"][...] = h
+    for stage in (1, 2, 3):
+        A["D"][...] = z_of_v(A["v"]) * b["vol"][None]
+        lib().ora_rk_save(stage, C.byref(oc.ob[0]))
+        lib().ora_rk(stage, C.byref(oc.ob[0]))
+
+
+@pytest.mark.parametrize("z", [-0.7, -2.1, 0.3])
+def test_rk3_linear_ode_amplification(z):
+    """On dV/dt = lambda V the three stages give V0 (1 + z + z^2/2 + z^3/6), z = lambda h --
+    the stability polynomial of the third-order TVD Runge-Kutta scheme, which fixes all six
+    stage weights.  k and omega are left to the k-omega LU-SGS; Wt2 keeps the initial state."""
+    b, prm, edges = _box()
+    oc = OracleCase([b], prm)
+    rng = np.random.Generator(np.random.MT19937(11))
+    V = oc[0]["v"]
+    V[...] = 1.0 + 0.5 * rng.uniform(0, 1, V.shape)
+    v0 = V.copy()
+    h = 0.37
+    _rk_stages(oc, lambda v: (z / h) * v, h)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    amp = 1 + z + z * z / 2 + z ** 3 / 6
+    np.testing.assert_allclose(V[:5, I, J, K], amp * v0[:5, I, J, K], rtol=2e-15 * max(1, abs(z)) * 8)
+    assert np.all(V[5:, I, J, K] == v0[5:, I, J, K])
+    assert np.all(oc[0]["wt2"][:, I, J, K] == v0[:, I, J, K])
+
+
+def test_rk_constant_residual_single_step():
+    """A residual that does not change between stages gives V0 + Dtm D / Vol after three
+    stages (the stage weights of each stage sum to one)."""
+    b, prm, edges = _box()
+    oc = OracleCase([b], prm)
+    rng = np.random.Generator(np.random.MT19937(12))
+    V = oc[0]["v"]
+    V[...] = 1.0 + rng.uniform(0, 1, V.shape)
+    v0 = V.copy()
+    D = oc[0]["D"]
+    D[...] = rng.uniform(-1, 1, D.shape)
+    d = D.copy()
+    oc[0]["Dtm"][...] = 0.2
+    for stage in (1, 2, 3):
+        lib().ora_rk_save(stage, C.byref(oc.ob[0]))
+        lib().ora_rk(stage, C.byref(oc.ob[0]))
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    want = v0[:5, I, J, K] + 0.2 * d[:5, I, J, K] / b["vol"][None, I, J, K]
+    np.testing.assert_allclose(V[:5, I, J, K], want, rtol=1e-14)
+
+
+def test_rk_density_norm_before_stage_last_max_wins():
+    """DRSa = rms of |D(1)/rho| over interior cells with rho taken before the stage; DUa the
+    maximum, located at the last cell (K, J, I loop order) among equal maxima."""
+    b, prm, edges = _box()
+    oc = OracleCase([b], prm)
+    rng = np.random.Generator(np.random.MT19937(13))
+    V, D = oc[0]["v"], oc[0]["D"]
+    V[...] = 1.0 + rng.uniform(0, 1, V.shape)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    D[...] = 0.0
+    D[0][I, J, K] = rng.uniform(-1, 1, D[0][I, J, K].shape)
+    r0 = V[0].copy()
+    big = 5.0
+    for (i, j, k) in ((2, 3, 1), (6, 1, 3), (3, 5, 2)):  # equal maxima; (6,1,3) is last
+        c = (i + 2, j + 2, k + 2)  # arrays start at reference index -2
+        D[0][c] = -big * r0[c]
+    oc[0]["Dtm"][...] = 0.05
+    lib().ora_rk(1, C.byref(oc.ob[0]))
+    rel = np.abs(D[0][I, J, K] / r0[I, J, K])
+    res = oc.ob[0].res
+    assert res.drsa == pytest.approx(np.sqrt(np.mean(rel ** 2)), rel=1e-13)
+    assert res.dua == big
+    assert (res.ima, res.jma, res.kma) == (6, 1, 3)
+
+
+def test_rk_update_v_thermo_and_kw_limiter():
+    """TIME_RK_V_Update_New: p = 0.4 (rho E - |rho u|^2 / (2 rho)), T = p / (rho R); k and
+    omega take the positivity-damped increments; at zero strain (Cmua = 1, no omega floor)
+    the k limiter min(k, 1e5 mu_L omega / rho) uses the k-omega Sutherland form
+    (1 + C)/(T + C) T^1.5, which differs from the mean-flow form by the Csth1 scaling of T,
+    so that mu_t = 1e5 mu_L where it is active."""
+    b, prm, edges = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1)
+    assert abs(prm.csth1 - 1.0) > 0.01
+    oc = OracleCase([b], prm)
+    rng = np.random.Generator(np.random.MT19937(14))
+    V, D = oc[0]["v"], oc[0]["D"]
+    shp = V[0].shape
+    V[0] = 1.0 + 0.2 * rng.uniform(0, 1, shp)
+    V[1:4] = 0.3 * rng.uniform(-1, 1, (3,) + shp)
+    V[4] = 1 / (1.4 * 0.4 * 0.25) + 0.5 * (V[1] ** 2 + V[2] ** 2 + V[3] ** 2) / V[0]
+    V[6] = 10.0 + rng.uniform(0, 1, shp)
+    mu_ref = (1 + prm.csthlnd) / (1 + prm.csthlnd)  # order of mu_L near T = 1
+    V[5] = np.where(rng.uniform(0, 1, shp) < 0.5, 1e-6, 1e6 * mu_ref) * V[6] / V[0]
+    D[...] = 0.0
+    D[5] = 1e-7 * rng.uniform(-1, 1, shp)
+    D[6] = 0.5 * rng.uniform(-1, 1, shp)
+    v0, d = V.copy(), D.copy()
+    oc[0]["DQdxyz"][...] = 0.0
+    oc.call("ora_rk_update_v", 0, 1, 1)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    rho = v0[0]
+    vv = v0[1] ** 2 + v0[2] ** 2 + v0[3] ** 2
+    p = 0.4 * (v0[4] - 0.5 * vv / rho)
+    np.testing.assert_allclose(oc[0]["pp"][I, J, K], p[I, J, K], rtol=1e-14)
+    np.testing.assert_allclose(oc[0]["t"][I, J, K], (prm.rxm2 * p / rho)[I, J, K], rtol=1e-14)
+
+    def upd(x, dx):
+        return np.maximum(np.where(dx >= 0, x + dx, x * np.exp(dx / (x + 1e-10))), 1e-10)
+    k1, w1 = upd(v0[5], d[5]), upd(v0[6], d[6])
+    tt = prm.rxm2 * p / rho
+    mul = (1 + prm.csthlnd) / (tt + prm.csthlnd) * tt ** 1.5
+    kl = np.minimum(k1, 1e5 * mul * w1 / rho)
+    assert np.any(kl < k1) and np.any(kl == k1)
+    np.testing.assert_allclose(oc[0]["v"][6][I, J, K], w1[I, J, K], rtol=1e-15)
+    np.testing.assert_allclose(oc[0]["v"][5][I, J, K], kl[I, J, K], rtol=1e-13)
+    np.testing.assert_allclose(oc[0]["rmiu"][I, J, K], (kl / w1 * rho)[I, J, K], rtol=1e-13)
+    n = (b.ni - 1) * (b.nj - 1) * (b.nk - 1)
+    res = oc.ob[0].res
+    assert res.drsk == pytest.approx(np.sqrt(np.sum((d[5] / rho)[I, J, K] ** 2) / n), rel=1e-13)
+    assert res.drso == pytest.approx(np.sqrt(np.sum((d[6] / rho)[I, J, K] ** 2) / n), rel=1e-13)
+    assert res.duo == pytest.approx(np.max(np.abs(d[6] / rho)[I, J, K]), rel=1e-15)
+
+
+def test_rk_iterate_preserves_turbulent_freestream():
+    """IF_RK = 1 through ora_iterate (three residual + stage passes per iteration) on a
+    stretched box in uniform turbulent flow, away from the far-field boundaries in i: the
+    mean flow stays uniform, k and omega only decay (SST destruction at zero strain), and Wt2
+    holds the state of the iteration's start."""
+    b, prm, edges = _box(ni=41, nj=8, nk=5, IF_Euler=0, IF_Turb_Org=1, KItm=1, Kind_Spatial=3,
+                         Kind_Roe=3, IF_RK=1, CFL_Sub_start=0.5, CFL_Sub_end=0.5)
+    oc = OracleCase([b], prm)
+    v0 = oc[0]["v"].copy()
+    ki = oc.iterate(1, 1)
+    assert ki == 2
+    region = (slice(None), sl(16, 25), sl(1, b.nj - 1), sl(1, b.nk - 1))
+    v1, w2 = oc[0]["v"][region], oc[0]["wt2"][region]
+    assert np.all(w2 == v0[region])
+    scale = np.array([v0[0].max(), v0[1].max(), v0[1].max(), v0[1].max(), v0[4].max()])
+    dv = np.abs(v1[:5] - v0[region][:5]).max(axis=(1, 2, 3))
+    assert np.all(dv < 1e-12 * scale), dv / scale
+    for q in (5, 6):  # destruction only (local time steps make the decay differ per cell)
+        assert np.all(v1[q] < v0[region][q]) and np.all(v1[q] > 0.5 * v0[region][q])

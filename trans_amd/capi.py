@@ -96,7 +96,8 @@ EXPORTS = [
     "trans_hip_create", "trans_hip_destroy", "trans_hip_last_error", "trans_hip_set_params",
     "trans_hip_set_iteration", "trans_hip_set_geometry", "trans_hip_set_state",
     "trans_hip_get_state", "trans_hip_get_field", "trans_hip_spectral_radius", "trans_hip_bc",
-    "trans_hip_space", "trans_hip_residual", "trans_hip_lusgs_update", "trans_hip_update_u",
+    "trans_hip_space", "trans_hip_residual", "trans_hip_lusgs_update", "trans_hip_rk_update",
+    "trans_hip_update_u",
     "trans_hip_iterate", "trans_hip_sync", "trans_hip_profile", "trans_hip_profile_count",
     "trans_hip_profile_get",
 ]
@@ -147,6 +148,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_space.argtypes = [C.c_void_p]
     lib.trans_hip_residual.argtypes = [C.c_void_p]
     lib.trans_hip_lusgs_update.argtypes = [C.c_void_p, P(TransResid)]
+    lib.trans_hip_rk_update.argtypes = [C.c_void_p, C.c_int, P(TransResid)]
     lib.trans_hip_update_u.argtypes = [C.c_void_p]
     lib.trans_hip_iterate.argtypes = [C.c_void_p, C.c_int, P(TransResid)]
     lib.trans_hip_sync.argtypes = [C.c_void_p]

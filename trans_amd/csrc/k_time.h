@@ -88,9 +88,10 @@ __global__ void __launch_bounds__(256) k_face_means(Fields f, const BlkInfo* __r
         }
 }
 
-// Increments of the line-march sweep back to D in the natural layout (TIME_Update_V reads D).
+// Increments D(c0+1..c0+neq) of the line-march sweep back to D in the natural layout (the
+// state updates read D).
 template <typename R>
-__global__ void __launch_bounds__(256) k_unskew(int neq, Fields f, const R* __restrict__ Wd,
+__global__ void __launch_bounds__(256) k_unskew(int c0, int neq, Fields f, const R* __restrict__ Wd,
                                                 const BlkInfo* __restrict__ blks,
                                                 const Seg* __restrict__ seg)
 {
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(256) k_unskew(int neq, Fields f, const R* __re
     decode3(lin, NI1, NJ1, i, j, k);
     const long c = cidx(b, i, j, k);
     const long w = widx(b, f.march, i, j, k);
-    for (int L = 0; L < neq; L++) f.D[(long)L * f.tot + c] = (double)Wd[(long)L * f.wtot + w];
+    for (int L = c0; L < c0 + neq; L++) f.D[(long)L * f.tot + c] = (double)Wd[(long)L * f.wtot + w];
 }
 
 // Time/TIME_LUSGS.F90:696-768 (LUSGS_ADQ) -- flux-Jacobian times increment, gamma = 1.4.
@@ -196,7 +197,8 @@ __device__ __forceinline__ void turb_nb(const Fields& f, const R* __restrict__ D
 // a hyperplane depends only on the previous one, so this order performs exactly the
 // reference's lexicographic Gauss-Seidel operations (Time/TIME_LUSGS.F90:172-436 forward,
 // :442-675 backward; NEQ = 7 adds Time/TURBTWO_TIME_LUSGS_SST.F90:550-766).
-// R = double or float (FP32 implicit operator); Dq is D or its FP32 copy.
+// R = double or float (FP32 implicit operator); Dq is D or its FP32 copy.  NEQ = 2 runs the
+// k-omega sweeps alone (the Runge-Kutta path, main.F90:160-163).
 template <typename R, bool FORWARD, int NEQ>
 __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, R* Dq,
                                                       const BlkInfo* __restrict__ blks)
@@ -227,19 +229,21 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
                 const bool edge = FORWARD ? pos[d] == 1 : pos[d] == nmax[d];
                 if (edge) continue;
                 const long cn = FORWARD ? c - st3[d] : c + st3[d];
-                lusgs_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), dn[d]);
-                if (NEQ > 5) turb_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), tn[d]);
+                if (NEQ != 2) lusgs_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), dn[d]);
+                if (NEQ != 5) turb_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), tn[d]);
             }
-            const R f1 = R(f.F1[c]);
+            if (NEQ != 2) {
+                const R f1 = R(f.F1[c]);
 #pragma unroll
-            for (int L = 0; L < 5; L++) {
-                const long e = (long)L * tot + c;
-                if (FORWARD)
-                    Dq[e] = (Dq[e] * R(stept) + (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs)) / f1;
-                else
-                    Dq[e] = Dq[e] - (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs) / f1;
+                for (int L = 0; L < 5; L++) {
+                    const long e = (long)L * tot + c;
+                    if (FORWARD)
+                        Dq[e] = (Dq[e] * R(stept) + (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs)) / f1;
+                    else
+                        Dq[e] = Dq[e] - (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs) / f1;
+                }
             }
-            if (NEQ > 5) {
+            if (NEQ != 5) {
                 const R f1t = R(f.F1t[c]);
 #pragma unroll
                 for (int q = 0; q < 2; q++) {
@@ -270,7 +274,10 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
 // NEQ = 7 adds the k-omega sweeps of Time/TURBTWO_TIME_LUSGS_SST.F90:550-766 to the same
 // wavefront: their flux Jacobian is the normal velocity V.S of the neighbour (no grid
 // velocity term) and their diagonal is F1t.  They read only their own increments, so running
-// them in the mean-flow wavefront performs the reference's operations in its order.
+// them in the mean-flow wavefront performs the reference's operations in its order.  NEQ = 2
+// runs the k-omega sweeps alone (components 6 and 7; the Runge-Kutta path).
+__host__ __device__ constexpr int sweep_c0(int neq) { return neq == 2 ? 5 : 0; }
+
 template <int NEQ>
 struct LineRaw {
     double d[NEQ];
@@ -287,15 +294,16 @@ __device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__
 {
     const long wt = f.wtot;
     const double* W = f.W;
+    constexpr int C0 = sweep_c0(NEQ);
 #pragma unroll
-    for (int L = 0; L < NEQ; L++) x.d[L] = (double)Dq[(long)L * wt + w];
+    for (int L = 0; L < NEQ; L++) x.d[L] = (double)Dq[(long)(C0 + L) * wt + w];
     x.stept = W[W_STEPT * wt + w];
     x.cvs = W[W_CVS * wt + w];
-    x.f1 = W[W_F1 * wt + w];
-    x.f1t = NEQ > 5 ? W[W_F1T * wt + w] : 1.0;
+    x.f1 = NEQ != 2 ? W[W_F1 * wt + w] : 1.0;
+    x.f1t = NEQ != 5 ? W[W_F1T * wt + w] : 1.0;
 #pragma unroll
     for (int L = 0; L < 4; L++) x.v[L] = W[(long)(W_V + L) * wt + w];
-    x.pp = W[W_PP * wt + w];
+    x.pp = NEQ != 2 ? W[W_PP * wt + w] : 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma unroll
@@ -352,6 +360,7 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
     // nplanes-1-t (backward); padding lanes read the last line's slots
     const long wline = b.woff + (active ? ln : nlines - 1);
     R* my_sink = sink + (long)blockIdx.x * NEQ * MAXT + ln;
+    constexpr int C0 = sweep_c0(NEQ);
     R tm[NEQ];  // term from the previous cell of this line
 #pragma unroll
     for (int L = 0; L < NEQ; L++) tm[L] = R(0);
@@ -377,12 +386,12 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
             const R x1 = march == 0 ? sa : (march == 1 ? sm : sb);
             const R x2 = march == 2 ? sm : sb;
             const R sum = x0 + x1 + x2;
-            const R f1 = L < 5 ? R(cur.f1) : R(cur.f1t);
+            const R f1 = C0 + L < 5 ? R(cur.f1) : R(cur.f1t);
             if (FORWARD)
                 dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
             else
                 dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
-            R* dst = act ? Dq + (long)L * f.wtot + w : my_sink + (long)L * MAXT;
+            R* dst = act ? Dq + (long)(C0 + L) * f.wtot + w : my_sink + (long)L * MAXT;
             *dst = dq[L];
         }
         const double roinv = 1.0 / cur.v[0];
@@ -391,16 +400,18 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             R o[NEQ];
-            R dif[5];
             const R sx = R(cur.sdm[d][0]), sy = R(cur.sdm[d][1]), sz = R(cur.sdm[d][2]);
-            lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, R(cur.pp), R(roinv), dq, dif);
             const R rd = R(cur.rds[d]);
+            if constexpr (NEQ != 2) {
+                R dif[5];
+                lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, R(cur.pp), R(roinv), dq, dif);
 #pragma unroll
-            for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
-            if (NEQ > 5) {
+                for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
+            }
+            if constexpr (NEQ != 5) {
                 const R VP = pVx * sx + pVy * sy + pVz * sz;
 #pragma unroll
-                for (int L = 5; L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
+                for (int L = (NEQ == 2 ? 0 : 5); L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
             }
             const bool to_a = d == (march == 0 ? 1 : 0);
             if (d == march) {
@@ -460,13 +471,93 @@ __device__ __forceinline__ void norm_combine(double& s, double& m, int& a, doubl
     }
 }
 
-__global__ void __launch_bounds__(256) k_update_v(trans_params p, int ki, int if_turb, Fields f,
-                                                  const BlkInfo* __restrict__ blks,
-                                                  const Seg* __restrict__ seg, NormPart* part)
+// k/omega update with positivity damping and the SST limiter on eddy viscosity
+// (Time/TIME_Update_V.F90:709-817, Time/TIME_RK_V_Update_New.F90:104-200): increments from
+// D(6:7), new k, omega and Rmiu written back.  pRo is the updated density and RmiuL the
+// laminar viscosity, which the two callers evaluate in their own forms.
+__device__ __forceinline__ void sst_kw_update(const trans_params& p, const Fields& f, long c, double pRo,
+                                              double RmiuL)
+{
+    const long tot = f.tot;
+    const double AinRTk = f.D[5 * tot + c], AinRTO = f.D[6 * tot + c];
+    double ARTk = f.V[5 * tot + c], ARTO = f.V[6 * tot + c];
+    if (AinRTk >= 0.0)
+        ARTk = ARTk + AinRTk;
+    else
+        ARTk = ARTk * exp(AinRTk / (ARTk + 1.e-10));
+    ARTk = fmax(ARTk, 1.e-10);
+    if (AinRTO >= 0.0)
+        ARTO = ARTO + AinRTO;
+    else
+        ARTO = ARTO * exp(AinRTO / (ARTO + 1.e-10));
+    ARTO = fmax(ARTO, 1.e-10);
+    const double DistN = f.Dst[c];
+    const double* Q = f.DQ;
+    const double dudx = Q[c], dudy = Q[tot + c], dudz = Q[2 * tot + c];
+    const double dvdx = Q[3 * tot + c], dvdy = Q[4 * tot + c], dvdz = Q[5 * tot + c];
+    const double dwdx = Q[6 * tot + c], dwdy = Q[7 * tot + c], dwdz = Q[8 * tot + c];
+    const double Div = dudx + dvdy + dwdz;
+    const double prod = (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sqr(dudy + dvdx) +
+                         sqr(dvdz + dwdy) + sqr(dwdx + dudz) - 2.0 / 3.0 * Div * Div);
+    if (p.kind_model == 1) {
+        double arg2 = 2.0 * sqrt(ARTk / pRo) / p.beta_star / (ARTO / pRo) / DistN / p.ref;
+        arg2 = fmax(arg2, 500.0 * RmiuL / DistN / DistN / ARTO / p.ref / p.ref);
+        arg2 = tanh(arg2 * arg2);
+        const double Sii = dudx, Sjj = dvdy, Skk = dwdz;
+        const double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
+        const double Skl = sqrt(2.0 * (1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk +
+                                       2.0 * Sij * Sij + 2.0 * Sik * Sik + 2.0 * Sjk * Sjk));
+        const double Cmua = fmin(1.0, p.a1 * ARTO / pRo / ((Skl)*arg2 + 1.e-10) * p.ref);
+        ARTO = fmax(ARTO, pow(pRo * Cmua * fabs(prod) / (p.prod_lmt * p.beta_star), 0.5) / p.ref);
+        ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo / Cmua);
+        f.V[5 * tot + c] = ARTk;
+        f.V[6 * tot + c] = ARTO;
+        f.Rmiu[c] = Cmua * ARTk / ARTO * pRo;
+    }
+}
+
+// Workgroup reduction of sum(dr^2) and max(dr) (last index wins ties) of the norms q >= q0
+// into part[blockIdx.x]; the entries q < q0 are left as they are.  All threads call it.
+__device__ __forceinline__ void norm_reduce(const double dr[3], int arg, NormPart* part, int q0)
 {
     __shared__ double ssum[3][256];
     __shared__ double smax[3][256];
     __shared__ int sarg[3][256];
+    for (int q = 0; q < 3; q++) {
+        ssum[q][threadIdx.x] = dr[q] * dr[q];
+        smax[q][threadIdx.x] = arg >= 0 ? dr[q] : -1.0;
+        sarg[q][threadIdx.x] = arg;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            for (int q = 0; q < 3; q++) {
+                double sm = ssum[q][threadIdx.x], mx = smax[q][threadIdx.x];
+                int a = sarg[q][threadIdx.x];
+                norm_combine(sm, mx, a, ssum[q][threadIdx.x + w], smax[q][threadIdx.x + w],
+                             sarg[q][threadIdx.x + w]);
+                ssum[q][threadIdx.x] = sm;
+                smax[q][threadIdx.x] = mx;
+                sarg[q][threadIdx.x] = a;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        NormPart* o = part + blockIdx.x;
+        for (int q = q0; q < 3; q++) {
+            o->sum[q] = ssum[q][0];
+            o->mx[q] = smax[q][0];
+            o->arg[q] = sarg[q][0];
+        }
+        o->pad = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_update_v(trans_params p, int ki, int if_turb, Fields f,
+                                                  const BlkInfo* __restrict__ blks,
+                                                  const Seg* __restrict__ seg, NormPart* part)
+{
     const Seg s = seg[blockIdx.x];
     const BlkInfo b = blks[s.blk];
     const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
@@ -522,78 +613,13 @@ __global__ void __launch_bounds__(256) k_update_v(trans_params p, int ki, int if
         f.V[4 * tot + c] = v5;
 
         if (p.if_euler == 0 && if_turb == 1 && ki >= p.kitm) {
-            const double AinRTk = f.D[5 * tot + c], AinRTO = f.D[6 * tot + c];
-            double ARTk = f.V[5 * tot + c], ARTO = f.V[6 * tot + c];
-            if (AinRTk >= 0.0)
-                ARTk = ARTk + AinRTk;
-            else
-                ARTk = ARTk * exp(AinRTk / (ARTk + 1.e-10));
-            ARTk = fmax(ARTk, 1.e-10);
-            if (AinRTO >= 0.0)
-                ARTO = ARTO + AinRTO;
-            else
-                ARTO = ARTO * exp(AinRTO / (ARTO + 1.e-10));
-            ARTO = fmax(ARTO, 1.e-10);
             const double VV = sqr(pRo * pVx) + sqr(pRo * pVy) + sqr(pRo * pVz);
             const double PPP = (1.4 - 1.0) * v5 - 0.5 * (1.4 - 1.0) * (VV / pRo - sqr(p.omega1 * rad) * pRo);
             const double TT = p.rxm2 * PPP / pRo;
-            const double RmiuL = sutherland(p, TT);
-            const double DistN = f.Dst[c];
-            const double* Q = f.DQ;
-            const double dudx = Q[c], dudy = Q[tot + c], dudz = Q[2 * tot + c];
-            const double dvdx = Q[3 * tot + c], dvdy = Q[4 * tot + c], dvdz = Q[5 * tot + c];
-            const double dwdx = Q[6 * tot + c], dwdy = Q[7 * tot + c], dwdz = Q[8 * tot + c];
-            const double Div = dudx + dvdy + dwdz;
-            const double prod = (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sqr(dudy + dvdx) +
-                                 sqr(dvdz + dwdy) + sqr(dwdx + dudz) - 2.0 / 3.0 * Div * Div);
-            if (p.kind_model == 1) {
-                double arg2 = 2.0 * sqrt(ARTk / pRo) / p.beta_star / (ARTO / pRo) / DistN / p.ref;
-                arg2 = fmax(arg2, 500.0 * RmiuL / DistN / DistN / ARTO / p.ref / p.ref);
-                arg2 = tanh(arg2 * arg2);
-                const double Sii = dudx, Sjj = dvdy, Skk = dwdz;
-                const double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
-                const double Skl = sqrt(2.0 * (1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk +
-                                               2.0 * Sij * Sij + 2.0 * Sik * Sik + 2.0 * Sjk * Sjk));
-                const double Cmua = fmin(1.0, p.a1 * ARTO / pRo / ((Skl)*arg2 + 1.e-10) * p.ref);
-                ARTO = fmax(ARTO, pow(pRo * Cmua * fabs(prod) / (p.prod_lmt * p.beta_star), 0.5) / p.ref);
-                ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo / Cmua);
-                f.V[5 * tot + c] = ARTk;
-                f.V[6 * tot + c] = ARTO;
-                f.Rmiu[c] = Cmua * ARTk / ARTO * pRo;
-            }
+            sst_kw_update(p, f, c, pRo, sutherland(p, TT));
         }
     }
-    // workgroup reduction of sum(dr^2) and max(dr) with the last index winning ties
-    for (int q = 0; q < 3; q++) {
-        ssum[q][threadIdx.x] = dr[q] * dr[q];
-        smax[q][threadIdx.x] = arg >= 0 ? dr[q] : -1.0;
-        sarg[q][threadIdx.x] = arg;
-    }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            for (int q = 0; q < 3; q++) {
-                double sm = ssum[q][threadIdx.x], mx = smax[q][threadIdx.x];
-                int a = sarg[q][threadIdx.x];
-                norm_combine(sm, mx, a, ssum[q][threadIdx.x + w], smax[q][threadIdx.x + w],
-                             sarg[q][threadIdx.x + w]);
-                ssum[q][threadIdx.x] = sm;
-                smax[q][threadIdx.x] = mx;
-                sarg[q][threadIdx.x] = a;
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        NormPart o;
-        for (int q = 0; q < 3; q++) {
-            o.sum[q] = ssum[q][0];
-            o.mx[q] = smax[q][0];
-            o.arg[q] = sarg[q][0];
-        }
-        o.pad = 0;
-        part[blockIdx.x] = o;
-    }
+    norm_reduce(dr, arg, part, 0);
 }
 
 // Final per-block norms from the partials of the block's workgroups [w0, w1).

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -14,6 +15,7 @@
 #include "dev_common.h"
 #include "k_bc.h"
 #include "k_space.h"
+#include "k_rk.h"
 #include "k_time.h"
 #include "k_visc.h"
 
@@ -196,7 +198,7 @@ static int check_params(trans_ctx* c, const trans_params* p)
     if (p->if_precondition != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_PRECONDITION must be 0");
     if (p->kind_rotmod != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_RotMod must be 0");
     if (p->cfl_sub_step < 1) return fail(c, TRANS_EINVAL, "CFL_Sub_step must be >= 1");
-    if (p->if_rk != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_RK=1 is not supported by this build");
+    if (p->if_rk != 0 && p->if_rk != 1) return fail(c, TRANS_EINVAL, "IF_RK must be 0 or 1");
     if (p->kind_spatial != 2 && p->kind_spatial != 3)
         return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 2 (MUSCL) or 3 (MDCD)");
     if (p->kind_roe != 1 && p->kind_roe != 3)
@@ -314,11 +316,14 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
     if (!bs.empty()) HIPCHK(c, hipMemcpy(c->d_bcseg, bs.data(), bs.size() * sizeof(BcSeg), hipMemcpyHostToDevice));
 
     {
-        // march along j (lanes on i: coalesced) unless the (i,k) sections do not fit a workgroup
+        // march along j (lanes on i: coalesced) unless the (i,k) sections do not fit a workgroup;
+        // TRANS_HIP_LINE_SWEEP=0 in the environment selects the hyperplane sweep instead
+        const char* env = std::getenv("TRANS_HIP_LINE_SWEEP");
+        const bool lines_ok = !(env && env[0] == '0');
         const int order[3] = {1, 2, 0};
         const size_t esz = c->p.implicit_fp32 ? sizeof(float) : sizeof(double);
         const size_t neq = (c->p.if_euler == 0 && c->p.if_turb_org == 1) ? 7 : 5;
-        for (int q = 0; q < 3 && !c->line_threads; q++) {
+        for (int q = 0; q < 3 && lines_ok && !c->line_threads; q++) {
             const int m = order[q];
             int mx = 0;
             for (int b = 0; b < nblk; b++) {
@@ -627,10 +632,11 @@ static void launch_lines(trans_ctx* c, R* dq)
     const int nblk = (int)c->hb.size();
     const size_t lds = (size_t)4 * NEQ * (c->max_lines + 1) * sizeof(R);
     R* sink = reinterpret_cast<R*>(c->line_sink);
-    KLAUNCH_LDS(c, "k_lusgs_line<fwd>", (k_lusgs_line<R, true, NEQ, MAXT>), dim3(nblk), dim3(c->line_threads), lds,
-                c->p, c->f, dq, c->d_blk, c->march, sink);
-    KLAUNCH_LDS(c, "k_lusgs_line<bwd>", (k_lusgs_line<R, false, NEQ, MAXT>), dim3(nblk), dim3(c->line_threads), lds,
-                c->p, c->f, dq, c->d_blk, c->march, sink);
+    const bool kw = NEQ == 2;  // k-omega sweeps alone
+    KLAUNCH_LDS(c, kw ? "k_turb_line<fwd>" : "k_lusgs_line<fwd>", (k_lusgs_line<R, true, NEQ, MAXT>), dim3(nblk),
+                dim3(c->line_threads), lds, c->p, c->f, dq, c->d_blk, c->march, sink);
+    KLAUNCH_LDS(c, kw ? "k_turb_line<bwd>" : "k_lusgs_line<bwd>", (k_lusgs_line<R, false, NEQ, MAXT>), dim3(nblk),
+                dim3(c->line_threads), lds, c->p, c->f, dq, c->d_blk, c->march, sink);
 }
 
 template <typename R, int NEQ>
@@ -646,10 +652,11 @@ static void launch_sweeps(trans_ctx* c, R* dq)
         else
             launch_lines<R, NEQ, 1024>(c, dq);
     } else {
-        KLAUNCH(c, "k_lusgs_sweep<fwd>", (k_lusgs_sweep<R, true, NEQ>), dim3(nblk), dim3(1024), c->p, c->f, dq,
-                c->d_blk);
-        KLAUNCH(c, "k_lusgs_sweep<bwd>", (k_lusgs_sweep<R, false, NEQ>), dim3(nblk), dim3(1024), c->p, c->f, dq,
-                c->d_blk);
+        const bool kw = NEQ == 2;
+        KLAUNCH(c, kw ? "k_turb_sweep<fwd>" : "k_lusgs_sweep<fwd>", (k_lusgs_sweep<R, true, NEQ>), dim3(nblk),
+                dim3(1024), c->p, c->f, dq, c->d_blk);
+        KLAUNCH(c, kw ? "k_turb_sweep<bwd>" : "k_lusgs_sweep<bwd>", (k_lusgs_sweep<R, false, NEQ>), dim3(nblk),
+                dim3(1024), c->p, c->f, dq, c->d_blk);
     }
 }
 
@@ -671,14 +678,14 @@ static int lusgs_update_async(trans_ctx* c)
                 launch_sweeps<float, 7>(c, c->f.Wf);
             else
                 launch_sweeps<float, 5>(c, c->f.Wf);
-            KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), neq, c->f, (const float*)c->f.Wf,
-                    c->d_blk, c->seg_cell.d);
+            KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), 0, neq, c->f,
+                    (const float*)c->f.Wf, c->d_blk, c->seg_cell.d);
         } else {
             if (if_turb)
                 launch_sweeps<double, 7>(c, c->f.W);
             else
                 launch_sweeps<double, 5>(c, c->f.W);
-            KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), neq, c->f,
+            KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), 0, neq, c->f,
                     (const double*)c->f.W, c->d_blk, c->seg_cell.d);
         }
     } else if (c->p.implicit_fp32) {
@@ -702,20 +709,68 @@ static int lusgs_update_async(trans_ctx* c)
     return launch_check(c, "lusgs_update");
 }
 
+// Runge-Kutta stage itr_chld (main.F90:158-170): TIME_RK_Save and TIME_RK, then with k-omega
+// TURBTWO_TIME_LUSGS_SST on the stage's state and TIME_RK_V_Update_New.
+static int rk_update_async(trans_ctx* c, int stage)
+{
+    const int nblk = (int)c->hb.size();
+    const int if_turb = if_turb_now(c);
+    KLAUNCH(c, "k_rk_stage", k_rk_stage, dim3(c->seg_cell.n), dim3(WG), c->p, stage, if_turb ? 0 : 1, c->f,
+            c->d_blk, c->seg_cell.d, c->d_part);
+    if (if_turb) {
+        KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+        if (c->f.W) {
+            if (c->p.implicit_fp32) {
+                launch_sweeps<float, 2>(c, c->f.Wf);
+                KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), 5, 2, c->f,
+                        (const float*)c->f.Wf, c->d_blk, c->seg_cell.d);
+            } else {
+                launch_sweeps<double, 2>(c, c->f.W);
+                KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), 5, 2, c->f,
+                        (const double*)c->f.W, c->d_blk, c->seg_cell.d);
+            }
+        } else if (c->p.implicit_fp32) {
+            KLAUNCH(c, "k_convert<double,float>", (k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), 2, 5,
+                    c->tot, (const double*)c->f.D, c->f.Ds, c->d_blk, c->seg_cell.d);
+            launch_sweeps<float, 2>(c, c->f.Ds);
+            KLAUNCH(c, "k_convert<float,double>", (k_convert<float, double>), dim3(c->seg_cell.n), dim3(WG), 2, 5,
+                    c->tot, (const float*)c->f.Ds, c->f.D, c->d_blk, c->seg_cell.d);
+        } else {
+            launch_sweeps<double, 2>(c, c->f.D);
+        }
+        KLAUNCH(c, "k_rk_update_v", k_rk_update_v, dim3(c->seg_cell.n), dim3(WG), c->p, c->ki, if_turb, c->f,
+                c->d_blk, c->seg_cell.d, c->d_part);
+    }
+    KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk, c->d_resid);
+    return launch_check(c, "rk_update");
+}
+
+static int fetch_resid(trans_ctx* c, trans_resid* per_block)
+{
+    HIPCHK(c, hipMemcpyAsync(per_block, c->d_resid, c->hb.size() * sizeof(trans_resid), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t b = 0; b < c->hb.size(); b++)
+        if (per_block[b].drsa != per_block[b].drsa)
+            return fail(c, TRANS_ENONFINITE, "residual of block " + std::to_string(b + 1) + " is NaN");
+    return 0;
+}
+
+int trans_hip_rk_update(trans_ctx* c, int itr_chld, trans_resid* per_block)
+{
+    if (!c) return TRANS_EINVAL;
+    if (itr_chld < 1 || itr_chld > 3) return fail(c, TRANS_EINVAL, "Runge-Kutta stage must be 1, 2 or 3");
+    int rc = rk_update_async(c, itr_chld);
+    if (rc) return rc;
+    return per_block ? fetch_resid(c, per_block) : 0;
+}
+
 int trans_hip_lusgs_update(trans_ctx* c, trans_resid* per_block)
 {
     if (!c) return TRANS_EINVAL;
     int rc = lusgs_update_async(c);
     if (rc) return rc;
-    if (per_block) {
-        HIPCHK(c, hipMemcpyAsync(per_block, c->d_resid, c->hb.size() * sizeof(trans_resid),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (size_t b = 0; b < c->hb.size(); b++)
-            if (per_block[b].drsa != per_block[b].drsa)
-                return fail(c, TRANS_ENONFINITE, "residual of block " + std::to_string(b + 1) + " is NaN");
-    }
-    return 0;
+    return per_block ? fetch_resid(c, per_block) : 0;
 }
 
 int trans_hip_update_u(trans_ctx* c)
@@ -731,24 +786,16 @@ int trans_hip_iterate(trans_ctx* c, int n_iter, trans_resid* per_block)
     if (!c || n_iter < 0) return TRANS_EINVAL;
     int rc;
     for (int it = 0; it < n_iter; it++) {
-        const int nsub = c->p.kind_dual != 0 ? c->p.kchld : 1;
+        const int nsub = c->p.if_rk == 1 ? 3 : (c->p.kind_dual != 0 ? c->p.kchld : 1);
         for (int s = 0; s < nsub; s++) {
             if ((rc = trans_hip_residual(c))) return rc;
-            if ((rc = lusgs_update_async(c))) return rc;
+            if ((rc = c->p.if_rk == 1 ? rk_update_async(c, s + 1) : lusgs_update_async(c))) return rc;
         }
         c->ki += 1;
         if (c->p.kind_dual != 0)
             if ((rc = trans_hip_update_u(c))) return rc;
     }
-    if (per_block) {
-        HIPCHK(c, hipMemcpyAsync(per_block, c->d_resid, c->hb.size() * sizeof(trans_resid),
-                                 hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (size_t b = 0; b < c->hb.size(); b++)
-            if (per_block[b].drsa != per_block[b].drsa)
-                return fail(c, TRANS_ENONFINITE, "residual of block " + std::to_string(b + 1) + " is NaN");
-    }
-    return 0;
+    return per_block ? fetch_resid(c, per_block) : 0;
 }
 
 int trans_hip_profile(trans_ctx* c, int on)

@@ -142,6 +142,12 @@ module trans_hip_mod
             type(c_ptr), value :: ctx
             type(trans_resid), intent(out) :: per_block(*)
         end function
+        integer(c_int) function trans_hip_rk_update(ctx, itr_chld, per_block) bind(C, name="trans_hip_rk_update")
+            import :: c_int, c_ptr, trans_resid
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: itr_chld
+            type(trans_resid), intent(out) :: per_block(*)
+        end function
         integer(c_int) function trans_hip_update_u(ctx) bind(C, name="trans_hip_update_u")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
@@ -160,7 +166,8 @@ module trans_hip_mod
     public :: trans_hip_create, trans_hip_destroy, trans_hip_last_error, trans_hip_set_params, &
         trans_hip_set_iteration, trans_hip_set_geometry, trans_hip_set_state, trans_hip_get_state, &
         trans_hip_get_field, trans_hip_spectral_radius, trans_hip_bc, trans_hip_space, &
-        trans_hip_residual, trans_hip_lusgs_update, trans_hip_update_u, trans_hip_iterate, trans_hip_sync
+        trans_hip_residual, trans_hip_lusgs_update, trans_hip_rk_update, trans_hip_update_u, trans_hip_iterate, &
+        trans_hip_sync
 
 contains
 

@@ -77,6 +77,11 @@ class Solver:
         self._check(self.lib.trans_hip_lusgs_update(self.ctx, res))
         return [r.as_dict() for r in res]
 
+    def rk_update(self, itr_chld):
+        res = (capi.TransResid * len(self.blocks))()
+        self._check(self.lib.trans_hip_rk_update(self.ctx, itr_chld, res))
+        return [r.as_dict() for r in res]
+
     def update_u(self):
         self._check(self.lib.trans_hip_update_u(self.ctx))
 
@@ -106,6 +111,16 @@ class Solver:
         self._check(self.lib.trans_hip_sync(self.ctx))
 
     # ---- data out
+    def set_state(self, blk, arrays):
+        """Upload v, pp, t, rmiu, wt0, wt1 and wt2 of block blk from `arrays` (reference
+        layout, bounds -2:N+2, as get_state returns them)."""
+        keep = {}
+        sv = capi.TransStateView()
+        for nm in ("v", "pp", "t", "rmiu", "wt0", "wt1", "wt2"):
+            keep[nm] = np.asfortranarray(arrays[nm], dtype=np.float64)
+            setattr(sv, nm, capi.fview(keep[nm]))
+        self._check(self.lib.trans_hip_set_state(self.ctx, blk, C.byref(sv)))
+
     def get_state(self, blk):
         b = self.blocks[blk]
         out = {}
