@@ -63,7 +63,8 @@ struct Fields {
     // stride wtot, components W_* below.  Null when the hyperplane fallback sweep is used.
     double* W;
     float* Wf;        // FP32 increments of the FP32 implicit operator (7 components)
-    long wtot;
+    long wtot;        // component stride: the slots of all blocks, then the sinks
+    long wsink;       // first sink slot (idle lanes of the line sweep store there)
     int march;        // march axis of the line sweep (0 i, 1 j, 2 k)
 };
 

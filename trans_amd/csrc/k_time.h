@@ -260,55 +260,104 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
 }
 
 // ---------------------------------------------------------------------------------------
-// Line-march wavefront of the same sweeps.  The block is cut into lines along a march axis m
-// (j when the (i,k) cross-section fits one workgroup, so that consecutive lanes hold
-// consecutive i and every load is coalesced).  At step t the thread of line (a,b) updates the
-// cell of its line on hyperplane s = (i-1)+(j-1)+(k-1).  After updating its increment a cell
-// evaluates the three terms its downstream neighbours need, 0.5*(A_d dQ +/- Rds_d dQ) (exactly
-// the expressions of Time/TIME_LUSGS.F90:190-256 the neighbour would evaluate), keeps the
-// march-axis term in registers for its own next cell and hands the two others to the
-// neighbouring lines through ping-pong LDS.  Incoming terms are summed in i, j, k order as
-// the reference does.  One LDS-only barrier per step; the next cell's inputs are loaded
-// while the current one is computed.
+// Line-march wavefront of the same sweeps.  The block is cut into lines along a march axis
+// MARCH (j when the (i,k) cross-section fits one workgroup).  At step t the thread of line
+// (a,b) updates the cell of its line on hyperplane s = (i-1)+(j-1)+(k-1).  After updating its
+// increment a cell evaluates the three terms its downstream neighbours need,
+// 0.5*(A_d dQ +/- Rds_d dQ) (exactly the expressions of Time/TIME_LUSGS.F90:190-256 the
+// neighbour would evaluate), keeps the march-axis term in registers for its own next cell and
+// hands the two others to the neighbouring lines through ping-pong LDS.  Incoming terms are
+// summed in i, j, k order as the reference does.  One LDS-only barrier per step; the next
+// cell's inputs are loaded while the current one is computed.
 //
-// NEQ = 7 adds the k-omega sweeps of Time/TURBTWO_TIME_LUSGS_SST.F90:550-766 to the same
-// wavefront: their flux Jacobian is the normal velocity V.S of the neighbour (no grid
-// velocity term) and their diagonal is F1t.  They read only their own increments, so running
-// them in the mean-flow wavefront performs the reference's operations in its order.  NEQ = 2
-// runs the k-omega sweeps alone (components 6 and 7; the Runge-Kutta path).
+// The k-omega sweeps of Time/TURBTWO_TIME_LUSGS_SST.F90:550-766 (flux Jacobian: the normal
+// velocity V.S of the neighbour, no grid velocity term; diagonal F1t) read only their own
+// increments, so they run as their own wavefront: NEQ = 2 sweeps components 6 and 7 alone,
+// in a second row of workgroups beside the mean-flow ones (NEQ = 5) when both are swept.
 __host__ __device__ constexpr int sweep_c0(int neq) { return neq == 2 ? 5 : 0; }
 
 template <int NEQ>
 struct LineRaw {
     double d[NEQ];
-    double stept, cvs, f1, f1t;
+    double stept, cvs, f1;  // f1: F1 (mean flow) or F1t (k-omega)
     double v[4], pp;
     double sdm[3][3];  // face mean of SD in each direction
     double rds[3];
 };
 
+// A wave-uniform pointer made opaque to the compiler (readfirstlane), so that a load from it
+// at a 32-bit byte offset compiles to the scalar-base + vector-offset form instead of a 64-bit
+// address per lane and component.
+template <typename T>
+__device__ __forceinline__ T* sbase(T* p)
+{
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
+}
+
+// Element w of a component array in global memory; w * sizeof(T) fits 32 bits (checked at
+// creation).
+#define TRANS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ T ldg(const T* base, unsigned w)
+{
+    const TRANS_GLOBAL char* b = (const TRANS_GLOBAL char*)base;
+    return *(const TRANS_GLOBAL T*)(b + w * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void stg(T* base, unsigned w, T v)
+{
+    TRANS_GLOBAL char* b = (TRANS_GLOBAL char*)base;
+    *(TRANS_GLOBAL T*)(b + w * (unsigned)sizeof(T)) = v;
+}
+
+// Scalar bases of the sweep's components in the hyperplane-major buffers.
+template <typename R, int NEQ>
+struct LineSrc {
+    R* d[NEQ];
+    const double *stept, *cvs, *f1, *v[4], *pp, *sdm[3][3], *rds[3];
+    __device__ __forceinline__ LineSrc(const Fields& f, R* Dq)
+    {
+        constexpr int C0 = sweep_c0(NEQ);
+        const size_t wt = (size_t)f.wtot;
+        const double* W = f.W;
+#pragma unroll
+        for (int L = 0; L < NEQ; L++) d[L] = sbase(Dq + (C0 + L) * wt);
+        stept = sbase(W + W_STEPT * wt);
+        cvs = sbase(W + W_CVS * wt);
+        f1 = sbase(W + (C0 == 0 ? W_F1 : W_F1T) * wt);
+#pragma unroll
+        for (int L = 0; L < 4; L++) v[L] = sbase(W + (W_V + L) * wt);
+        pp = sbase(W + W_PP * wt);
+#pragma unroll
+        for (int dd = 0; dd < 3; dd++) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) sdm[dd][q] = sbase(W + (W_SDM + dd + 3 * q) * wt);
+            rds[dd] = sbase(W + (W_RDS + dd) * wt);
+        }
+    }
+};
+
 // Inputs of one cell of the sweep from the hyperplane-major buffer (slot w).  For the FP32
 // operator R(rds) rounds the spectral radius exactly as the FP32 copy Rdsf does.
 template <typename R, int NEQ>
-__device__ __forceinline__ void line_load(const Fields& f, const R* __restrict__ Dq, long w, LineRaw<NEQ>& x)
+__device__ __forceinline__ void line_load(const LineSrc<R, NEQ>& src, unsigned w, LineRaw<NEQ>& x)
 {
-    const long wt = f.wtot;
-    const double* W = f.W;
-    constexpr int C0 = sweep_c0(NEQ);
 #pragma unroll
-    for (int L = 0; L < NEQ; L++) x.d[L] = (double)Dq[(long)(C0 + L) * wt + w];
-    x.stept = W[W_STEPT * wt + w];
-    x.cvs = W[W_CVS * wt + w];
-    x.f1 = NEQ != 2 ? W[W_F1 * wt + w] : 1.0;
-    x.f1t = NEQ != 5 ? W[W_F1T * wt + w] : 1.0;
+    for (int L = 0; L < NEQ; L++) x.d[L] = (double)ldg(src.d[L], w);
+    x.stept = ldg(src.stept, w);
+    x.cvs = ldg(src.cvs, w);
+    x.f1 = ldg(src.f1, w);
 #pragma unroll
-    for (int L = 0; L < 4; L++) x.v[L] = W[(long)(W_V + L) * wt + w];
-    x.pp = NEQ != 2 ? W[W_PP * wt + w] : 0.0;
+    for (int L = 0; L < 4; L++) x.v[L] = ldg(src.v[L], w);
+    x.pp = sweep_c0(NEQ) == 0 ? ldg(src.pp, w) : 0.0;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) x.sdm[d][q] = W[(long)(W_SDM + d + 3 * q) * wt + w];
-        x.rds[d] = W[(long)(W_RDS + d) * wt + w];
+        for (int q = 0; q < 3; q++) x.sdm[d][q] = ldg(src.sdm[d][q], w);
+        x.rds[d] = ldg(src.rds[d], w);
     }
 }
 
@@ -319,48 +368,47 @@ __device__ __forceinline__ void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// march: 0 = i, 1 = j, 2 = k.  Lines are indexed a-fastest over the two other axes (a < b).
-// Dynamic LDS: 2 parities x 2 directions x NEQ x (nlines + 1) values of R; slot nlines of each
-// row is a sink for the lanes that have nothing to hand on.  Dq is W (FP64 operator) or Wf.
+// One block's wavefront for NEQ = 5 (mean flow) or 2 (k-omega).  Lines are indexed
+// a-fastest over the two axes other than MARCH (a < b).  LDS `buf`: 2 parities x 2 directions
+// x NEQ rows of nlines + 2 values; slot nlines of a row is a sink for lanes with nothing to
+// hand on, slot nlines + 1 stays zero for lanes with no upstream line.  `sink` is this
+// workgroup's slot range past the data of each component (Fields::wsink).
 //
 // The step loop is free of wave-level branches: lanes outside their line's window compute on
-// whatever their slot holds and send their results to sinks (`sink` in global memory, slot
-// nlines in LDS).  Inputs and increments live in the hyperplane-major buffer (widx), so the
-// lanes of a wave read and write consecutive addresses at every step.  Every step then issues the same memory operations, which keeps the compiler's
-// memory-counter waits precise across the loop (the next cell's loads stay in flight while
-// the current cell is computed).
-template <typename R, bool FORWARD, int NEQ, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R* Dq,
-                                                     const BlkInfo* __restrict__ blks, int march,
-                                                     R* __restrict__ sink)
+// whatever their slot holds and send their results to the sinks.  Every step issues the same
+// memory operations, which keeps the compiler's memory-counter waits precise across the loop
+// (the next cell's loads stay in flight while the current cell is computed).
+template <typename R, bool FORWARD, int NEQ, int MARCH>
+__device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, const BlkInfo& b, unsigned sink,
+                                           R* __restrict__ buf)
 {
-    extern __shared__ unsigned char lds_raw[];
-    const BlkInfo b = blks[blockIdx.x];
+    static_assert(NEQ == 5 || NEQ == 2, "mean flow or k-omega");
+    constexpr bool MF = NEQ == 5;
     const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
-    // axes: march m, the two others a < b; all selects, so that nothing is indexed at run time
-    const int Nm = march == 0 ? NI1 : (march == 1 ? NJ1 : NK1);
-    const int Na = march == 0 ? NJ1 : NI1;
-    const int Nb = march == 2 ? NJ1 : NK1;
+    const int Nm = MARCH == 0 ? NI1 : (MARCH == 1 ? NJ1 : NK1);
+    const int Na = MARCH == 0 ? NJ1 : NI1;
+    const int Nb = MARCH == 2 ? NJ1 : NK1;
     const int nlines = Na * Nb;
-    const int row = nlines + 1;
+    const int row = nlines + 2;
+    const int zero = nlines + 1;
     const int nplanes = NI1 + NJ1 + NK1 - 2;
-    R* buf = reinterpret_cast<R*>(lds_raw);  // [2 parity][2 dir][NEQ][nlines + 1]
     const int ln = threadIdx.x;
     const bool active = ln < nlines;
     const int ia = active ? ln % Na + 1 : 1, ib = active ? ln / Na + 1 : 1;
     const R sgn = FORWARD ? R(1) : R(-1);
-    // LDS slots this lane writes (the sink when there is no downstream line)
+    // LDS slots this lane writes (the sink when there is no downstream line) and reads
     const int wa = active && (FORWARD ? ia < Na : ia > 1) ? (FORWARD ? ln + 1 : ln - 1) : nlines;
     const int wb = active && (FORWARD ? ib < Nb : ib > 1) ? (FORWARD ? ln + Na : ln - Na) : nlines;
-    const int rd_slot = active ? ln : nlines;  // padding lanes read the sink slot
-    const bool in_a = FORWARD ? (ia > 1) : (ia < Na);
-    const bool in_b = FORWARD ? (ib > 1) : (ib < Nb);
+    const int ra = !active ? nlines : ((FORWARD ? ia > 1 : ia < Na) ? ln : zero);
+    const int rb = !active ? nlines : ((FORWARD ? ib > 1 : ib < Nb) ? ln : zero);
     const int start = FORWARD ? (ia - 1) + (ib - 1) : (Na - ia) + (Nb - ib);
-    // hyperplane-major slots: the lane's cell at step t is on hyperplane t (forward) or
-    // nplanes-1-t (backward); padding lanes read the last line's slots
-    const long wline = b.woff + (active ? ln : nlines - 1);
-    R* my_sink = sink + (long)blockIdx.x * NEQ * MAXT + ln;
-    constexpr int C0 = sweep_c0(NEQ);
+    for (int q = ln; q < 4 * NEQ; q += blockDim.x) buf[q * row + zero] = R(0);
+    __syncthreads();
+    // hyperplane-major slot of the lane's line on hyperplane s is lw + s*nlines; padding
+    // lanes read the last line's slots
+    const unsigned lw = (unsigned)(b.woff + (active ? ln : nlines - 1));
+    const unsigned my_sink = sink + (unsigned)ln;
+    const LineSrc<R, NEQ> src(f, Dq);
     R tm[NEQ];  // term from the previous cell of this line
 #pragma unroll
     for (int L = 0; L < NEQ; L++) tm[L] = R(0);
@@ -370,29 +418,31 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
         const int par = t & 1;
         const int rel = t - start;
         const bool act = active && rel >= 0 && rel < Nm;
-        const long w = wline + (long)(FORWARD ? t : nplanes - 1 - t) * nlines;
-        line_load<R, NEQ>(f, Dq, wline + (long)(FORWARD ? min(t + 1, nplanes - 1) : max(nplanes - 2 - t, 0)) * nlines,
-                          nxt);
+        const int s = FORWARD ? t : nplanes - 1 - t;
+        const int sn = FORWARD ? min(s + 1, nplanes - 1) : max(s - 1, 0);
+        line_load<R, NEQ>(src, lw + (unsigned)sn * (unsigned)nlines, nxt);
+        const unsigned wst = act ? lw + (unsigned)s * (unsigned)nlines : my_sink;
         const double stept = cur.stept;
         const double cvs = cur.cvs;
         R dq[NEQ];
 #pragma unroll
         for (int L = 0; L < NEQ; L++) {
-            const R sa = in_a ? buf[((par * 2 + 0) * NEQ + L) * row + rd_slot] : R(0);
-            const R sb = in_b ? buf[((par * 2 + 1) * NEQ + L) * row + rd_slot] : R(0);
+            const R sa = buf[((par * 2 + 0) * NEQ + L) * row + ra];
+            const R sb = buf[((par * 2 + 1) * NEQ + L) * row + rb];
             const R sm = rel == 0 ? R(0) : tm[L];
             // incoming terms summed in i, j, k order
-            const R x0 = march == 0 ? sm : sa;
-            const R x1 = march == 0 ? sa : (march == 1 ? sm : sb);
-            const R x2 = march == 2 ? sm : sb;
-            const R sum = x0 + x1 + x2;
-            const R f1 = C0 + L < 5 ? R(cur.f1) : R(cur.f1t);
-            if (FORWARD)
-                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
+            R sum;
+            if constexpr (MARCH == 0)
+                sum = sm + sa + sb;
+            else if constexpr (MARCH == 1)
+                sum = sa + sm + sb;
             else
-                dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
-            R* dst = act ? Dq + (long)(C0 + L) * f.wtot + w : my_sink + (long)L * MAXT;
-            *dst = dq[L];
+                sum = sa + sb + sm;
+            if (FORWARD)
+                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / R(cur.f1);
+            else
+                dq[L] = R(cur.d[L]) - sum * R(cvs) / R(cur.f1);
+            stg(src.d[L], wst, dq[L]);
         }
         const double roinv = 1.0 / cur.v[0];
         const R pVx = R(cur.v[1] * roinv), pVy = R(cur.v[2] * roinv), pVz = R(cur.v[3] * roinv);
@@ -402,22 +452,21 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
             R o[NEQ];
             const R sx = R(cur.sdm[d][0]), sy = R(cur.sdm[d][1]), sz = R(cur.sdm[d][2]);
             const R rd = R(cur.rds[d]);
-            if constexpr (NEQ != 2) {
+            if constexpr (MF) {
                 R dif[5];
                 lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, R(cur.pp), R(roinv), dq, dif);
 #pragma unroll
                 for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
-            }
-            if constexpr (NEQ != 5) {
+            } else {
                 const R VP = pVx * sx + pVy * sy + pVz * sz;
 #pragma unroll
-                for (int L = (NEQ == 2 ? 0 : 5); L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
+                for (int L = 0; L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
             }
-            const bool to_a = d == (march == 0 ? 1 : 0);
-            if (d == march) {
+            if (d == MARCH) {
 #pragma unroll
                 for (int L = 0; L < NEQ; L++) tm[L] = o[L];
             } else {
+                const bool to_a = d == (MARCH == 0 ? 1 : 0);
                 const int slot = to_a ? wa : wb;
                 const int dir = to_a ? 0 : 1;
 #pragma unroll
@@ -427,10 +476,29 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(trans_params p, Fields f, R
         lds_barrier();
     };
     LineRaw<NEQ> bufA, bufB;
-    line_load<R, NEQ>(f, Dq, wline + (long)(FORWARD ? 0 : nplanes - 1) * nlines, bufA);
+    line_load<R, NEQ>(src, lw + (unsigned)(FORWARD ? 0 : nplanes - 1) * (unsigned)nlines, bufA);
     for (int t = 0; t < nplanes; t += 2) {
         step(t, bufA, bufB);
         if (t + 1 < nplanes) step(t + 1, bufB, bufA);
+    }
+}
+
+// NEQ = 7: blockIdx.y 0 sweeps the mean flow and 1 the k-omega equations of block
+// blockIdx.x; NEQ = 5 or 2: one row of workgroups.  Dq is W (FP64 operator) or Wf.
+template <typename R, bool FORWARD, int NEQ, int MAXT, int MARCH>
+__global__ void __launch_bounds__(MAXT) k_lusgs_line(Fields f, R* Dq, const BlkInfo* __restrict__ blks)
+{
+    extern __shared__ unsigned char lds_raw[];
+    R* buf = reinterpret_cast<R*>(lds_raw);
+    const BlkInfo b = blks[blockIdx.x];
+    const unsigned sink = (unsigned)(f.wsink + (long)(blockIdx.x * gridDim.y + blockIdx.y) * MAXT);
+    if constexpr (NEQ == 7) {
+        if (blockIdx.y == 0)
+            line_sweep<R, FORWARD, 5, MARCH>(f, Dq, b, sink, buf);
+        else
+            line_sweep<R, FORWARD, 2, MARCH>(f, Dq, b, sink, buf);
+    } else {
+        line_sweep<R, FORWARD, NEQ, MARCH>(f, Dq, b, sink, buf);
     }
 }
 

@@ -72,7 +72,6 @@ struct trans_ctx {
     int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
     int max_lines = 0;     // largest line count over the blocks for that axis
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
-    void* line_sink = nullptr;  // write sink of the idle lanes of the line-march sweep
     std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
     std::vector<void*> allocs;
     std::string err;
@@ -330,7 +329,7 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
                 const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
                 mx = std::max(mx, n[(m + 1) % 3] * n[(m + 2) % 3]);
             }
-            if (mx <= 1024 && 4 * neq * (mx + 1) * esz <= 128 * 1024) {
+            if (mx <= 1024 && 4 * std::min(neq, (size_t)5) * (mx + 2) * esz <= 128 * 1024) {
                 c->march = m;
                 c->max_lines = mx;
                 c->line_threads = (mx + 63) / 64 * 64;
@@ -339,16 +338,19 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
     }
     std::vector<long> woff(nblk, 0);
     if (c->line_threads) {
-        double* sk = nullptr;
-        if ((rc = dalloc(c, &sk, (size_t)nblk * 7 * 1024))) return rc;
-        c->line_sink = sk;
-        // hyperplane-major sweep buffer: nplanes x nlines slots per block
+        // hyperplane-major sweep buffer: nplanes x nlines slots per block, then per
+        // component a sink of 1024 slots for each of the 2 workgroups of every block
         long wt = 0;
         for (int b = 0; b < nblk; b++) {
             const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
             woff[b] = wt;
             wt += (long)(n[0] + n[1] + n[2] - 2) * n[(c->march + 1) % 3] * n[(c->march + 2) % 3];
         }
+        f.wsink = wt;
+        wt += (long)nblk * 2 * 1024;
+        // the sweep addresses a component with 32-bit byte offsets
+        if (wt * (long)sizeof(double) >= (1L << 32))
+            return fail(c, TRANS_EUNSUPPORTED, "too many cells for the line-march sweep buffer");
         f.wtot = wt;
         f.march = c->march;
         if ((rc = dalloc(c, &f.W, (size_t)W_NCOMP * wt))) return rc;
@@ -626,17 +628,29 @@ int trans_hip_residual(trans_ctx* c)
 
 }  // extern "C"
 
-template <typename R, int NEQ, int MAXT>
+template <typename R, int NEQ, int MAXT, int MARCH>
 static void launch_lines(trans_ctx* c, R* dq)
 {
     const int nblk = (int)c->hb.size();
-    const size_t lds = (size_t)4 * NEQ * (c->max_lines + 1) * sizeof(R);
-    R* sink = reinterpret_cast<R*>(c->line_sink);
+    // LDS: 2 parities x 2 directions x (5 or 2) equations x (lines + 2) values
+    const size_t lds = (size_t)4 * (NEQ == 2 ? 2 : 5) * (c->max_lines + 2) * sizeof(R);
+    const dim3 grid(nblk, NEQ == 7 ? 2 : 1);
     const bool kw = NEQ == 2;  // k-omega sweeps alone
-    KLAUNCH_LDS(c, kw ? "k_turb_line<fwd>" : "k_lusgs_line<fwd>", (k_lusgs_line<R, true, NEQ, MAXT>), dim3(nblk),
-                dim3(c->line_threads), lds, c->p, c->f, dq, c->d_blk, c->march, sink);
-    KLAUNCH_LDS(c, kw ? "k_turb_line<bwd>" : "k_lusgs_line<bwd>", (k_lusgs_line<R, false, NEQ, MAXT>), dim3(nblk),
-                dim3(c->line_threads), lds, c->p, c->f, dq, c->d_blk, c->march, sink);
+    KLAUNCH_LDS(c, kw ? "k_turb_line<fwd>" : "k_lusgs_line<fwd>", (k_lusgs_line<R, true, NEQ, MAXT, MARCH>), grid,
+                dim3(c->line_threads), lds, c->f, dq, c->d_blk);
+    KLAUNCH_LDS(c, kw ? "k_turb_line<bwd>" : "k_lusgs_line<bwd>", (k_lusgs_line<R, false, NEQ, MAXT, MARCH>), grid,
+                dim3(c->line_threads), lds, c->f, dq, c->d_blk);
+}
+
+template <typename R, int NEQ, int MAXT>
+static void launch_lines(trans_ctx* c, R* dq)
+{
+    if (c->march == 0)
+        launch_lines<R, NEQ, MAXT, 0>(c, dq);
+    else if (c->march == 1)
+        launch_lines<R, NEQ, MAXT, 1>(c, dq);
+    else
+        launch_lines<R, NEQ, MAXT, 2>(c, dq);
 }
 
 template <typename R, int NEQ>
