@@ -59,26 +59,40 @@ struct Fields {
     float* Ds;        // FP32 copy of D for the FP32 implicit operator (7)
     float* Rdsf;      // FP32 Rds (3)
     int* mw;          // wall marks per block: i0 (NJ1*NK1), im, j0 (NI1*NK1), jm, k0 (NI1*NJ1), km
-    // Inputs of the line-march LU-SGS in hyperplane-major order (see widx): component
-    // stride wtot, components W_* below.  Null when the hyperplane fallback sweep is used.
+    // Inputs and increments of the line-march LU-SGS in hyperplane-major order (see widx):
+    // component stride wtot, components W_* below; W for the FP64 operator, Wf (FP32) for
+    // the FP32 one.  Both null when the hyperplane fallback sweep is used.
     double* W;
-    float* Wf;        // FP32 increments of the FP32 implicit operator (7 components)
+    float* Wf;
     long wtot;        // component stride: the slots of all blocks, then the sinks
     long wsink;       // first sink slot (idle lanes of the line sweep store there)
     int march;        // march axis of the line sweep (0 i, 1 j, 2 k)
 };
 
-// components of Fields::W
-constexpr int W_D = 0;      // 7: right-hand side, then the increment (FP64 operator)
+// components of Fields::W / Fields::Wf (the mean-flow diagonal F1 = 1 + sum(Rds)*cvs is
+// re-formed by the sweep from W_RDS and W_CVS)
+constexpr int W_D = 0;      // 7: right-hand side, then the increment
 constexpr int W_STEPT = 7;  // stept
 constexpr int W_CVS = 8;    // stept / Vol
-constexpr int W_F1 = 9;     // mean-flow diagonal
-constexpr int W_F1T = 10;   // k-omega diagonal
-constexpr int W_V = 11;     // 4: rho, rho*u, rho*v, rho*w
-constexpr int W_PP = 15;    // pressure
-constexpr int W_RDS = 16;   // 3: spectral radii
-constexpr int W_SDM = 19;   // 9: face-mean vectors (geometry)
-constexpr int W_NCOMP = 28;
+constexpr int W_F1T = 9;    // k-omega diagonal
+constexpr int W_V = 10;     // 4: rho, rho*u, rho*v, rho*w
+constexpr int W_PP = 14;    // pressure
+constexpr int W_RDS = 15;   // 3: spectral radii
+constexpr int W_SDM = 18;   // 9: face-mean vectors (geometry)
+constexpr int W_NCOMP = 27;
+
+// Whether the sweep inputs live in the hyperplane-major buffer.
+__device__ __forceinline__ bool has_w(const Fields& f) { return f.W != nullptr || f.Wf != nullptr; }
+
+// Component `comp` of the hyperplane-major buffer at slot w (rounded to FP32 for the FP32
+// operator, whose sweep computes in FP32 anyway).
+__device__ __forceinline__ void wput(const Fields& f, int comp, long w, double v)
+{
+    if (f.Wf)
+        f.Wf[(long)comp * f.wtot + w] = (float)v;
+    else
+        f.W[(long)comp * f.wtot + w] = v;
+}
 
 __device__ __forceinline__ long cidx(const BlkInfo& b, int i, int j, int k)
 {

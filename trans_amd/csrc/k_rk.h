@@ -53,18 +53,17 @@ __global__ void __launch_bounds__(256) k_rk_stage(trans_params p, int stage, int
             f.T[c] = p.rxm2 * pp / vn[0];
             dr[1] = fabs(f.D[5 * tot + c] / vn[0]);
             dr[2] = fabs(f.D[6 * tot + c] / vn[0]);
-        } else if (f.W) {
+        } else if (has_w(f)) {
             // step sizes as Time/TURBTWO_TIME_LUSGS_SST.F90:567-568 forms them
             const double dtl = p.dt_lusgs_main;
             const double stept = 2.0 * dtm / (3.0 * dtm / dtl + 2.0);
             const long w = widx(b, f.march, i, j, k);
-            const long wt = f.wtot;
-            f.W[W_STEPT * wt + w] = stept;
-            f.W[W_CVS * wt + w] = stept / vol;
+            wput(f, W_STEPT, w, stept);
+            wput(f, W_CVS, w, stept / vol);
 #pragma unroll
-            for (int L = 0; L < 4; L++) f.W[(long)(W_V + L) * wt + w] = vn[L];
+            for (int L = 0; L < 4; L++) wput(f, W_V + L, w, vn[L]);
 #pragma unroll
-            for (int d = 0; d < 3; d++) f.W[(long)(W_RDS + d) * wt + w] = f.Rds[(long)d * tot + c];
+            for (int d = 0; d < 3; d++) wput(f, W_RDS + d, w, f.Rds[(long)d * tot + c]);
         }
     }
     norm_reduce(dr, arg, part, 0);

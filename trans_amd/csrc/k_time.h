@@ -6,8 +6,8 @@
 
 // ---------------------------------------------------------------------------------------
 // Time/TIME_LUSGS.F90:60-108: BDF2 right-hand side and the scalar diagonal F1.  With the
-// line-march sweep the results, and the state the sweep needs, go to the hyperplane-major
-// buffer W (FP32 increments to Wf for the FP32 operator); otherwise to D, F1 and Stp.
+// line-march sweep the right-hand side and the state the sweep needs go to the
+// hyperplane-major buffer (the sweep re-forms F1 from Rds and cvs); otherwise D, F1 and Stp.
 __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
                                                    const BlkInfo* __restrict__ blks,
                                                    const Seg* __restrict__ seg)
@@ -33,24 +33,17 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
     const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
     const double cvs = stept / vol;
     const double f1 = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * cvs;
-    if (f.W) {
+    if (has_w(f)) {
         const long w = widx(b, f.march, i, j, k);
-        const long wt = f.wtot;
 #pragma unroll
-        for (int L = 0; L < 5; L++) {
-            if (f.Wf)
-                f.Wf[(long)L * wt + w] = (float)rhs[L];
-            else
-                f.W[(long)(W_D + L) * wt + w] = rhs[L];
-        }
-        f.W[W_STEPT * wt + w] = stept;
-        f.W[W_CVS * wt + w] = cvs;
-        f.W[W_F1 * wt + w] = f1;
+        for (int L = 0; L < 5; L++) wput(f, W_D + L, w, rhs[L]);
+        wput(f, W_STEPT, w, stept);
+        wput(f, W_CVS, w, cvs);
 #pragma unroll
-        for (int L = 0; L < 4; L++) f.W[(long)(W_V + L) * wt + w] = f.V[(long)L * tot + c];
-        f.W[W_PP * wt + w] = f.PP[c];
+        for (int L = 0; L < 4; L++) wput(f, W_V + L, w, f.V[(long)L * tot + c]);
+        wput(f, W_PP, w, f.PP[c]);
 #pragma unroll
-        for (int d = 0; d < 3; d++) f.W[(long)(W_RDS + d) * wt + w] = f.Rds[(long)d * tot + c];
+        for (int d = 0; d < 3; d++) wput(f, W_RDS + d, w, f.Rds[(long)d * tot + c]);
     } else {
 #pragma unroll
         for (int L = 0; L < 5; L++) f.D[(long)L * tot + c] = rhs[L];
@@ -76,7 +69,7 @@ __global__ void __launch_bounds__(256) k_face_means(Fields f, const BlkInfo* __r
     const long c = cidx(b, i, j, k);
     const long tot = f.tot;
     const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
-    const long w = f.W ? widx(b, f.march, i, j, k) : 0;
+    const long w = has_w(f) ? widx(b, f.march, i, j, k) : 0;
 #pragma unroll
     for (int d = 0; d < 3; d++)
 #pragma unroll
@@ -84,7 +77,7 @@ __global__ void __launch_bounds__(256) k_face_means(Fields f, const BlkInfo* __r
             const long o = (long)(d + 3 * q) * tot;
             const double m = 0.5 * (f.SD[o + c + st[d]] + f.SD[o + c]);
             f.SDm[o + c] = m;
-            if (f.W) f.W[(long)(W_SDM + d + 3 * q) * f.wtot + w] = m;
+            if (has_w(f)) wput(f, W_SDM + d + 3 * q, w, m);
         }
 }
 
@@ -276,13 +269,13 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
 // in a second row of workgroups beside the mean-flow ones (NEQ = 5) when both are swept.
 __host__ __device__ constexpr int sweep_c0(int neq) { return neq == 2 ? 5 : 0; }
 
-template <int NEQ>
+template <typename S, int NEQ>
 struct LineRaw {
-    double d[NEQ];
-    double stept, cvs, f1;  // f1: F1 (mean flow) or F1t (k-omega)
-    double v[4], pp;
-    double sdm[3][3];  // face mean of SD in each direction
-    double rds[3];
+    S d[NEQ];
+    S stept, cvs, f1;  // f1: F1t (k-omega); the mean-flow F1 is re-formed from rds and cvs
+    S v[4], pp;
+    S sdm[3][3];  // face mean of SD in each direction
+    S rds[3];
 };
 
 // A wave-uniform pointer made opaque to the compiler (readfirstlane), so that a load from it
@@ -313,21 +306,22 @@ __device__ __forceinline__ void stg(T* base, unsigned w, T v)
     *(TRANS_GLOBAL T*)(b + w * (unsigned)sizeof(T)) = v;
 }
 
-// Scalar bases of the sweep's components in the hyperplane-major buffers.
+// Scalar bases of the sweep's components in the hyperplane-major buffer (W for the FP64
+// operator, Wf for the FP32 one; R is the element type).
 template <typename R, int NEQ>
 struct LineSrc {
     R* d[NEQ];
-    const double *stept, *cvs, *f1, *v[4], *pp, *sdm[3][3], *rds[3];
+    const R *stept, *cvs, *f1, *v[4], *pp, *sdm[3][3], *rds[3];
     __device__ __forceinline__ LineSrc(const Fields& f, R* Dq)
     {
         constexpr int C0 = sweep_c0(NEQ);
         const size_t wt = (size_t)f.wtot;
-        const double* W = f.W;
+        const R* W = Dq;  // the increments share the buffer with the other inputs
 #pragma unroll
         for (int L = 0; L < NEQ; L++) d[L] = sbase(Dq + (C0 + L) * wt);
         stept = sbase(W + W_STEPT * wt);
         cvs = sbase(W + W_CVS * wt);
-        f1 = sbase(W + (C0 == 0 ? W_F1 : W_F1T) * wt);
+        f1 = sbase(W + W_F1T * wt);
 #pragma unroll
         for (int L = 0; L < 4; L++) v[L] = sbase(W + (W_V + L) * wt);
         pp = sbase(W + W_PP * wt);
@@ -340,19 +334,19 @@ struct LineSrc {
     }
 };
 
-// Inputs of one cell of the sweep from the hyperplane-major buffer (slot w).  For the FP32
-// operator R(rds) rounds the spectral radius exactly as the FP32 copy Rdsf does.
+// Inputs of one cell of the sweep from the hyperplane-major buffer (slot w).
 template <typename R, int NEQ>
-__device__ __forceinline__ void line_load(const LineSrc<R, NEQ>& src, unsigned w, LineRaw<NEQ>& x)
+__device__ __forceinline__ void line_load(const LineSrc<R, NEQ>& src, unsigned w, LineRaw<R, NEQ>& x)
 {
+    constexpr bool MF = sweep_c0(NEQ) == 0;
 #pragma unroll
-    for (int L = 0; L < NEQ; L++) x.d[L] = (double)ldg(src.d[L], w);
+    for (int L = 0; L < NEQ; L++) x.d[L] = ldg(src.d[L], w);
     x.stept = ldg(src.stept, w);
     x.cvs = ldg(src.cvs, w);
-    x.f1 = ldg(src.f1, w);
+    x.f1 = MF ? R(1) : ldg(src.f1, w);
 #pragma unroll
     for (int L = 0; L < 4; L++) x.v[L] = ldg(src.v[L], w);
-    x.pp = sweep_c0(NEQ) == 0 ? ldg(src.pp, w) : 0.0;
+    x.pp = MF ? ldg(src.pp, w) : R(0);
 #pragma unroll
     for (int d = 0; d < 3; d++) {
 #pragma unroll
@@ -414,7 +408,7 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
     for (int L = 0; L < NEQ; L++) tm[L] = R(0);
     // one step of the wavefront on cell `cur`, loading the next cell into `nxt`; the loop
     // below alternates two buffers so that loads stay in flight across a whole step
-    auto step = [&](const int t, const LineRaw<NEQ>& cur, LineRaw<NEQ>& nxt) {
+    auto step = [&](const int t, const LineRaw<R, NEQ>& cur, LineRaw<R, NEQ>& nxt) {
         const int par = t & 1;
         const int rel = t - start;
         const bool act = active && rel >= 0 && rel < Nm;
@@ -424,6 +418,9 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
         const unsigned wst = act ? lw + (unsigned)s * (unsigned)nlines : my_sink;
         const double stept = cur.stept;
         const double cvs = cur.cvs;
+        // F1 as Time/TIME_LUSGS.F90:92-108 forms it (k_lusgs_rhs evaluates the same expression)
+        const R f1 = MF ? R(1.0 + (double(cur.rds[0]) + double(cur.rds[1]) + double(cur.rds[2])) * cvs)
+                        : R(cur.f1);
         R dq[NEQ];
 #pragma unroll
         for (int L = 0; L < NEQ; L++) {
@@ -439,12 +436,12 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
             else
                 sum = sa + sb + sm;
             if (FORWARD)
-                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / R(cur.f1);
+                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
             else
-                dq[L] = R(cur.d[L]) - sum * R(cvs) / R(cur.f1);
+                dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
             stg(src.d[L], wst, dq[L]);
         }
-        const double roinv = 1.0 / cur.v[0];
+        const double roinv = 1.0 / double(cur.v[0]);
         const R pVx = R(cur.v[1] * roinv), pVy = R(cur.v[2] * roinv), pVz = R(cur.v[3] * roinv);
         const int npar = par ^ 1;
 #pragma unroll
@@ -475,7 +472,7 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
         }
         lds_barrier();
     };
-    LineRaw<NEQ> bufA, bufB;
+    LineRaw<R, NEQ> bufA, bufB;
     line_load<R, NEQ>(src, lw + (unsigned)(FORWARD ? 0 : nplanes - 1) * (unsigned)nlines, bufA);
     for (int t = 0; t < nplanes; t += 2) {
         step(t, bufA, bufB);

@@ -620,17 +620,11 @@ __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const
     const double Dt = f.Dtm[c];
     const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
     const double f1t = 1. + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * (stept / vol) + (CnegTk + CnegTO) * stept;
-    if (f.W) {  // hyperplane-major inputs of the line-march sweep
+    if (has_w(f)) {  // hyperplane-major inputs of the line-march sweep
         const long w = widx(b, f.march, i, j, k);
-        const long wt = f.wtot;
-        f.W[W_F1T * wt + w] = f1t;
-        if (f.Wf) {
-            f.Wf[5 * wt + w] = (float)d6;
-            f.Wf[6 * wt + w] = (float)d7;
-        } else {
-            f.W[(W_D + 5) * wt + w] = d6;
-            f.W[(W_D + 6) * wt + w] = d7;
-        }
+        wput(f, W_F1T, w, f1t);
+        wput(f, W_D + 5, w, d6);
+        wput(f, W_D + 6, w, d7);
     } else {
         f.F1t[c] = f1t;
         f.D[5 * tot + c] = d6;

@@ -353,8 +353,11 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
             return fail(c, TRANS_EUNSUPPORTED, "too many cells for the line-march sweep buffer");
         f.wtot = wt;
         f.march = c->march;
-        if ((rc = dalloc(c, &f.W, (size_t)W_NCOMP * wt))) return rc;
-        if (c->p.implicit_fp32 && (rc = dalloc(c, &f.Wf, (size_t)7 * wt))) return rc;
+        if (c->p.implicit_fp32) {
+            if ((rc = dalloc(c, &f.Wf, (size_t)W_NCOMP * wt))) return rc;
+        } else if ((rc = dalloc(c, &f.W, (size_t)W_NCOMP * wt))) {
+            return rc;
+        }
     }
     std::vector<BlkInfo> bi(nblk);
     for (int b = 0; b < nblk; b++) {
@@ -685,7 +688,7 @@ static int lusgs_update_async(trans_ctx* c)
     if (if_turb)
         KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
     const int neq = if_turb ? 7 : 5;
-    if (c->f.W) {
+    if (c->f.W || c->f.Wf) {
         // line-march sweeps on the hyperplane-major buffer, increments back to D after
         if (c->p.implicit_fp32) {
             if (if_turb)
@@ -733,7 +736,7 @@ static int rk_update_async(trans_ctx* c, int stage)
             c->d_blk, c->seg_cell.d, c->d_part);
     if (if_turb) {
         KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
-        if (c->f.W) {
+        if (c->f.W || c->f.Wf) {
             if (c->p.implicit_fp32) {
                 launch_sweeps<float, 2>(c, c->f.Wf);
                 KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), 5, 2, c->f,
