@@ -110,7 +110,6 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
     const int n0 = c.point_end[0] - c.point_start[0] + 1;
     const int n1 = c.point_end[1] - c.point_start[1] + 1;
     const int n2 = c.point_end[2] - c.point_start[2] + 1;
-    if (c.bctype == 7 && sweep == -1) return;  // outlet ghosts: k_bc_outlet
     if (lin >= n0 * n1 * n2) return;
     const BlkInfo b = blks[dp.blk];
     const long tot = f.tot;
@@ -324,9 +323,6 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
                             0.5 * sqr(p.omega1 * f.rad[il]) * Roin;
         f.PP[il] = PsIn;
         f.T[il] = TsIn;
-        for (int d = 0; d < 3; d++) f.Rds[(long)d * tot + il] = f.Rds[(long)d * tot + is];
-        if (f.Rdsf)
-            for (int d = 0; d < 3; d++) f.Rdsf[(long)d * tot + il] = f.Rdsf[(long)d * tot + is];
         for (int lay = 1; lay < 3; lay++) {
             const int* ea = lay_dst(c, lay);
             const long d2 = gidx(b, i + ea[0], j + ea[1], k + ea[2]);
@@ -353,8 +349,14 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
         }
         break;
     }
-    case 7:  // outlets: k_bc_outlet
+    case 7: {  // outlet ghosts come from k_bc_outlet; the layer-0 Rds copy is here
+        const long is = gidx(b, i + ad[0], j + ad[1], k + ad[2]);
+        const long il = gidx(b, i + e[0], j + e[1], k + e[2]);
+        for (int d = 0; d < 3; d++) f.Rds[(long)d * tot + il] = f.Rds[(long)d * tot + is];
+        if (f.Rdsf)
+            for (int d = 0; d < 3; d++) f.Rdsf[(long)d * tot + il] = f.Rdsf[(long)d * tot + is];
         break;
+    }
     case 9: {  // BC/BC_P2P.F90:76-139
         const DPatch& tp = pt[dp.tpatch];
         const BlkInfo br = blks[tp.blk];
@@ -411,7 +413,9 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
 // ---------------------------------------------------------------------------------------
 // BC/BC_Outlet.F90:49-107: static-pressure outlet with the radial-equilibrium march along j
 // from the hub.  One workgroup per (i,k) line of an outlet patch: the march inputs are staged
-// in LDS by all lanes, lane 0 runs the pressure recurrence, and all lanes write the ghosts.
+// in LDS by all lanes, lane 0 runs the pressure recurrence, and all lanes write the ghosts
+// (the layer-0 Rds copy of the same routine is done by k_bc, after the spectral radius).
+// It reads interior cells and geometry only.
 constexpr int OUTLET_MAXJ = 512;
 
 struct OutLine {
@@ -477,9 +481,6 @@ __global__ void __launch_bounds__(256) k_bc_outlet(trans_params p, int if_turb, 
         f.V[tot + il] = f.V[tot + is] * den_ratio;
         f.V[2 * tot + il] = s_v3[jj] * den_ratio;
         f.V[3 * tot + il] = s_v4[jj] * den_ratio;
-        for (int d = 0; d < 3; d++) f.Rds[(long)d * tot + il] = f.Rds[(long)d * tot + is];
-        if (f.Rdsf)
-            for (int d = 0; d < 3; d++) f.Rdsf[(long)d * tot + il] = f.Rdsf[(long)d * tot + is];
         for (int lay = 1; lay < 3; lay++) {
             const int* ea = lay_dst(c, lay);
             const long d2 = gidx(b, i + ea[0], j + ea[1], k + ea[2]);

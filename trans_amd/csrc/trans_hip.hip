@@ -69,6 +69,10 @@ struct trans_ctx {
     NormPart* d_part = nullptr;
     trans_resid* d_resid = nullptr;
     hipStream_t stream = nullptr;
+    // helper streams for independent kernels of one call (forked from and joined back
+    // into `stream` inside the call, so that calls stay ordered on `stream`)
+    hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
     int max_lines = 0;     // largest line count over the blocks for that axis
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
@@ -114,19 +118,20 @@ static void prof_flush(trans_ctx* c)
 // Records HIP events around one kernel launch when profiling is on.
 struct ProfScope {
     trans_ctx* c;
+    hipStream_t s;
     ProfRec r{};
-    ProfScope(trans_ctx* cc, const char* name) : c(cc)
+    ProfScope(trans_ctx* cc, const char* name, hipStream_t st) : c(cc), s(st)
     {
         if (!c->prof) return;
         r.name = name;
         r.a = prof_event(c);
         r.b = prof_event(c);
-        (void)hipEventRecord(r.a, c->stream);
+        (void)hipEventRecord(r.a, s);
     }
     ~ProfScope()
     {
         if (!c->prof) return;
-        (void)hipEventRecord(r.b, c->stream);
+        (void)hipEventRecord(r.b, s);
         c->prof_pending.push_back(r);
         if (c->prof_pending.size() >= 4096) prof_flush(c);
     }
@@ -134,13 +139,20 @@ struct ProfScope {
 
 #define KLAUNCH_LDS(c, name, kern, grid, blk, lds, ...)                    \
     do {                                                                   \
-        ProfScope ps_(c, name);                                            \
+        ProfScope ps_(c, name, (c)->stream);                               \
         hipLaunchKernelGGL(kern, grid, blk, lds, (c)->stream, __VA_ARGS__); \
+    } while (0)
+
+// launch on helper stream `st` (between fork and join)
+#define KLAUNCH_ON(c, st, name, kern, grid, blk, ...)               \
+    do {                                                            \
+        ProfScope ps_(c, name, st);                                 \
+        hipLaunchKernelGGL(kern, grid, blk, 0, st, __VA_ARGS__);    \
     } while (0)
 
 #define KLAUNCH(c, name, kern, grid, blk, ...)                          \
     do {                                                                 \
-        ProfScope ps_(c, name);                                          \
+        ProfScope ps_(c, name, (c)->stream);                             \
         hipLaunchKernelGGL(kern, grid, blk, 0, (c)->stream, __VA_ARGS__); \
     } while (0)
 
@@ -148,6 +160,22 @@ static int fail(trans_ctx* c, int code, const std::string& msg)
 {
     if (c) c->err = msg;
     return code;
+}
+
+// Helper streams 0..n-1 wait for the work queued so far on the context's stream.
+static void fork_aux(trans_ctx* c, int n)
+{
+    (void)hipEventRecord(c->ev_fork, c->stream);
+    for (int q = 0; q < n; q++) (void)hipStreamWaitEvent(c->aux[q], c->ev_fork, 0);
+}
+
+// The context's stream waits for the work queued on helper streams 0..n-1.
+static void join_aux(trans_ctx* c, int n)
+{
+    for (int q = 0; q < n; q++) {
+        (void)hipEventRecord(c->ev_join[q], c->aux[q]);
+        (void)hipStreamWaitEvent(c->stream, c->ev_join[q], 0);
+    }
 }
 
 #define HIPCHK(ctx, x)                                                                   \
@@ -243,6 +271,11 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
                 return fail(c, TRANS_EINVAL, "interface target out of range");
         }
     HIPCHK(c, hipStreamCreate(&c->stream));
+    for (int q = 0; q < 3; q++) {
+        HIPCHK(c, hipStreamCreate(&c->aux[q]));
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming));
+    }
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
 
     Fields& f = c->f;
     f.tot = off;
@@ -385,6 +418,11 @@ int trans_hip_destroy(trans_ctx* c)
     }
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
+    for (int q = 0; q < 3; q++) {
+        if (c->aux[q]) (void)hipStreamDestroy(c->aux[q]);
+        if (c->ev_join[q]) (void)hipEventDestroy(c->ev_join[q]);
+    }
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -571,21 +609,21 @@ int trans_hip_bc(trans_ctx* c, int sweep)
 
 }  // extern "C"
 
+// Face kernels of direction d+1 on helper stream d: the three directions are independent
+// until k_accumulate_visc, which runs after the join.
 template <int SPATIAL, int ROE>
 static void launch_conv(trans_ctx* c, int if_turb)
 {
-    KLAUNCH(c, "k_conv_face<1>", (k_conv_face<1, SPATIAL, ROE>), dim3(c->seg_face[0].n), dim3(WG), c->p, if_turb,
-            c->f, c->d_blk, c->seg_face[0].d);
-    KLAUNCH(c, "k_conv_face<2>", (k_conv_face<2, SPATIAL, ROE>), dim3(c->seg_face[1].n), dim3(WG), c->p, if_turb,
-            c->f, c->d_blk, c->seg_face[1].d);
-    KLAUNCH(c, "k_conv_face<3>", (k_conv_face<3, SPATIAL, ROE>), dim3(c->seg_face[2].n), dim3(WG), c->p, if_turb,
-            c->f, c->d_blk, c->seg_face[2].d);
+    KLAUNCH_ON(c, c->aux[0], "k_conv_face<1>", (k_conv_face<1, SPATIAL, ROE>), dim3(c->seg_face[0].n), dim3(WG),
+               c->p, if_turb, c->f, c->d_blk, c->seg_face[0].d);
+    KLAUNCH_ON(c, c->aux[1], "k_conv_face<2>", (k_conv_face<2, SPATIAL, ROE>), dim3(c->seg_face[1].n), dim3(WG),
+               c->p, if_turb, c->f, c->d_blk, c->seg_face[1].d);
+    KLAUNCH_ON(c, c->aux[2], "k_conv_face<3>", (k_conv_face<3, SPATIAL, ROE>), dim3(c->seg_face[2].n), dim3(WG),
+               c->p, if_turb, c->f, c->d_blk, c->seg_face[2].d);
 }
 
 extern "C" {
 
-// Residual after BC(-1): gradients and BC(0) when viscous, face fluxes, accumulation into D
-// (main.F90:86-133).
 int trans_hip_space(trans_ctx* c)
 {
     if (!c) return TRANS_EINVAL;
@@ -597,12 +635,15 @@ int trans_hip_space(trans_ctx* c)
                 c->seg_cell.d);
         if ((rc = launch_check(c, "k_gradients"))) return rc;
         if ((rc = trans_hip_bc(c, 0))) return rc;
-        KLAUNCH(c, "k_visc_face<1>", k_visc_face<1>, dim3(c->seg_face[0].n), dim3(WG), c->p, if_turb, c->f,
-                c->d_blk, c->seg_face[0].d);
-        KLAUNCH(c, "k_visc_face<2>", k_visc_face<2>, dim3(c->seg_face[1].n), dim3(WG), c->p, if_turb, c->f,
-                c->d_blk, c->seg_face[1].d);
-        KLAUNCH(c, "k_visc_face<3>", k_visc_face<3>, dim3(c->seg_face[2].n), dim3(WG), c->p, if_turb, c->f,
-                c->d_blk, c->seg_face[2].d);
+    }
+    fork_aux(c, 3);
+    if (visc) {
+        KLAUNCH_ON(c, c->aux[0], "k_visc_face<1>", k_visc_face<1>, dim3(c->seg_face[0].n), dim3(WG), c->p, if_turb,
+                   c->f, c->d_blk, c->seg_face[0].d);
+        KLAUNCH_ON(c, c->aux[1], "k_visc_face<2>", k_visc_face<2>, dim3(c->seg_face[1].n), dim3(WG), c->p, if_turb,
+                   c->f, c->d_blk, c->seg_face[1].d);
+        KLAUNCH_ON(c, c->aux[2], "k_visc_face<3>", k_visc_face<3>, dim3(c->seg_face[2].n), dim3(WG), c->p, if_turb,
+                   c->f, c->d_blk, c->seg_face[2].d);
     }
     if (c->p.kind_spatial == 2) {
         if (c->p.kind_roe == 1)
@@ -615,6 +656,7 @@ int trans_hip_space(trans_ctx* c)
         else
             launch_conv<3, 3>(c, if_turb);
     }
+    join_aux(c, 3);
     KLAUNCH(c, "k_accumulate_visc", k_accumulate_visc, dim3(c->seg_cell.n), dim3(WG), c->p, c->p.if_euler,
             if_turb, c->f, c->d_blk, c->seg_cell.d);
     return launch_check(c, "space");
@@ -623,8 +665,20 @@ int trans_hip_space(trans_ctx* c)
 int trans_hip_residual(trans_ctx* c)
 {
     int rc;
+    // The outlet march reads interior cells only and writes only its own ghosts, so it runs
+    // beside the spectral radius and the other patches' BC (which copies its Rds ghosts).
+    const bool outl = c->n_outl > 0;
+    if (outl) {
+        fork_aux(c, 1);
+        KLAUNCH_ON(c, c->aux[0], "k_bc_outlet", k_bc_outlet, dim3(c->n_outl), dim3(256), c->p, if_turb_now(c), c->f,
+                   c->d_blk, c->d_patch, c->d_outl);
+    }
     if ((rc = trans_hip_spectral_radius(c))) return rc;
-    if ((rc = trans_hip_bc(c, -1))) return rc;
+    if (c->n_bcseg)
+        KLAUNCH(c, "k_bc", k_bc, dim3(c->n_bcseg), dim3(128), c->p, if_turb_now(c), -1, c->f, c->d_blk, c->d_patch,
+                c->d_bcseg);
+    if (outl) join_aux(c, 1);
+    if ((rc = launch_check(c, "k_bc"))) return rc;
     if ((rc = trans_hip_space(c))) return rc;
     return 0;
 }
