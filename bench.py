@@ -53,9 +53,11 @@ def alg_bytes(name, neq, fp32):
         "k_lusgs_rhs": ("cell", d * (5 + 5 + 5 + 5 + 3 + 1 + 1 + 5 + 1)),
         # D(6:7), V(1,6,7), Wt0, Wt1 (2 each), DQ(15), Rmiu, T, Dst, Dtm, Rds(3), Vol in; D(6:7), F1t out
         "k_sst_rhs": ("cell", d * (2 + 3 + 2 + 2 + 15 + 1 + 1 + 1 + 1 + 3 + 1 + 2 + 1)),
-        # D(neq) in/out, Dtm, Vol, F1(,F1t), V(1:4), PP, SD(9), Rds(3)
-        "k_lusgs_line<fwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
-        "k_lusgs_line<bwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
+        # hyperplane-major inputs in the operator's precision: D(neq) in/out, stept, cvs,
+        # (F1t), V(1:4), PP, face means SD(9), Rds(3); F1 is re-formed from Rds and cvs
+        "k_lusgs_line<fwd>": ("cell", r * (2 * neq + 2 + (1 if turb else 0) + 4 + 1 + 9 + 3)),
+        "k_lusgs_line<bwd>": ("cell", r * (2 * neq + 2 + (1 if turb else 0) + 4 + 1 + 9 + 3)),
+        # D(neq), Dtm, Vol, F1(,F1t), V(1:4), PP, SD(9), Rds(3) in the natural layout
         "k_lusgs_sweep<fwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
         "k_lusgs_sweep<bwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
         # D(7), V(7), PP, rad, DQ(9), Dst in; V(7), PP, T, Rmiu out
@@ -193,6 +195,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-implicit leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup+steps of the FP64 path only and print the kernel table")
+    ap.add_argument("--implicit-fp32", action="store_true",
+                    help="with --profile-only: profile the FP32-implicit path instead")
     ap.add_argument("--converge", type=int, default=0,
                     help="instead of the timed run: iterate up to N times to a 1e-8 drop of the "
                          "global DRSa and print the history")
@@ -214,7 +218,7 @@ def main():
                                    "blocks is a design only", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    blocks, prm, desc = build_workload(args.workload)
+    blocks, prm, desc = build_workload(args.workload, 1 if (args.profile_only and args.implicit_fp32) else 0)
     cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
     if args.converge:
         converge(blocks, prm, desc, args.converge)
@@ -223,7 +227,8 @@ def main():
     print(f"[bench] {desc}: {cells} cells, warmup {args.warmup}, steps {args.steps}", flush=True)
     prof_steps = min(args.steps, 50)
     dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps)
-    print(f"[bench] FP64: {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)
+    tag = "FP32 implicit" if prm.implicit_fp32 else "FP64"
+    print(f"[bench] {tag}: {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)
     if args.profile_only:
         for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0]):
             print(f"[bench] {k:28s} {v[0] / v[1] * 1e3:9.2f} us x {v[1]}")
