@@ -163,6 +163,20 @@ typedef struct trans_resid {
 #define TRANS_FIELD_DQDXYZ 3 /* DQdxyz(18,...) */
 #define TRANS_FIELD_SHOCK 4  /* shock sensor */
 
+/* Integrals over one constant-index face plane of a block (all faces of direction dir at
+ * face index `index`, i.e. between cells index-1 and index).  The face state is the
+ * reference's mass-monitor weighting of the two adjacent cells
+ * (Output/OUTPUT_Residual_NS.F90:19-32): q_f = (Vol(index-1) q(index) + Vol(index) q(index-1))
+ * / (Vol(index-1) + Vol(index)); dm = (rho u)_f . S.  Total pressure and temperature use the
+ * reference's output formula p0 = p (1 + 0.2 M^2)^3.5 (Output/Average_Field.F90:66) and
+ * T0 = T (1 + 0.2 M^2).  Nondimensional, as the state is. */
+typedef struct trans_plane_integrals {
+    double mass_flow;   /* sum of dm (the reference's msum before its Roref*Vref scaling) */
+    double area;        /* sum of |S| */
+    double p0_mass_avg; /* sum(p0 dm) / sum(dm) */
+    double t0_mass_avg; /* sum(T0 dm) / sum(dm) */
+} trans_plane_integrals;
+
 typedef struct trans_ctx trans_ctx;
 
 /* Allocate device state for all blocks.  Replaces INPUT_Read_BlockDimensions'
@@ -206,6 +220,11 @@ int trans_hip_update_u(trans_ctx* ctx);            /* TIME_Update_U, main.F90:19
  * when IF_RK = 1; update of Wt0/Wt1, KI += 1).
  * per_block (nblk entries, may be NULL) receives the norms of the last sub-iteration. */
 int trans_hip_iterate(trans_ctx* ctx, int n_iter, trans_resid* per_block);
+/* Plane integrals (struct above) of block blk (0-based), direction dir (1 i, 2 j, 3 k), face
+ * index 1..N_dir.  Generalises OUTPUT_Residual_NS's mass monitor, which the reference
+ * hard-wires to i = 4 of block 1 and i = NI-4 of block 2 (Output/OUTPUT_Residual_NS.F90:15-60);
+ * the ratio and efficiency of two planes follow on the host. */
+int trans_hip_plane_integrals(trans_ctx* ctx, int blk, int dir, int index, trans_plane_integrals* out);
 /* Wait for all queued device work of the context. */
 int trans_hip_sync(trans_ctx* ctx);
 

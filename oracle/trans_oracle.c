@@ -1195,6 +1195,46 @@ void ora_rk_update(const trans_params* p, int ki, int itr_chld, ora_block* block
     }
 }
 
+/* ------------------------------------------------------------------------------------ */
+/* Plane integrals.  Face state: the volume weighting of Output/OUTPUT_Residual_NS.F90:19-32
+ * (k outer, then the other in-plane index); p0 = p (1 + 0.2 M^2)^3.5 as
+ * Output/Average_Field.F90:60-66 forms it per cell, T0 = T (1 + 0.2 M^2). */
+void ora_plane_integrals(ora_block* b, int dir, int index, double out[4])
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    const int n0 = dir == 1 ? NJ1 : NI1;  /* fast in-plane index */
+    const int n1 = dir == 3 ? NJ1 : NK1;  /* slow in-plane index */
+    double msum = 0.0, area = 0.0, psum = 0.0, tsum = 0.0;
+    for (int q1 = 1; q1 <= n1; q1++)
+        for (int q0 = 1; q0 <= n0; q0++) {
+            int i, j, k, im, jm, km;
+            if (dir == 1) { i = index; j = q0; k = q1; }
+            else if (dir == 2) { i = q0; j = index; k = q1; }
+            else { i = q0; j = q1; k = index; }
+            im = i - (dir == 1); jm = j - (dir == 2); km = k - (dir == 3);
+            double wei1 = VOL(im, jm, km), wei2 = VOL(i, j, k), wei = wei1 + wei2;
+            double ro = (wei1 * V(1, i, j, k) + wei2 * V(1, im, jm, km)) / wei;
+            double ru = (wei1 * V(2, i, j, k) + wei2 * V(2, im, jm, km)) / wei;
+            double rv = (wei1 * V(3, i, j, k) + wei2 * V(3, im, jm, km)) / wei;
+            double rw = (wei1 * V(4, i, j, k) + wei2 * V(4, im, jm, km)) / wei;
+            double pf = (wei1 * PP(i, j, k) + wei2 * PP(im, jm, km)) / wei;
+            double tf = (wei1 * T(i, j, k) + wei2 * T(im, jm, km)) / wei;
+            double s1 = SD(dir, 1, i, j, k), s2 = SD(dir, 2, i, j, k), s3 = SD(dir, 3, i, j, k);
+            double dm = ru * s1 + rv * s2 + rw * s3;
+            double u = ru / ro, v = rv / ro, w = rw / ro;
+            double m2 = (u * u + v * v + w * w) / (1.4 * pf / ro);
+            double fac = 1.0 + 0.2 * m2;
+            msum = msum + dm;
+            area = area + sqrt(s1 * s1 + s2 * s2 + s3 * s3);
+            psum = psum + pf * pow(fac, 3.5) * dm;
+            tsum = tsum + tf * fac * dm;
+        }
+    out[0] = msum;
+    out[1] = area;
+    out[2] = psum / msum;
+    out[3] = tsum / msum;
+}
+
 /* Threads an OpenMP region of the oracle uses. */
 int ora_num_threads(void)
 {

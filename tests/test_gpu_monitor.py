@@ -1,0 +1,81 @@
+"""GPU parity of the plane integrals (mass flow, mass-averaged p0 and T0) and the
+precision report of the FP32 implicit operator on them.
+
+Bars: plane integrals of identical states within 1e-12 relative (the device sums in a
+fixed tree order and evaluates pow in its own math library, the oracle sums sequentially);
+after iterating, the FP64 path within 1e-6 of the oracle (as the other multi-iteration
+tests); the FP32 implicit operator's deviation from the FP64 oracle is printed and held to
+1e-4 (its increments carry ~1e-6 relative rounding that accumulates over the iterations).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleCase
+from trans_amd import capi, cases, solver
+from trans_amd.solver import Solver, stage_performance
+
+from helpers import perturb
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("mass_flow", "area", "p0_mass_avg", "t0_mass_avg")
+
+
+def _plate(implicit_fp32=0):
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
+    prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32)
+    return blocks, prm
+
+
+def _planes(blocks):
+    out = []
+    for n, b in enumerate(blocks):
+        out += [(n, 1, 1), (n, 1, 4), (n, 1, b.ni), (n, 2, 1), (n, 2, 12), (n, 3, 2)]
+    return out
+
+
+def test_plane_integrals_match_oracle():
+    print("case: flat plate 3 blocks, perturbed state, plane integrals", flush=True)
+    blocks, prm = _plate()
+    perturb(blocks, prm, seed=7, turb=True)
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    for (n, d, idx) in _planes(blocks):
+        g = sv.plane_integrals(n, d, idx)
+        o = oc.plane_integrals(n, d, idx)
+        for k in KEYS:
+            assert abs(g[k] - o[k]) <= 1e-12 * abs(o[k]) + 1e-300, (n, d, idx, k, g[k], o[k])
+    sv.close()
+
+
+def test_integrals_after_iterations_fp64_and_fp32():
+    n_iter = 30
+    print(f"case: flat plate 3 blocks, {n_iter} iterations, FP64 and FP32-implicit vs oracle", flush=True)
+    blocks, prm = _plate()
+    oc = OracleCase(blocks, prm)
+    oc.iterate(1, n_iter)
+    sv = Solver(blocks, prm, ki=1)
+    sv.iterate(n_iter)
+    b32, p32 = _plate(implicit_fp32=1)
+    s32 = Solver(b32, p32, ki=1)
+    s32.iterate(n_iter)
+    last = len(blocks) - 1
+    inlet, outlet = (0, 1, 1), (last, 1, blocks[last].ni)
+    worst32 = 0.0
+    for plane in (inlet, outlet, (1, 2, 12)):
+        o = oc.plane_integrals(*plane)
+        g = sv.plane_integrals(*plane)
+        h = s32.plane_integrals(*plane)
+        for k in ("mass_flow", "p0_mass_avg", "t0_mass_avg"):
+            e64 = abs(g[k] - o[k]) / abs(o[k])
+            e32 = abs(h[k] - o[k]) / abs(o[k])
+            print(plane, k, "FP64 relerr %.2e  FP32-implicit relerr %.2e" % (e64, e32), flush=True)
+            assert e64 < 1e-6, (plane, k)
+            worst32 = max(worst32, e32)
+    po = stage_performance(oc.plane_integrals(*inlet), oc.plane_integrals(*outlet))
+    pg = stage_performance(sv.plane_integrals(*inlet), sv.plane_integrals(*outlet))
+    assert abs(pg["pressure_ratio"] - po["pressure_ratio"]) < 1e-6 * po["pressure_ratio"]
+    print("FP32-implicit worst relerr vs FP64 oracle %.2e" % worst32, flush=True)
+    assert worst32 < 1e-4
+    sv.close()
+    s32.close()

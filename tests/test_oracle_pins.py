@@ -793,3 +793,52 @@ def test_rk_iterate_preserves_turbulent_freestream():
     assert np.all(dv < 1e-12 * scale), dv / scale
     for q in (5, 6):  # destruction only (local time steps make the decay differ per cell)
         assert np.all(v1[q] < v0[region][q]) and np.all(v1[q] > 0.5 * v0[region][q])
+
+
+# ---------------------------------------------------------------------------------------
+# Plane integrals (mass monitor of Output/OUTPUT_Residual_NS.F90:19-32, p0 of
+# Output/Average_Field.F90:60-66)
+
+def _uniform_state(b, prm, rho, u, v, w, p):
+    V = b["v"]
+    V[0], V[1], V[2], V[3] = rho, rho * u, rho * v, rho * w
+    b["pp"][...] = p
+    b["t"][...] = prm.rxm2 * p / rho
+    V[4] = p / 0.4 + 0.5 * rho * (u * u + v * v + w * w)
+
+
+@pytest.mark.parametrize("direction", [1, 2, 3])
+def test_plane_integrals_uniform_flow(direction):
+    """Uniform flow: mass flow = rho U_n A through any plane of the box, and the mass
+    averages are the freestream p0 = p (1 + 0.2 M^2)^3.5 and T0 = T (1 + 0.2 M^2)."""
+    b, prm, (x1, y1, z1) = _box()
+    rho, u, v, w, p = 1.1, 0.6, -0.25, 0.15, 1.0 / (1.4 * 0.25)
+    _uniform_state(b, prm, rho, u, v, w, p)
+    oc = OracleCase([b], prm)
+    L = (x1[-1] - x1[0], y1[-1] - y1[0], z1[-1] - z1[0])
+    area = L[0] * L[1] * L[2] / L[direction - 1]
+    un = (u, v, w)[direction - 1]
+    idx = 3 if direction != 3 else 2
+    r = oc.plane_integrals(0, direction, idx)
+    assert r["area"] == pytest.approx(area, rel=1e-13)
+    assert r["mass_flow"] == pytest.approx(rho * un * area, rel=1e-13)
+    m2 = (u * u + v * v + w * w) / (1.4 * p / rho)
+    assert r["p0_mass_avg"] == pytest.approx(p * (1 + 0.2 * m2) ** 3.5, rel=1e-13)
+    assert r["t0_mass_avg"] == pytest.approx(prm.rxm2 * p / rho * (1 + 0.2 * m2), rel=1e-13)
+
+
+def test_plane_integrals_face_state_is_linear_interpolation():
+    """The monitor's crossed volume weights (Vol(i-1) q(i) + Vol(i) q(i-1)) / (Vol(i-1) + Vol(i))
+    interpolate linearly to the face: with rho v linear in y on the box's geometric y-spacing,
+    the mass flow through a j-plane is rho v(y_face) times the plane's area, which an
+    uncrossed weighting would miss."""
+    b, prm, (x1, y1, z1) = _box()
+    _uniform_state(b, prm, 1.0, 0.3, 0.0, 0.0, 1.0 / (1.4 * 0.25))
+    yc = _centres(y1)
+    a0, a1 = 0.2, 1.7
+    b["v"][2] = (a0 + a1 * yc)[None, :, None]
+    oc = OracleCase([b], prm)
+    area = (x1[-1] - x1[0]) * (z1[-1] - z1[0])
+    for jf in (2, 4, 6):
+        r = oc.plane_integrals(0, 2, jf)
+        assert r["mass_flow"] == pytest.approx((a0 + a1 * y1[jf - 1]) * area, rel=1e-12), jf

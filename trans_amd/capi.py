@@ -90,6 +90,14 @@ class TransResid(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class TransPlaneIntegrals(C.Structure):
+    _fields_ = [("mass_flow", C.c_double), ("area", C.c_double), ("p0_mass_avg", C.c_double),
+                ("t0_mass_avg", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK = 0, 1, 2, 3, 4
 
 EXPORTS = [
@@ -98,7 +106,7 @@ EXPORTS = [
     "trans_hip_get_state", "trans_hip_get_field", "trans_hip_spectral_radius", "trans_hip_bc",
     "trans_hip_space", "trans_hip_residual", "trans_hip_lusgs_update", "trans_hip_rk_update",
     "trans_hip_update_u",
-    "trans_hip_iterate", "trans_hip_sync", "trans_hip_profile", "trans_hip_profile_count",
+    "trans_hip_iterate", "trans_hip_plane_integrals", "trans_hip_sync", "trans_hip_profile", "trans_hip_profile_count",
     "trans_hip_profile_get",
 ]
 
@@ -149,6 +157,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_residual.argtypes = [C.c_void_p]
     lib.trans_hip_lusgs_update.argtypes = [C.c_void_p, P(TransResid)]
     lib.trans_hip_rk_update.argtypes = [C.c_void_p, C.c_int, P(TransResid)]
+    lib.trans_hip_plane_integrals.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, P(TransPlaneIntegrals)]
     lib.trans_hip_update_u.argtypes = [C.c_void_p]
     lib.trans_hip_iterate.argtypes = [C.c_void_p, C.c_int, P(TransResid)]
     lib.trans_hip_sync.argtypes = [C.c_void_p]

@@ -14,6 +14,7 @@
 
 #include "dev_common.h"
 #include "k_bc.h"
+#include "k_monitor.h"
 #include "k_space.h"
 #include "k_rk.h"
 #include "k_time.h"
@@ -68,11 +69,17 @@ struct trans_ctx {
     int* d_wrange = nullptr;
     NormPart* d_part = nullptr;
     trans_resid* d_resid = nullptr;
+    double* d_mon = nullptr;  // 4 doubles: result of k_plane_integrals
     hipStream_t stream = nullptr;
     // helper streams for independent kernels of one call (forked from and joined back
     // into `stream` inside the call, so that calls stay ordered on `stream`)
     hipStream_t aux[3] = {nullptr, nullptr, nullptr};
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
+    // one outer iteration captured as a HIP graph and replayed while the iteration's kernel
+    // arguments stay the same (set_params and profiling drop it)
+    bool graphs = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
     int max_lines = 0;     // largest line count over the blocks for that axis
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
@@ -178,6 +185,14 @@ static void join_aux(trans_ctx* c, int n)
     }
 }
 
+static void drop_graph(trans_ctx* c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
 #define HIPCHK(ctx, x)                                                                   \
     do {                                                                                 \
         hipError_t e_ = (x);                                                             \
@@ -276,6 +291,10 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
         HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming));
     }
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    {
+        const char* env = std::getenv("TRANS_HIP_GRAPH");
+        c->graphs = !(env && env[0] == '0');
+    }
 
     Fields& f = c->f;
     f.tot = off;
@@ -404,7 +423,9 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
     HIPCHK(c, hipMemcpy(c->d_wrange, c->seg_cell.wrange.data(), (nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
-    if ((rc = dalloc(c, &c->d_part, c->seg_cell.n)) || (rc = dalloc(c, &c->d_resid, nblk))) return rc;
+    if ((rc = dalloc(c, &c->d_part, c->seg_cell.n)) || (rc = dalloc(c, &c->d_resid, nblk)) ||
+        (rc = dalloc(c, &c->d_mon, 4)))
+        return rc;
     return 0;
 }
 
@@ -418,6 +439,8 @@ int trans_hip_destroy(trans_ctx* c)
     }
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     for (int q = 0; q < 3; q++) {
         if (c->aux[q]) (void)hipStreamDestroy(c->aux[q]);
         if (c->ev_join[q]) (void)hipEventDestroy(c->ev_join[q]);
@@ -438,6 +461,7 @@ int trans_hip_set_params(trans_ctx* c, const trans_params* p)
     if (p->if_euler != c->p.if_euler || p->if_turb_org != c->p.if_turb_org)
         return fail(c, TRANS_EINVAL, "IF_Euler and IF_Turb_Org are fixed at context creation");
     c->p = *p;
+    drop_graph(c);
     return 0;
 }
 
@@ -852,19 +876,69 @@ int trans_hip_update_u(trans_ctx* c)
     return launch_check(c, "k_update_u");
 }
 
+// The device work of one outer iteration (main.F90:65-197 without I/O), queued only.
+static int iteration_body(trans_ctx* c)
+{
+    int rc;
+    const int nsub = c->p.if_rk == 1 ? 3 : (c->p.kind_dual != 0 ? c->p.kchld : 1);
+    for (int s = 0; s < nsub; s++) {
+        if ((rc = trans_hip_residual(c))) return rc;
+        if ((rc = c->p.if_rk == 1 ? rk_update_async(c, s + 1) : lusgs_update_async(c))) return rc;
+    }
+    if (c->p.kind_dual != 0)
+        if ((rc = trans_hip_update_u(c))) return rc;
+    return 0;
+}
+
+// KI enters the kernels only through the CFL ramp of the local step (constant once the
+// ramp has reached a non-decreasing end value, or without dual time) and the turbulence
+// switch (constant from KItm on), so from then on every iteration launches the same
+// kernels with the same arguments.
+static bool iteration_is_static(const trans_ctx* c)
+{
+    const trans_params& p = c->p;
+    const bool ramp = p.kind_dual == 0 || (p.cfl_sub_end >= p.cfl_sub_start && c->ki - p.ki_ini >= p.cfl_sub_step);
+    const bool turb = !(p.if_euler == 0 && p.if_turb_org == 1) || p.kind_hybrid > 0 || c->ki >= p.kitm;
+    return ramp && turb;
+}
+
+// Capture one iteration into c->gexec; on any failure the iteration runs eagerly instead.
+static int capture_iteration(trans_ctx* c)
+{
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
+    const int rc = iteration_body(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return -1;
+    }
+    hipGraphExec_t x = nullptr;
+    if (hipGraphInstantiate(&x, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return -1;
+    }
+    c->graph = g;
+    c->gexec = x;
+    return 0;
+}
+
 int trans_hip_iterate(trans_ctx* c, int n_iter, trans_resid* per_block)
 {
     if (!c || n_iter < 0) return TRANS_EINVAL;
     int rc;
     for (int it = 0; it < n_iter; it++) {
-        const int nsub = c->p.if_rk == 1 ? 3 : (c->p.kind_dual != 0 ? c->p.kchld : 1);
-        for (int s = 0; s < nsub; s++) {
-            if ((rc = trans_hip_residual(c))) return rc;
-            if ((rc = c->p.if_rk == 1 ? rk_update_async(c, s + 1) : lusgs_update_async(c))) return rc;
+        const bool replay = c->graphs && !c->prof && iteration_is_static(c);
+        if (replay && !c->gexec && capture_iteration(c) != 0) c->graphs = false;  // eager from now on
+        if (replay && c->gexec) {
+            HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        } else {
+            drop_graph(c);
+            if ((rc = iteration_body(c))) return rc;
         }
         c->ki += 1;
-        if (c->p.kind_dual != 0)
-            if ((rc = trans_hip_update_u(c))) return rc;
     }
     return per_block ? fetch_resid(c, per_block) : 0;
 }
@@ -900,6 +974,26 @@ int trans_hip_profile_get(trans_ctx* c, int idx, char* name, int name_len, doubl
     }
     if (total_ms) *total_ms = c->prof_ms[idx];
     if (launches) *launches = c->prof_n[idx];
+    return 0;
+}
+
+int trans_hip_plane_integrals(trans_ctx* c, int blk, int dir, int index, trans_plane_integrals* out)
+{
+    if (!c || !out) return TRANS_EINVAL;
+    if (blk < 0 || blk >= (int)c->hb.size()) return fail(c, TRANS_EINVAL, "block out of range");
+    if (dir < 1 || dir > 3) return fail(c, TRANS_EINVAL, "dir must be 1, 2 or 3");
+    const HostBlock& h = c->hb[blk];
+    const int n = dir == 1 ? h.ni : (dir == 2 ? h.nj : h.nk);
+    if (index < 1 || index > n) return fail(c, TRANS_EINVAL, "face index out of range");
+    BlkInfo b{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, 0, 0, 0, 0};
+    KLAUNCH(c, "k_plane_integrals", k_plane_integrals, dim3(1), dim3(256), c->f, b, dir, index, c->d_mon);
+    double r[4];
+    HIPCHK(c, hipMemcpyAsync(r, c->d_mon, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    out->mass_flow = r[0];
+    out->area = r[1];
+    out->p0_mass_avg = r[2];
+    out->t0_mass_avg = r[3];
     return 0;
 }
 

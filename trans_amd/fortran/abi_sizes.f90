@@ -11,6 +11,7 @@ program abi_sizes
     type(trans_geom_view) :: e
     type(trans_state_view) :: f
     type(trans_resid) :: g
+    type(trans_plane_integrals) :: h
     print '(a, i0)', "trans_params ", c_sizeof(a)
     print '(a, i0)', "trans_patch ", c_sizeof(b)
     print '(a, i0)', "trans_block_desc ", c_sizeof(c)
@@ -18,4 +19,5 @@ program abi_sizes
     print '(a, i0)', "trans_geom_view ", c_sizeof(e)
     print '(a, i0)', "trans_state_view ", c_sizeof(f)
     print '(a, i0)', "trans_resid ", c_sizeof(g)
+    print '(a, i0)', "trans_plane_integrals ", c_sizeof(h)
 end program abi_sizes

@@ -55,6 +55,10 @@ module trans_hip_mod
         type(c_ptr) :: markwall(6) = c_null_ptr
     end type trans_geom_view
 
+    type, bind(C), public :: trans_plane_integrals
+        real(c_double) :: mass_flow, area, p0_mass_avg, t0_mass_avg
+    end type trans_plane_integrals
+
     type, bind(C), public :: trans_state_view
         type(trans_fview) :: v, pp, t, rmiu, wt0, wt1, wt2
     end type trans_state_view
@@ -158,6 +162,13 @@ module trans_hip_mod
             integer(c_int), value :: n_iter
             type(trans_resid), intent(out) :: per_block(*)
         end function
+        integer(c_int) function trans_hip_plane_integrals(ctx, blk, dir, index, out) &
+                bind(C, name="trans_hip_plane_integrals")
+            import :: c_int, c_ptr, trans_plane_integrals
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk, dir, index
+            type(trans_plane_integrals), intent(out) :: out
+        end function
         integer(c_int) function trans_hip_sync(ctx) bind(C, name="trans_hip_sync")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
@@ -167,7 +178,7 @@ module trans_hip_mod
         trans_hip_set_iteration, trans_hip_set_geometry, trans_hip_set_state, trans_hip_get_state, &
         trans_hip_get_field, trans_hip_spectral_radius, trans_hip_bc, trans_hip_space, &
         trans_hip_residual, trans_hip_lusgs_update, trans_hip_rk_update, trans_hip_update_u, trans_hip_iterate, &
-        trans_hip_sync
+        trans_hip_plane_integrals, trans_hip_sync
 
 contains
 

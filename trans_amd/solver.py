@@ -15,6 +15,15 @@ class TransError(RuntimeError):
         self.rc = rc
 
 
+def stage_performance(inlet, outlet, gamma=1.4):
+    """Total-pressure ratio and isentropic efficiency between two plane integrals:
+    pi = p0_out / p0_in, eta = (pi^((gamma-1)/gamma) - 1) / (T0_out / T0_in - 1)."""
+    pi = outlet["p0_mass_avg"] / inlet["p0_mass_avg"]
+    tau = outlet["t0_mass_avg"] / inlet["t0_mass_avg"]
+    return {"pressure_ratio": pi, "temperature_ratio": tau,
+            "efficiency": (pi ** ((gamma - 1.0) / gamma) - 1.0) / (tau - 1.0) if tau != 1.0 else float("nan")}
+
+
 class Solver:
     def __init__(self, blocks, params: capi.TransParams, ki: int = 1):
         self.lib = capi.load_library()
@@ -81,6 +90,13 @@ class Solver:
         res = (capi.TransResid * len(self.blocks))()
         self._check(self.lib.trans_hip_rk_update(self.ctx, itr_chld, res))
         return [r.as_dict() for r in res]
+
+    def plane_integrals(self, blk, direction, index):
+        """Mass flow, area and mass-averaged p0, T0 over face plane `index` of direction
+        `direction` (1 i, 2 j, 3 k) of block `blk` (0-based)."""
+        out = capi.TransPlaneIntegrals()
+        self._check(self.lib.trans_hip_plane_integrals(self.ctx, blk, direction, index, C.byref(out)))
+        return out.as_dict()
 
     def update_u(self):
         self._check(self.lib.trans_hip_update_u(self.ctx))
