@@ -53,6 +53,9 @@ def lib():
         L.ora_bc.argtypes = [PP, C.c_int, PB, C.c_int, C.c_int]
         L.ora_spaceinterp.argtypes = [PP, C.c_int, PB]
         L.ora_convect_muscl.argtypes = [PP, PB]
+        L.ora_convect_muscl2.argtypes = [PP, PB]
+        L.ora_muscl2.argtypes = [C.c_double, C.c_double]
+        L.ora_muscl2.restype = C.c_double
         L.ora_convect_roe.argtypes = [PP, PB]
         L.ora_lusgs.argtypes = [PP, PB]
         L.ora_update_v.argtypes = [PP, C.c_int, C.c_int, PB]

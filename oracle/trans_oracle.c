@@ -634,10 +634,20 @@ double ora_alimiter(double var1, double var2)
            fabs(copysign(one, var1) + copysign(one, var2));
 }
 
-/* Space/CONVECT_MUSCL.F90:40-323 */
-void ora_convect_muscl(const trans_params* p, ora_block* b)
+/* Space/UTIL_MUSCL.F90:21-35 (MUSCL2, the smoothness-weighted slope of CONVECT_MUSCL2). */
+double ora_muscl2(double var1, double var2)
 {
-    (void)p;
+    const double epsm = 1.e-6;
+    double smooth = (2. * var1 * var2 + epsm) / (var1 * var1 + var2 * var2 + epsm);
+    return ((1. + smooth) * var1 + (1. - smooth) * var2) * smooth / 2.;
+}
+
+typedef double (*ora_limiter)(double, double);
+
+/* Space/CONVECT_MUSCL.F90:40-323 (ALIMITER) and Space/CONVECT_MUSCL2.F90:30-290 (MUSCL2):
+ * the same reconstruction from the face averages with a different limiter. */
+static void convect_muscl_lim(ora_block* b, ora_limiter lim)
+{
     const int NI = b->ni, NJ = b->nj, NK = b->nk;
     const int NI1 = NI - 1, NJ1 = NJ - 1, NK1 = NK - 1;
     for (int i = 0; i <= NI + 1; i++)
@@ -666,16 +676,28 @@ void ora_convect_muscl(const trans_params* p, ora_block* b)
                     for (int L = 1; L <= 5; L++) {
                         double DR = VL(d, L, ip, jp, kp) - q[L - 1];
                         double DL = q[L - 1] - VR(d, L, i, j, k);
-                        VL(d, L, ip, jp, kp) = q[L - 1] + ora_alimiter(DR, DL);
-                        VR(d, L, i, j, k) = q[L - 1] - ora_alimiter(DL, DR);
+                        VL(d, L, ip, jp, kp) = q[L - 1] + lim(DR, DL);
+                        VR(d, L, i, j, k) = q[L - 1] - lim(DL, DR);
                     }
                     double rrr = RAD(i, j, k);
                     double DR = RCCL(d, ip, jp, kp) - rrr;
                     double DL = rrr - RCCR(d, i, j, k);
-                    RCCL(d, ip, jp, kp) = rrr + ora_alimiter(DR, DL);
-                    RCCR(d, i, j, k) = rrr - ora_alimiter(DL, DR);
+                    RCCL(d, ip, jp, kp) = rrr + lim(DR, DL);
+                    RCCR(d, i, j, k) = rrr - lim(DL, DR);
                 }
     }
+}
+
+void ora_convect_muscl(const trans_params* p, ora_block* b)
+{
+    (void)p;
+    convect_muscl_lim(b, ora_alimiter);
+}
+
+void ora_convect_muscl2(const trans_params* p, ora_block* b)
+{
+    (void)p;
+    convect_muscl_lim(b, ora_muscl2);
 }
 
 /* Space/CONVECT_Roe_Original.F90:242-407 (CONVECT_Roe_Common, IF_PRECONDITION = 0) */
@@ -876,6 +898,29 @@ void ora_lusgs(const trans_params* p, ora_block* b)
             }
 }
 
+/* WD+ limiter of the k-omega update (Time/TIME_Update_V.F90:821-858,
+ * Time/TIME_RK_V_Update_New.F90:195-230): omega floor from the production, k cap, then
+ * mu_t from a second omega floor on the deviatoric strain that is not stored back. */
+static void wdp_limit(const trans_params* p, ora_block* b, int I, int J, int K, double pRo, double RmiuL,
+                      double prod, double ARTk, double ARTO)
+{
+    double dudx = DQ(1, I, J, K), dudy = DQ(2, I, J, K), dudz = DQ(3, I, J, K);
+    double dvdx = DQ(4, I, J, K), dvdy = DQ(5, I, J, K), dvdz = DQ(6, I, J, K);
+    double dwdx = DQ(7, I, J, K), dwdy = DQ(8, I, J, K), dwdz = DQ(9, I, J, K);
+    double Sii = dudx - (dudx + dvdy + dwdz) / 3.;
+    double Sjj = dvdy - (dudx + dvdy + dwdz) / 3.;
+    double Skk = dwdz - (dudx + dvdy + dwdz) / 3.;
+    double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
+    double Skl = sqrt(1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk + 2.0 * Sij * Sij + 2.0 * Sik * Sik +
+                      2.0 * Sjk * Sjk);
+    ARTO = fmax(ARTO, pow(pRo * fabs(prod) / (p->prod_lmt * p->beta_star), 0.5) / p->ref);
+    ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo);
+    V(6, I, J, K) = ARTk;
+    V(7, I, J, K) = ARTO;
+    ARTO = fmax(ARTO, 7. / 8. * pRo * sqrt(Skl * Skl * 2. / 0.09) / p->ref);
+    RMIU(I, J, K) = ARTk / ARTO * pRo;
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Time/TIME_Update_V.F90:1-917 (non-preconditioned path; the smoothing passes only write
  * to the scratch CC and are omitted) */
@@ -990,6 +1035,8 @@ void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b)
                     V(6, I, J, K) = ARTk;
                     V(7, I, J, K) = ARTO;
                     RMIU(I, J, K) = Cmua * ARTk / ARTO * pRo;
+                } else if (p->kind_model == 2) {
+                    wdp_limit(p, b, I, J, K, pRo, RmiuL, prod, ARTk, ARTO);
                 }
             }
     b->res.drsa = DRSa; b->res.dua = DUa; b->res.ima = Ima; b->res.jma = Jma; b->res.kma = Kma;
@@ -1039,6 +1086,7 @@ void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk)
         memset(b->D, 0, sizeof(double) * (size_t)n);
         if (p->if_euler == 0) ora_diffusion(p, b);
         if (if_turb) ora_turb_diffusion(p, b);
+        if (p->kind_spatial == 1) ora_convect_muscl2(p, b);
         if (p->kind_spatial == 2) ora_convect_muscl(p, b);
         if (p->kind_spatial == 3) ora_convect_mdcd(p, b);
         if (if_turb) ora_turb_convect_muscl(p, b);
@@ -1158,6 +1206,8 @@ void ora_rk_update_v(const trans_params* p, int ki, int if_turb, ora_block* b)
                     V(6, I, J, K) = ARTk;
                     V(7, I, J, K) = ARTO;
                     RMIU(I, J, K) = Cmua * ARTk / ARTO * pRo;
+                } else if (p->kind_model == 2) {
+                    wdp_limit(p, b, I, J, K, pRo, RmiuL, prod, ARTk, ARTO);
                 }
             }
     double DRSk = 0.0, DUk = 0.0, DRSo = 0.0, DUo = 0.0;

@@ -49,6 +49,8 @@ void ora_spectral_radius(const trans_params* p, int ki, ora_block* b);
 void ora_bc(const trans_params* p, int if_turb, ora_block* blocks, int nblk, int sweep);
 void ora_spaceinterp(const trans_params* p, int if_turb, ora_block* b);
 void ora_convect_muscl(const trans_params* p, ora_block* b);
+void ora_convect_muscl2(const trans_params* p, ora_block* b);
+double ora_muscl2(double var1, double var2);
 void ora_convect_roe(const trans_params* p, ora_block* b);
 void ora_lusgs(const trans_params* p, ora_block* b);
 void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b);

@@ -259,7 +259,10 @@ void ora_turb_diffusion(const trans_params* p, ora_block* b)
                     double RmiuTL = all * RMIU(im, jm, km) + alr * RMIU(i, j, k);
                     if (wall) RmiuTL = 0.0;
                     double SigkAB, SigoAB;
-                    {
+                    if (p->kind_model == 2) { /* WD+ (:168-170): unblended sigmas */
+                        SigkAB = p->sigk1;
+                        SigoAB = p->sigo1;
+                    } else {
                         double DEN0 = V(1, i, j, k);
                         double TKK0 = V(6, i, j, k) / DEN0, TOO0 = V(7, i, j, k) / DEN0;
                         double DEN1 = V(1, im, jm, km);
@@ -546,6 +549,55 @@ void ora_turb_lusgs_sst(const trans_params* p, ora_block* b)
                 double prod = (2.0 * (dudx * dudx + dvdy * dvdy + dwdz * dwdz) + sqr(dudy + dvdx) +
                                sqr(dvdz + dwdy) + sqr(dwdx + dudz) - 2.0 / 3.0 * Div * Div);
                 double cdif = dtkdx * dtodx + dtkdy * dtody + dtkdz * dtodz;
+                if (p->kind_model == 2) { /* WD+ (Time/TURBTWO_TIME_LUSGS_SST.F90:418-547) */
+                    double W[3][3], S[3][3];
+                    W[0][0] = 0.; W[1][1] = 0.; W[2][2] = 0.;
+                    W[0][1] = 0.5 * (dudy - dvdx);
+                    W[0][2] = 0.5 * (dudz - dwdx);
+                    W[1][2] = 0.5 * (dvdz - dwdy);
+                    W[1][0] = -W[0][1];
+                    W[2][0] = -W[0][2];
+                    W[2][1] = -W[1][2];
+                    S[0][0] = 0.5 * (dudx + dudx);
+                    S[1][1] = 0.5 * (dvdy + dvdy);
+                    S[2][2] = 0.5 * (dwdz + dwdz);
+                    S[0][1] = 0.5 * (dudy + dvdx);
+                    S[0][2] = 0.5 * (dudz + dwdx);
+                    S[1][2] = 0.5 * (dvdz + dwdy);
+                    S[1][0] = S[0][1];
+                    S[2][0] = S[0][2];
+                    S[2][1] = S[1][2];
+                    double tprod = fmin(prod, Votg) * RMIU(I, J, K) / Ref;
+                    double ARo = V(1, I, J, K), ARTk = V(6, I, J, K), ARTO = V(7, I, J, K);
+                    double tprodl1 = p->prod_lmt * p->beta_star * ARTk * ARTO / ARo * Ref;
+                    tprod = fmin(tprod, tprodl1);
+                    double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+                    double SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p->beta_star * ARTO / ARo * Ref;
+                    double Source = SourceP + SourceN1 * ARTk;
+                    double CnegTk = fabs(SourceN1);
+                    D(6, I, J, K) = D(6, I, J, K) + Source;
+                    tprod = fmin(prod, Votg) / Ref;
+                    double WWSg = 0.;
+                    for (int II = 0; II < 3; II++)
+                        for (int JJ = 0; JJ < 3; JJ++)
+                            for (int KK = 0; KK < 3; KK++) WWSg = WWSg + W[II][JJ] * W[JJ][KK] * S[KK][II];
+                    double aK_omg = fabs(WWSg / pow(0.09 * ARTO / ARo * Ref, 3.));
+                    double FBeta = (1. + 85. * aK_omg) / (1. + 100. * aK_omg);
+                    double Beta_06 = p->beta1 * FBeta;
+                    double dltCD = cdif <= 0. ? 0. : 0.125;
+                    SourceP = p->gama1 * ARo * tprod - 2.0 / 3.0 * p->gama1 * fmin(Div, 0.0) * ARTO;
+                    SourceN1 = -2.0 / 3.0 * p->gama1 * fmax(Div, 0.0) - Beta_06 * ARTO / ARo * Ref;
+                    Source = SourceP + SourceN1 * ARTO;
+                    double SourceCross = dltCD * ARo * ARo / ARTO / Ref * cdif;
+                    Source = Source + SourceCross;
+                    double CnegTO = 2.0 * fabs(SourceN1);
+                    D(7, I, J, K) = D(7, I, J, K) + Source;
+                    double stept = 2.0 * DTM(I, J, K) / (3.0 * DTM(I, J, K) / dtm + 2.0);
+                    F(1, I, J, K) = 1. + (RDS(1, I, J, K) + RDS(2, I, J, K) + RDS(3, I, J, K)) * (stept / VOL(I, J, K)) +
+                                    (CnegTk + CnegTO) * stept;
+                    F(2, I, J, K) = F(1, I, J, K);
+                    continue;
+                }
                 double tprod = fmin(prod, Votg) * RMIU(I, J, K) / Ref;
                 double ARo = V(1, I, J, K), ARTk = V(6, I, J, K), ARTO = V(7, I, J, K);
                 double tprodl1 = p->prod_lmt * p->beta_star * ARTk * ARTO / ARo * Ref;

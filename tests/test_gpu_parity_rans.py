@@ -80,7 +80,7 @@ def _post_bc_case(spatial, roe):
     return blocks, prm, oc
 
 
-@pytest.mark.parametrize("spatial,roe", [(3, 3), (2, 1), (3, 1), (2, 3)])
+@pytest.mark.parametrize("spatial,roe", [(3, 3), (2, 1), (3, 1), (2, 3), (1, 1), (1, 3)])
 def test_single_residual(spatial, roe):
     """One residual evaluation (gradients, shock sensor, BC(0), viscous and k-omega diffusion,
     reconstruction, inviscid and k-omega fluxes) from identical post-BC states.
@@ -192,4 +192,39 @@ def test_fp32_implicit_operator():
         e = relerr(dv_g, dv_o)
         print(n, "FP32 increment relerr", e)
         assert np.all(e < 1e-5)
+    sv.close()
+
+
+@pytest.mark.parametrize("rk", [0, 1])
+def test_wdp_kw_model(rk):
+    """WD+ k-omega (Kind_Model = 2): unblended diffusion sigmas, WD+ sources with the
+    vortex-stretching function, and the WD+ limiter of the LU-SGS or RK update, one update
+    from identical inputs at 1e-12 with norms and argmax locations exact."""
+    print(f"case: flat plate, WD+ k-omega, IF_RK={rk}", flush=True)
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
+    org.Kind_Model = 2
+    if rk:
+        org.IF_RK, org.CFL_Sub_start, org.CFL_Sub_end = 1, 0.8, 0.8
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=11, turb=True)
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    oc.residual(1)
+    sv.residual()
+    for n, b in enumerate(blocks):
+        e = relerr(interior(sv.get_field(n, capi.FIELD_D), b), interior(oc[n]["D"], b))
+        assert np.all(e < TOL), (n, "D", e)
+    if rk:
+        ores, gres = oc.rk_update(1, 1), sv.rk_update(1)
+    else:
+        ores, gres = oc.lusgs_update(1), sv.lusgs_update()
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        for nm in ("v", "pp", "t", "rmiu"):
+            e = relerr(interior(st[nm], b), interior(oc[n][nm], b))
+            assert np.all(np.asarray(e) < TOL), (n, nm, e)
+        for key in ("drsa", "dua", "drsk", "duk", "drso", "duo"):
+            assert abs(gres[n][key] - ores[n][key]) <= TOL * abs(ores[n][key]), (n, key)
+        for key in ("ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo"):
+            assert gres[n][key] == ores[n][key], (n, key)
     sv.close()

@@ -514,11 +514,12 @@ def test_inlet_freestream_state():
         np.testing.assert_allclose(oc[0]["pp"][g, J, K], 1 / (1.4 * 0.25), rtol=1e-12)
 
 
-@pytest.mark.parametrize("routine,spatial", [("ora_convect_muscl", 2), ("ora_convect_mdcd", 3)])
+@pytest.mark.parametrize("routine,spatial", [("ora_convect_muscl", 2), ("ora_convect_muscl2", 1),
+                                             ("ora_convect_mdcd", 3)])
 def test_reconstruction_exact_on_linear_data(routine, spatial):
     """On a uniform grid, u, p and T linear in (x, y, z): both reconstructed states of every
-    interior face equal the exact face value (ALIMITER of equal slopes is the slope; MDCD is
-    exact on linear data)."""
+    interior face equal the exact face value (ALIMITER and MUSCL2 of equal slopes are the
+    slope; MDCD is exact on linear data)."""
     b, prm, edges = _box(ni=12, nj=10, nk=9, ratio=1.0)
     cu, ct = (0.5, 0.3, -0.2, 0.1), (1.0, 0.1, 0.05, -0.02)
     _set_linear(b, prm, cu, ct, edges)
@@ -842,3 +843,111 @@ def test_plane_integrals_face_state_is_linear_interpolation():
     for jf in (2, 4, 6):
         r = oc.plane_integrals(0, 2, jf)
         assert r["mass_flow"] == pytest.approx((a0 + a1 * y1[jf - 1]) * area, rel=1e-12), jf
+
+
+@pytest.mark.parametrize("a", [1.0, -0.3, 2.5e-3, 40.0])
+def test_muscl2_equal_slopes_and_blend(a):
+    """MUSCL2 (Space/UTIL_MUSCL.F90:21-35): equal slopes give the slope; its weight
+    s = (2 a b + eps) / (a^2 + b^2 + eps) blends ((1 + s) a + (1 - s) b) s / 2, so b = 0
+    gives a (eps / (a^2 + eps))^2 (1 + eps/(a^2+eps)) / 2 -> ~0 for |a| >> sqrt(eps), and
+    the two argument orders weight the first argument more."""
+    L = lib()
+    assert L.ora_muscl2(a, a) == pytest.approx(a, rel=1e-14)
+    eps = 1e-6
+    sm = eps / (a * a + eps)
+    assert L.ora_muscl2(a, 0.0) == pytest.approx((1 + sm) * a * sm / 2, rel=1e-12)
+    b = 0.5 * a
+    assert abs(L.ora_muscl2(a, b) - b) > abs(L.ora_muscl2(b, a) - b)
+
+
+# ---------------------------------------------------------------------------------------
+# WD+ k-omega (Kind_Model = 2): Time/TURBTWO_TIME_LUSGS_SST.F90:418-547, limiter of
+# Time/TIME_Update_V.F90:821-858
+
+def _kw_box(**deck):
+    b, prm, edges = _box(ni=5, nj=5, nk=5, IF_Euler=0, IF_Turb_Org=1, **deck)
+    oc = OracleCase([b], prm)
+    V = oc[0]["v"]
+    V[1:4] = 0.0
+    oc[0]["wt0"][...] = V
+    oc[0]["wt1"][...] = V
+    oc[0]["Rds"][...] = 0.0
+    oc[0]["Dtm"][...] = 0.7
+    oc[0]["D"][...] = 0.0
+    oc[0]["DQdxyz"][...] = 0.0
+    return b, prm, oc
+
+
+def test_wdp_sources_at_zero_strain():
+    """WD+ at rest: the omega destruction uses beta_1 exactly (chi_omega = 0 gives
+    f_beta = 1, no blending), no cross diffusion (dk.domega = 0), k as SST."""
+    b, prm, oc = _kw_box(Kind_Model=2)
+    oc.call("ora_turb_lusgs_sst", 0)
+    V = oc[0]["v"]
+    rho, rk, rw = V[0], V[5], V[6]
+    om = rw / rho
+    Re = prm.ref
+    stept = 2.0 * 0.7 / (3.0 * 0.7 / prm.dt_lusgs_main + 2.0)
+    f = 1.0 + (prm.beta_star * om * Re + 2.0 * prm.beta1 * om * Re) * stept
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    np.testing.assert_allclose(oc[0]["D"][5][I, J, K], (-prm.beta_star * om * Re * rk * stept / f)[I, J, K], rtol=1e-12)
+    np.testing.assert_allclose(oc[0]["D"][6][I, J, K], (-prm.beta1 * om * Re * rw * stept / f)[I, J, K], rtol=1e-12)
+
+
+def test_wdp_vortex_stretching_in_2d():
+    """For a two-dimensional velocity gradient the vortex-stretching invariant of WD+ is
+    W_ij W_jk S_ki = -w12^2 (du/dx + dv/dy), w12 = (du/dy - dv/dx)/2, which sets
+    chi = |.|/(0.09 omega Re)^3 and f_beta = (1 + 85 chi)/(1 + 100 chi) in the omega
+    destruction; with mu_t = 0 the k production vanishes."""
+    b, prm, oc = _kw_box(Kind_Model=2)
+    dudx, dudy, dvdx, dvdy = 0.8, 2.5, -1.1, 0.3
+    Q = oc[0]["DQdxyz"]
+    Q[0], Q[1], Q[3], Q[4] = dudx, dudy, dvdx, dvdy
+    oc[0]["rmiu"][...] = 0.0
+    oc.call("ora_turb_lusgs_sst", 0)
+    V = oc[0]["v"]
+    rho, rk, rw = V[0], V[5], V[6]
+    om = rw / rho
+    Re = prm.ref
+    w12 = 0.5 * (dudy - dvdx)
+    chi = abs(-w12 * w12 * (dudx + dvdy)) / (0.09 * om * Re) ** 3
+    fb = (1 + 85 * chi) / (1 + 100 * chi)
+    div = dudx + dvdy
+    votg = (dudy - dvdx) ** 2
+    prod = 2 * (dudx ** 2 + dvdy ** 2) + (dudy + dvdx) ** 2 - 2.0 / 3.0 * div * div
+    sk_n = -2.0 / 3.0 * max(div, 0) - prm.beta_star * om * Re
+    so_n = -2.0 / 3.0 * prm.gama1 * max(div, 0) - prm.beta1 * fb * om * Re
+    so = prm.gama1 * rho * min(prod, votg) / Re + so_n * rw
+    sk = sk_n * rk
+    stept = 2.0 * 0.7 / (3.0 * 0.7 / prm.dt_lusgs_main + 2.0)
+    f = 1.0 + (abs(sk_n) + 2.0 * abs(so_n)) * stept
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    np.testing.assert_allclose(oc[0]["D"][5][I, J, K], (sk * stept / f)[I, J, K], rtol=1e-12)
+    np.testing.assert_allclose(oc[0]["D"][6][I, J, K], (so * stept / f)[I, J, K], rtol=1e-12)
+
+
+def test_wdp_limiter_at_zero_strain():
+    """WD+ limiter with zero gradients: omega keeps its update, k is capped at
+    1e5 mu_L omega / rho, and mu_t = rho k / omega (no SST a1 limiter)."""
+    b, prm, (x1, y1, z1) = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1, Kind_Model=2)
+    oc = OracleCase([b], prm)
+    rng = np.random.Generator(np.random.MT19937(21))
+    V, D = oc[0]["v"], oc[0]["D"]
+    shp = V[0].shape
+    V[6] = 10.0 + rng.uniform(0, 1, shp)
+    V[5] = np.where(rng.uniform(0, 1, shp) < 0.5, 1e-6, 1e6) * V[6] / V[0]
+    D[...] = 0.0
+    D[6] = 0.5 * rng.uniform(-1, 1, shp)
+    v0, d = V.copy(), D.copy()
+    oc[0]["DQdxyz"][...] = 0.0
+    oc.call("ora_update_v", 0, 1, 1)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    rho = oc[0]["v"][0]
+    w1 = np.maximum(np.where(d[6] >= 0, v0[6] + d[6], v0[6] * np.exp(d[6] / (v0[6] + 1e-10))), 1e-10)
+    tt = oc[0]["t"]
+    mul = (1 + prm.csthlnd) / (tt * prm.csth1 + prm.csthlnd) * (prm.csth1 * tt) ** 1.5
+    kl = np.minimum(v0[5], 1e5 * mul * w1 / rho)
+    assert np.any(kl[I, J, K] < v0[5][I, J, K]) and np.any(kl[I, J, K] == v0[5][I, J, K])
+    np.testing.assert_allclose(oc[0]["v"][6][I, J, K], w1[I, J, K], rtol=1e-15)
+    np.testing.assert_allclose(oc[0]["v"][5][I, J, K], kl[I, J, K], rtol=1e-12)
+    np.testing.assert_allclose(oc[0]["rmiu"][I, J, K], (kl / w1 * rho)[I, J, K], rtol=1e-12)

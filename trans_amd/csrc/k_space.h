@@ -68,6 +68,14 @@ __device__ __forceinline__ double alimiter(double var1, double var2)
            fabs(copysign(1.0, var1) + copysign(1.0, var2));
 }
 
+// Space/UTIL_MUSCL.F90:21-35 (MUSCL2): smoothness-weighted blend of the two slopes.
+__device__ __forceinline__ double muscl2(double var1, double var2)
+{
+    const double epsm = 1.e-6;
+    const double smooth = (2. * var1 * var2 + epsm) / (var1 * var1 + var2 * var2 + epsm);
+    return ((1. + smooth) * var1 + (1. - smooth) * var2) * smooth / 2.;
+}
+
 // Space/CONVECT_Roe_Original.F90:242-407, IF_PRECONDITION = 0.  Returns the flux per unit
 // area (droR..dreR).
 __device__ __forceinline__ void roe_common(const trans_params& p, double PLo, double PRo,

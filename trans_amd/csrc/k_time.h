@@ -536,8 +536,8 @@ __device__ __forceinline__ void norm_combine(double& s, double& m, int& a, doubl
     }
 }
 
-// k/omega update with positivity damping and the SST limiter on eddy viscosity
-// (Time/TIME_Update_V.F90:709-817, Time/TIME_RK_V_Update_New.F90:104-200): increments from
+// k/omega update with positivity damping and the SST (or WD+) limiter on eddy viscosity
+// (Time/TIME_Update_V.F90:709-858, Time/TIME_RK_V_Update_New.F90:104-230): increments from
 // D(6:7), new k, omega and Rmiu written back.  pRo is the updated density and RmiuL the
 // laminar viscosity, which the two callers evaluate in their own forms.
 __device__ __forceinline__ void sst_kw_update(const trans_params& p, const Fields& f, long c, double pRo,
@@ -578,6 +578,19 @@ __device__ __forceinline__ void sst_kw_update(const trans_params& p, const Field
         f.V[5 * tot + c] = ARTk;
         f.V[6 * tot + c] = ARTO;
         f.Rmiu[c] = Cmua * ARTk / ARTO * pRo;
+    } else if (p.kind_model == 2) {  // WD+ (Time/TIME_Update_V.F90:821-858)
+        const double Sii = dudx - (dudx + dvdy + dwdz) / 3.;
+        const double Sjj = dvdy - (dudx + dvdy + dwdz) / 3.;
+        const double Skk = dwdz - (dudx + dvdy + dwdz) / 3.;
+        const double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
+        const double Skl = sqrt(1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk + 2.0 * Sij * Sij +
+                                2.0 * Sik * Sik + 2.0 * Sjk * Sjk);
+        ARTO = fmax(ARTO, pow(pRo * fabs(prod) / (p.prod_lmt * p.beta_star), 0.5) / p.ref);
+        ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo);
+        f.V[5 * tot + c] = ARTk;
+        f.V[6 * tot + c] = ARTO;
+        ARTO = fmax(ARTO, 7. / 8. * pRo * sqrt(Skl * Skl * 2. / 0.09) / p.ref);
+        f.Rmiu[c] = ARTk / ARTO * pRo;
     }
 }
 

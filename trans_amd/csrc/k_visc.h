@@ -252,8 +252,9 @@ __global__ void __launch_bounds__(256) k_visc_face(trans_params p, int if_turb, 
     arg1 = fmin(arg1, 4.0 * Den * p.sigo2 * TKK / CDKO / distfc / distfc);
     const double a2 = arg1 * arg1;
     const double fc1c = tanh(a2 * a2);
-    const double SigkAB = fc1c * p.sigk1 + (1. - fc1c) * p.sigk2;
-    const double SigoAB = fc1c * p.sigo1 + (1. - fc1c) * p.sigo2;
+    // WD+ (Kind_Model = 2, :168-170) takes the unblended sigmas
+    const double SigkAB = p.kind_model == 2 ? p.sigk1 : fc1c * p.sigk1 + (1. - fc1c) * p.sigk2;
+    const double SigoAB = p.kind_model == 2 ? p.sigo1 : fc1c * p.sigo1 + (1. - fc1c) * p.sigo2;
     const double vmute = RmiuLt + RmiuTL * SigkAB;
     const double vmuoq = RmiuLt + RmiuTL * SigoAB;
     G[4 * tot + c0] = (vmute * (gk[0] * Dx + gk[1] * Dy + gk[2] * Dz)) / p.ref;
@@ -352,7 +353,7 @@ __device__ __forceinline__ void ausmpw_common(const trans_params& p, double PLo,
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-// One thread per face of direction DIR: reconstruction (SPATIAL 2 = CONVECT_MUSCL,
+// One thread per face of direction DIR: reconstruction (SPATIAL 1 = CONVECT_MUSCL2, 2 = CONVECT_MUSCL,
 // 3 = CONVECT_MDCD), inviscid flux (ROE 1 = CONVECT_Roe_Original, 3 = CONVECT_AUSMPW_Plus)
 // and, when turbulent, TURBTWO_CONVECT_MUSCL + TURBTWO_CONVECT_Upwind.  G[DIR-1] receives
 // flux*|S| (5 comps) and the area-weighted k, omega fluxes (comps 5, 6).
@@ -426,8 +427,13 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
             const double avm = face_avg(am2, am1, qm2.q[L], qm1.q[L]);
             const double av0 = face_avg(am1, a0, qm1.q[L], q0.q[L]);
             const double avp = face_avg(a0, ap1, q0.q[L], qp1.q[L]);
-            qL[L] = qm1.q[L] + alimiter(av0 - qm1.q[L], qm1.q[L] - avm);
-            qR[L] = q0.q[L] - alimiter(q0.q[L] - av0, avp - q0.q[L]);
+            if (SPATIAL == 1) {  // Space/CONVECT_MUSCL2.F90
+                qL[L] = qm1.q[L] + muscl2(av0 - qm1.q[L], qm1.q[L] - avm);
+                qR[L] = q0.q[L] - muscl2(q0.q[L] - av0, avp - q0.q[L]);
+            } else {  // Space/CONVECT_MUSCL.F90
+                qL[L] = qm1.q[L] + alimiter(av0 - qm1.q[L], qm1.q[L] - avm);
+                qR[L] = q0.q[L] - alimiter(q0.q[L] - av0, avp - q0.q[L]);
+            }
         }
     }
     const double* S = f.SD + (long)d * tot;
@@ -588,35 +594,67 @@ __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const
     const double ARo = f.V[c], ARTk = f.V[5 * tot + c], ARTO = f.V[6 * tot + c];
     const double tprodl1 = p.prod_lmt * p.beta_star * ARTk * ARTO / ARo * Ref;
     tprod = fmin(tprod, tprodl1);
-    const double RmiuL = sutherland(p, f.T[c]);
-    const double Fdes = 1.0;
-    const double DistN = f.Dst[c];
-    const double CDKO = fmax(2.0 * ARo * p.sigo2 / (ARTO / ARo) * cdif, 1.0e-10);
-    double arg1 = fmax(sqrt(ARTk / ARo) / p.beta_star / (ARTO / ARo) / DistN / Ref,
-                       500.0 * RmiuL / DistN / DistN / ARTO / Ref / Ref);
-    arg1 = fmin(arg1, 4.0 * p.sigo2 * ARTk / CDKO / DistN / DistN);
-    {
-        const double a2 = arg1 * arg1;
-        arg1 = tanh(a2 * a2);
+    double CnegTk, CnegTO;
+    if (p.kind_model == 2) {  // WD+ (Time/TURBTWO_TIME_LUSGS_SST.F90:418-547)
+        double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+        double SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p.beta_star * ARTO / ARo * Ref;
+        d6 = d6 + (SourceP + SourceN1 * ARTk);
+        CnegTk = fabs(SourceN1);
+        tprod = fmin(prod, Votg) / Ref;
+        // W(II,JJ) W(JJ,KK) S(KK,II) summed with II outermost, zero entries included
+        const double w12 = 0.5 * (dudy - dvdx), w13 = 0.5 * (dudz - dwdx), w23 = 0.5 * (dvdz - dwdy);
+        const double W[3][3] = {{0., w12, w13}, {-w12, 0., w23}, {-w13, -w23, 0.}};
+        const double s12 = 0.5 * (dudy + dvdx), s13 = 0.5 * (dudz + dwdx), s23 = 0.5 * (dvdz + dwdy);
+        const double S[3][3] = {{0.5 * (dudx + dudx), s12, s13},
+                                {s12, 0.5 * (dvdy + dvdy), s23},
+                                {s13, s23, 0.5 * (dwdz + dwdz)}};
+        double WWSg = 0.;
+#pragma unroll
+        for (int II = 0; II < 3; II++)
+#pragma unroll
+            for (int JJ = 0; JJ < 3; JJ++)
+#pragma unroll
+                for (int KK = 0; KK < 3; KK++) WWSg = WWSg + W[II][JJ] * W[JJ][KK] * S[KK][II];
+        const double aK_omg = fabs(WWSg / pow(0.09 * ARTO / ARo * Ref, 3.));
+        const double FBeta = (1. + 85. * aK_omg) / (1. + 100. * aK_omg);
+        const double Beta_06 = p.beta1 * FBeta;
+        const double dltCD = cdif <= 0. ? 0. : 0.125;
+        SourceP = p.gama1 * ARo * tprod - 2.0 / 3.0 * p.gama1 * fmin(Div, 0.0) * ARTO;
+        SourceN1 = -2.0 / 3.0 * p.gama1 * fmax(Div, 0.0) - Beta_06 * ARTO / ARo * Ref;
+        const double SourceCross = dltCD * ARo * ARo / ARTO / Ref * cdif;
+        d7 = d7 + (SourceP + SourceN1 * ARTO + SourceCross);
+        CnegTO = 2.0 * fabs(SourceN1);
+    } else {  // SST (:139-415)
+        const double RmiuL = sutherland(p, f.T[c]);
+        const double Fdes = 1.0;
+        const double DistN = f.Dst[c];
+        const double CDKO = fmax(2.0 * ARo * p.sigo2 / (ARTO / ARo) * cdif, 1.0e-10);
+        double arg1 = fmax(sqrt(ARTk / ARo) / p.beta_star / (ARTO / ARo) / DistN / Ref,
+                           500.0 * RmiuL / DistN / DistN / ARTO / Ref / Ref);
+        arg1 = fmin(arg1, 4.0 * p.sigo2 * ARTk / CDKO / DistN / DistN);
+        {
+            const double a2 = arg1 * arg1;
+            arg1 = tanh(a2 * a2);
+        }
+        double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+        SourceP = SourceP * 1.0;  // f_r1 = 1 (Kind_RotMod = 0)
+        double SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p.beta_star * ARTO / ARo * Ref * Fdes;
+        double Source = SourceP + SourceN1 * ARTk;
+        d6 = d6 + Source;
+        CnegTk = fabs(SourceN1);
+        tprod = fmin(prod, Votg) / Ref;
+        const double GamaAB = p.gama1 * arg1 + p.gama2 * (1.0 - arg1);
+        const double BetaAB = p.beta1 * arg1 + p.beta2 * (1.0 - arg1);
+        SourceP = GamaAB * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
+        SourceP = SourceP * 1.0;
+        SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) - BetaAB * ARTO / ARo * Ref;
+        Source = SourceP + SourceN1 * ARTO;
+        double SourceCross = 2.0 * (1 - arg1) * ARo * p.sigo2 / (ARTO / ARo) * cdif / Ref;
+        SourceCross = copysign(1.0, SourceCross) * fmin(fabs(SourceCross), fabs(SourceP) * 0.5);
+        Source = Source + SourceCross;
+        d7 = d7 + Source;
+        CnegTO = 2.0 * fabs(SourceN1);
     }
-    double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
-    SourceP = SourceP * 1.0;  // f_r1 = 1 (Kind_RotMod = 0)
-    double SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p.beta_star * ARTO / ARo * Ref * Fdes;
-    double Source = SourceP + SourceN1 * ARTk;
-    const double CnegTk = fabs(SourceN1);
-    d6 = d6 + Source;
-    tprod = fmin(prod, Votg) / Ref;
-    const double GamaAB = p.gama1 * arg1 + p.gama2 * (1.0 - arg1);
-    const double BetaAB = p.beta1 * arg1 + p.beta2 * (1.0 - arg1);
-    SourceP = GamaAB * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
-    SourceP = SourceP * 1.0;
-    SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) - BetaAB * ARTO / ARo * Ref;
-    Source = SourceP + SourceN1 * ARTO;
-    double SourceCross = 2.0 * (1 - arg1) * ARo * p.sigo2 / (ARTO / ARo) * cdif / Ref;
-    SourceCross = copysign(1.0, SourceCross) * fmin(fabs(SourceCross), fabs(SourceP) * 0.5);
-    Source = Source + SourceCross;
-    d7 = d7 + Source;
-    const double CnegTO = 2.0 * fabs(SourceN1);
     const double Dt = f.Dtm[c];
     const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
     const double f1t = 1. + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * (stept / vol) + (CnegTk + CnegTO) * stept;

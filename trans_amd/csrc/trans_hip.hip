@@ -241,12 +241,13 @@ static int check_params(trans_ctx* c, const trans_params* p)
     if (p->kind_rotmod != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_RotMod must be 0");
     if (p->cfl_sub_step < 1) return fail(c, TRANS_EINVAL, "CFL_Sub_step must be >= 1");
     if (p->if_rk != 0 && p->if_rk != 1) return fail(c, TRANS_EINVAL, "IF_RK must be 0 or 1");
-    if (p->kind_spatial != 2 && p->kind_spatial != 3)
-        return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 2 (MUSCL) or 3 (MDCD)");
+    if (p->kind_spatial < 1 || p->kind_spatial > 3)
+        return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 1 (MUSCL2), 2 (MUSCL) or 3 (MDCD)");
     if (p->kind_roe != 1 && p->kind_roe != 3)
         return fail(c, TRANS_EUNSUPPORTED, "Kind_Roe must be 1 (Roe) or 3 (AUSMPW+)");
     if (p->if_euler == 0 && p->if_turb_org == 1) {
-        if (p->kind_model != 1) return fail(c, TRANS_EUNSUPPORTED, "Kind_Model must be 1 (SST)");
+        if (p->kind_model != 1 && p->kind_model != 2)
+            return fail(c, TRANS_EUNSUPPORTED, "Kind_Model must be 1 (SST) or 2 (WD+ k-omega)");
         if (p->kind_hybrid != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_Hybrid must be 0 (RANS)");
         if (p->if_comppress != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_Comppress must be 0");
     }
@@ -669,7 +670,12 @@ int trans_hip_space(trans_ctx* c)
         KLAUNCH_ON(c, c->aux[2], "k_visc_face<3>", k_visc_face<3>, dim3(c->seg_face[2].n), dim3(WG), c->p, if_turb,
                    c->f, c->d_blk, c->seg_face[2].d);
     }
-    if (c->p.kind_spatial == 2) {
+    if (c->p.kind_spatial == 1) {
+        if (c->p.kind_roe == 1)
+            launch_conv<1, 1>(c, if_turb);
+        else
+            launch_conv<1, 3>(c, if_turb);
+    } else if (c->p.kind_spatial == 2) {
         if (c->p.kind_roe == 1)
             launch_conv<2, 1>(c, if_turb);
         else
