@@ -40,11 +40,11 @@ extern "C" {
  */
 typedef struct trans_params {
     /* switches */
-    int kind_spatial;    /* 2 = MUSCL/ALIMITER, 3 = MDCD */
+    int kind_spatial;    /* 1 = MUSCL2, 2 = MUSCL/ALIMITER, 3 = MDCD, 4 = MDCDHYB */
     int kind_roe;        /* 1 = Roe, 3 = AUSMPW+ */
     int if_euler;
     int if_turb_org;
-    int kind_model;      /* 1 = SST */
+    int kind_model;      /* 1 = SST, 2 = WD+ k-omega */
     int kind_dual;       /* 0, 1, 2 */
     int kchld;           /* sub-iterations per outer iteration */
     int if_rk;           /* 0 = LU-SGS, 1 = SSP-RK3 */
@@ -73,6 +73,8 @@ typedef struct trans_params {
     double sigk2, sigo2, beta2, gama2;
     double p0in, t0in, pouthub, ps00, ts00;
     double rkf, rof, rmiuf;
+    double f_limiter1, f_limiter2; /* deck: DES blending floor; MDCDHYB switch to MUSCL at sensor >= f_limiter2 */
+    double alscale;                /* deck: DES length-scale floor of the blending function */
 } trans_params;
 
 /*

@@ -72,6 +72,7 @@ def lib():
         L.ora_diffusion.argtypes = [PP, PB]
         L.ora_turb_diffusion.argtypes = [PP, PB]
         L.ora_convect_mdcd.argtypes = [PP, PB]
+        L.ora_convect_mdcdhyb.argtypes = [PP, PB]
         L.ora_convect_ausmpw.argtypes = [PP, PB]
         L.ora_turb_lusgs_sst.argtypes = [PP, PB]
         L.ora_ausmpw_common.argtypes = [PP] + [C.c_double] * 14 + [DP]

@@ -1089,6 +1089,7 @@ void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk)
         if (p->kind_spatial == 1) ora_convect_muscl2(p, b);
         if (p->kind_spatial == 2) ora_convect_muscl(p, b);
         if (p->kind_spatial == 3) ora_convect_mdcd(p, b);
+        if (p->kind_spatial == 4) ora_convect_mdcdhyb(p, b);
         if (if_turb) ora_turb_convect_muscl(p, b);
         if (p->kind_roe == 1) ora_convect_roe(p, b);
         if (p->kind_roe == 3) ora_convect_ausmpw(p, b);

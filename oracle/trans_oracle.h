@@ -60,6 +60,7 @@ void ora_diff_dqdxyz(const trans_params* p, int if_turb, ora_block* b);
 void ora_diffusion(const trans_params* p, ora_block* b);
 void ora_turb_diffusion(const trans_params* p, ora_block* b);
 void ora_convect_mdcd(const trans_params* p, ora_block* b);
+void ora_convect_mdcdhyb(const trans_params* p, ora_block* b);
 void ora_turb_convect_muscl(const trans_params* p, ora_block* b);
 void ora_convect_ausmpw(const trans_params* p, ora_block* b);
 void ora_turb_convect_upwind(const trans_params* p, ora_block* b);

@@ -330,6 +330,45 @@ double ora_mdcd_rec(double gama, const double s[6], int right)
     return mdcd_rec(gama, s[5], s[4], s[3], s[2], s[1], s[0]);
 }
 
+/* Space/CONVECT_MDCDHYB.F90:1-477: the MDCD reconstruction of u, v, w, p, T with the
+ * six-cell shock maximum as sensor in all three directions, switched to a half-slope
+ * ALIMITER reconstruction from cell values where the sensor reaches F_limiter2; the face
+ * radii keep the SPACEINTERP averages. */
+double ora_alimiter(double var1, double var2);
+void ora_convect_mdcdhyb(const trans_params* p, ora_block* b)
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    for (int d = 1; d <= 3; d++) {
+        const int di = d == 1, dj = d == 2, dk = d == 3;
+        const int ihi = di ? b->ni : NI1, jhi = dj ? b->nj : NJ1, khi = dk ? b->nk : NK1;
+        for (int k = 1; k <= khi; k++)
+            for (int j = 1; j <= jhi; j++)
+                for (int i = 1; i <= ihi; i++) {
+                    double q[5][6], sensor = 0.0;
+                    for (int n = 0; n < 6; n++) {
+                        const int ii = i + di * (n - 3), jj = j + dj * (n - 3), kk = k + dk * (n - 3);
+                        q[0][n] = V(2, ii, jj, kk) / V(1, ii, jj, kk);
+                        q[1][n] = V(3, ii, jj, kk) / V(1, ii, jj, kk);
+                        q[2][n] = V(4, ii, jj, kk) / V(1, ii, jj, kk);
+                        q[3][n] = PP(ii, jj, kk);
+                        q[4][n] = T(ii, jj, kk);
+                        const double sh = SHOCK(ii, jj, kk);
+                        sensor = n == 0 ? sh : fmax(sensor, sh);
+                    }
+                    const double gama = p->gama_min + sensor * (p->gama_max - p->gama_min);
+                    for (int L = 1; L <= 5; L++) {
+                        const double* v = q[L - 1];
+                        VL(d, L, i, j, k) = mdcd_rec(gama, v[0], v[1], v[2], v[3], v[4], v[5]);
+                        if (sensor >= p->f_limiter2)
+                            VL(d, L, i, j, k) = v[2] + 0.5 * ora_alimiter(v[3] - v[2], v[2] - v[1]);
+                        VR(d, L, i, j, k) = mdcd_rec(gama, v[5], v[4], v[3], v[2], v[1], v[0]);
+                        if (sensor >= p->f_limiter2)
+                            VR(d, L, i, j, k) = v[3] - 0.5 * ora_alimiter(v[3] - v[2], v[4] - v[3]);
+                    }
+                }
+    }
+}
+
 /* Space/CONVECT_MDCD.F90:1-668.  The k-direction radius block stands outside its loop
  * (:450-521) and writes one ghost corner only: k-face radii keep the face averages. */
 void ora_convect_mdcd(const trans_params* p, ora_block* b)

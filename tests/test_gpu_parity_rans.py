@@ -27,12 +27,13 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def _plate(spatial=3, roe=3, seed=20261202, **kw):
+def _plate(spatial=3, roe=3, seed=20261202, flim2=0.0, **kw):
     args = dict(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
     args.update(kw)
     blocks, org, glob = cases.flat_plate(**args)
     org.Kind_Spatial = spatial
     org.Kind_Roe = roe
+    org.F_limiter2 = flim2
     prm = solver.prepare_case(blocks, org, glob)
     perturb(blocks, prm, seed=seed, turb=True)
     return blocks, prm
@@ -67,10 +68,10 @@ def test_bc_interfaces_and_ghosts():
     sv.close()
 
 
-def _post_bc_case(spatial, roe):
+def _post_bc_case(spatial, roe, flim2=0.0):
     """Oracle spectral radius + BC(-1) on the perturbed case; the blocks then carry the
     oracle's ghost layers, so that device and oracle residuals start from identical bits."""
-    blocks, prm = _plate(spatial, roe)
+    blocks, prm = _plate(spatial, roe, flim2=flim2)
     oc = OracleCase(blocks, prm)
     oc.spectral_radius(1)
     oc.bc(1, -1)
@@ -80,8 +81,9 @@ def _post_bc_case(spatial, roe):
     return blocks, prm, oc
 
 
-@pytest.mark.parametrize("spatial,roe", [(3, 3), (2, 1), (3, 1), (2, 3), (1, 1), (1, 3)])
-def test_single_residual(spatial, roe):
+@pytest.mark.parametrize("spatial,roe,flim2", [(3, 3, 0.0), (2, 1, 0.0), (3, 1, 0.0), (2, 3, 0.0), (1, 1, 0.0),
+                                               (1, 3, 0.0), (4, 3, 0.0), (4, 3, 2.0), (4, 1, 2.0)])
+def test_single_residual(spatial, roe, flim2):
     """One residual evaluation (gradients, shock sensor, BC(0), viscous and k-omega diffusion,
     reconstruction, inviscid and k-omega fluxes) from identical post-BC states.
 
@@ -89,8 +91,8 @@ def test_single_residual(spatial, roe):
     (factor |sgn a + sgn b| / 2), so a last-bit difference in a ghost value from a far-field
     pow() can move a face state by 2/3 of the other argument.  The BC test above bounds
     those ghost differences separately."""
-    print(f"case: flat plate, residual, Kind_Spatial={spatial} Kind_Roe={roe}", flush=True)
-    blocks, prm, oc = _post_bc_case(spatial, roe)
+    print(f"case: flat plate, residual, Kind_Spatial={spatial} Kind_Roe={roe} F_limiter2={flim2}", flush=True)
+    blocks, prm, oc = _post_bc_case(spatial, roe, flim2)
     sv = Solver(blocks, prm, ki=1)
     sv.spectral_radius()
     sv.space()

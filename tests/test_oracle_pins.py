@@ -514,18 +514,20 @@ def test_inlet_freestream_state():
         np.testing.assert_allclose(oc[0]["pp"][g, J, K], 1 / (1.4 * 0.25), rtol=1e-12)
 
 
-@pytest.mark.parametrize("routine,spatial", [("ora_convect_muscl", 2), ("ora_convect_muscl2", 1),
-                                             ("ora_convect_mdcd", 3)])
-def test_reconstruction_exact_on_linear_data(routine, spatial):
+@pytest.mark.parametrize("routine,spatial,flim2", [("ora_convect_muscl", 2, 0.0), ("ora_convect_muscl2", 1, 0.0),
+                                                   ("ora_convect_mdcd", 3, 0.0), ("ora_convect_mdcdhyb", 4, 2.0),
+                                                   ("ora_convect_mdcdhyb", 4, 0.0)])
+def test_reconstruction_exact_on_linear_data(routine, spatial, flim2):
     """On a uniform grid, u, p and T linear in (x, y, z): both reconstructed states of every
     interior face equal the exact face value (ALIMITER and MUSCL2 of equal slopes are the
-    slope; MDCD is exact on linear data)."""
-    b, prm, edges = _box(ni=12, nj=10, nk=9, ratio=1.0)
+    slope; MDCD is exact on linear data; MDCDHYB is MDCD below F_limiter2 and the half-slope
+    ALIMITER step from the upwind cell, also exact, at or above it)."""
+    b, prm, edges = _box(ni=12, nj=10, nk=9, ratio=1.0, F_limiter2=flim2)
     cu, ct = (0.5, 0.3, -0.2, 0.1), (1.0, 0.1, 0.05, -0.02)
     _set_linear(b, prm, cu, ct, edges)
     oc = OracleCase([b], prm)
     oc.call("ora_spaceinterp", 0, 0)
-    if spatial == 3:
+    if spatial >= 3:
         oc.call("ora_diff_dqdxyz", 0, 0)
     oc.call(routine, 0)
     x1, y1, z1 = edges
@@ -951,3 +953,46 @@ def test_wdp_limiter_at_zero_strain():
     np.testing.assert_allclose(oc[0]["v"][6][I, J, K], w1[I, J, K], rtol=1e-15)
     np.testing.assert_allclose(oc[0]["v"][5][I, J, K], kl[I, J, K], rtol=1e-12)
     np.testing.assert_allclose(oc[0]["rmiu"][I, J, K], (kl / w1 * rho)[I, J, K], rtol=1e-12)
+
+
+def _random_reconstruction(routine, flim2, seed=31):
+    b, prm, edges = _box(ni=10, nj=9, nk=8, ratio=1.1, F_limiter2=flim2, IF_Euler=0)
+    rng = np.random.Generator(np.random.MT19937(seed))
+    V = b["v"]
+    V[1] = V[0] * (0.5 + rng.uniform(-0.2, 0.2, V[0].shape))
+    V[2] = V[0] * rng.uniform(-0.2, 0.2, V[0].shape)
+    b["pp"][...] *= 1 + 0.1 * rng.uniform(-1, 1, V[0].shape)
+    oc = OracleCase([b], prm)
+    oc.call("ora_spaceinterp", 0, 0)
+    oc.call("ora_diff_dqdxyz", 0, 0)
+    oc[0]["shock"][...] = rng.uniform(0, 1, oc[0]["shock"].shape)
+    oc.call(routine, 0)
+    return b, oc
+
+
+def test_mdcdhyb_equals_mdcd_where_it_does_not_switch():
+    """With F_limiter2 above every sensor value MDCDHYB performs CONVECT_MDCD's
+    reconstruction of u, v, w, p, T face by face (the i and j directions, where MDCD has no
+    radius quirk)."""
+    b, oc = _random_reconstruction("ora_convect_mdcdhyb", 2.0)
+    _, om = _random_reconstruction("ora_convect_mdcd", 2.0)
+    for d in (0, 1):
+        for L in range(5):
+            np.testing.assert_array_equal(oc[0]["VL"][d + 3 * L], om[0]["VL"][d + 3 * L])
+            np.testing.assert_array_equal(oc[0]["VR"][d + 3 * L], om[0]["VR"][d + 3 * L])
+
+
+def test_mdcdhyb_limited_states_are_monotone():
+    """With F_limiter2 = 0 every face takes the half-slope ALIMITER step from the upwind
+    cell, which stays between the two cells adjacent to the face (0.5 lim(a, b) lies between
+    0 and a for same-sign slopes, and lim = 0 at extrema)."""
+    b, oc = _random_reconstruction("ora_convect_mdcdhyb", 0.0)
+    V = oc[0]["v"]
+    u = V[1] / V[0]
+    I, J, K = sl(2, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)  # i-faces 2..NI-1
+    um = u[sl(1, b.ni - 2), J, K]
+    up = u[I, J, K]
+    lo, hi = np.minimum(um, up), np.maximum(um, up)
+    for A in (oc[0]["VL"][0], oc[0]["VR"][0]):
+        a = A[I, J, K]
+        assert np.all(a >= lo - 1e-15) and np.all(a <= hi + 1e-15)

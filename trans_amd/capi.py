@@ -37,6 +37,7 @@ class TransParams(C.Structure):
         ("p0in", C.c_double), ("t0in", C.c_double), ("pouthub", C.c_double),
         ("ps00", C.c_double), ("ts00", C.c_double),
         ("rkf", C.c_double), ("rof", C.c_double), ("rmiuf", C.c_double),
+        ("f_limiter1", C.c_double), ("f_limiter2", C.c_double), ("alscale", C.c_double),
     ]
 
 

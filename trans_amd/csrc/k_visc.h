@@ -354,7 +354,7 @@ __device__ __forceinline__ void ausmpw_common(const trans_params& p, double PLo,
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 // One thread per face of direction DIR: reconstruction (SPATIAL 1 = CONVECT_MUSCL2, 2 = CONVECT_MUSCL,
-// 3 = CONVECT_MDCD), inviscid flux (ROE 1 = CONVECT_Roe_Original, 3 = CONVECT_AUSMPW_Plus)
+// 3 = CONVECT_MDCD, 4 = CONVECT_MDCDHYB), inviscid flux (ROE 1 = CONVECT_Roe_Original, 3 = CONVECT_AUSMPW_Plus)
 // and, when turbulent, TURBTWO_CONVECT_MUSCL + TURBTWO_CONVECT_Upwind.  G[DIR-1] receives
 // flux*|S| (5 comps) and the area-weighted k, omega fluxes (comps 5, 6).
 template <int DIR, int SPATIAL, int ROE>
@@ -378,7 +378,7 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
     const int d = DIR - 1;
     const double* Al = f.Alagm + (long)d * tot;
     double qL[6], qR[6];
-    if (SPATIAL == 3) {
+    if (SPATIAL == 3 || SPATIAL == 4) {
         // six cells f-3..f+2; the shock sensor's ghost values are those of the nearest
         // interior cell (Space/DIFF_DqDxyz.F90:388-397)
         double sv[6][6];
@@ -401,20 +401,29 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
             const double sh = f.shock[cs];
             sensor = n == 0 ? sh : fmax(sensor, sh);
         }
-        if (DIR == 3) sensor = fmax(sensor, 0.0);
+        if (SPATIAL == 3 && DIR == 3) sensor = fmax(sensor, 0.0);
         const double gama = p.gama_min + sensor * (p.gama_max - p.gama_min);
 #pragma unroll
         for (int L = 0; L < 5; L++) {
             qL[L] = mdcd_rec(gama, sv[L][0], sv[L][1], sv[L][2], sv[L][3], sv[L][4], sv[L][5]);
             qR[L] = mdcd_rec(gama, sv[L][5], sv[L][4], sv[L][3], sv[L][2], sv[L][1], sv[L][0]);
         }
-        if (DIR != 3) {
-            qL[5] = mdcd_rec(gama, sv[5][0], sv[5][1], sv[5][2], sv[5][3], sv[5][4], sv[5][5]);
-            qR[5] = mdcd_rec(gama, sv[5][5], sv[5][4], sv[5][3], sv[5][2], sv[5][1], sv[5][0]);
-        } else {
-            // k-face radii keep the face average (CONVECT_MDCD.F90:450-521 stands outside its loop)
+        if (SPATIAL == 4 && sensor >= p.f_limiter2) {
+            // Space/CONVECT_MDCDHYB.F90: half-slope ALIMITER from cell values near shocks
+#pragma unroll
+            for (int L = 0; L < 5; L++) {
+                qL[L] = sv[L][2] + 0.5 * alimiter(sv[L][3] - sv[L][2], sv[L][2] - sv[L][1]);
+                qR[L] = sv[L][3] - 0.5 * alimiter(sv[L][3] - sv[L][2], sv[L][4] - sv[L][3]);
+            }
+        }
+        if (SPATIAL == 4 || DIR == 3) {
+            // MDCDHYB leaves all face radii, and MDCD the k-face radii (CONVECT_MDCD.F90:450-521
+            // stands outside its loop), at the face average
             qL[5] = face_avg(Al[c0 - st], Al[c0], sv[5][2], sv[5][3]);
             qR[5] = qL[5];
+        } else {
+            qL[5] = mdcd_rec(gama, sv[5][0], sv[5][1], sv[5][2], sv[5][3], sv[5][4], sv[5][5]);
+            qR[5] = mdcd_rec(gama, sv[5][5], sv[5][4], sv[5][3], sv[5][2], sv[5][1], sv[5][0]);
         }
     } else {
         const Prim qm2 = load_prim(f, c0 - 2 * st);

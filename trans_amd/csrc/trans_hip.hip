@@ -241,8 +241,8 @@ static int check_params(trans_ctx* c, const trans_params* p)
     if (p->kind_rotmod != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_RotMod must be 0");
     if (p->cfl_sub_step < 1) return fail(c, TRANS_EINVAL, "CFL_Sub_step must be >= 1");
     if (p->if_rk != 0 && p->if_rk != 1) return fail(c, TRANS_EINVAL, "IF_RK must be 0 or 1");
-    if (p->kind_spatial < 1 || p->kind_spatial > 3)
-        return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 1 (MUSCL2), 2 (MUSCL) or 3 (MDCD)");
+    if (p->kind_spatial < 1 || p->kind_spatial > 4)
+        return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 1 (MUSCL2), 2 (MUSCL), 3 (MDCD) or 4 (MDCDHYB)");
     if (p->kind_roe != 1 && p->kind_roe != 3)
         return fail(c, TRANS_EUNSUPPORTED, "Kind_Roe must be 1 (Roe) or 3 (AUSMPW+)");
     if (p->if_euler == 0 && p->if_turb_org == 1) {
@@ -680,11 +680,16 @@ int trans_hip_space(trans_ctx* c)
             launch_conv<2, 1>(c, if_turb);
         else
             launch_conv<2, 3>(c, if_turb);
-    } else {
+    } else if (c->p.kind_spatial == 3) {
         if (c->p.kind_roe == 1)
             launch_conv<3, 1>(c, if_turb);
         else
             launch_conv<3, 3>(c, if_turb);
+    } else {
+        if (c->p.kind_roe == 1)
+            launch_conv<4, 1>(c, if_turb);
+        else
+            launch_conv<4, 3>(c, if_turb);
     }
     join_aux(c, 3);
     KLAUNCH(c, "k_accumulate_visc", k_accumulate_visc, dim3(c->seg_cell.n), dim3(WG), c->p, c->p.if_euler,

@@ -77,6 +77,7 @@ program box_demo
     prm%sigk2 = 1.0d0; prm%sigo2 = 0.856d0; prm%beta2 = 0.0828d0; prm%gama2 = 0.44d0
     prm%p0in = 0.0d0; prm%t0in = 1.0d0; prm%pouthub = 0.0d0; prm%ps00 = 0.0d0; prm%ts00 = 1.0d0
     prm%rkf = 1.5d-4; prm%rof = 1.2d0 * 1.5d-4 / 1.5d-5; prm%rmiuf = 1.5d-5
+    prm%f_limiter1 = 0.0d0; prm%f_limiter2 = 0.0d0; prm%alscale = 1.0d0
 
     ! uniform freestream: rho = 1, u = 1, p = 1/(gamma M^2), T = gamma M^2 p / rho = 1
     p_inf = prm%ppf

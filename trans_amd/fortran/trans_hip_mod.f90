@@ -24,6 +24,7 @@ module trans_hip_mod
         real(c_double) :: prod_lmt, visratio, fsti, a1, akapa, beta_star
         real(c_double) :: sigk1, sigo1, beta1, gama1, sigk2, sigo2, beta2, gama2
         real(c_double) :: p0in, t0in, pouthub, ps00, ts00, rkf, rof, rmiuf
+        real(c_double) :: f_limiter1, f_limiter2, alscale
     end type trans_params
 
     ! ConnectivityStruct fields (Module_Block.F90:4-23); target_block/target_conn 0-based

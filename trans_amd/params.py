@@ -177,6 +177,7 @@ def make_params(org: OrgDeck, glob: GlobalInfo, implicit_fp32: int = 0) -> capi.
     p.sigk2, p.sigo2, p.beta2, p.gama2 = sigk2, sigo2, beta2, gama2
     p.p0in, p.t0in, p.pouthub, p.ps00, p.ts00 = p0in, t0in, pouthub, ps00, ts00
     p.rkf, p.rof, p.rmiuf = rkf, rof, rmiuf
+    p.f_limiter1, p.f_limiter2, p.alscale = org.F_limiter1, org.F_limiter2, org.Alscale
     return p
 
 
