@@ -41,7 +41,7 @@ extern "C" {
 typedef struct trans_params {
     /* switches */
     int kind_spatial;    /* 1 = MUSCL2, 2 = MUSCL/ALIMITER, 3 = MDCD, 4 = MDCDHYB */
-    int kind_roe;        /* 1 = Roe, 3 = AUSMPW+ */
+    int kind_roe;        /* 1 = Roe, 2 = rotated Roe, 3 = AUSMPW+ */
     int if_euler;
     int if_turb_org;
     int kind_model;      /* 1 = SST, 2 = WD+ k-omega */
@@ -164,6 +164,7 @@ typedef struct trans_resid {
 #define TRANS_FIELD_DTM 2    /* Dtm */
 #define TRANS_FIELD_DQDXYZ 3 /* DQdxyz(18,...) */
 #define TRANS_FIELD_SHOCK 4  /* shock sensor */
+#define TRANS_FIELD_FSCHEME 5 /* DES blending Fscheme of the rotated Roe flux (Kind_Roe = 2) */
 
 /* Integrals over one constant-index face plane of a block (all faces of direction dir at
  * face index `index`, i.e. between cells index-1 and index).  The face state is the

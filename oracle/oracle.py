@@ -57,6 +57,8 @@ def lib():
         L.ora_muscl2.argtypes = [C.c_double, C.c_double]
         L.ora_muscl2.restype = C.c_double
         L.ora_convect_roe.argtypes = [PP, PB]
+        L.ora_convect_roe_rotated.argtypes = [PP, PB]
+        L.ora_roe_rotated_face.argtypes = [PP] + [C.c_double] * 17 + [DP]
         L.ora_lusgs.argtypes = [PP, PB]
         L.ora_update_v.argtypes = [PP, C.c_int, C.c_int, PB]
         L.ora_update_u.argtypes = [PP, PB]

@@ -31,6 +31,7 @@
 #define DTM(i, j, k) b->Dtm[IX(i, j, k)]
 #define DQ(L, i, j, k) b->DQdxyz[(L)-1 + 18 * IX(i, j, k)]
 #define SHOCK(i, j, k) b->shock[IX(i, j, k)]
+#define FSCHEME(i, j, k) b->Fscheme[IX(i, j, k)]
 #define F_R1(i, j, k) b->f_r1[IX(i, j, k)]
 #define ALENG(i, j, k) b->aLeng[IX(i, j, k)]
 

@@ -796,6 +796,127 @@ void ora_convect_roe(const trans_params* p, ora_block* b)
     }
 }
 
+/* Space/CONVECT_Roe_Rotated.F90:597-752 (its own Roe_Common, IF_PRECONDITION = 0): adds the
+ * Roe dissipation through the sub-face (sav1, sav2, sav3), |.| = sav, weighted by Funscheme,
+ * to the area-weighted flux acc[5]. */
+static void roe_rot_common(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
+                           double URo, double VLo, double VRo, double WLo, double WRo, double sav1, double sav2,
+                           double sav3, double sav, double Funscheme, double rLo, double rRo, double acc[5])
+{
+    double rcpcv = p->ppf;
+    double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
+    double ga = 1.4, ga1 = 0.4;
+    double VAMLo = ULo * ULo + VLo * VLo + WLo * WLo, VAMRo = URo * URo + VRo * VRo + WRo * WRo;
+    double HLo = PLo * ga / (DENLo * ga1) + 0.5 * VAMLo - 0.5 * (rLo * rLo);
+    double HRo = PRo * ga / (DENRo * ga1) + 0.5 * VAMRo - 0.5 * (rRo * rRo);
+    double rrorl = sqrt(DENRo / DENLo), rrorlp1 = 1.0 + rrorl;
+    double rm = sqrt(DENRo * DENLo);
+    double um = (ULo + URo * rrorl) / rrorlp1, vm = (VLo + VRo * rrorl) / rrorlp1;
+    double wm = (WLo + WRo * rrorl) / rrorlp1, hm = (HLo + HRo * rrorl) / rrorlp1;
+    double vm2 = um * um + vm * vm + wm * wm;
+    double rccm = (rLo + rRo * rrorl) / rrorlp1;
+    double am2 = ga1 * fabs(hm - 0.5 * vm2 + 0.5 * (rccm * rccm));
+    double am = sqrt(am2);
+    double sav1n = sav1 / sav, sav2n = sav2 / sav, sav3n = sav3 / sav;
+    double unormaln = (sav1n * um + sav2n * vm + sav3n * wm);
+    double anormaln = am;
+    double eig1 = fabs(unormaln), eig2 = fabs(unormaln + anormaln), eig3 = fabs(unormaln - anormaln);
+    double alf = 0.0, ubetac2 = 1.0, amrev = am;
+    double dunormaln = (sav1n * (URo - ULo) + sav2n * (VRo - VLo) + sav3n * (WRo - WLo));
+    double astar = (eig2 + eig3) / 2.0;
+    double Mstar = (eig2 - eig3) / 2.0 / amrev;
+    double dW1 = DENRo - DENLo, dU1 = URo - ULo, dU2 = VRo - VLo, dU3 = WRo - WLo, dP1 = PRo - PLo;
+    double dW2 = rm * dU1 + dW1 * um, dW3 = rm * dU2 + dW1 * vm, dW4 = rm * dU3 + dW1 * wm;
+    double dW5 = dP1 / ga1 + 0.5 * vm2 * dW1 + rm * um * dU1 + rm * vm * dU2 + rm * wm * dU3;
+    double dUroe = Mstar * dunormaln + (astar - (1 - 2.0 * alf) * eig1 - alf * unormaln * Mstar) * (PRo - PLo) / rm /
+                                           (ubetac2 * am2);
+    double dProe = Mstar * (PRo - PLo) + (astar - eig1 + alf * unormaln * Mstar) * rm * dunormaln;
+    double r1[5];
+    r1[0] = eig1 * dW1 + dUroe * rm;
+    r1[1] = eig1 * dW2 + dUroe * rm * um + dProe * sav1n;
+    r1[2] = eig1 * dW3 + dUroe * rm * vm + dProe * sav2n;
+    r1[3] = eig1 * dW4 + dUroe * rm * wm + dProe * sav3n;
+    r1[4] = eig1 * dW5 + dUroe * rm * hm + dProe * unormaln;
+    for (int L = 0; L < 5; L++) acc[L] = acc[L] + 0.5 * r1[L] * sav * Funscheme;
+}
+
+/* Space/CONVECT_Roe_Rotated.F90:84-176 for one face: central flux through the face, then
+ * the Roe dissipation through the face (velocity jump below Tiny), or through its parts
+ * along and across the velocity jump.  The grid-velocity term vibc of :169 is never set in
+ * the routine; it is taken as 0.  out = area-weighted flux droR..dreR. */
+void ora_roe_rotated_face(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
+                          double URo, double VLo, double VRo, double WLo, double WRo, double s1, double s2,
+                          double s3, double grad, double Funscheme, double rLo, double rRo, double out[5])
+{
+    double rcpcv = p->ppf;
+    double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
+    double ga = 1.4, ga1 = 0.4;
+    double VAMLo = ULo * ULo + VLo * VLo + WLo * WLo, VAMRo = URo * URo + VRo * VRo + WRo * WRo;
+    double HLo = PLo * ga / (DENLo * ga1) + 0.5 * VAMLo - 0.5 * (rLo * rLo);
+    double HRo = PRo * ga / (DENRo * ga1) + 0.5 * VAMRo - 0.5 * (rRo * rRo);
+    double sav = grad;
+    double sav1n = s1 / sav, sav2n = s2 / sav, sav3n = s3 / sav;
+    double ulnormaln = (sav1n * ULo + sav2n * VLo + sav3n * WLo);
+    double urnormaln = (sav1n * URo + sav2n * VRo + sav3n * WRo);
+    double rulnormaln = DENLo * ulnormaln, rurnormaln = DENRo * urnormaln;
+    const double vibc = 0.0;
+    out[0] = -0.5 * (rulnormaln + rurnormaln) * sav;
+    out[1] = -0.5 * (rulnormaln * ULo + rurnormaln * URo + sav1n * (PLo + PRo)) * sav;
+    out[2] = -0.5 * (rulnormaln * VLo + rurnormaln * VRo + sav2n * (PLo + PRo)) * sav;
+    out[3] = -0.5 * (rulnormaln * WLo + rurnormaln * WRo + sav3n * (PLo + PRo)) * sav;
+    out[4] = -0.5 * (rulnormaln * HLo + rurnormaln * HRo - vibc * (PLo + PRo)) * sav;
+    double DeltaU = sqrt((URo - ULo) * (URo - ULo) + (VRo - VLo) * (VRo - VLo) + (WRo - WLo) * (WRo - WLo));
+    if (DeltaU <= p->tiny) {
+        roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, s1, s2, s3, grad, Funscheme, rLo, rRo,
+                       out);
+    } else {
+        sav1n = (URo - ULo) / DeltaU;
+        sav2n = (VRo - VLo) / DeltaU;
+        sav3n = (WRo - WLo) / DeltaU;
+        sav = sav1n * s1 + sav2n * s2 + sav3n * s3;
+        sav1n = copysign(1.0, sav) * sav1n;
+        sav2n = copysign(1.0, sav) * sav2n;
+        sav3n = copysign(1.0, sav) * sav3n;
+        sav = fabs(sav);
+        double a1 = sav1n * sav, a2 = sav2n * sav, a3 = sav3n * sav;
+        double b1 = s1 - a1, b2 = s2 - a2, b3 = s3 - a3;
+        sav = fmax(sqrt(a1 * a1 + a2 * a2 + a3 * a3), p->tiny);
+        roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, a1, a2, a3, sav, Funscheme, rLo, rRo,
+                       out);
+        sav = fmax(sqrt(b1 * b1 + b2 * b2 + b3 * b3), p->tiny);
+        roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, b1, b2, b3, sav, Funscheme, rLo, rRo,
+                       out);
+    }
+}
+
+/* Space/CONVECT_Roe_Rotated.F90:1-595: the scatter of CONVECT_Roe_Original with the rotated
+ * face flux and Funscheme = (Fscheme of the two cells)/2. */
+void ora_convect_roe_rotated(const trans_params* p, ora_block* b)
+{
+    const int NI = b->ni, NJ = b->nj, NK = b->nk;
+    const int NI1 = NI - 1, NJ1 = NJ - 1, NK1 = NK - 1;
+    for (int d = 1; d <= 3; d++) {
+        const int di = d == 1, dj = d == 2, dk = d == 3;
+        const int ihi = di ? NI : NI1, jhi = dj ? NJ : NJ1, khi = dk ? NK : NK1;
+        for (int k = 1; k <= khi; k++)
+            for (int j = 1; j <= jhi; j++)
+                for (int i = 1; i <= ihi; i++) {
+                    double f[5];
+                    double Funscheme = (FSCHEME(i - di, j - dj, k - dk) + FSCHEME(i, j, k)) / 2.;
+                    ora_roe_rotated_face(p, VL(d, 4, i, j, k), VR(d, 4, i, j, k), VL(d, 5, i, j, k),
+                                         VR(d, 5, i, j, k), VL(d, 1, i, j, k), VR(d, 1, i, j, k),
+                                         VL(d, 2, i, j, k), VR(d, 2, i, j, k), VL(d, 3, i, j, k),
+                                         VR(d, 3, i, j, k), SD(d, 1, i, j, k), SD(d, 2, i, j, k),
+                                         SD(d, 3, i, j, k), GRAD(d, i, j, k), Funscheme,
+                                         RCCL(d, i, j, k) * p->omega1, RCCR(d, i, j, k) * p->omega1, f);
+                    for (int L = 1; L <= 5; L++) {
+                        D(L, i, j, k) = D(L, i, j, k) - f[L - 1];
+                        D(L, i - di, j - dj, k - dk) = D(L, i - di, j - dj, k - dk) + f[L - 1];
+                    }
+                }
+    }
+}
+
 /* ------------------------------------------------------------------------------------ */
 /* Time/TIME_LUSGS.F90:696-768 (LUSGS_ADQ, gamma = 1.4, grid velocity 0) */
 void ora_lusgs_adq(double pVx, double pVy, double pVz, double aSDx, double aSDy, double aSDz,
@@ -1092,6 +1213,7 @@ void ora_residual(const trans_params* p, int ki, ora_block* blocks, int nblk)
         if (p->kind_spatial == 4) ora_convect_mdcdhyb(p, b);
         if (if_turb) ora_turb_convect_muscl(p, b);
         if (p->kind_roe == 1) ora_convect_roe(p, b);
+        if (p->kind_roe == 2) ora_convect_roe_rotated(p, b);
         if (p->kind_roe == 3) ora_convect_ausmpw(p, b);
         if (if_turb) ora_turb_convect_upwind(p, b);
     }

@@ -10,6 +10,13 @@
  * Layout: every array of a block spans -2:N+2 in each direction (N = points), i.e.
  * N+5 entries, with i fastest after the component index.  Component arrays put the
  * component first: V(L,i,j,k) is V[(L-1) + 7*IX(i,j,k)].
+ *
+ * Every routine is pinned by tests/test_oracle_pins.py against closed forms, invariants or
+ * brute force, except for these parts, whose parity is unpinned (checked GPU-vs-oracle
+ * only): the crossed/reversed orientations of the block-interface copy (ora_bc, lcross = 1
+ * or revMatrix != 0), the Cmua / rho*omega floor / rho*k cap branch of ora_update_v, the SST
+ * cross-diffusion and blending at F1 != 0, and the length-scale blend coefficients of
+ * Fscheme inside its blended range (its limits and monotonicity are pinned).
  */
 #ifndef TRANS_ORACLE_H
 #define TRANS_ORACLE_H
@@ -52,6 +59,10 @@ void ora_convect_muscl(const trans_params* p, ora_block* b);
 void ora_convect_muscl2(const trans_params* p, ora_block* b);
 double ora_muscl2(double var1, double var2);
 void ora_convect_roe(const trans_params* p, ora_block* b);
+void ora_convect_roe_rotated(const trans_params* p, ora_block* b);
+void ora_roe_rotated_face(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
+                          double URo, double VLo, double VRo, double WLo, double WRo, double s1, double s2,
+                          double s3, double grad, double Funscheme, double rLo, double rRo, double out[5]);
 void ora_lusgs(const trans_params* p, ora_block* b);
 void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b);
 void ora_update_u(const trans_params* p, ora_block* b);

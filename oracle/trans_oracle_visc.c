@@ -50,7 +50,6 @@ static void grad_cell(ora_block* b, int i, int j, int k, int turb)
 /* Space/DIFF_DqDxyz.F90:1-408 */
 void ora_diff_dqdxyz(const trans_params* p, int if_turb, ora_block* b)
 {
-    (void)p;
     const int NI = b->ni, NJ = b->nj, NK = b->nk;
     const int NI1 = NI - 1, NJ1 = NJ - 1, NK1 = NK - 1;
     for (int k = 1; k <= NK1; k++)
@@ -80,7 +79,9 @@ void ora_diff_dqdxyz(const trans_params* p, int if_turb, ora_block* b)
                         for (int x = 1; x <= 3; x++)
                             DQ(13 + 3 * q + x - 1, i, j, k) = volinv * gg(b, 6 + q, x, i, j, k);
                 }
-    /* shock sensor (:300-384); Fscheme feeds only the rotated Roe scheme and is omitted */
+    /* DES blending Fscheme and shock sensor (:295-384; CH1 = 3, CH2 = 1, CH3 = 2,
+     * Cdes_epsi = 0.61, Cdes_omeg = 0.78, cmu = 0.09 from Module_Global.F90:64-124).
+     * Fscheme feeds the rotated Roe flux. */
     for (int K = 1; K <= NK1; K++)
         for (int J = 1; J <= NJ1; J++)
             for (int I = 1; I <= NI1; I++) {
@@ -88,10 +89,48 @@ void ora_diff_dqdxyz(const trans_params* p, int if_turb, ora_block* b)
                 double DvDx = DQ(4, I, J, K), DvDy = DQ(5, I, J, K), DvDz = DQ(6, I, J, K);
                 double DwDx = DQ(7, I, J, K), DwDy = DQ(8, I, J, K), DwDz = DQ(9, I, J, K);
                 double W12 = (DuDy - DvDx) / 2., W13 = (DuDz - DwDx) / 2., W23 = (DvDz - DwDy) / 2.;
+                double S11 = (DuDx + DuDx) / 2., S22 = (DvDy + DvDy) / 2., S33 = (DwDz + DwDz) / 2.;
+                double S12 = (DuDy + DvDx) / 2., S13 = (DuDz + DwDx) / 2., S23 = (DwDy + DvDz) / 2.;
                 double Vort = sqrt(2. * (2. * W12 * W12 + 2. * W13 * W13 + 2. * W23 * W23));
+                double Skl = sqrt(2. * (1. * S11 * S11 + 1. * S22 * S22 + 1. * S33 * S33 + 2. * S12 * S12 +
+                                        2. * S13 * S13 + 2. * S23 * S23));
                 double DivU = sqrt(DuDx * DuDx + DvDy * DvDy + DwDz * DwDz);
+                double Den = V(1, I, J, K), RmiuT = RMIU(I, J, K), TT = T(I, J, K);
+                double RMIUL = (1. + p->csthlnd) / (TT * p->csth1 + p->csthlnd) * pow(p->csth1 * TT, 1.5);
+                double vK = V(6, I, J, K) / Den, vOmega = V(7, I, J, K) / Den;
+                double DLTxyz = ALENG(I, J, K);
+                double tau = 1.0;
+                double dkdx = DQ(13, I, J, K), dkdy = DQ(14, I, J, K), dkdz = DQ(15, I, J, K);
+                double dodx = DQ(16, I, J, K), dody = DQ(17, I, J, K), dodz = DQ(18, I, J, K);
+                double Dist = DST(I, J, K);
+                double dTKO = dkdx * dodx + dkdy * dody + dkdz * dodz;
+                double CDKO = fmax(2.0 * Den * p->sigo2 / vOmega * (dTKO), 1.0e-10);
+                double arg1 = fmax(sqrt(vK) / p->beta_star / vOmega / Dist / p->ref,
+                                   500.0 * RMIUL / Den / Dist / Dist / vOmega / p->ref / p->ref);
+                arg1 = fmin(arg1, 4.0 * Den * p->sigo2 * vK / CDKO / Dist / Dist);
+                double FunC1 = tanh(pow4(arg1));
+                double aLgrd = ((1. - FunC1) * 0.61 + FunC1 * 0.78) * DLTxyz;
+                double B0 = 2. * Vort * fmax(Vort, Skl) / fmax((Vort * Vort + Skl * Skl) / 2., p->alscale * 0.1);
+                double g0 = tanh(B0 * B0 * B0 * B0);
+                double sok = fmax(pow((Vort * Vort + Skl * Skl) / 2., 0.5), 0.1 / tau);
+                double alturb = pow((RmiuT + RMIUL) / Den / (p->ref * sok * pow(0.09, 1.5)), 0.5);
+                double A0 = 1. * fmax(((aLgrd / alturb / g0) - 0.1), 0.0);
+                FSCHEME(I, J, K) = fmax(1. * tanh(pow(A0, 3.)), p->f_limiter1);
                 SHOCK(I, J, K) = DivU * DivU / fmax(DivU * DivU + Vort * Vort, 1.0);
             }
+    /* Fscheme's first ghost layer (:399-404) */
+    for (int k = -2; k <= NK + 2; k++)
+        for (int j = -2; j <= NJ + 2; j++) FSCHEME(0, j, k) = FSCHEME(1, j, k);
+    for (int k = -2; k <= NK + 2; k++)
+        for (int i = -2; i <= NI + 2; i++) FSCHEME(i, 0, k) = FSCHEME(i, 1, k);
+    for (int j = -2; j <= NJ + 2; j++)
+        for (int i = -2; i <= NI + 2; i++) FSCHEME(i, j, 0) = FSCHEME(i, j, 1);
+    for (int k = -2; k <= NK + 2; k++)
+        for (int j = -2; j <= NJ + 2; j++) FSCHEME(NI, j, k) = FSCHEME(NI1, j, k);
+    for (int k = -2; k <= NK + 2; k++)
+        for (int i = -2; i <= NI + 2; i++) FSCHEME(i, NJ, k) = FSCHEME(i, NJ1, k);
+    for (int j = -2; j <= NJ + 2; j++)
+        for (int i = -2; i <= NI + 2; i++) FSCHEME(i, j, NK) = FSCHEME(i, j, NK1);
     /* ghost extension (:388-397) */
     for (int L = 1; L <= 3; L++) {
         for (int k = -2; k <= NK + 2; k++)

@@ -996,3 +996,206 @@ def test_mdcdhyb_limited_states_are_monotone():
     for A in (oc[0]["VL"][0], oc[0]["VR"][0]):
         a = A[I, J, K]
         assert np.all(a >= lo - 1e-15) and np.all(a <= hi + 1e-15)
+
+
+# ---------------------------------------------------------------------------------------
+# Rotated Roe flux (Space/CONVECT_Roe_Rotated.F90) and its DES blending Fscheme
+# (Space/DIFF_DqDxyz.F90:295-404)
+
+def _rot_face(prm, L_, R_, S, fun):
+    TL, TR = L_[4] / (prm.ppf * L_[0]), R_[4] / (prm.ppf * R_[0])
+    out = (C.c_double * 5)()
+    sav = float(np.sqrt((S ** 2).sum()))
+    lib().ora_roe_rotated_face(C.byref(prm), L_[4], R_[4], TL, TR, L_[1], R_[1], L_[2], R_[2], L_[3], R_[3],
+                               *S, sav, fun, 0.0, 0.0, out)
+    return np.array(out[:])
+
+
+def _roe_orig(prm, L_, R_, S):
+    """CONVECT_Roe_Original's flux per unit area (pinned above)."""
+    TL, TR = L_[4] / (prm.ppf * L_[0]), R_[4] / (prm.ppf * R_[0])
+    out = (C.c_double * 5)()
+    sav = float(np.sqrt((S ** 2).sum()))
+    lib().ora_roe_common(C.byref(prm), L_[4], R_[4], TL, TR, L_[1], R_[1], L_[2], R_[2], L_[3], R_[3],
+                         *S, sav, 0.0, 0.0, out)
+    return np.array(out[:])
+
+
+@pytest.mark.parametrize("st", STATES)
+@pytest.mark.parametrize("S", FACES)
+@pytest.mark.parametrize("fun", [0.0, 0.4, 1.0])
+def test_roe_rotated_consistency(st, S, fun):
+    """Equal states: the area-weighted physical flux, whatever the blending."""
+    prm = _prm(XM=0.5)
+    sav = float(np.sqrt((S ** 2).sum()))
+    np.testing.assert_allclose(_rot_face(prm, st, st, S, fun), -_euler_flux(*st, S) * sav,
+                               rtol=1e-13, atol=1e-13 * sav)
+
+
+def test_roe_rotated_central_without_dissipation():
+    """Funscheme = 0: the mean of the two physical fluxes (no dissipation)."""
+    prm = _prm(XM=0.5)
+    for L_, R_ in ((STATES[0], STATES[1]), (STATES[1], STATES[2]), (STATES[2], STATES[0])):
+        for S in FACES:
+            sav = float(np.sqrt((S ** 2).sum()))
+            want = -0.5 * (_euler_flux(*L_, S) + _euler_flux(*R_, S)) * sav
+            np.testing.assert_allclose(_rot_face(prm, L_, R_, S, 0.0), want, rtol=1e-13, atol=1e-14 * sav)
+
+
+@pytest.mark.parametrize("S", FACES)
+def test_roe_rotated_is_roe_for_a_normal_velocity_jump(S):
+    """A velocity jump along the face normal leaves the whole face to the first sub-face:
+    with Funscheme = 1 the flux is the original Roe flux times the area (eigenvalues kept
+    above the original's entropy-fix threshold 0.1)."""
+    prm = _prm(XM=0.5)
+    n = S / np.sqrt((S ** 2).sum())
+    L_ = (1.0, 0.5, 0.2, -0.1, 1 / (1.4 * 0.35))
+    uR = np.array(L_[1:4]) - 0.07 * n
+    R_ = (0.9, *uR, 1.1 * L_[4])
+    sav = float(np.sqrt((S ** 2).sum()))
+    np.testing.assert_allclose(_rot_face(prm, L_, R_, S, 1.0), _roe_orig(prm, L_, R_, S) * sav,
+                               rtol=1e-12, atol=1e-13 * sav)
+
+
+def test_roe_rotated_splits_the_face_along_the_velocity_jump():
+    """Oblique velocity jump: central flux through S plus the Roe dissipation through
+    a = (S.e) e (e the unit jump direction, sign taken so that S.e >= 0) and through
+    b = S - a, each of which is the original Roe flux minus its central part."""
+    prm = _prm(XM=0.5)
+    S = np.array([0.3, 0.0, 0.0])
+    L_ = (1.0, 0.6, -0.3, 0.1, 1 / (1.4 * 0.35))
+    R_ = (0.95, 0.65, -0.25, 0.1, 1.05 / (1.4 * 0.35))
+    e = np.array(R_[1:4]) - np.array(L_[1:4])
+    e /= np.sqrt((e ** 2).sum())
+    a = (S @ e) * e
+    b = S - a
+    want = -0.5 * (_euler_flux(*L_, S) + _euler_flux(*R_, S)) * 0.3
+    for sub in (a, b):
+        m = float(np.sqrt((sub ** 2).sum()))
+        central = -0.5 * (_euler_flux(*L_, sub) + _euler_flux(*R_, sub))
+        want = want + (_roe_orig(prm, L_, R_, sub) - central) * m
+    got = _rot_face(prm, L_, R_, S, 1.0)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+    # and the dissipation scales linearly with Funscheme
+    g0, gh = _rot_face(prm, L_, R_, S, 0.0), _rot_face(prm, L_, R_, S, 0.5)
+    np.testing.assert_allclose(gh, 0.5 * (g0 + got), rtol=1e-13, atol=1e-15)
+
+
+def test_roe_rotated_rotation_invariance():
+    """Rotating both velocities and the face vector rotates the momentum flux and leaves
+    the mass and energy fluxes unchanged."""
+    prm = _prm(XM=0.5)
+    rng = np.random.Generator(np.random.MT19937(41))
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    S = np.array([0.2, -0.1, 0.15])
+    L_ = (1.0, 0.4, 0.3, -0.2, 2.0)
+    R_ = (0.8, 0.1, 0.5, 0.05, 1.7)
+    f = _rot_face(prm, L_, R_, S, 0.8)
+    rot = lambda s: (s[0], *(Q @ np.array(s[1:4])), s[4])
+    g = _rot_face(prm, rot(L_), rot(R_), Q @ S, 0.8)
+    np.testing.assert_allclose(g[[0, 4]], f[[0, 4]], rtol=1e-12)
+    np.testing.assert_allclose(g[1:4], Q @ f[1:4], rtol=1e-12, atol=1e-14)
+
+
+def _des_box(seed=43, **deck):
+    b, prm, edges = _box(ni=8, nj=7, nk=5, ratio=1.1, IF_Euler=0, IF_Turb_Org=1, KItm=1, Kind_Roe=2, **deck)
+    rng = np.random.Generator(np.random.MT19937(seed))
+    V = b["v"]
+    V[1] = V[0] * (0.5 + rng.uniform(-0.2, 0.2, V[0].shape))
+    V[2] = V[0] * rng.uniform(-0.2, 0.2, V[0].shape)
+    V[5] *= 1 + 0.3 * rng.uniform(-1, 1, V[0].shape)
+    return b, prm
+
+
+def test_fscheme_limits():
+    """Fscheme stays in [F_limiter1, 1]; it is exactly F_limiter1 where the eddy viscosity
+    makes the turbulent length scale dwarf the grid scale (A0 = 0), and exactly 1 where
+    vorticity and strain vanish (g0 = 0)."""
+    b, prm = _des_box(F_limiter1=0.25)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    oc = OracleCase([b], prm)
+    oc.call("ora_spaceinterp", 0, 1)
+    oc.call("ora_diff_dqdxyz", 0, 1)
+    fs = oc[0]["Fscheme"][I, J, K]
+    assert np.all(fs >= 0.25) and np.all(fs <= 1.0)
+    # a spread of eddy viscosities reaches the blended range; doubling mu_t lengthens the
+    # turbulent scale and never raises Fscheme
+    rng = np.random.Generator(np.random.MT19937(47))
+    mut = prm.ref * 10.0 ** rng.uniform(-8, -2, b["rmiu"].shape)
+    out = []
+    for scale in (1.0, 2.0):
+        oc = OracleCase([b], prm)
+        oc[0]["rmiu"][...] = scale * mut
+        oc.call("ora_spaceinterp", 0, 1)
+        oc.call("ora_diff_dqdxyz", 0, 1)
+        out.append(oc[0]["Fscheme"][I, J, K].copy())
+    assert np.mean((out[0] > 0.25) & (out[0] < 1.0)) > 0.05
+    assert np.all(out[1] <= out[0]) and np.any(out[1] < out[0])
+    oc = OracleCase([b], prm)
+    oc[0]["rmiu"][...] = 1e6 * prm.ref
+    oc.call("ora_spaceinterp", 0, 1)
+    oc.call("ora_diff_dqdxyz", 0, 1)
+    np.testing.assert_array_equal(oc[0]["Fscheme"][I, J, K], 0.25)
+    b2, prm2 = _des_box(F_limiter1=0.25)
+    b2["v"][:5] = b2["v"][:5].mean(axis=(1, 2, 3))[:, None, None, None]
+    oc = OracleCase([b2], prm2)
+    oc.call("ora_spaceinterp", 0, 1)
+    oc.call("ora_diff_dqdxyz", 0, 1)
+    np.testing.assert_array_equal(oc[0]["Fscheme"][I, J, K], 1.0)
+
+
+def test_roe_rotated_scatter():
+    """CONVECT_Roe_Rotated's loop: every face flux (from the reconstructed states, the face
+    vector and Funscheme = mean of the two cells' Fscheme, first ghost layer = the nearest
+    interior cell) subtracted from the cell above the face and added to the cell below."""
+    b, prm = _des_box()
+    oc = OracleCase([b], prm)
+    oc.residual(1)
+    A = oc[0]
+    fs = A["Fscheme"]
+    D = np.zeros_like(A["D"])
+    ni, nj, nk = b.ni, b.nj, b.nk
+    sd, grad, VL, VR, rl, rr = b["sd"], b["grad"], A["VL"], A["VR"], A["rccl"], A["rccr"]
+    clip = lambda x, n: min(max(x, 1), n - 1)
+    for d in range(3):
+        hi = [ni - 1, nj - 1, nk - 1]
+        hi[d] += 1
+        for k in range(1, hi[2] + 1):
+            for j in range(1, hi[1] + 1):
+                for i in range(1, hi[0] + 1):
+                    lo = [i, j, k]
+                    lo[d] -= 1
+                    fa = fs[clip(lo[0], ni) + O, clip(lo[1], nj) + O, clip(lo[2], nk) + O]
+                    fb = fs[clip(i, ni) + O, clip(j, nj) + O, clip(k, nk) + O]
+                    c = (i + O, j + O, k + O)
+                    q = lambda X, L: X[(d + 3 * L,) + c]
+                    out = (C.c_double * 5)()
+                    lib().ora_roe_rotated_face(
+                        C.byref(prm), q(VL, 3), q(VR, 3), q(VL, 4), q(VR, 4), q(VL, 0), q(VR, 0), q(VL, 1),
+                        q(VR, 1), q(VL, 2), q(VR, 2), sd[(d,) + c], sd[(d + 3,) + c], sd[(d + 6,) + c],
+                        grad[(d,) + c], (fa + fb) / 2., rl[(d,) + c] * prm.omega1, rr[(d,) + c] * prm.omega1, out)
+                    cm = (lo[0] + O, lo[1] + O, lo[2] + O)
+                    D[(slice(0, 5),) + c] -= out[:]
+                    D[(slice(0, 5),) + cm] += out[:]
+    A["D"][...] = 0.0
+    oc.call("ora_convect_roe_rotated", 0)
+    I, J, K = sl(1, ni - 1), sl(1, nj - 1), sl(1, nk - 1)
+    np.testing.assert_allclose(A["D"][:5, I, J, K], D[:5, I, J, K], rtol=0, atol=1e-14 * np.abs(D).max())
+    assert np.abs(D[:5, I, J, K]).max() > 0
+
+
+def test_roe_rotated_turbulent_freestream_preservation():
+    """Uniform flow on the curvilinear bump channel with the rotated Roe flux: the interior
+    residual vanishes (central part and dissipation both cancel)."""
+    blocks, org, glob = cases.bump_channel(33, 17, 5)
+    org.Kind_Spatial, org.Kind_Roe, org.IF_Euler, org.IF_Turb_Org = 3, 2, 0, 1
+    prm = solver.prepare_case(blocks, org, glob)
+    b = blocks[0]
+    oc = OracleCase(blocks, prm)
+    oc.residual(1)
+    D = oc[0]["D"]
+    flux = (1 / (1.4 * 0.25)) * np.max(b["grad"])
+    region = D[:, sl(5, b.ni - 5), sl(8, b.nj - 5), sl(1, b.nk - 1)]
+    assert np.max(np.abs(region[:5])) < 1e-12 * flux
+    fs = oc[0]["Fscheme"][sl(5, b.ni - 5), sl(8, b.nj - 5), sl(1, b.nk - 1)]
+    np.testing.assert_array_equal(fs, 1.0)

@@ -99,7 +99,7 @@ class TransPlaneIntegrals(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
-FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK = 0, 1, 2, 3, 4
+FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK, FIELD_FSCHEME = 0, 1, 2, 3, 4, 5
 
 EXPORTS = [
     "trans_hip_create", "trans_hip_destroy", "trans_hip_last_error", "trans_hip_set_params",

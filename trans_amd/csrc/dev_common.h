@@ -51,6 +51,7 @@ struct Fields {
     double* Wt2;      // 7
     double* DQ;       // 18
     double* shock;
+    double* Fscheme;  // DES blending of the rotated Roe dissipation (zero for inviscid runs)
     double* F1;       // LU-SGS diagonal
     double* Stp;      // 2: dual-time pseudo step stept and stept/Vol of the LU-SGS
     double* F1t;      // k-omega LU-SGS diagonal

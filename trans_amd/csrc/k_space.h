@@ -76,6 +76,99 @@ __device__ __forceinline__ double muscl2(double var1, double var2)
     return ((1. + smooth) * var1 + (1. - smooth) * var2) * smooth / 2.;
 }
 
+// Space/CONVECT_Roe_Rotated.F90:597-752 (that file's own Roe_Common, IF_PRECONDITION = 0):
+// adds the Roe dissipation through the sub-face (sav1, sav2, sav3), |.| = sav, weighted by
+// Funscheme, to the area-weighted flux acc.
+__device__ __forceinline__ void roe_rot_common(const trans_params& p, double PLo, double PRo, double TLo,
+                                               double TRo, double ULo, double URo, double VLo, double VRo,
+                                               double WLo, double WRo, double sav1, double sav2, double sav3,
+                                               double sav, double Funscheme, double rLo, double rRo,
+                                               double acc[5])
+{
+    const double rcpcv = p.ppf;
+    const double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
+    const double ga = 1.4, ga1 = 0.4;
+    const double VAMLo = ULo * ULo + VLo * VLo + WLo * WLo, VAMRo = URo * URo + VRo * VRo + WRo * WRo;
+    const double HLo = PLo * ga / (DENLo * ga1) + 0.5 * VAMLo - 0.5 * (rLo * rLo);
+    const double HRo = PRo * ga / (DENRo * ga1) + 0.5 * VAMRo - 0.5 * (rRo * rRo);
+    const double rrorl = sqrt(DENRo / DENLo), rrorlp1 = 1.0 + rrorl;
+    const double rm = sqrt(DENRo * DENLo);
+    const double um = (ULo + URo * rrorl) / rrorlp1, vm = (VLo + VRo * rrorl) / rrorlp1;
+    const double wm = (WLo + WRo * rrorl) / rrorlp1, hm = (HLo + HRo * rrorl) / rrorlp1;
+    const double vm2 = um * um + vm * vm + wm * wm;
+    const double rccm = (rLo + rRo * rrorl) / rrorlp1;
+    const double am2 = ga1 * fabs(hm - 0.5 * vm2 + 0.5 * (rccm * rccm));
+    const double am = sqrt(am2);
+    const double sav1n = sav1 / sav, sav2n = sav2 / sav, sav3n = sav3 / sav;
+    const double unormaln = (sav1n * um + sav2n * vm + sav3n * wm);
+    const double anormaln = am;
+    const double eig1 = fabs(unormaln), eig2 = fabs(unormaln + anormaln), eig3 = fabs(unormaln - anormaln);
+    const double alf = 0.0, ubetac2 = 1.0, amrev = am;
+    const double dunormaln = (sav1n * (URo - ULo) + sav2n * (VRo - VLo) + sav3n * (WRo - WLo));
+    const double astar = (eig2 + eig3) / 2.0;
+    const double Mstar = (eig2 - eig3) / 2.0 / amrev;
+    const double dW1 = DENRo - DENLo, dU1 = URo - ULo, dU2 = VRo - VLo, dU3 = WRo - WLo, dP1 = PRo - PLo;
+    const double dW2 = rm * dU1 + dW1 * um, dW3 = rm * dU2 + dW1 * vm, dW4 = rm * dU3 + dW1 * wm;
+    const double dW5 = dP1 / ga1 + 0.5 * vm2 * dW1 + rm * um * dU1 + rm * vm * dU2 + rm * wm * dU3;
+    const double dUroe = Mstar * dunormaln + (astar - (1 - 2.0 * alf) * eig1 - alf * unormaln * Mstar) *
+                                                 (PRo - PLo) / rm / (ubetac2 * am2);
+    const double dProe = Mstar * (PRo - PLo) + (astar - eig1 + alf * unormaln * Mstar) * rm * dunormaln;
+    const double r1[5] = {eig1 * dW1 + dUroe * rm, eig1 * dW2 + dUroe * rm * um + dProe * sav1n,
+                          eig1 * dW3 + dUroe * rm * vm + dProe * sav2n, eig1 * dW4 + dUroe * rm * wm + dProe * sav3n,
+                          eig1 * dW5 + dUroe * rm * hm + dProe * unormaln};
+#pragma unroll
+    for (int L = 0; L < 5; L++) acc[L] = acc[L] + 0.5 * r1[L] * sav * Funscheme;
+}
+
+// Space/CONVECT_Roe_Rotated.F90:84-176 for one face: the central flux through the face, then
+// the Roe dissipation through the face (velocity jump at or below Tiny), or through its parts
+// along and across the velocity jump.  The grid-velocity term vibc of :169 is never set in the
+// routine and is taken as 0.  out = area-weighted flux.
+__device__ __forceinline__ void roe_rotated_face(const trans_params& p, double PLo, double PRo, double TLo,
+                                                 double TRo, double ULo, double URo, double VLo, double VRo,
+                                                 double WLo, double WRo, double s1, double s2, double s3,
+                                                 double grad, double Funscheme, double rLo, double rRo,
+                                                 double out[5])
+{
+    const double rcpcv = p.ppf;
+    const double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
+    const double ga = 1.4, ga1 = 0.4;
+    const double VAMLo = ULo * ULo + VLo * VLo + WLo * WLo, VAMRo = URo * URo + VRo * VRo + WRo * WRo;
+    const double HLo = PLo * ga / (DENLo * ga1) + 0.5 * VAMLo - 0.5 * (rLo * rLo);
+    const double HRo = PRo * ga / (DENRo * ga1) + 0.5 * VAMRo - 0.5 * (rRo * rRo);
+    double sav = grad;
+    double sav1n = s1 / sav, sav2n = s2 / sav, sav3n = s3 / sav;
+    const double ulnormaln = (sav1n * ULo + sav2n * VLo + sav3n * WLo);
+    const double urnormaln = (sav1n * URo + sav2n * VRo + sav3n * WRo);
+    const double rulnormaln = DENLo * ulnormaln, rurnormaln = DENRo * urnormaln;
+    const double vibc = 0.0;
+    out[0] = -0.5 * (rulnormaln + rurnormaln) * sav;
+    out[1] = -0.5 * (rulnormaln * ULo + rurnormaln * URo + sav1n * (PLo + PRo)) * sav;
+    out[2] = -0.5 * (rulnormaln * VLo + rurnormaln * VRo + sav2n * (PLo + PRo)) * sav;
+    out[3] = -0.5 * (rulnormaln * WLo + rurnormaln * WRo + sav3n * (PLo + PRo)) * sav;
+    out[4] = -0.5 * (rulnormaln * HLo + rurnormaln * HRo - vibc * (PLo + PRo)) * sav;
+    const double DeltaU = sqrt((URo - ULo) * (URo - ULo) + (VRo - VLo) * (VRo - VLo) + (WRo - WLo) * (WRo - WLo));
+    if (DeltaU <= p.tiny) {
+        roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, s1, s2, s3, grad, Funscheme, rLo, rRo,
+                       out);
+    } else {
+        sav1n = (URo - ULo) / DeltaU;
+        sav2n = (VRo - VLo) / DeltaU;
+        sav3n = (WRo - WLo) / DeltaU;
+        sav = sav1n * s1 + sav2n * s2 + sav3n * s3;
+        sav1n = copysign(1.0, sav) * sav1n;
+        sav2n = copysign(1.0, sav) * sav2n;
+        sav3n = copysign(1.0, sav) * sav3n;
+        sav = fabs(sav);
+        const double a1 = sav1n * sav, a2 = sav2n * sav, a3 = sav3n * sav;
+        const double b1 = s1 - a1, b2 = s2 - a2, b3 = s3 - a3;
+        roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, a1, a2, a3,
+                       fmax(sqrt(a1 * a1 + a2 * a2 + a3 * a3), p.tiny), Funscheme, rLo, rRo, out);
+        roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, b1, b2, b3,
+                       fmax(sqrt(b1 * b1 + b2 * b2 + b3 * b3), p.tiny), Funscheme, rLo, rRo, out);
+    }
+}
+
 // Space/CONVECT_Roe_Original.F90:242-407, IF_PRECONDITION = 0.  Returns the flux per unit
 // area (droR..dreR).
 __device__ __forceinline__ void roe_common(const trans_params& p, double PLo, double PRo,

@@ -109,6 +109,37 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
     const double Vort = sqrt(2. * (2. * W12 * W12 + 2. * W13 * W13 + 2. * W23 * W23));
     const double DivU = sqrt(DuDx * DuDx + DvDy * DvDy + DwDz * DwDz);
     f.shock[c] = DivU * DivU / fmax(DivU * DivU + Vort * Vort, 1.0);
+    if (p.kind_roe == 2) {
+        // DES blending Fscheme (Space/DIFF_DqDxyz.F90:295-381; CH1 = 3, CH2 = 1, CH3 = 2,
+        // Cdes_epsi = 0.61, Cdes_omeg = 0.78, cmu = 0.09), read by the rotated Roe flux only
+        const double S11 = (DuDx + DuDx) / 2., S22 = (DvDy + DvDy) / 2., S33 = (DwDz + DwDz) / 2.;
+        const double S12 = (DuDy + DvDx) / 2., S13 = (DuDz + DwDx) / 2., S23 = (DwDy + DvDz) / 2.;
+        const double Skl = sqrt(2. * (1. * S11 * S11 + 1. * S22 * S22 + 1. * S33 * S33 + 2. * S12 * S12 +
+                                      2. * S13 * S13 + 2. * S23 * S23));
+        const double Den = f.V[c], RmiuT = f.Rmiu[c];
+        const double RMIUL = sutherland(p, f.T[c]);
+        const double vK = f.V[5 * tot + c] / Den, vOmega = f.V[6 * tot + c] / Den;
+        const double DLTxyz = f.aLeng[c];
+        const double tau = 1.0;
+        double gko[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) gko[q] = turb ? g[12 + q] : f.DQ[(long)(12 + q) * tot + c];
+        const double Dist = f.Dst[c];
+        const double dTKO = gko[0] * gko[3] + gko[1] * gko[4] + gko[2] * gko[5];
+        const double CDKO = fmax(2.0 * Den * p.sigo2 / vOmega * (dTKO), 1.0e-10);
+        double arg1 = fmax(sqrt(vK) / p.beta_star / vOmega / Dist / p.ref,
+                           500.0 * RMIUL / Den / Dist / Dist / vOmega / p.ref / p.ref);
+        arg1 = fmin(arg1, 4.0 * Den * p.sigo2 * vK / CDKO / Dist / Dist);
+        const double a2 = arg1 * arg1;
+        const double FunC1 = tanh(a2 * a2);
+        const double aLgrd = ((1. - FunC1) * 0.61 + FunC1 * 0.78) * DLTxyz;
+        const double B0 = 2. * Vort * fmax(Vort, Skl) / fmax((Vort * Vort + Skl * Skl) / 2., p.alscale * 0.1);
+        const double g0 = tanh(B0 * B0 * B0 * B0);
+        const double sok = fmax(pow((Vort * Vort + Skl * Skl) / 2., 0.5), 0.1 / tau);
+        const double alturb = pow((RmiuT + RMIUL) / Den / (p.ref * sok * pow(0.09, 1.5)), 0.5);
+        const double A0 = 1. * fmax(((aLgrd / alturb / g0) - 0.1), 0.0);
+        f.Fscheme[c] = fmax(1. * tanh(pow(A0, 3.)), p.f_limiter1);
+    }
 }
 
 // Wall mark of face (i,j,k) in direction DIR (MARKWALL arrays, Input/Read_BlockDimensions.F90:146-283)
@@ -354,7 +385,8 @@ __device__ __forceinline__ void ausmpw_common(const trans_params& p, double PLo,
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 // One thread per face of direction DIR: reconstruction (SPATIAL 1 = CONVECT_MUSCL2, 2 = CONVECT_MUSCL,
-// 3 = CONVECT_MDCD, 4 = CONVECT_MDCDHYB), inviscid flux (ROE 1 = CONVECT_Roe_Original, 3 = CONVECT_AUSMPW_Plus)
+// 3 = CONVECT_MDCD, 4 = CONVECT_MDCDHYB), inviscid flux (ROE 1 = CONVECT_Roe_Original,
+// 2 = CONVECT_Roe_Rotated, 3 = CONVECT_AUSMPW_Plus)
 // and, when turbulent, TURBTWO_CONVECT_MUSCL + TURBTWO_CONVECT_Upwind.  G[DIR-1] receives
 // flux*|S| (5 comps) and the area-weighted k, omega fluxes (comps 5, 6).
 template <int DIR, int SPATIAL, int ROE>
@@ -448,15 +480,30 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
     const double* S = f.SD + (long)d * tot;
     const double sav = f.Grad[(long)d * tot + c0];
     double fl[5];
-    if (ROE == 1)
+    if (ROE == 2) {  // Space/CONVECT_Roe_Rotated.F90: area-weighted flux, scattered as it is
+        // Funscheme from the two cells; the first ghost layer of Fscheme is the nearest
+        // interior cell (Space/DIFF_DqDxyz.F90:399-404)
+        int pm[3] = {i, j, k};
+        pm[d] -= 1;
+        const int n3[3] = {b.ni - 1, b.nj - 1, b.nk - 1};
+        const long ca = cidx(b, clampi(pm[0], 1, n3[0]), clampi(pm[1], 1, n3[1]), clampi(pm[2], 1, n3[2]));
+        const long cb = cidx(b, clampi(i, 1, n3[0]), clampi(j, 1, n3[1]), clampi(k, 1, n3[2]));
+        const double Funscheme = (f.Fscheme[ca] + f.Fscheme[cb]) / 2.;
+        roe_rotated_face(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
+                         S[3 * tot + c0], S[6 * tot + c0], sav, Funscheme, qL[5] * p.omega1, qR[5] * p.omega1, fl);
+        double* G = f.G[d];
+#pragma unroll
+        for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L];
+    } else if (ROE == 1)
         roe_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0], S[3 * tot + c0],
                    S[6 * tot + c0], sav, qL[5] * p.omega1, qR[5] * p.omega1, fl);
     else
         ausmpw_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
                       S[3 * tot + c0], S[6 * tot + c0], sav, fl);
     double* G = f.G[d];
+    if (ROE != 2)
 #pragma unroll
-    for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L] * sav;
+        for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L] * sav;
     if (!turb) return;
     // k/rho, omega/rho: face averages + ALIMITER reconstruction (Space/TURBTWO_CONVECT_MUSCL.F90)
     double tL[2], tR[2];

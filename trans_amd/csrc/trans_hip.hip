@@ -243,8 +243,8 @@ static int check_params(trans_ctx* c, const trans_params* p)
     if (p->if_rk != 0 && p->if_rk != 1) return fail(c, TRANS_EINVAL, "IF_RK must be 0 or 1");
     if (p->kind_spatial < 1 || p->kind_spatial > 4)
         return fail(c, TRANS_EUNSUPPORTED, "Kind_Spatial must be 1 (MUSCL2), 2 (MUSCL), 3 (MDCD) or 4 (MDCDHYB)");
-    if (p->kind_roe != 1 && p->kind_roe != 3)
-        return fail(c, TRANS_EUNSUPPORTED, "Kind_Roe must be 1 (Roe) or 3 (AUSMPW+)");
+    if (p->kind_roe < 1 || p->kind_roe > 3)
+        return fail(c, TRANS_EUNSUPPORTED, "Kind_Roe must be 1 (Roe), 2 (rotated Roe) or 3 (AUSMPW+)");
     if (p->if_euler == 0 && p->if_turb_org == 1) {
         if (p->kind_model != 1 && p->kind_model != 2)
             return fail(c, TRANS_EUNSUPPORTED, "Kind_Model must be 1 (SST) or 2 (WD+ k-omega)");
@@ -307,7 +307,7 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
         (rc = dalloc(c, &f.Zc, n)) || (rc = dalloc(c, &f.thtc, n)) || (rc = dalloc(c, &f.D, 7 * n)) ||
         (rc = dalloc(c, &f.Rds, 3 * n)) || (rc = dalloc(c, &f.Dtm, n)) || (rc = dalloc(c, &f.Wt0, 7 * n)) ||
         (rc = dalloc(c, &f.Wt1, 7 * n)) || (rc = dalloc(c, &f.Wt2, 7 * n)) || (rc = dalloc(c, &f.DQ, 18 * n)) ||
-        (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.F1t, n)) ||
+        (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.Fscheme, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.F1t, n)) ||
         (rc = dalloc(c, &f.SDm, 9 * n)) || (rc = dalloc(c, &f.Stp, 2 * n)) ||
         (rc = dalloc(c, &f.G[0], 7 * n)) || (rc = dalloc(c, &f.G[1], 7 * n)) || (rc = dalloc(c, &f.G[2], 7 * n)))
         return rc;
@@ -592,6 +592,7 @@ int trans_hip_get_field(trans_ctx* c, int blk, int which, const trans_fview* out
     case TRANS_FIELD_DTM: return copy_view(c, blk, *out, f.Dtm, 1, false);
     case TRANS_FIELD_DQDXYZ: return copy_view(c, blk, *out, f.DQ, 18, false);
     case TRANS_FIELD_SHOCK: return copy_view(c, blk, *out, f.shock, 1, false);
+    case TRANS_FIELD_FSCHEME: return copy_view(c, blk, *out, f.Fscheme, 1, false);
     default: return fail(c, TRANS_EINVAL, "unknown field");
     }
 }
@@ -647,6 +648,17 @@ static void launch_conv(trans_ctx* c, int if_turb)
                c->p, if_turb, c->f, c->d_blk, c->seg_face[2].d);
 }
 
+template <int SPATIAL>
+static void launch_conv_roe(trans_ctx* c, int if_turb)
+{
+    if (c->p.kind_roe == 1)
+        launch_conv<SPATIAL, 1>(c, if_turb);
+    else if (c->p.kind_roe == 2)
+        launch_conv<SPATIAL, 2>(c, if_turb);
+    else
+        launch_conv<SPATIAL, 3>(c, if_turb);
+}
+
 extern "C" {
 
 int trans_hip_space(trans_ctx* c)
@@ -670,27 +682,14 @@ int trans_hip_space(trans_ctx* c)
         KLAUNCH_ON(c, c->aux[2], "k_visc_face<3>", k_visc_face<3>, dim3(c->seg_face[2].n), dim3(WG), c->p, if_turb,
                    c->f, c->d_blk, c->seg_face[2].d);
     }
-    if (c->p.kind_spatial == 1) {
-        if (c->p.kind_roe == 1)
-            launch_conv<1, 1>(c, if_turb);
-        else
-            launch_conv<1, 3>(c, if_turb);
-    } else if (c->p.kind_spatial == 2) {
-        if (c->p.kind_roe == 1)
-            launch_conv<2, 1>(c, if_turb);
-        else
-            launch_conv<2, 3>(c, if_turb);
-    } else if (c->p.kind_spatial == 3) {
-        if (c->p.kind_roe == 1)
-            launch_conv<3, 1>(c, if_turb);
-        else
-            launch_conv<3, 3>(c, if_turb);
-    } else {
-        if (c->p.kind_roe == 1)
-            launch_conv<4, 1>(c, if_turb);
-        else
-            launch_conv<4, 3>(c, if_turb);
-    }
+    if (c->p.kind_spatial == 1)
+        launch_conv_roe<1>(c, if_turb);
+    else if (c->p.kind_spatial == 2)
+        launch_conv_roe<2>(c, if_turb);
+    else if (c->p.kind_spatial == 3)
+        launch_conv_roe<3>(c, if_turb);
+    else
+        launch_conv_roe<4>(c, if_turb);
     join_aux(c, 3);
     KLAUNCH(c, "k_accumulate_visc", k_accumulate_visc, dim3(c->seg_cell.n), dim3(WG), c->p, c->p.if_euler,
             if_turb, c->f, c->d_blk, c->seg_cell.d);

@@ -13,7 +13,7 @@ module trans_hip_mod
     integer(c_int), parameter, public :: TRANS_OK = 0, TRANS_EINVAL = -1, TRANS_EHIP = -2, &
         TRANS_ENOMEM = -3, TRANS_EUNSUPPORTED = -4, TRANS_ENONFINITE = 1
     integer(c_int), parameter, public :: TRANS_FIELD_D = 0, TRANS_FIELD_RDS = 1, TRANS_FIELD_DTM = 2, &
-        TRANS_FIELD_DQDXYZ = 3, TRANS_FIELD_SHOCK = 4
+        TRANS_FIELD_DQDXYZ = 3, TRANS_FIELD_SHOCK = 4, TRANS_FIELD_FSCHEME = 5
 
     type, bind(C), public :: trans_params
         integer(c_int) :: kind_spatial, kind_roe, if_euler, if_turb_org, kind_model, kind_dual, kchld, &

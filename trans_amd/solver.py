@@ -152,7 +152,7 @@ class Solver:
     def get_field(self, blk, which):
         b = self.blocks[blk]
         nc = {capi.FIELD_D: 7, capi.FIELD_RDS: 3, capi.FIELD_DTM: 1, capi.FIELD_DQDXYZ: 18,
-              capi.FIELD_SHOCK: 1}[which]
+              capi.FIELD_SHOCK: 1, capi.FIELD_FSCHEME: 1}[which]
         shp = b.shape()
         a = np.zeros(shp if nc == 1 else (nc,) + shp, order="F")
         v = capi.fview(a)
