@@ -19,6 +19,8 @@
 #ifndef TRANS_HIP_H
 #define TRANS_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -239,6 +241,78 @@ int trans_hip_profile(trans_ctx* ctx, int on);
 int trans_hip_profile_count(trans_ctx* ctx);
 int trans_hip_profile_get(trans_ctx* ctx, int idx, char* name, int name_len, double* total_ms,
                           long* launches);
+
+/* Device bytes a context for these inputs would allocate (trans_hip_create's own sizing,
+ * run without allocating or touching the device), and the bytes a context holds.  No
+ * counterpart in the reference; used to admit solves by HBM footprint. */
+int trans_hip_footprint(const trans_params* params, int nblk, const trans_block_desc* blocks,
+                        size_t* bytes);
+int trans_hip_device_bytes(const trans_ctx* ctx, size_t* bytes);
+
+/* ---- Serving: continuous batching of independent steady solves -------------------------
+ * No counterpart in the reference, which runs one case per process.  A server admits
+ * submitted solves, in arrival order and first fit, while their footprints fit hbm_budget
+ * and fewer than max_active are resident; each resident solve keeps its state in HBM and
+ * advances in chunks of `chunk` iterations on its own stream, so the chunks of all resident
+ * solves overlap on the device.  A solve finishes when its global residual
+ * DRSa = sqrt(sum_b N_b DRSa_b^2 / sum_b N_b) (N_b cells of block b) has dropped to
+ * target_drop times its first-iteration value, or after max_iter iterations; it then leaves
+ * the device and its final state waits on the host for trans_srv_result.  The iteration that
+ * first reached the target is reported exactly; the state is the one at the end of that
+ * chunk.  A solve's iterations depend only on its own inputs and on `chunk`, so its result
+ * is the same alone and in any mix.  One host thread per server. */
+typedef struct trans_srv trans_srv;
+
+typedef struct trans_srv_config {
+    size_t hbm_budget; /* device bytes the resident solves may hold together */
+    int max_active;    /* most solves resident at once */
+    int chunk;         /* iterations queued per solve between convergence checks */
+} trans_srv_config;
+
+typedef struct trans_srv_request {
+    trans_params params;
+    int nblk;
+    const trans_block_desc* blocks; /* nblk */
+    const trans_geom_view* geom;    /* nblk */
+    const trans_state_view* state;  /* nblk: initial state */
+    double target_drop;             /* e.g. 1e-8 */
+    int max_iter;
+} trans_srv_request;
+
+#define TRANS_SRV_QUEUED 0
+#define TRANS_SRV_RUNNING 1
+#define TRANS_SRV_CONVERGED 2
+#define TRANS_SRV_MAX_ITER 3
+#define TRANS_SRV_CANCELLED 4
+#define TRANS_SRV_FAILED 5
+
+typedef struct trans_srv_status {
+    int state;                /* TRANS_SRV_* */
+    int error;                /* TRANS_E* when FAILED */
+    int iterations;           /* iterations run */
+    int iterations_to_target; /* first iteration at or below the target drop, 0 = not reached */
+    double drsa_first;        /* global DRSa of iteration 1 */
+    double drsa_last;         /* global DRSa of the last iteration run */
+    double t_submit, t_admit, t_done; /* seconds since trans_srv_create; 0 = not yet */
+    size_t footprint;         /* device bytes while resident */
+} trans_srv_status;
+
+int trans_srv_create(const trans_srv_config* cfg, trans_srv** srv);
+/* Copies the request's inputs (the caller's arrays are not used after the call); rejects a
+ * request the solver rejects or that could never fit the budget. */
+int trans_srv_submit(trans_srv* srv, const trans_srv_request* req, int* id);
+/* One scheduling round: collect finished chunks, retire finished solves, admit waiting ones,
+ * queue the next chunks.  wait != 0 first blocks until the oldest queued chunk has ended.
+ * n_unfinished (may be NULL) receives the number of queued or running solves. */
+int trans_srv_progress(trans_srv* srv, int wait, int* n_unfinished);
+int trans_srv_poll(trans_srv* srv, int id, trans_srv_status* out);
+int trans_srv_cancel(trans_srv* srv, int id);
+/* Final state of block blk (0-based) of a finished solve into the caller's views. */
+int trans_srv_result(trans_srv* srv, int id, int blk, const trans_state_view* out);
+/* Drops the host copy of a finished solve's state. */
+int trans_srv_release(trans_srv* srv, int id);
+int trans_srv_destroy(trans_srv* srv);
+const char* trans_srv_last_error(const trans_srv* srv);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     src = open(os.path.join(ROOT, "include", "trans_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(trans_hip_\w+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(trans_(?:hip|srv)_\w+)\s*\(", src)))
 
 
 def test_header_matches_python_binding():
@@ -32,9 +32,10 @@ def test_struct_sizes_match_header():
 #include <stdio.h>
 #include <stddef.h>
 #include "trans_hip.h"
-int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(trans_params), sizeof(trans_patch),
- sizeof(trans_block_desc), sizeof(trans_fview), sizeof(trans_geom_view), sizeof(trans_state_view),
- sizeof(trans_resid)); return 0;}
+int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(trans_params),
+ sizeof(trans_patch), sizeof(trans_block_desc), sizeof(trans_fview), sizeof(trans_geom_view),
+ sizeof(trans_state_view), sizeof(trans_resid), sizeof(trans_plane_integrals), sizeof(trans_srv_config),
+ sizeof(trans_srv_request), sizeof(trans_srv_status)); return 0;}
 '''
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "p.c")
@@ -45,5 +46,21 @@ int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(trans_params), siz
     got = [int(x) for x in out]
     want = [ctypes.sizeof(t) for t in (capi.TransParams, capi.TransPatch, capi.TransBlockDesc,
                                        capi.TransFView, capi.TransGeomView, capi.TransStateView,
-                                       capi.TransResid)]
+                                       capi.TransResid, capi.TransPlaneIntegrals, capi.TransSrvConfig,
+                                       capi.TransSrvRequest, capi.TransSrvStatus)]
     assert got == want
+
+
+def test_footprint_without_a_device():
+    """trans_hip_footprint runs trans_hip_create's sizing without touching the device: it
+    works here, grows with the cell count, and adds the viscous face fluxes and the k-omega
+    sweep inputs for a turbulent case."""
+    from trans_amd import cases, serving, solver
+    sizes = []
+    for ni in (33, 65):
+        blocks, org, glob = cases.bump_channel(ni, 17, 5)
+        sizes.append(serving.footprint(blocks, solver.prepare_case(blocks, org, glob)))
+    assert 0 < sizes[0] < sizes[1]
+    blocks, org, glob = cases.bump_channel(33, 17, 5)
+    org.IF_Euler, org.IF_Turb_Org = 0, 1
+    assert serving.footprint(blocks, solver.prepare_case(blocks, org, glob)) > sizes[0]

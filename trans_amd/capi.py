@@ -101,6 +101,29 @@ class TransPlaneIntegrals(C.Structure):
 
 FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK, FIELD_FSCHEME = 0, 1, 2, 3, 4, 5
 
+
+class TransSrvConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int), ("chunk", C.c_int)]
+
+
+class TransSrvRequest(C.Structure):
+    _fields_ = [("params", TransParams), ("nblk", C.c_int), ("blocks", C.POINTER(TransBlockDesc)),
+                ("geom", C.POINTER(TransGeomView)), ("state", C.POINTER(TransStateView)),
+                ("target_drop", C.c_double), ("max_iter", C.c_int)]
+
+
+class TransSrvStatus(C.Structure):
+    _fields_ = [("state", C.c_int), ("error", C.c_int), ("iterations", C.c_int),
+                ("iterations_to_target", C.c_int), ("drsa_first", C.c_double),
+                ("drsa_last", C.c_double), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("footprint", C.c_size_t)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+SRV_QUEUED, SRV_RUNNING, SRV_CONVERGED, SRV_MAX_ITER, SRV_CANCELLED, SRV_FAILED = range(6)
+
 EXPORTS = [
     "trans_hip_create", "trans_hip_destroy", "trans_hip_last_error", "trans_hip_set_params",
     "trans_hip_set_iteration", "trans_hip_set_geometry", "trans_hip_set_state",
@@ -108,7 +131,9 @@ EXPORTS = [
     "trans_hip_space", "trans_hip_residual", "trans_hip_lusgs_update", "trans_hip_rk_update",
     "trans_hip_update_u",
     "trans_hip_iterate", "trans_hip_plane_integrals", "trans_hip_sync", "trans_hip_profile", "trans_hip_profile_count",
-    "trans_hip_profile_get",
+    "trans_hip_profile_get", "trans_hip_footprint", "trans_hip_device_bytes",
+    "trans_srv_create", "trans_srv_submit", "trans_srv_progress", "trans_srv_poll", "trans_srv_cancel",
+    "trans_srv_result", "trans_srv_release", "trans_srv_destroy", "trans_srv_last_error",
 ]
 
 
@@ -166,8 +191,20 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_profile_count.argtypes = [C.c_void_p]
     lib.trans_hip_profile_get.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int,
                                           C.POINTER(C.c_double), C.POINTER(C.c_long)]
+    lib.trans_hip_footprint.argtypes = [P(TransParams), C.c_int, P(TransBlockDesc), P(C.c_size_t)]
+    lib.trans_hip_device_bytes.argtypes = [C.c_void_p, P(C.c_size_t)]
+    lib.trans_srv_create.argtypes = [P(TransSrvConfig), P(C.c_void_p)]
+    lib.trans_srv_submit.argtypes = [C.c_void_p, P(TransSrvRequest), P(C.c_int)]
+    lib.trans_srv_progress.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]
+    lib.trans_srv_poll.argtypes = [C.c_void_p, C.c_int, P(TransSrvStatus)]
+    lib.trans_srv_cancel.argtypes = [C.c_void_p, C.c_int]
+    lib.trans_srv_result.argtypes = [C.c_void_p, C.c_int, C.c_int, P(TransStateView)]
+    lib.trans_srv_release.argtypes = [C.c_void_p, C.c_int]
+    lib.trans_srv_destroy.argtypes = [C.c_void_p]
+    lib.trans_srv_last_error.argtypes = [C.c_void_p]
+    lib.trans_srv_last_error.restype = C.c_char_p
     for name in EXPORTS:
-        if name != "trans_hip_last_error":
+        if name not in ("trans_hip_last_error", "trans_srv_last_error"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

@@ -85,6 +85,8 @@ struct trans_ctx {
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
     std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
     std::vector<void*> allocs;
+    size_t bytes = 0;  // device bytes allocated by the context
+    bool dry = false;  // sizing run of trans_hip_footprint: no HIP call, no allocation
     std::string err;
 };
 
@@ -200,15 +202,40 @@ static void drop_graph(trans_ctx* c)
             return fail(ctx, TRANS_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
     } while (0)
 
+// Copies ordered on the context's stream and complete on return (host memory may be
+// pageable).  The context's streams are non-blocking, so nothing here may use the null
+// stream: work of other contexts on the device is never waited for.
+static int h2d(trans_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    if (c->dry) return 0;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int d2h(trans_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 template <typename T>
 static int dalloc(trans_ctx* c, T** out, size_t n)
 {
     void* p = nullptr;
     if (n == 0) n = 1;
+    if (c->dry) {  // sizing run: count only
+        c->bytes += n * sizeof(T);
+        *out = nullptr;
+        return 0;
+    }
     hipError_t e = hipMalloc(&p, n * sizeof(T));
     if (e != hipSuccess) return fail(c, TRANS_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    if (hipMemset(p, 0, n * sizeof(T)) != hipSuccess) return fail(c, TRANS_EHIP, "hipMemset");
+    // stream-ordered clear: the context's streams do not synchronise with the null stream
+    if (hipMemsetAsync(p, 0, n * sizeof(T), c->stream) != hipSuccess) return fail(c, TRANS_EHIP, "hipMemsetAsync");
     c->allocs.push_back(p);
+    c->bytes += n * sizeof(T);
     *out = (T*)p;
     return 0;
 }
@@ -227,7 +254,7 @@ static SegTable make_segs(trans_ctx* c, int kind)  // kind: 0 cells, 1..3 faces 
     }
     t.n = (int)segs.size();
     if (dalloc(c, &t.d, segs.size()) == 0)
-        (void)hipMemcpy(t.d, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice);
+        (void)h2d(c, t.d, segs.data(), segs.size() * sizeof(Seg));
     return t;
 }
 
@@ -254,12 +281,13 @@ static int check_params(trans_ctx* c, const trans_params* p)
     return 0;
 }
 
-int trans_hip_create(const trans_params* params, int nblk, const trans_block_desc* blocks,
-                     trans_ctx** out)
+static int create_ctx(const trans_params* params, int nblk, const trans_block_desc* blocks, trans_ctx** out,
+                      bool dry)
 {
     if (!params || !blocks || !out || nblk <= 0) return TRANS_EINVAL;
     trans_ctx* c = new trans_ctx();
     *out = c;
+    c->dry = dry;
     int rc = check_params(c, params);
     if (rc) return rc;
     c->p = *params;
@@ -286,12 +314,14 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
                                    pt.target_conn >= c->patch0[pt.target_block + 1] - c->patch0[pt.target_block]))
                 return fail(c, TRANS_EINVAL, "interface target out of range");
         }
-    HIPCHK(c, hipStreamCreate(&c->stream));
-    for (int q = 0; q < 3; q++) {
-        HIPCHK(c, hipStreamCreate(&c->aux[q]));
-        HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming));
+    if (!dry) {
+        HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (int q = 0; q < 3; q++) {
+            HIPCHK(c, hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
+            HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming));
+        }
+        HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     }
-    HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     {
         const char* env = std::getenv("TRANS_HIP_GRAPH");
         c->graphs = !(env && env[0] == '0');
@@ -360,12 +390,12 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
     c->n_outl = (int)ol.size();
     if (c->n_outl) {
         if ((rc = dalloc(c, &c->d_outl, ol.size()))) return rc;
-        HIPCHK(c, hipMemcpy(c->d_outl, ol.data(), ol.size() * sizeof(OutLine), hipMemcpyHostToDevice));
+        if ((rc = h2d(c, c->d_outl, ol.data(), ol.size() * sizeof(OutLine)))) return rc;
     }
     c->n_bcseg = (int)bs.size();
     if ((rc = dalloc(c, &c->d_patch, dp.size())) || (rc = dalloc(c, &c->d_bcseg, bs.size()))) return rc;
-    if (!dp.empty()) HIPCHK(c, hipMemcpy(c->d_patch, dp.data(), dp.size() * sizeof(DPatch), hipMemcpyHostToDevice));
-    if (!bs.empty()) HIPCHK(c, hipMemcpy(c->d_bcseg, bs.data(), bs.size() * sizeof(BcSeg), hipMemcpyHostToDevice));
+    if (!dp.empty() && (rc = h2d(c, c->d_patch, dp.data(), dp.size() * sizeof(DPatch)))) return rc;
+    if (!bs.empty() && (rc = h2d(c, c->d_bcseg, bs.data(), bs.size() * sizeof(BcSeg)))) return rc;
 
     {
         // march along j (lanes on i: coalesced) unless the (i,k) sections do not fit a workgroup;
@@ -419,14 +449,38 @@ int trans_hip_create(const trans_params* params, int nblk, const trans_block_des
                         c->mwoff[b], woff[b]};
     }
     if ((rc = dalloc(c, &c->d_blk, nblk))) return rc;
-    HIPCHK(c, hipMemcpy(c->d_blk, bi.data(), nblk * sizeof(BlkInfo), hipMemcpyHostToDevice));
+    if ((rc = h2d(c, c->d_blk, bi.data(), nblk * sizeof(BlkInfo)))) return rc;
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
-    HIPCHK(c, hipMemcpy(c->d_wrange, c->seg_cell.wrange.data(), (nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
+    if ((rc = h2d(c, c->d_wrange, c->seg_cell.wrange.data(), (nblk + 1) * sizeof(int)))) return rc;
     if ((rc = dalloc(c, &c->d_part, c->seg_cell.n)) || (rc = dalloc(c, &c->d_resid, nblk)) ||
         (rc = dalloc(c, &c->d_mon, 4)))
         return rc;
+    if (!dry) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int trans_hip_create(const trans_params* params, int nblk, const trans_block_desc* blocks,
+                     trans_ctx** out)
+{
+    return create_ctx(params, nblk, blocks, out, false);
+}
+
+int trans_hip_footprint(const trans_params* params, int nblk, const trans_block_desc* blocks, size_t* bytes)
+{
+    if (!bytes) return TRANS_EINVAL;
+    trans_ctx* c = nullptr;
+    const int rc = create_ctx(params, nblk, blocks, &c, true);
+    *bytes = c ? c->bytes : 0;
+    trans_hip_destroy(c);
+    return rc;
+}
+
+int trans_hip_device_bytes(const trans_ctx* c, size_t* bytes)
+{
+    if (!c || !bytes) return TRANS_EINVAL;
+    *bytes = c->bytes;
     return 0;
 }
 
@@ -494,15 +548,9 @@ static int copy_view(trans_ctx* c, int blk, const trans_fview& v, double* dbase,
     std::vector<double> stage((size_t)ncomp * blkn);
     for (int cc = 0; cc < ncomp; cc++) {
         double* dst = dbase + (long)cc * c->tot + h.off;
-        if (!upload) {
-            hipError_t e = hipMemcpy(stage.data() + (size_t)cc * blkn, dst, blkn * sizeof(double),
-                                     hipMemcpyDeviceToHost);
-            if (e != hipSuccess) return fail(c, TRANS_EHIP, hipGetErrorString(e));
-        } else {
-            hipError_t e = hipMemcpy(stage.data() + (size_t)cc * blkn, dst, blkn * sizeof(double),
-                                     hipMemcpyDeviceToHost);
-            if (e != hipSuccess) return fail(c, TRANS_EHIP, hipGetErrorString(e));
-        }
+        // (an upload also reads the block first: the view may cover only part of it)
+        int rc = d2h(c, stage.data() + (size_t)cc * blkn, dst, blkn * sizeof(double));
+        if (rc) return rc;
         for (int k = lo[2]; k <= hi[2]; k++)
             for (int j = lo[1]; j <= hi[1]; j++)
                 for (int i = lo[0]; i <= hi[0]; i++) {
@@ -513,11 +561,7 @@ static int copy_view(trans_ctx* c, int blk, const trans_fview& v, double* dbase,
                     else
                         v.p[vi] = stage[di];
                 }
-        if (upload) {
-            hipError_t e = hipMemcpy(dst, stage.data() + (size_t)cc * blkn, blkn * sizeof(double),
-                                     hipMemcpyHostToDevice);
-            if (e != hipSuccess) return fail(c, TRANS_EHIP, hipGetErrorString(e));
-        }
+        if (upload && (rc = h2d(c, dst, stage.data() + (size_t)cc * blkn, blkn * sizeof(double)))) return rc;
     }
     return 0;
 }
@@ -546,7 +590,7 @@ int trans_hip_set_geometry(trans_ctx* c, int blk, const trans_geom_view* g)
         else
             mw.insert(mw.end(), (size_t)fsz[q], 0);
     }
-    HIPCHK(c, hipMemcpy(f.mw + c->mwoff[blk], mw.data(), mw.size() * sizeof(int), hipMemcpyHostToDevice));
+    if ((rc = h2d(c, f.mw + c->mwoff[blk], mw.data(), mw.size() * sizeof(int)))) return rc;
     hipLaunchKernelGGL(k_face_means, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->f, c->d_blk, c->seg_cell.d);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1015,3 +1059,5 @@ int trans_hip_sync(trans_ctx* c)
 }
 
 }  // extern "C"
+
+#include "serve.h"
