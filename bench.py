@@ -82,6 +82,60 @@ def build_workload(name, implicit_fp32=0):
     return blocks, prm, desc
 
 
+def serving_leg(n_req=16, seed=20261202, target=1e-3, max_iter=20000, mean_gap_s=0.25,
+                budget_gib=64, max_active=16, chunk=100):
+    """Continuous-batching serving of a Poisson stream of steady bump-channel solves.
+
+    Requests draw a grid (65x33x9, 97x49x9 or 129x65x9 points) and a freestream Mach number
+    in [0.50, 0.52] from the seed and arrive with exponential gaps; each runs to a residual
+    drop of `target` (or `max_iter`).  The server admits them by HBM footprint (budget
+    `budget_gib`) into at most `max_active` resident slots and interleaves their iterations.
+    Latency = completion - submission."""
+    import numpy as np
+    from trans_amd import cases, serving, solver
+    rng = np.random.Generator(np.random.MT19937(seed))
+    grids = [(65, 33, 9), (97, 49, 9), (129, 65, 9)]
+    reqs = []
+    t = 0.0
+    for _ in range(n_req):
+        t += rng.exponential(mean_gap_s)
+        reqs.append((t, grids[rng.integers(0, 3)], float(rng.uniform(0.50, 0.52))))
+    built = []
+    for (_, g, m) in reqs:
+        blocks, org, glob = cases.bump_channel(*g)
+        org.XM = m
+        built.append((blocks, solver.prepare_case(blocks, org, glob)))
+    srv = serving.Server(budget_gib << 30, max_active=max_active, chunk=chunk)
+    ids = []
+    t0 = time.perf_counter()
+    nxt = 0
+    unfinished = 0
+    while nxt < n_req or unfinished:
+        now = time.perf_counter() - t0
+        while nxt < n_req and reqs[nxt][0] <= now:
+            ids.append(srv.submit(built[nxt][0], built[nxt][1], target, max_iter))
+            nxt += 1
+        due = nxt < n_req and reqs[nxt][0] - now < 0.01
+        unfinished = srv.progress(wait=not due)
+        if not unfinished and nxt < n_req:
+            time.sleep(max(0.0, min(0.01, reqs[nxt][0] - (time.perf_counter() - t0))))
+    sts = [srv.poll(i) for i in ids]
+    srv.close()
+    lat = np.array([s["t_done"] - s["t_submit"] for s in sts])
+    span = max(s["t_done"] for s in sts) - min(s["t_submit"] for s in sts)
+    cell_iters = sum((g[0] - 1) * (g[1] - 1) * (g[2] - 1) * s["iterations"] for (_, g, _), s in zip(reqs, sts))
+    return {"requests": n_req, "converged": sum(s["state"] == 2 for s in sts),
+            "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
+            "solves_per_hr": round(n_req / span * 3600, 1), "p50_s": round(float(np.percentile(lat, 50)), 3),
+            "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 3),
+            "mcell_iter_per_s": round(cell_iters / span / 1e6, 2),
+            "iterations": [s["iterations"] for s in sts],
+            "workload": f"Poisson stream (mean gap {mean_gap_s} s, seed {seed}) of bump-channel Euler solves "
+                        f"(MUSCL + Roe, LU-SGS CFL 5), grids 65x33x9 / 97x49x9 / 129x65x9, Mach U[0.50, 0.52], "
+                        f"to a {target:g} residual drop (cap {max_iter}); budget {budget_gib} GiB, "
+                        f"{max_active} slots, {chunk}-iteration chunks"}
+
+
 def units(blocks):
     u = {"cell": 0, "face1": 0, "face2": 0, "face3": 0}
     for b in blocks:
@@ -192,6 +246,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default=os.environ.get("TRANS_BENCH_WORKLOAD", "flatplate"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--serving", type=int, default=16, metavar="N",
+                    help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serving-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-implicit leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup+steps of the FP64 path only and print the kernel table")
@@ -203,6 +260,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.serving_child:
+        print(json.dumps(serving_leg(args.serving)), flush=True)
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 8
@@ -261,6 +321,18 @@ def main():
                 "drsa_block1": res32[0]["drsa"], "drsa_block1_fp64": res[0]["drsa"]}
         print(f"[bench] FP32 implicit: {args.steps} steps in {dt32:.3f} s", flush=True)
     cpu = None if args.no_cpu else cpu_baseline(blocks, prm)
+    serve = None
+    if args.serving > 0:
+        # in a child process: it spreads the resident solves' streams over 16 hardware queues
+        # (HIP's default is 4), a per-process setting that must precede HIP's initialisation
+        import subprocess
+        env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
+        cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--serving-child",
+                             "--serving", str(args.serving)], env=env, capture_output=True, text=True)
+        if cp.returncode != 0:
+            raise SystemExit(f"serving leg failed ({cp.returncode}): {cp.stderr[-2000:]}")
+        serve = json.loads(cp.stdout.strip().splitlines()[-1])
+        print(f"[bench] serving: {serve['requests']} solves in {serve['makespan_s']} s", flush=True)
     # whole-iteration figure: compulsory bytes of one fused iteration per cell (state in and
     # out once, geometry once, one round trip of the increment between the two sweeps)
     b_it = ITER_BYTES_RANS if neq == 7 else ITER_BYTES_EULER
@@ -274,7 +346,7 @@ def main():
         "data": "synthetic (generated grid, uniform start)",
         "config": {"workload": desc, "cells": cells, "blocks": len(blocks), "implicit": "fp64"},
         "roofline": roof, "iteration_roofline": it_roof, "cpu_baseline": cpu, "fp32_implicit": fp32,
-        "kernels": kern,
+        "kernels": kern, "serving": serve,
     }
     print(json.dumps(out))
 

@@ -32,7 +32,9 @@ def test_bind_c_sizes_match_c_structs():
     want = {"trans_params": capi.TransParams, "trans_patch": capi.TransPatch,
             "trans_block_desc": capi.TransBlockDesc, "trans_fview": capi.TransFView,
             "trans_geom_view": capi.TransGeomView, "trans_state_view": capi.TransStateView,
-            "trans_resid": capi.TransResid, "trans_plane_integrals": capi.TransPlaneIntegrals}
+            "trans_resid": capi.TransResid, "trans_plane_integrals": capi.TransPlaneIntegrals,
+            "trans_srv_config": capi.TransSrvConfig, "trans_srv_request": capi.TransSrvRequest,
+            "trans_srv_status": capi.TransSrvStatus}
     for name, ty in want.items():
         assert sizes[name] == C.sizeof(ty), (name, sizes[name], C.sizeof(ty))
 

@@ -1,8 +1,10 @@
 // Serving: continuous batching of independent steady solves on one device.
 //
 // Each admitted request owns a context (its state stays resident in HBM) and advances in
-// chunks of iterations queued on that context's own non-blocking stream, so the chunks of
-// all admitted requests overlap on the device.  Requests wait in arrival order and are
+// chunks of iterations queued on that context's one non-blocking stream, so the chunks of
+// all admitted requests overlap on the device: streams are spread over the hardware queues
+// in creation order, and one stream per solve puts different solves on different queues
+// (GPU_MAX_HW_QUEUES, 4 by default, caps how many run side by side).  Requests wait in arrival order and are
 // admitted first-fit while their footprint (trans_hip_footprint) fits the budget left by the
 // resident ones and a slot is free; a finished request leaves the device at once (its final
 // state is kept on the host), which admits the next ones.  The reference runs one case per
@@ -139,7 +141,7 @@ static void srv_retire(trans_srv* s, SrvReq* r, int state, int error)
 static int srv_admit(trans_srv* s, SrvReq* r)
 {
     trans_ctx* c = nullptr;
-    int rc = trans_hip_create(&r->p, (int)r->desc.size(), r->desc.data(), &c);
+    int rc = create_ctx(&r->p, (int)r->desc.size(), r->desc.data(), &c, false, true);
     r->ctx = c;
     if (!rc)
         for (size_t b = 0; b < r->desc.size() && !rc; b++) {

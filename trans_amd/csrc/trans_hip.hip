@@ -174,6 +174,7 @@ static int fail(trans_ctx* c, int code, const std::string& msg)
 // Helper streams 0..n-1 wait for the work queued so far on the context's stream.
 static void fork_aux(trans_ctx* c, int n)
 {
+    if (c->aux[0] == c->stream) return;  // single-stream context
     (void)hipEventRecord(c->ev_fork, c->stream);
     for (int q = 0; q < n; q++) (void)hipStreamWaitEvent(c->aux[q], c->ev_fork, 0);
 }
@@ -181,6 +182,7 @@ static void fork_aux(trans_ctx* c, int n)
 // The context's stream waits for the work queued on helper streams 0..n-1.
 static void join_aux(trans_ctx* c, int n)
 {
+    if (c->aux[0] == c->stream) return;
     for (int q = 0; q < n; q++) {
         (void)hipEventRecord(c->ev_join[q], c->aux[q]);
         (void)hipStreamWaitEvent(c->stream, c->ev_join[q], 0);
@@ -281,8 +283,11 @@ static int check_params(trans_ctx* c, const trans_params* p)
     return 0;
 }
 
+// dry: sizing run only.  single_stream: the helper streams are the context's own stream, so
+// that every launch of the context is ordered on one stream (the serving layer's contexts:
+// one stream each spreads the resident solves over the device's hardware queues).
 static int create_ctx(const trans_params* params, int nblk, const trans_block_desc* blocks, trans_ctx** out,
-                      bool dry)
+                      bool dry, bool single_stream = false)
 {
     if (!params || !blocks || !out || nblk <= 0) return TRANS_EINVAL;
     trans_ctx* c = new trans_ctx();
@@ -317,10 +322,14 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     if (!dry) {
         HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         for (int q = 0; q < 3; q++) {
+            if (single_stream) {
+                c->aux[q] = c->stream;
+                continue;
+            }
             HIPCHK(c, hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
             HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming));
         }
-        HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        if (!single_stream) HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
     }
     {
         const char* env = std::getenv("TRANS_HIP_GRAPH");
@@ -497,7 +506,7 @@ int trans_hip_destroy(trans_ctx* c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (int q = 0; q < 3; q++) {
-        if (c->aux[q]) (void)hipStreamDestroy(c->aux[q]);
+        if (c->aux[q] && c->aux[q] != c->stream) (void)hipStreamDestroy(c->aux[q]);
         if (c->ev_join[q]) (void)hipEventDestroy(c->ev_join[q]);
     }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);

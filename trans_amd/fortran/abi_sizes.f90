@@ -12,6 +12,9 @@ program abi_sizes
     type(trans_state_view) :: f
     type(trans_resid) :: g
     type(trans_plane_integrals) :: h
+    type(trans_srv_config) :: q1
+    type(trans_srv_request) :: q2
+    type(trans_srv_status) :: q3
     print '(a, i0)', "trans_params ", c_sizeof(a)
     print '(a, i0)', "trans_patch ", c_sizeof(b)
     print '(a, i0)', "trans_block_desc ", c_sizeof(c)
@@ -20,4 +23,7 @@ program abi_sizes
     print '(a, i0)', "trans_state_view ", c_sizeof(f)
     print '(a, i0)', "trans_resid ", c_sizeof(g)
     print '(a, i0)', "trans_plane_integrals ", c_sizeof(h)
+    print '(a, i0)', "trans_srv_config ", c_sizeof(q1)
+    print '(a, i0)', "trans_srv_request ", c_sizeof(q2)
+    print '(a, i0)', "trans_srv_status ", c_sizeof(q3)
 end program abi_sizes

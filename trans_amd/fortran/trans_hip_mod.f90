@@ -73,6 +73,29 @@ module trans_hip_mod
         integer(c_int) :: imo, jmo, kmo
     end type trans_resid
 
+    ! serving (trans_srv_* in include/trans_hip.h)
+    integer(c_int), parameter, public :: TRANS_SRV_QUEUED = 0, TRANS_SRV_RUNNING = 1, &
+        TRANS_SRV_CONVERGED = 2, TRANS_SRV_MAX_ITER = 3, TRANS_SRV_CANCELLED = 4, TRANS_SRV_FAILED = 5
+
+    type, bind(C), public :: trans_srv_config
+        integer(c_size_t) :: hbm_budget
+        integer(c_int) :: max_active, chunk
+    end type trans_srv_config
+
+    type, bind(C), public :: trans_srv_request
+        type(trans_params) :: params
+        integer(c_int) :: nblk
+        type(c_ptr) :: blocks = c_null_ptr, geom = c_null_ptr, state = c_null_ptr
+        real(c_double) :: target_drop
+        integer(c_int) :: max_iter
+    end type trans_srv_request
+
+    type, bind(C), public :: trans_srv_status
+        integer(c_int) :: state, error, iterations, iterations_to_target
+        real(c_double) :: drsa_first, drsa_last, t_submit, t_admit, t_done
+        integer(c_size_t) :: footprint
+    end type trans_srv_status
+
     public :: trans_view, trans_error_text
 
     interface
@@ -174,12 +197,63 @@ module trans_hip_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function
+        integer(c_int) function trans_hip_footprint(params, nblk, blocks, bytes) bind(C, name="trans_hip_footprint")
+            import :: c_int, c_size_t, trans_params, trans_block_desc
+            type(trans_params), intent(in) :: params
+            integer(c_int), value :: nblk
+            type(trans_block_desc), intent(in) :: blocks(*)
+            integer(c_size_t), intent(out) :: bytes
+        end function
+        integer(c_int) function trans_srv_create(cfg, srv) bind(C, name="trans_srv_create")
+            import :: c_int, c_ptr, trans_srv_config
+            type(trans_srv_config), intent(in) :: cfg
+            type(c_ptr), intent(out) :: srv
+        end function
+        integer(c_int) function trans_srv_submit(srv, req, id) bind(C, name="trans_srv_submit")
+            import :: c_int, c_ptr, trans_srv_request
+            type(c_ptr), value :: srv
+            type(trans_srv_request), intent(in) :: req
+            integer(c_int), intent(out) :: id
+        end function
+        integer(c_int) function trans_srv_progress(srv, wait, n_unfinished) bind(C, name="trans_srv_progress")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: srv
+            integer(c_int), value :: wait
+            integer(c_int), intent(out) :: n_unfinished
+        end function
+        integer(c_int) function trans_srv_poll(srv, id, out) bind(C, name="trans_srv_poll")
+            import :: c_int, c_ptr, trans_srv_status
+            type(c_ptr), value :: srv
+            integer(c_int), value :: id
+            type(trans_srv_status), intent(out) :: out
+        end function
+        integer(c_int) function trans_srv_cancel(srv, id) bind(C, name="trans_srv_cancel")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: srv
+            integer(c_int), value :: id
+        end function
+        integer(c_int) function trans_srv_result(srv, id, blk, out) bind(C, name="trans_srv_result")
+            import :: c_int, c_ptr, trans_state_view
+            type(c_ptr), value :: srv
+            integer(c_int), value :: id, blk
+            type(trans_state_view), intent(in) :: out
+        end function
+        integer(c_int) function trans_srv_release(srv, id) bind(C, name="trans_srv_release")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: srv
+            integer(c_int), value :: id
+        end function
+        integer(c_int) function trans_srv_destroy(srv) bind(C, name="trans_srv_destroy")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: srv
+        end function
     end interface
     public :: trans_hip_create, trans_hip_destroy, trans_hip_last_error, trans_hip_set_params, &
         trans_hip_set_iteration, trans_hip_set_geometry, trans_hip_set_state, trans_hip_get_state, &
         trans_hip_get_field, trans_hip_spectral_radius, trans_hip_bc, trans_hip_space, &
         trans_hip_residual, trans_hip_lusgs_update, trans_hip_rk_update, trans_hip_update_u, trans_hip_iterate, &
-        trans_hip_plane_integrals, trans_hip_sync
+        trans_hip_plane_integrals, trans_hip_sync, trans_hip_footprint, trans_srv_create, trans_srv_submit, &
+        trans_srv_progress, trans_srv_poll, trans_srv_cancel, trans_srv_result, trans_srv_release, trans_srv_destroy
 
 contains
 
