@@ -51,8 +51,8 @@ typedef struct trans_params {
     int kchld;           /* sub-iterations per outer iteration */
     int if_rk;           /* 0 = LU-SGS, 1 = SSP-RK3 */
     int kitm;            /* first iteration with turbulence on */
-    int kind_hybrid;     /* 0 = RANS, 3 = DES/DDES */
-    int kind_subhy;
+    int kind_hybrid;     /* 0 = RANS, 3 = DES/DDES (any value > 0 keeps k-omega on from KI = 1) */
+    int kind_subhy;      /* with Kind_Hybrid 3: 0 DES, 1/3 DDES with Menter's F1, 2 DDES with F2 */
     int if_comppress;    /* SST compressibility correction */
     int kind_cc;
     int kind_rotmod;     /* must be 0 (RC_Correction.F90:107 reads an unset index otherwise) */

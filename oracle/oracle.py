@@ -59,6 +59,9 @@ def lib():
         L.ora_convect_roe.argtypes = [PP, PB]
         L.ora_convect_roe_rotated.argtypes = [PP, PB]
         L.ora_roe_rotated_face.argtypes = [PP] + [C.c_double] * 17 + [DP]
+        L.ora_fsst.argtypes = [PP] + [C.c_double] * 6
+        L.ora_fsst.restype = C.c_double
+        L.ora_sst_sources.argtypes = [PP, PB]
         L.ora_lusgs.argtypes = [PP, PB]
         L.ora_update_v.argtypes = [PP, C.c_int, C.c_int, PB]
         L.ora_update_u.argtypes = [PP, PB]

@@ -51,6 +51,9 @@ typedef struct ora_block {
 
 /* IF_turb as main.F90:59-62 derives it for iteration ki. */
 int ora_if_turb(const trans_params* p, int ki);
+double ora_fsst(const trans_params* p, double Den, double vK, double vOmega, double Dist, double RmiuL,
+                double CDiff);
+void ora_sst_sources(const trans_params* p, ora_block* b);
 
 void ora_spectral_radius(const trans_params* p, int ki, ora_block* b);
 void ora_bc(const trans_params* p, int if_turb, ora_block* blocks, int nblk, int sweep);

@@ -603,11 +603,43 @@ void ora_turb_convect_upwind(const trans_params* p, ora_block* b)
 }
 
 /* ------------------------------------------------------------------------------------ */
-/* Time/TURBTWO_TIME_LUSGS_SST.F90:2-775 (SST, RANS: Kind_Hybrid = 0, f_r1 = 1) */
-void ora_turb_lusgs_sst(const trans_params* p, ora_block* b)
+/* Time/Hybrid_function.F90:1-39 (FSST): DES/DDES shielding function */
+double ora_fsst(const trans_params* p, double Den, double vK, double vOmega, double Dist, double RmiuL,
+                double CDiff)
+{
+    const double Ref = p->ref;
+    double Fsst = 0.;
+    if (p->kind_subhy == 0) {
+        Fsst = 0.;
+    } else if (p->kind_subhy == 1 || p->kind_subhy == 3) {
+        double arg1 = sqrt(vK) / (0.09 * vOmega * Dist * Ref);
+        double arg2 = 500. * RmiuL / (Den * Dist * Dist * vOmega * Ref * Ref);
+        double arga = fmax(arg1, arg2);
+        double temp = fmax(Den * CDiff * Ref, 1.e-10);
+        double argb = 4. * Den * 0.856 * vK / (temp * Dist * Dist);
+        double arg = fmin(arga, argb);
+        Fsst = tanh(arg * arg * arg * arg);
+    } else if (p->kind_subhy == 2) {
+        double arg1 = 2. * sqrt(vK) / (0.09 * vOmega * Dist * Ref);
+        double arg2 = 500. * RmiuL / (Den * Dist * Dist * vOmega * Ref * Ref);
+        double arg = fmax(arg1, arg2);
+        Fsst = tanh(arg * arg);
+    }
+    return Fsst;
+}
+
+/* Time/TURBTWO_TIME_LUSGS_SST.F90:2-415 (and :418-547 for WD+): right-hand side, sources
+ * (SST with the DES/DDES multiplier of Kind_Hybrid = 3, or WD+; f_r1 = 1) and the diagonal
+ * F(1:2) of the k-omega LU-SGS. */
+void ora_sst_sources(const trans_params* p, ora_block* b)
 {
     const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
     const double dtm = p->dt_lusgs_main, Ref = p->ref;
+    /* Vomega and arg1 of the previous cell: the DES block (:192-214) reads both before this
+     * cell assigns them.  They are unset for the first cell of a call (a local of the
+     * reference routine); the first cell takes its own. */
+    double Vomega = 0., arg1_prev = 0.;
+    int have_prev = 0;
     for (int K = 1; K <= NK1; K++)
         for (int J = 1; J <= NJ1; J++)
             for (int I = 1; I <= NI1; I++)
@@ -682,13 +714,29 @@ void ora_turb_lusgs_sst(const trans_params* p, ora_block* b)
                 tprod = fmin(tprod, tprodl1);
                 double TT = T(I, J, K);
                 double RmiuL = sutherland(p, TT);
-                const double Fdes = 1.0;
                 double DistN = DST(I, J, K);
                 double CDKO = fmax(2.0 * ARo * p->sigo2 / (ARTO / ARo) * cdif, 1.0e-10);
                 double arg1 = fmax(sqrt(ARTk / ARo) / p->beta_star / (ARTO / ARo) / DistN / Ref,
                                    500.0 * RmiuL / DistN / DistN / ARTO / Ref / Ref);
                 arg1 = fmin(arg1, 4.0 * p->sigo2 * ARTk / CDKO / DistN / DistN);
                 arg1 = tanh(pow4(arg1));
+                double Fdes = 1.0;
+                if (p->kind_hybrid == 3) { /* :192-214 */
+                    if (!have_prev) {
+                        Vomega = ARTO / ARo;
+                        arg1_prev = arg1;
+                    }
+                    double aLt = sqrt(ARTk / ARo) / (p->beta_star * ARTO / ARo * Ref);
+                    double Cdiff_DDES_S = (2. * 0.856 / Vomega) / Ref * cdif;
+                    double Den = ARo, vk = ARTk / ARo;
+                    Vomega = ARTO / ARo;
+                    double Fsst_DES = ora_fsst(p, Den, vk, Vomega, DistN, RmiuL, Cdiff_DDES_S);
+                    double aLgrd = ((1. - arg1_prev) * 0.61 + arg1_prev * 0.78) * ALENG(I, J, K);
+                    Fdes = fmax(aLt * (1. - Fsst_DES) / aLgrd, 1.);
+                }
+                arg1_prev = arg1; /* arg1 of this cell, read by the next one */
+                Vomega = ARTO / ARo;
+                have_prev = 1;
                 /* IF_Comppress = 0 */
                 double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
                 SourceP = SourceP * b->f_r1[IX(I, J, K)];
@@ -713,7 +761,15 @@ void ora_turb_lusgs_sst(const trans_params* p, ora_block* b)
                                 (CnegTk + CnegTO) * stept;
                 F(2, I, J, K) = F(1, I, J, K);
             }
-    /* sweeps (:550-766): flux Jacobian reduces to the normal velocity */
+}
+
+/* Time/TURBTWO_TIME_LUSGS_SST.F90:2-775: sources, then the sweeps (:550-766), whose flux
+ * Jacobian reduces to the normal velocity. */
+void ora_turb_lusgs_sst(const trans_params* p, ora_block* b)
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    const double dtm = p->dt_lusgs_main;
+    ora_sst_sources(p, b);
     for (int pass = 0; pass < 2; pass++) {
         const int fwd = pass == 0;
         for (int K0 = 1; K0 <= NK1; K0++)

@@ -242,3 +242,50 @@ def test_wdp_kw_model(rk):
         for key in ("ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo"):
             assert gres[n][key] == ores[n][key], (n, key)
     sv.close()
+
+
+@pytest.mark.parametrize("sub,rk", [(0, 0), (1, 0), (2, 0), (3, 1)])
+def test_des_kw_model(sub, rk):
+    """SST-DES/DDES (Kind_Hybrid = 3, Kind_SubHY = sub): the k destruction multiplied by
+    Fdes, including the reference's carry of Vomega and arg1 from the previous cell of its
+    loop (the oracle carries them through its sequential loop; the device recomputes them
+    from the previous cell).  The DES length scale aLeng is shrunk so that Fdes > 1 in most
+    cells.  One LU-SGS update or RK stage from identical inputs at 1e-12."""
+    print(f"case: flat plate, DES Kind_SubHY={sub}, IF_RK={rk}", flush=True)
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
+    org.Kind_Hybrid, org.Kind_SubHY = 3, sub
+    if rk:
+        org.IF_RK, org.CFL_Sub_start, org.CFL_Sub_end = 1, 0.8, 0.8
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=13, turb=True)
+    for b in blocks:
+        b["aleng"][...] *= 1e-9
+    oc = OracleCase(blocks, prm)
+    # the multiplier is active: the RANS sources differ
+    p0 = capi.TransParams.from_buffer_copy(prm)
+    p0.kind_hybrid = 0
+    o0 = OracleCase(blocks, p0)
+    oc.residual(1)
+    o0.residual(1)
+    sv = Solver(blocks, prm, ki=1)
+    sv.residual()
+    if rk:
+        ores, gres = oc.rk_update(1, 1), sv.rk_update(1)
+        o0.rk_update(1, 1)
+    else:
+        ores, gres = oc.lusgs_update(1), sv.lusgs_update()
+        o0.lusgs_update(1)
+    changed = np.mean([np.mean(interior(oc[n]["v"], b)[5] != interior(o0[n]["v"], b)[5])
+                       for n, b in enumerate(blocks)])
+    print("fraction of cells whose k differs from RANS", changed, flush=True)
+    assert changed > 0.5
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        for nm in ("v", "pp", "t", "rmiu"):
+            e = relerr(interior(st[nm], b), interior(oc[n][nm], b))
+            assert np.all(np.asarray(e) < TOL), (n, nm, e)
+        for key in ("drsa", "dua", "drsk", "duk", "drso", "duo"):
+            assert abs(gres[n][key] - ores[n][key]) <= TOL * abs(ores[n][key]), (n, key)
+        for key in ("ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo"):
+            assert gres[n][key] == ores[n][key], (n, key)
+    sv.close()

@@ -1199,3 +1199,101 @@ def test_roe_rotated_turbulent_freestream_preservation():
     assert np.max(np.abs(region[:5])) < 1e-12 * flux
     fs = oc[0]["Fscheme"][sl(5, b.ni - 5), sl(8, b.nj - 5), sl(1, b.nk - 1)]
     np.testing.assert_array_equal(fs, 1.0)
+
+
+# ---------------------------------------------------------------------------------------
+# DES/DDES multiplier of the k destruction (Time/TURBTWO_TIME_LUSGS_SST.F90:192-214,
+# Time/Hybrid_function.F90:1-39)
+
+def _des_fields(b, prm, rng, dst_rows=None, aleng=1e-12):
+    """Rows j of uniform state (k, omega vary with j only); wall distance per row."""
+    V = b["v"]
+    nj = V.shape[2]
+    V[0] = 1.0
+    V[1] = 0.5
+    V[2] = 0.0
+    V[3] = 0.0
+    kj = 10.0 ** rng.uniform(-5, -3, nj)
+    oj = 10.0 ** rng.uniform(-1, 1, nj)
+    V[5] = kj[None, :, None]
+    V[6] = oj[None, :, None]
+    b["rmiu"][...] = 1e-3
+    if dst_rows is not None:
+        b["dst"][...] = dst_rows[None, :, None]
+    b["aleng"][...] = aleng
+
+
+def _sst_sources(b, prm):
+    oc = OracleCase([b], prm)
+    A = oc[0]
+    A["D"][...] = 0.0
+    A["DQdxyz"][...] = 0.0
+    A["Rds"][...] = 0.0
+    A["Dtm"][...] = 1.0
+    oc.call("ora_sst_sources", 0)
+    return A["D"].copy(), A["F"].copy()
+
+
+def capi_params_copy(prm, **changes):
+    from trans_amd import capi
+    q = capi.TransParams.from_buffer_copy(prm)
+    for k, v in changes.items():
+        setattr(q, k, v)
+    return q
+
+
+def _fdes(b, prm_des, prm_rans):
+    """Fdes from the k-equation sources of the same cells with and without the multiplier:
+    D6_rans - D6_des = beta* omega Ref (Fdes - 1) rho k."""
+    d_des, _ = _sst_sources(b, prm_des)
+    d_rans, _ = _sst_sources(b, prm_rans)
+    V = b["v"]
+    return 1.0 + (d_rans[5] - d_des[5]) / (prm_des.beta_star * V[6] / V[0] * prm_des.ref * V[5])
+
+
+def test_fsst_shielding_limits():
+    prm = _prm(Kind_Hybrid=3, Kind_SubHY=0)
+    assert lib().ora_fsst(C.byref(prm), 1.0, 1e-4, 1.0, 1e-9, 1e-3, 0.0) == 0.0
+    for sub in (1, 2, 3):
+        prm = _prm(Kind_Hybrid=3, Kind_SubHY=sub)
+        near = lib().ora_fsst(C.byref(prm), 1.0, 1e-4, 1.0, 1e-12, 1e-3, 1e-3)
+        far = lib().ora_fsst(C.byref(prm), 1.0, 1e-4, 1.0, 1e6, 1e-3, 1e-3)
+        assert near == 1.0 and 0.0 <= far < 1e-12, (sub, near, far)
+
+
+def test_des_is_rans_where_the_grid_scale_is_large():
+    """Fdes = max(L_t (1 - F) / L_grid, 1) = 1 when the grid scale dwarfs the turbulent one:
+    the sources and the diagonal are then exactly those of RANS."""
+    for sub in (0, 1, 2):
+        b, prm, _ = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1, Kind_Hybrid=3, Kind_SubHY=sub)
+        _des_fields(b, prm, np.random.Generator(np.random.MT19937(61)), aleng=1e6)
+        prm0 = capi_params_copy(prm, kind_hybrid=0)
+        d3, f3 = _sst_sources(b, prm)
+        d0, f0 = _sst_sources(b, prm0)
+        np.testing.assert_array_equal(d3, d0)
+        np.testing.assert_array_equal(f3, f0)
+
+
+def test_des_multiplier_and_its_carry_over():
+    """Strelets DES (F = 0) on rows of uniform state: rows alternate between a tiny wall
+    distance (F1 = 1, C_DES = 0.78) and a huge one (F1 = 0, C_DES = 0.61).  Fdes is
+    L_t / (C_DES aLeng) with C_DES from the previous cell of the reference's K, J, I loop:
+    the same row for i > 1, the previous row at i = 1 (the cell's own row for the first
+    cell)."""
+    b, prm, _ = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1, Kind_Hybrid=3, Kind_SubHY=0)
+    nj = b["v"].shape[2]
+    rows = np.where(np.arange(nj) % 2 == 0, 1e-9, 1e9)
+    _des_fields(b, prm, np.random.Generator(np.random.MT19937(67)), dst_rows=rows)
+    prm0 = capi_params_copy(prm, kind_hybrid=0)
+    fdes = _fdes(b, prm, prm0)
+    V = b["v"]
+    lt = np.sqrt(V[5] / V[0]) / (prm.beta_star * V[6] / V[0] * prm.ref)
+    cdes = np.where(rows < 1, 0.78, 0.61)
+    ni1, nj1, nk1 = b.ni - 1, b.nj - 1, b.nk - 1
+    for k in range(1, nk1 + 1):
+        for j in range(1, nj1 + 1):
+            for i in range(1, ni1 + 1):
+                prev_row = j if (i > 1 or (j == 1 and k == 1)) else (j - 1 if j > 1 else nj1)
+                want = lt[i + O, j + O, k + O] / (cdes[prev_row + O] * 1e-12)
+                got = fdes[i + O, j + O, k + O]
+                assert abs(got - want) <= 1e-9 * want, (i, j, k, got, want)

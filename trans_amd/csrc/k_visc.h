@@ -619,6 +619,50 @@ __global__ void __launch_bounds__(256) k_accumulate_visc(trans_params p, int if_
 // ---------------------------------------------------------------------------------------
 // Time/TURBTWO_TIME_LUSGS_SST.F90:60-415: k-omega right-hand side, SST sources (RANS,
 // IF_Comppress = 0, f_r1 = 1) and the diagonal F1t = 1 + sum(Rds)*stept/Vol + (Cneg)*stept.
+// SST blending F1 = tanh(arg^4) of cell c as the SST source forms it
+// (Time/TURBTWO_TIME_LUSGS_SST.F90:263-270)
+__device__ __forceinline__ double sst_fc1(const trans_params& p, const Fields& f, long c)
+{
+    const long tot = f.tot;
+    const double Ref = p.ref;
+    const double ARo = f.V[c], ARTk = f.V[5 * tot + c], ARTO = f.V[6 * tot + c];
+    const double* Q = f.DQ;
+    const double cdif = Q[12 * tot + c] * Q[15 * tot + c] + Q[13 * tot + c] * Q[16 * tot + c] +
+                        Q[14 * tot + c] * Q[17 * tot + c];
+    const double RmiuL = sutherland(p, f.T[c]);
+    const double DistN = f.Dst[c];
+    const double CDKO = fmax(2.0 * ARo * p.sigo2 / (ARTO / ARo) * cdif, 1.0e-10);
+    double arg1 = fmax(sqrt(ARTk / ARo) / p.beta_star / (ARTO / ARo) / DistN / Ref,
+                       500.0 * RmiuL / DistN / DistN / ARTO / Ref / Ref);
+    arg1 = fmin(arg1, 4.0 * p.sigo2 * ARTk / CDKO / DistN / DistN);
+    const double a2 = arg1 * arg1;
+    return tanh(a2 * a2);
+}
+
+// DES/DDES shielding function FSST (Time/Hybrid_function.F90:1-39): 0 for Kind_SubHY 0
+// (Strelets DES), Menter's F1 for 1 and 3, F2 for 2
+__device__ __forceinline__ double fsst_des(const trans_params& p, double Den, double vK, double vOmega, double Dist,
+                                           double RmiuL, double CDiff)
+{
+    const double Ref = p.ref;
+    if (p.kind_subhy == 1 || p.kind_subhy == 3) {
+        const double arg1 = sqrt(vK) / (0.09 * vOmega * Dist * Ref);
+        const double arg2 = 500. * RmiuL / (Den * Dist * Dist * vOmega * Ref * Ref);
+        const double arga = fmax(arg1, arg2);
+        const double temp = fmax(Den * CDiff * Ref, 1.e-10);
+        const double argb = 4. * Den * 0.856 * vK / (temp * Dist * Dist);
+        const double arg = fmin(arga, argb);
+        return tanh(arg * arg * arg * arg);
+    }
+    if (p.kind_subhy == 2) {
+        const double arg1 = 2. * sqrt(vK) / (0.09 * vOmega * Dist * Ref);
+        const double arg2 = 500. * RmiuL / (Den * Dist * Dist * vOmega * Ref * Ref);
+        const double arg = fmax(arg1, arg2);
+        return tanh(arg * arg);
+    }
+    return 0.;
+}
+
 __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
                                                  const Seg* __restrict__ seg)
 {
@@ -681,16 +725,27 @@ __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const
         d7 = d7 + (SourceP + SourceN1 * ARTO + SourceCross);
         CnegTO = 2.0 * fabs(SourceN1);
     } else {  // SST (:139-415)
-        const double RmiuL = sutherland(p, f.T[c]);
-        const double Fdes = 1.0;
-        const double DistN = f.Dst[c];
-        const double CDKO = fmax(2.0 * ARo * p.sigo2 / (ARTO / ARo) * cdif, 1.0e-10);
-        double arg1 = fmax(sqrt(ARTk / ARo) / p.beta_star / (ARTO / ARo) / DistN / Ref,
-                           500.0 * RmiuL / DistN / DistN / ARTO / Ref / Ref);
-        arg1 = fmin(arg1, 4.0 * p.sigo2 * ARTk / CDKO / DistN / DistN);
-        {
-            const double a2 = arg1 * arg1;
-            arg1 = tanh(a2 * a2);
+        const double arg1 = sst_fc1(p, f, c);
+        double Fdes = 1.0;
+        if (p.kind_hybrid == 3) {
+            // DES/DDES destruction multiplier (:192-214).  The reference reads Vomega and arg1
+            // there before assigning them in the cell, i.e. the values left by the previous cell
+            // of its K, J, I loop (for a block's first cell they are unset: its own are taken)
+            const double RmiuL = sutherland(p, f.T[c]);
+            const double aLt = sqrt(ARTk / ARo) / (p.beta_star * ARTO / ARo * Ref);
+            long cp = c;
+            if (i > 1)
+                cp = cidx(b, i - 1, j, k);
+            else if (j > 1)
+                cp = cidx(b, NI1, j - 1, k);
+            else if (k > 1)
+                cp = cidx(b, NI1, NJ1, k - 1);
+            const double Vomega_prev = f.V[6 * tot + cp] / f.V[cp];
+            const double arg1_prev = cp == c ? arg1 : sst_fc1(p, f, cp);
+            const double Cdiff_DDES_S = (2. * 0.856 / Vomega_prev) / Ref * cdif;
+            const double Fsst_DES = fsst_des(p, ARo, ARTk / ARo, ARTO / ARo, f.Dst[c], RmiuL, Cdiff_DDES_S);
+            const double aLgrd = ((1. - arg1_prev) * 0.61 + arg1_prev * 0.78) * f.aLeng[c];
+            Fdes = fmax(aLt * (1. - Fsst_DES) / aLgrd, 1.);
         }
         double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
         SourceP = SourceP * 1.0;  // f_r1 = 1 (Kind_RotMod = 0)

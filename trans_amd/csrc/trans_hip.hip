@@ -277,7 +277,9 @@ static int check_params(trans_ctx* c, const trans_params* p)
     if (p->if_euler == 0 && p->if_turb_org == 1) {
         if (p->kind_model != 1 && p->kind_model != 2)
             return fail(c, TRANS_EUNSUPPORTED, "Kind_Model must be 1 (SST) or 2 (WD+ k-omega)");
-        if (p->kind_hybrid != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_Hybrid must be 0 (RANS)");
+        if (p->kind_hybrid < 0) return fail(c, TRANS_EINVAL, "Kind_Hybrid must be >= 0");
+        if (p->kind_hybrid == 3 && (p->kind_subhy < 0 || p->kind_subhy > 3))
+            return fail(c, TRANS_EUNSUPPORTED, "Kind_SubHY must be 0-3 (DES, DDES); IDDES (5) is not supported");
         if (p->if_comppress != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_Comppress must be 0");
     }
     return 0;
