@@ -52,7 +52,7 @@ typedef struct trans_params {
     int if_rk;           /* 0 = LU-SGS, 1 = SSP-RK3 */
     int kitm;            /* first iteration with turbulence on */
     int kind_hybrid;     /* 0 = RANS, 3 = DES/DDES (any value > 0 keeps k-omega on from KI = 1) */
-    int kind_subhy;      /* with Kind_Hybrid 3: 0 DES, 1/3 DDES with Menter's F1, 2 DDES with F2 */
+    int kind_subhy;      /* with Kind_Hybrid 3: 0 DES, 1/3 DDES with Menter's F1, 2 DDES with F2, 5 IDDES */
     int if_comppress;    /* SST compressibility correction */
     int kind_cc;
     int kind_rotmod;     /* must be 0 (RC_Correction.F90:107 reads an unset index otherwise) */
@@ -133,6 +133,7 @@ typedef struct trans_geom_view {
     trans_fview rad;   /* rad = sqrt(yc^2+zc^2) */
     trans_fview dst;   /* wall distance */
     trans_fview aleng; /* largest neighbour distance (DES length) */
+    trans_fview aseng; /* smallest neighbour distance (IDDES; may be unset otherwise) */
     trans_fview yc, zc;
     trans_fview thtc;  /* circumferential angle atan2(zc, yc) (outlet radial equilibrium) */
     /* MARKWALLi0(NJ1,NK1), MARKWALLim, j0(NI1,NK1), jm, k0(NI1,NJ1), km; NULL = none */

@@ -24,7 +24,7 @@ class OraBlock(C.Structure):
                 ("patches", C.POINTER(capi.TransPatch)),
                 ("V", DP), ("PP", DP), ("T", DP), ("Rmiu", DP),
                 ("SD", DP), ("Grad", DP), ("Alagm", DP),
-                ("Vol", DP), ("rad", DP), ("Dst", DP), ("aLeng", DP), ("Yc", DP), ("Zc", DP),
+                ("Vol", DP), ("rad", DP), ("Dst", DP), ("aLeng", DP), ("aSeng", DP), ("Yc", DP), ("Zc", DP),
                 ("thtc", DP),
                 ("mw", C.POINTER(C.c_int) * 6),
                 ("VL", DP), ("VR", DP), ("rccl", DP), ("rccr", DP), ("D", DP), ("F", DP),
@@ -62,6 +62,10 @@ def lib():
         L.ora_fsst.argtypes = [PP] + [C.c_double] * 6
         L.ora_fsst.restype = C.c_double
         L.ora_sst_sources.argtypes = [PP, PB]
+        L.ora_iddes_fd.argtypes = [C.c_double] * 4 + [DP, C.c_double]
+        L.ora_iddes_fd.restype = C.c_double
+        L.ora_iddes_fe.argtypes = [C.c_double] * 5 + [DP, C.c_double]
+        L.ora_iddes_fe.restype = C.c_double
         L.ora_lusgs.argtypes = [PP, PB]
         L.ora_update_v.argtypes = [PP, C.c_int, C.c_int, PB]
         L.ora_update_u.argtypes = [PP, PB]
@@ -129,7 +133,7 @@ class OracleCase:
             o.patches = C.cast(pa, C.POINTER(capi.TransPatch))
             for f, key in (("V", "v"), ("PP", "pp"), ("T", "t"), ("Rmiu", "rmiu"), ("SD", "sd"),
                            ("Grad", "grad"), ("Alagm", "alagm"), ("Vol", "vol"), ("rad", "rad"),
-                           ("Dst", "dst"), ("aLeng", "aleng"), ("Yc", "yc"), ("Zc", "zc"),
+                           ("Dst", "dst"), ("aLeng", "aleng"), ("aSeng", "aseng"), ("Yc", "yc"), ("Zc", "zc"),
                            ("thtc", "thtc"), ("Wt0", "wt0"), ("Wt1", "wt1"), ("Wt2", "wt2")):
                 setattr(o, f, _p(A[key]))
             for f in ("VL", "VR", "rccl", "rccr", "D", "F", "Rds", "Dtm", "DQdxyz", "shock",

@@ -34,4 +34,5 @@
 #define FSCHEME(i, j, k) b->Fscheme[IX(i, j, k)]
 #define F_R1(i, j, k) b->f_r1[IX(i, j, k)]
 #define ALENG(i, j, k) b->aLeng[IX(i, j, k)]
+#define ASENG(i, j, k) b->aSeng[IX(i, j, k)]
 

@@ -35,7 +35,7 @@ typedef struct ora_block {
     double *V, *PP, *T, *Rmiu;         /* V: 7 comps */
     /* geometry */
     double *SD, *Grad, *Alagm;         /* 9, 3, 3 comps */
-    double *Vol, *rad, *Dst, *aLeng, *Yc, *Zc, *thtc; /* scalars */
+    double *Vol, *rad, *Dst, *aLeng, *aSeng, *Yc, *Zc, *thtc; /* scalars */
     const int* mw[6];                  /* MARKWALLi0, im, j0, jm, k0, km */
     /* work */
     double *VL, *VR;                   /* (3,8): comp (d-1)+3*(L-1) */
@@ -54,6 +54,9 @@ int ora_if_turb(const trans_params* p, int ki);
 double ora_fsst(const trans_params* p, double Den, double vK, double vOmega, double Dist, double RmiuL,
                 double CDiff);
 void ora_sst_sources(const trans_params* p, ora_block* b);
+double ora_iddes_fd(double Den1, double RmiuT1, double aDst1, double aLmax1, const double g[9], double Ref);
+double ora_iddes_fe(double Den2, double RmiuT2, double RMIUL2, double aDst2, double aLmax2, const double g[9],
+                    double Ref);
 
 void ora_spectral_radius(const trans_params* p, int ki, ora_block* b);
 void ora_bc(const trans_params* p, int if_turb, ora_block* blocks, int nblk, int sweep);

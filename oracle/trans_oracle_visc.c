@@ -628,6 +628,44 @@ double ora_fsst(const trans_params* p, double Den, double vK, double vOmega, dou
     return Fsst;
 }
 
+/* Time/Hybrid_function.F90:42-68 (fd_function), kappa = 0.41 (Module_Global.F90:126) */
+double ora_iddes_fd(double Den1, double RmiuT1, double aDst1, double aLmax1, const double g[9], double Ref)
+{
+    double alpha1 = 0.25 - aDst1 / aLmax1;
+    double fb = fmin(2. * exp(-9.0 * alpha1 * alpha1), 1.);
+    double Sij_square = g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + g[3] * g[3] + g[4] * g[4] + g[5] * g[5] +
+                        g[6] * g[6] + g[7] * g[7] + g[8] * g[8];
+    Sij_square = sqrt(Sij_square);
+    Sij_square = fmax(Sij_square, 1.e-10);
+    const double akapa = 0.41;
+    double rdt = (RmiuT1 / Den1) / (akapa * akapa * (aDst1 * aDst1) * Sij_square * Ref);
+    double fdt = 1. - tanh((8. * rdt) * (8. * rdt) * (8. * rdt));
+    return fmax((1. - fdt), fb);
+}
+
+/* Time/Hybrid_function.F90:72-105 (fe_function), Ct = 1.87, Cl = 5 */
+double ora_iddes_fe(double Den2, double RmiuT2, double RMIUL2, double aDst2, double aLmax2, const double g[9],
+                    double Ref)
+{
+    double alpha2 = 0.25 - aDst2 / aLmax2;
+    double fe1;
+    if (alpha2 >= 0.)
+        fe1 = 2. * exp(-11.09 * alpha2 * alpha2);
+    else
+        fe1 = 2. * exp(-9. * alpha2 * alpha2);
+    double Sij_square = g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + g[3] * g[3] + g[4] * g[4] + g[5] * g[5] +
+                        g[6] * g[6] + g[7] * g[7] + g[8] * g[8];
+    Sij_square = sqrt(Sij_square);
+    Sij_square = fmax(Sij_square, 1.e-10);
+    const double akapa = 0.41, Ct = 1.87, Cl = 5.00;
+    double rdt = (RmiuT2 / Den2) / (akapa * akapa * (aDst2 * aDst2) * Sij_square * Ref);
+    double ft = tanh((Ct * Ct * rdt) * (Ct * Ct * rdt) * (Ct * Ct * rdt));
+    double rdl = (RMIUL2 / Den2) / (akapa * akapa * (aDst2 * aDst2) * Sij_square * Ref);
+    double fl = tanh(pow(Cl * Cl * rdl, 10.));
+    double fe2 = 1. - fmax(ft, fl);
+    return fe2 * fmax((fe1 - 1.), 0.);
+}
+
 /* Time/TURBTWO_TIME_LUSGS_SST.F90:2-415 (and :418-547 for WD+): right-hand side, sources
  * (SST with the DES/DDES multiplier of Kind_Hybrid = 3, or WD+; f_r1 = 1) and the diagonal
  * F(1:2) of the k-omega LU-SGS. */
@@ -720,8 +758,8 @@ void ora_sst_sources(const trans_params* p, ora_block* b)
                                    500.0 * RmiuL / DistN / DistN / ARTO / Ref / Ref);
                 arg1 = fmin(arg1, 4.0 * p->sigo2 * ARTk / CDKO / DistN / DistN);
                 arg1 = tanh(pow4(arg1));
-                double Fdes = 1.0;
-                if (p->kind_hybrid == 3) { /* :192-214 */
+                double Fdes = 1.0, alIDDES = 0.0;
+                if (p->kind_hybrid == 3) { /* :192-247 */
                     if (!have_prev) {
                         Vomega = ARTO / ARo;
                         arg1_prev = arg1;
@@ -733,23 +771,65 @@ void ora_sst_sources(const trans_params* p, ora_block* b)
                     double Fsst_DES = ora_fsst(p, Den, vk, Vomega, DistN, RmiuL, Cdiff_DDES_S);
                     double aLgrd = ((1. - arg1_prev) * 0.61 + arg1_prev * 0.78) * ALENG(I, J, K);
                     Fdes = fmax(aLt * (1. - Fsst_DES) / aLgrd, 1.);
+                    if (p->kind_subhy == 5) { /* IDDES, Cw = 0.15 */
+                        double aLmax = ALENG(I, J, K), aDst = DST(I, J, K), aLmin = ASENG(I, J, K);
+                        double Cdes = (1. - arg1_prev) * 0.61 + arg1_prev * 0.78;
+                        double dlt_IDDES = fmin(fmax(fmax(0.15 * aLmax, 0.15 * aDst), aLmin), aLmax);
+                        aLgrd = Cdes * dlt_IDDES;
+                        const double g9[9] = {dudx, dudy, dudz, dvdx, dvdy, dvdz, dwdx, dwdy, dwdz};
+                        double fd = ora_iddes_fd(Den, RMIU(I, J, K), aDst, aLmax, g9, Ref);
+                        double fe = ora_iddes_fe(Den, RMIU(I, J, K), RmiuL, aDst, aLmax, g9, Ref);
+                        alIDDES = (1. + fe) * fd * aLt + (1. - fd) * aLgrd;
+                    }
                 }
                 arg1_prev = arg1; /* arg1 of this cell, read by the next one */
                 Vomega = ARTO / ARo;
                 have_prev = 1;
-                /* IF_Comppress = 0 */
-                double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
-                SourceP = SourceP * b->f_r1[IX(I, J, K)];
-                double SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p->beta_star * ARTO / ARo * Ref * Fdes;
-                double Source = SourceP + SourceN1 * ARTk;
-                double CnegTk = fabs(SourceN1);
-                D(6, I, J, K) = D(6, I, J, K) + Source;
-                tprod = fmin(prod, Votg) / Ref;
-                double GamaAB = p->gama1 * arg1 + p->gama2 * (1.0 - arg1);
-                double BetaAB = p->beta1 * arg1 + p->beta2 * (1.0 - arg1);
-                SourceP = GamaAB * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
-                SourceP = SourceP * b->f_r1[IX(I, J, K)];
-                SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) - BetaAB * ARTO / ARo * Ref;
+                double AA = sqrt(1.4 * PP(I, J, K) / ARo); /* :271-273 */
+                double Mt = sqrt(ARTk / ARo) / AA;
+                const double Alf1 = 1.0, Alf2 = 0.4, Alf3 = 0.2; /* :75-76 */
+                double SourceP, SourceN1, Source, CnegTk, GamaAB, BetaAB;
+                if (p->if_comppress == 0) { /* :282-330 */
+                    SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+                    SourceP = SourceP * b->f_r1[IX(I, J, K)];
+                    SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p->beta_star * ARTO / ARo * Ref * Fdes;
+                    Source = SourceP + SourceN1 * ARTk;
+                    CnegTk = fabs(SourceN1);
+                    if (p->kind_hybrid == 3 && p->kind_subhy == 5) { /* :299-311 */
+                        SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - sqrt(ARTk / ARo) / alIDDES;
+                        Source = SourceP + SourceN1 * ARTk;
+                        CnegTk = fabs(-2.0 / 3.0 * fmax(Div, 0.0) - 1.5 * sqrt(ARTk / ARo) / alIDDES);
+                    }
+                    D(6, I, J, K) = D(6, I, J, K) + Source;
+                    tprod = fmin(prod, Votg) / Ref;
+                    GamaAB = p->gama1 * arg1 + p->gama2 * (1.0 - arg1);
+                    BetaAB = p->beta1 * arg1 + p->beta2 * (1.0 - arg1);
+                    SourceP = GamaAB * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
+                    SourceP = SourceP * b->f_r1[IX(I, J, K)];
+                    SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) - BetaAB * ARTO / ARo * Ref;
+                } else { /* Suzen (:331-396) */
+                    SourceP = (1. - (1. - arg1) * Alf2 * Mt * Mt) * tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+                    SourceP = SourceP * b->f_r1[IX(I, J, K)];
+                    SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) -
+                               (1. + (1. - arg1) * (Alf1 - Alf3) * Mt * Mt) * p->beta_star * ARTO / ARo * Ref * Fdes;
+                    Source = SourceP + SourceN1 * ARTk;
+                    CnegTk = fabs(SourceN1);
+                    if (p->kind_hybrid == 3 && p->kind_subhy == 5) { /* :354-366 */
+                        SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) -
+                                   (1. + (1. - arg1) * (Alf1 - Alf3) * Mt * Mt) * sqrt(ARTk / ARo) / alIDDES;
+                        Source = SourceP + SourceN1 * ARTk;
+                        CnegTk = fabs(-2.0 / 3.0 * fmax(Div, 0.0) - 1.5 * sqrt(ARTk / ARo) / alIDDES);
+                    }
+                    D(6, I, J, K) = D(6, I, J, K) + Source;
+                    tprod = fmin(prod, Votg) / Ref;
+                    GamaAB = p->gama1 * arg1 + p->gama2 * (1.0 - arg1);
+                    BetaAB = p->beta1 * arg1 + p->beta2 * (1.0 - arg1);
+                    SourceP = (GamaAB + (1. - arg1) * Alf2 * Mt * Mt) * ARo * tprod -
+                              2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
+                    SourceP = SourceP * b->f_r1[IX(I, J, K)];
+                    SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) -
+                               (1. - (1. - arg1) * (Alf1 - Alf3) * Mt * Mt) * BetaAB * ARTO / ARo * Ref;
+                }
                 Source = SourceP + SourceN1 * ARTO;
                 double SourceCross = 2.0 * (1 - arg1) * ARo * p->sigo2 / (ARTO / ARo) * cdif / Ref;
                 SourceCross = copysign(1.0, SourceCross) * fmin(fabs(SourceCross), fabs(SourceP) * 0.5);

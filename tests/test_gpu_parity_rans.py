@@ -244,13 +244,14 @@ def test_wdp_kw_model(rk):
     sv.close()
 
 
-@pytest.mark.parametrize("sub,rk", [(0, 0), (1, 0), (2, 0), (3, 1)])
+@pytest.mark.parametrize("sub,rk", [(0, 0), (1, 0), (2, 0), (3, 1), (5, 0), (5, 1)])
 def test_des_kw_model(sub, rk):
     """SST-DES/DDES (Kind_Hybrid = 3, Kind_SubHY = sub): the k destruction multiplied by
     Fdes, including the reference's carry of Vomega and arg1 from the previous cell of its
     loop (the oracle carries them through its sequential loop; the device recomputes them
     from the previous cell).  The DES length scale aLeng is shrunk so that Fdes > 1 in most
-    cells.  One LU-SGS update or RK stage from identical inputs at 1e-12."""
+    cells.  Kind_SubHY = 5 is IDDES (fd, fe, the min/max cell lengths).  One LU-SGS update
+    or RK stage from identical inputs at 1e-12."""
     print(f"case: flat plate, DES Kind_SubHY={sub}, IF_RK={rk}", flush=True)
     blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
     org.Kind_Hybrid, org.Kind_SubHY = 3, sub
@@ -260,6 +261,7 @@ def test_des_kw_model(sub, rk):
     perturb(blocks, prm, seed=13, turb=True)
     for b in blocks:
         b["aleng"][...] *= 1e-9
+        b["aseng"][...] *= 1e-9
     oc = OracleCase(blocks, prm)
     # the multiplier is active: the RANS sources differ
     p0 = capi.TransParams.from_buffer_copy(prm)
@@ -279,6 +281,32 @@ def test_des_kw_model(sub, rk):
                        for n, b in enumerate(blocks)])
     print("fraction of cells whose k differs from RANS", changed, flush=True)
     assert changed > 0.5
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        for nm in ("v", "pp", "t", "rmiu"):
+            e = relerr(interior(st[nm], b), interior(oc[n][nm], b))
+            assert np.all(np.asarray(e) < TOL), (n, nm, e)
+        for key in ("drsa", "dua", "drsk", "duk", "drso", "duo"):
+            assert abs(gres[n][key] - ores[n][key]) <= TOL * abs(ores[n][key]), (n, key)
+        for key in ("ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo"):
+            assert gres[n][key] == ores[n][key], (n, key)
+    sv.close()
+
+
+@pytest.mark.parametrize("hybrid,sub", [(0, 0), (3, 1), (3, 5)])
+def test_suzen_compressibility(hybrid, sub):
+    """IF_Comppress = 1 (Suzen's correction with the turbulent Mach number), with RANS,
+    DDES (Kind_SubHY = 1) or IDDES (5): one LU-SGS update from identical inputs at 1e-12."""
+    print(f"case: flat plate, IF_Comppress=1, Kind_Hybrid={hybrid}, Kind_SubHY={sub}", flush=True)
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
+    org.IF_Comppress, org.Kind_Hybrid, org.Kind_SubHY = 1, hybrid, sub
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=17, turb=True)
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    oc.residual(1)
+    sv.residual()
+    ores, gres = oc.lusgs_update(1), sv.lusgs_update()
     for n, b in enumerate(blocks):
         st = sv.get_state(n)
         for nm in ("v", "pp", "t", "rmiu"):

@@ -1297,3 +1297,101 @@ def test_des_multiplier_and_its_carry_over():
                 want = lt[i + O, j + O, k + O] / (cdes[prev_row + O] * 1e-12)
                 got = fdes[i + O, j + O, k + O]
                 assert abs(got - want) <= 1e-9 * want, (i, j, k, got, want)
+
+
+def test_suzen_compressibility_closed_form():
+    """IF_Comppress = 1 (Time/TURBTWO_TIME_LUSGS_SST.F90:331-396) against IF_Comppress = 0 on
+    a state without gradients (no production, no cross diffusion, Fdes = 1): where F1 = 0 the
+    k destruction grows by 0.8 Mt^2 beta* omega rho k and the omega destruction shrinks by
+    0.8 Mt^2 beta_2 omega rho omega (Mt^2 = k / (1.4 p / rho)); where F1 = 1 nothing changes."""
+    b, prm, _ = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1)
+    nj = b["v"].shape[2]
+    rows = np.where(np.arange(nj) % 2 == 0, 1e-9, 1e9)  # F1 = 1 on even rows, 0 on odd ones
+    rng = np.random.Generator(np.random.MT19937(71))
+    _des_fields(b, prm, rng, dst_rows=rows, aleng=1.0)
+    V = b["v"]
+    V[5] = 10.0 ** rng.uniform(-2, -0.3, V[5].shape)
+    b["pp"][...] = 1 / (1.4 * 0.25) * (1 + 0.1 * rng.uniform(-1, 1, b["pp"].shape))
+    ps = capi_params_copy(prm, if_comppress=1)
+    d1, _ = _sst_sources(b, ps)
+    d0, _ = _sst_sources(b, prm)
+    mt2 = (V[5] / V[0]) / (1.4 * b["pp"] / V[0])
+    om = V[6] / V[0]
+    I, K = sl(1, b.ni - 1), sl(1, b.nk - 1)
+    for j in range(1, b.nj - 1 + 1):
+        J = j + O
+        if rows[J] < 1:
+            np.testing.assert_array_equal(d1[5:7, I, J, K], d0[5:7, I, J, K])
+        else:
+            dk = -(1.0 - 0.2) * mt2 * prm.beta_star * om * prm.ref * V[5]
+            do = (1.0 - 0.2) * mt2 * prm.beta2 * om * prm.ref * V[6]
+            np.testing.assert_allclose(d1[5, I, J, K] - d0[5, I, J, K], dk[I, J, K], rtol=1e-9)
+            np.testing.assert_allclose(d1[6, I, J, K] - d0[6, I, J, K], do[I, J, K], rtol=1e-9)
+
+
+# ---------------------------------------------------------------------------------------
+# IDDES (Kind_SubHY = 5; Time/Hybrid_function.F90:42-105, Time/TURBTWO_TIME_LUSGS_SST.F90:
+# 218-247, 299-311), checked against the published functions of Shur et al. (2008):
+# fb = min(2 exp(-9 a^2), 1), fe1 = 2 exp(-11.09 a^2) (a >= 0) or 2 exp(-9 a^2), a = 1/4 - d/h
+
+def _fdfe(dst, hmax, mut, mul, g=None, ref=1.0):
+    g9 = (C.c_double * 9)(*(g if g is not None else [0.0] * 9))
+    fd = lib().ora_iddes_fd(1.0, mut, dst, hmax, g9, ref)
+    fe = lib().ora_iddes_fe(1.0, mut, mul, dst, hmax, g9, ref)
+    return fd, fe
+
+
+def test_iddes_functions_against_published_forms():
+    for ratio in (0.1, 0.25, 0.4, 0.75, 1.5):
+        a = 0.25 - ratio
+        fb = min(2 * np.exp(-9 * a * a), 1.0)
+        fe1 = 2 * np.exp((-11.09 if a >= 0 else -9.0) * a * a)
+        # no eddy or molecular viscosity: f_dt = 1 (fd = fb), f_t = f_l = 0 (fe = max(fe1 - 1, 0))
+        fd, fe = _fdfe(ratio, 1.0, 0.0, 0.0)
+        assert fd == pytest.approx(fb, rel=1e-14) and fe == pytest.approx(max(fe1 - 1, 0.0), rel=1e-14, abs=1e-300)
+        # an overwhelming eddy viscosity: r_dt -> inf, f_dt = 0, fd = 1, and f_t = 1 kills fe
+        fd, fe = _fdfe(ratio, 1.0, 1e30, 0.0, g=[1.0] * 9)
+        assert fd == 1.0 and fe == 0.0
+
+
+def _iddes_box(mut, seed):
+    b, prm, _ = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1, Kind_Hybrid=3, Kind_SubHY=5)
+    nj = b["v"].shape[2]
+    rows = np.where(np.arange(nj) % 2 == 0, 1e-9, 1e9)
+    _des_fields(b, prm, np.random.Generator(np.random.MT19937(seed)), dst_rows=rows, aleng=1.0)
+    b["aseng"][...] = 0.5
+    b["rmiu"][...] = mut
+    return b, prm, rows
+
+
+def test_iddes_rans_and_les_limits():
+    """RANS limit (fd = 1, fe = 0 with an overwhelming mu_t and d/h >= 0.53): the k source is
+    RANS's.  LES limit (mu_t = 0, d >> h: fd = fe = 0): the k destruction is
+    sqrt(k) / (C_DES Delta) rho k with Delta = min(max(Cw h, Cw d, h_min), h) = h and C_DES
+    from the previous cell of the loop (0.61 after an F1 = 0 cell, 0.78 after an F1 = 1 one)."""
+    b, prm, rows = _iddes_box(1e30, 73)
+    prm0 = capi_params_copy(prm, kind_hybrid=0)
+    odd = rows > 1
+    d5, _ = _sst_sources(b, prm)
+    d0, _ = _sst_sources(b, prm0)
+    I, K = sl(1, b.ni - 1), sl(1, b.nk - 1)
+    for j in range(1, b.nj):
+        if odd[j + O]:
+            np.testing.assert_allclose(d5[5, I, j + O, K], d0[5, I, j + O, K], rtol=1e-13)
+    b, prm, rows = _iddes_box(0.0, 79)
+    prm0 = capi_params_copy(prm, kind_hybrid=0)
+    d5, _ = _sst_sources(b, prm)
+    d0, _ = _sst_sources(b, prm0)
+    V = b["v"]
+    for k in range(1, b.nk):
+        for j in range(1, b.nj):
+            if not odd[j + O]:
+                continue
+            for i in range(1, b.ni):
+                c = (i + O, j + O, k + O)
+                prev_row = j if (i > 1 or (j == 1 and k == 1)) else (j - 1 if j > 1 else b.nj - 1)
+                cdes = 0.78 if rows[prev_row + O] < 1 else 0.61
+                rk, om = V[(5,) + c] / V[(0,) + c], V[(6,) + c] / V[(0,) + c]
+                want = (-np.sqrt(rk) / (cdes * 1.0) + prm.beta_star * om * prm.ref) * V[(5,) + c]
+                got = d5[(5,) + c] - d0[(5,) + c]
+                assert abs(got - want) <= 1e-9 * abs(want), (i, j, k, got, want)

@@ -69,7 +69,7 @@ class TransFView(C.Structure):
 class TransGeomView(C.Structure):
     _fields_ = [("sd", TransFView), ("grad", TransFView), ("alagm", TransFView),
                 ("vol", TransFView), ("rad", TransFView), ("dst", TransFView),
-                ("aleng", TransFView), ("yc", TransFView), ("zc", TransFView),
+                ("aleng", TransFView), ("aseng", TransFView), ("yc", TransFView), ("zc", TransFView),
                 ("thtc", TransFView),
                 ("markwall", C.POINTER(C.c_int) * 6)]
 

@@ -40,6 +40,7 @@ struct Fields {
     double* rad;
     double* Dst;
     double* aLeng;
+    double* aSeng;    // smallest neighbour distance (IDDES)
     double* Yc;
     double* Zc;
     double* thtc;

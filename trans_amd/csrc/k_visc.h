@@ -663,6 +663,31 @@ __device__ __forceinline__ double fsst_des(const trans_params& p, double Den, do
     return 0.;
 }
 
+// IDDES functions (Time/Hybrid_function.F90:42-105; kappa = 0.41, Ct = 1.87, Cl = 5,
+// Module_Global.F90:126).  S = max(|grad u|, 1e-10) (Frobenius norm of the 9 gradients).
+__device__ __forceinline__ double iddes_fd(double Den, double RmiuT, double aDst, double aLmax, double Sn, double Ref)
+{
+    const double alpha1 = 0.25 - aDst / aLmax;
+    const double fb = fmin(2. * exp(-9.0 * alpha1 * alpha1), 1.);
+    const double rdt = (RmiuT / Den) / (0.41 * 0.41 * (aDst * aDst) * Sn * Ref);
+    const double fdt = 1. - tanh((8. * rdt) * (8. * rdt) * (8. * rdt));
+    return fmax(1. - fdt, fb);
+}
+
+__device__ __forceinline__ double iddes_fe(double Den, double RmiuT, double RmiuL, double aDst, double aLmax, double Sn,
+                                           double Ref)
+{
+    const double alpha2 = 0.25 - aDst / aLmax;
+    const double fe1 = alpha2 >= 0. ? 2. * exp(-11.09 * alpha2 * alpha2) : 2. * exp(-9. * alpha2 * alpha2);
+    const double rdt = (RmiuT / Den) / (0.41 * 0.41 * (aDst * aDst) * Sn * Ref);
+    const double ct2 = 1.87 * 1.87;
+    const double ft = tanh((ct2 * rdt) * (ct2 * rdt) * (ct2 * rdt));
+    const double rdl = (RmiuL / Den) / (0.41 * 0.41 * (aDst * aDst) * Sn * Ref);
+    const double fl = tanh(pow(5.00 * 5.00 * rdl, 10.));
+    const double fe2 = 1. - fmax(ft, fl);
+    return fe2 * fmax((fe1 - 1.), 0.);
+}
+
 __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
                                                  const Seg* __restrict__ seg)
 {
@@ -727,11 +752,12 @@ __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const
     } else {  // SST (:139-415)
         const double arg1 = sst_fc1(p, f, c);
         double Fdes = 1.0;
+        double alIDDES = 0.0;  // IDDES length scale (Kind_SubHY 5)
         if (p.kind_hybrid == 3) {
             // DES/DDES destruction multiplier (:192-214).  The reference reads Vomega and arg1
             // there before assigning them in the cell, i.e. the values left by the previous cell
             // of its K, J, I loop (for a block's first cell they are unset: its own are taken)
-            const double RmiuL = sutherland(p, f.T[c]);
+            const double RmiuL = sutherland(p, f.T[c]);  // mean-flow form (:185)
             const double aLt = sqrt(ARTk / ARo) / (p.beta_star * ARTO / ARo * Ref);
             long cp = c;
             if (i > 1)
@@ -746,19 +772,60 @@ __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const
             const double Fsst_DES = fsst_des(p, ARo, ARTk / ARo, ARTO / ARo, f.Dst[c], RmiuL, Cdiff_DDES_S);
             const double aLgrd = ((1. - arg1_prev) * 0.61 + arg1_prev * 0.78) * f.aLeng[c];
             Fdes = fmax(aLt * (1. - Fsst_DES) / aLgrd, 1.);
+            if (p.kind_subhy == 5) {  // IDDES (:218-247); its k destruction replaces Fdes's below
+                const double aLmax = f.aLeng[c], aDst = f.Dst[c], aLmin = f.aSeng[c];
+                const double Cdes = (1. - arg1_prev) * 0.61 + arg1_prev * 0.78;
+                const double dlt_IDDES = fmin(fmax(fmax(0.15 * aLmax, 0.15 * aDst), aLmin), aLmax);
+                const double aLgrd_i = Cdes * dlt_IDDES;
+                double Sn = dudx * dudx + dudy * dudy + dudz * dudz + dvdx * dvdx + dvdy * dvdy + dvdz * dvdz +
+                            dwdx * dwdx + dwdy * dwdy + dwdz * dwdz;
+                Sn = fmax(sqrt(Sn), 1.e-10);
+                const double RmiuT = f.Rmiu[c];
+                const double fd = iddes_fd(ARo, RmiuT, aDst, aLmax, Sn, Ref);
+                const double fe = iddes_fe(ARo, RmiuT, RmiuL, aDst, aLmax, Sn, Ref);
+                alIDDES = (1. + fe) * fd * aLt + (1. - fd) * aLgrd_i;
+            }
         }
-        double SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
-        SourceP = SourceP * 1.0;  // f_r1 = 1 (Kind_RotMod = 0)
-        double SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p.beta_star * ARTO / ARo * Ref * Fdes;
-        double Source = SourceP + SourceN1 * ARTk;
-        d6 = d6 + Source;
-        CnegTk = fabs(SourceN1);
-        tprod = fmin(prod, Votg) / Ref;
+        // IF_Comppress 0 (:282-330) or 1, Suzen's compressibility correction with the turbulent
+        // Mach number Mt = sqrt(k) / a and Alf1 = 1, Alf2 = 0.4, Alf3 = 0.2 (:331-396)
+        const bool suzen = p.if_comppress == 1;
+        double Mt = 0.;
+        if (suzen) {
+            const double AA = sqrt(1.4 * f.PP[c] / ARo);
+            Mt = sqrt(ARTk / ARo) / AA;
+        }
         const double GamaAB = p.gama1 * arg1 + p.gama2 * (1.0 - arg1);
         const double BetaAB = p.beta1 * arg1 + p.beta2 * (1.0 - arg1);
-        SourceP = GamaAB * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
+        double SourceP, SourceN1;
+        if (suzen) {
+            SourceP = (1. - (1. - arg1) * 0.4 * Mt * Mt) * tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+            SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) -
+                       (1. + (1. - arg1) * (1.0 - 0.2) * Mt * Mt) * p.beta_star * ARTO / ARo * Ref * Fdes;
+        } else {
+            SourceP = tprod - 2.0 / 3.0 * fmin(Div, 0.0) * ARTk;
+            SourceN1 = -2.0 / 3.0 * fmax(Div, 0.0) - p.beta_star * ARTO / ARo * Ref * Fdes;
+        }
+        SourceP = SourceP * 1.0;  // f_r1 = 1 (Kind_RotMod = 0)
+        double Source = SourceP + SourceN1 * ARTk;
+        CnegTk = fabs(SourceN1);
+        if (p.kind_hybrid == 3 && p.kind_subhy == 5) {  // (:299-311, :354-366)
+            SourceN1 = suzen ? -2.0 / 3.0 * fmax(Div, 0.0) -
+                                   (1. + (1. - arg1) * (1.0 - 0.2) * Mt * Mt) * sqrt(ARTk / ARo) / alIDDES
+                             : -2.0 / 3.0 * fmax(Div, 0.0) - sqrt(ARTk / ARo) / alIDDES;
+            Source = SourceP + SourceN1 * ARTk;
+            CnegTk = fabs(-2.0 / 3.0 * fmax(Div, 0.0) - 1.5 * sqrt(ARTk / ARo) / alIDDES);
+        }
+        d6 = d6 + Source;
+        tprod = fmin(prod, Votg) / Ref;
+        if (suzen) {
+            SourceP = (GamaAB + (1. - arg1) * 0.4 * Mt * Mt) * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
+            SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) -
+                       (1. - (1. - arg1) * (1.0 - 0.2) * Mt * Mt) * BetaAB * ARTO / ARo * Ref;
+        } else {
+            SourceP = GamaAB * ARo * tprod - 2.0 / 3.0 * GamaAB * fmin(Div, 0.0) * ARTO;
+            SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) - BetaAB * ARTO / ARo * Ref;
+        }
         SourceP = SourceP * 1.0;
-        SourceN1 = -2.0 / 3.0 * GamaAB * fmax(Div, 0.0) - BetaAB * ARTO / ARo * Ref;
         Source = SourceP + SourceN1 * ARTO;
         double SourceCross = 2.0 * (1 - arg1) * ARo * p.sigo2 / (ARTO / ARo) * cdif / Ref;
         SourceCross = copysign(1.0, SourceCross) * fmin(fabs(SourceCross), fabs(SourceP) * 0.5);

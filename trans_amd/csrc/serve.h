@@ -267,7 +267,7 @@ int trans_srv_submit(trans_srv* s, const trans_srv_request* q, int* id)
     r->patches.resize(nblk);
     r->geom.assign(q->geom, q->geom + nblk);
     r->st.assign(q->state, q->state + nblk);
-    r->geom_buf.resize((size_t)nblk * 10);
+    r->geom_buf.resize((size_t)nblk * 11);
     r->state_buf.resize((size_t)nblk * 7);
     r->mw_buf.resize((size_t)nblk * 6);
     for (int b = 0; b < nblk; b++) {
@@ -276,8 +276,9 @@ int trans_srv_submit(trans_srv* s, const trans_srv_request* q, int* id)
         d.patches = r->patches[b].data();
         r->ncell.push_back((double)(d.ni - 1) * (d.nj - 1) * (d.nk - 1));
         trans_geom_view& g = r->geom[b];
-        trans_fview* gv[10] = {&g.sd, &g.grad, &g.alagm, &g.vol, &g.rad, &g.dst, &g.aleng, &g.yc, &g.zc, &g.thtc};
-        for (int k = 0; k < 10; k++) stage_view(*gv[k], r->geom_buf[(size_t)b * 10 + k]);
+        trans_fview* gv[11] = {&g.sd,    &g.grad,  &g.alagm, &g.vol, &g.rad, &g.dst,
+                               &g.aleng, &g.aseng, &g.yc,    &g.zc,  &g.thtc};
+        for (int k = 0; k < 11; k++) stage_view(*gv[k], r->geom_buf[(size_t)b * 11 + k]);
         const size_t fsz[6] = {(size_t)(d.nj - 1) * (d.nk - 1), (size_t)(d.nj - 1) * (d.nk - 1),
                                (size_t)(d.ni - 1) * (d.nk - 1), (size_t)(d.ni - 1) * (d.nk - 1),
                                (size_t)(d.ni - 1) * (d.nj - 1), (size_t)(d.ni - 1) * (d.nj - 1)};

@@ -278,9 +278,10 @@ static int check_params(trans_ctx* c, const trans_params* p)
         if (p->kind_model != 1 && p->kind_model != 2)
             return fail(c, TRANS_EUNSUPPORTED, "Kind_Model must be 1 (SST) or 2 (WD+ k-omega)");
         if (p->kind_hybrid < 0) return fail(c, TRANS_EINVAL, "Kind_Hybrid must be >= 0");
-        if (p->kind_hybrid == 3 && (p->kind_subhy < 0 || p->kind_subhy > 3))
-            return fail(c, TRANS_EUNSUPPORTED, "Kind_SubHY must be 0-3 (DES, DDES); IDDES (5) is not supported");
-        if (p->if_comppress != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_Comppress must be 0");
+        if (p->kind_hybrid == 3 && (p->kind_subhy < 0 || (p->kind_subhy > 3 && p->kind_subhy != 5)))
+            return fail(c, TRANS_EUNSUPPORTED, "Kind_SubHY must be 0-3 (DES, DDES) or 5 (IDDES)");
+        if (p->if_comppress != 0 && p->if_comppress != 1)
+            return fail(c, TRANS_EUNSUPPORTED, "IF_Comppress must be 0 or 1 (Suzen)");
     }
     return 0;
 }
@@ -344,7 +345,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     if ((rc = dalloc(c, &f.V, 7 * n)) || (rc = dalloc(c, &f.PP, n)) || (rc = dalloc(c, &f.T, n)) ||
         (rc = dalloc(c, &f.Rmiu, n)) || (rc = dalloc(c, &f.SD, 9 * n)) || (rc = dalloc(c, &f.Grad, 3 * n)) ||
         (rc = dalloc(c, &f.Alagm, 3 * n)) || (rc = dalloc(c, &f.Vol, n)) || (rc = dalloc(c, &f.rad, n)) ||
-        (rc = dalloc(c, &f.Dst, n)) || (rc = dalloc(c, &f.aLeng, n)) || (rc = dalloc(c, &f.Yc, n)) ||
+        (rc = dalloc(c, &f.Dst, n)) || (rc = dalloc(c, &f.aLeng, n)) || (rc = dalloc(c, &f.aSeng, n)) || (rc = dalloc(c, &f.Yc, n)) ||
         (rc = dalloc(c, &f.Zc, n)) || (rc = dalloc(c, &f.thtc, n)) || (rc = dalloc(c, &f.D, 7 * n)) ||
         (rc = dalloc(c, &f.Rds, 3 * n)) || (rc = dalloc(c, &f.Dtm, n)) || (rc = dalloc(c, &f.Wt0, 7 * n)) ||
         (rc = dalloc(c, &f.Wt1, 7 * n)) || (rc = dalloc(c, &f.Wt2, 7 * n)) || (rc = dalloc(c, &f.DQ, 18 * n)) ||
@@ -586,7 +587,7 @@ int trans_hip_set_geometry(trans_ctx* c, int blk, const trans_geom_view* g)
     if ((rc = copy_view(c, blk, g->sd, f.SD, 9, true)) || (rc = copy_view(c, blk, g->grad, f.Grad, 3, true)) ||
         (rc = copy_view(c, blk, g->alagm, f.Alagm, 3, true)) || (rc = copy_view(c, blk, g->vol, f.Vol, 1, true)) ||
         (rc = copy_view(c, blk, g->rad, f.rad, 1, true)) || (rc = copy_view(c, blk, g->dst, f.Dst, 1, true)) ||
-        (rc = copy_view(c, blk, g->aleng, f.aLeng, 1, true)) || (rc = copy_view(c, blk, g->yc, f.Yc, 1, true)) ||
+        (rc = copy_view(c, blk, g->aleng, f.aLeng, 1, true)) || (rc = copy_view(c, blk, g->aseng, f.aSeng, 1, true)) || (rc = copy_view(c, blk, g->yc, f.Yc, 1, true)) ||
         (rc = copy_view(c, blk, g->zc, f.Zc, 1, true)) || (rc = copy_view(c, blk, g->thtc, f.thtc, 1, true)))
         return rc;
     // wall marks (Input/Read_BlockDimensions.F90:146-283): faces i0, im, j0, jm, k0, km

@@ -52,7 +52,7 @@ module trans_hip_mod
     end type trans_fview
 
     type, bind(C), public :: trans_geom_view
-        type(trans_fview) :: sd, grad, alagm, vol, rad, dst, aleng, yc, zc, thtc
+        type(trans_fview) :: sd, grad, alagm, vol, rad, dst, aleng, aseng, yc, zc, thtc
         type(c_ptr) :: markwall(6) = c_null_ptr
     end type trans_geom_view
 

@@ -290,6 +290,8 @@ def _metrics(b: Block, tiny: float):
     if NK1 == 2:
         cand = [dw, de, ds, dn]
     aleng[I, J, K] = np.maximum.reduce(cand)
+    aseng = b.alloc("aseng")
+    aseng[I, J, K] = np.minimum.reduce(cand)
 
     # volumes (CalcDst.F90:481-515)
     vol = b.alloc("vol")
@@ -574,7 +576,7 @@ def initial_state(blocks: list, prm, q_dst: float = 0.0, if_ns_uniform: int = 1)
 def geom_view(b: Block) -> capi.TransGeomView:
     g = capi.TransGeomView()
     A = b.arrays
-    for nm in ("sd", "grad", "alagm", "vol", "rad", "dst", "aleng", "yc", "zc", "thtc"):
+    for nm in ("sd", "grad", "alagm", "vol", "rad", "dst", "aleng", "aseng", "yc", "zc", "thtc"):
         setattr(g, nm, capi.fview(A[nm]))
     import ctypes as C
     arr = (C.POINTER(C.c_int) * 6)()
