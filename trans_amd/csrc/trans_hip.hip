@@ -267,7 +267,10 @@ const char* trans_hip_last_error(const trans_ctx* ctx) { return ctx ? ctx->err.c
 static int check_params(trans_ctx* c, const trans_params* p)
 {
     if (p->if_precondition != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_PRECONDITION must be 0");
-    if (p->kind_rotmod != 0) return fail(c, TRANS_EUNSUPPORTED, "Kind_RotMod must be 0");
+    // RCCorrection (Space/RC_Correction.F90): mode 2 only writes f_r2, which no routine reads,
+    // so it leaves the state as mode 0 does; mode 1 reads omega(M) with M unset (:107)
+    if (p->kind_rotmod != 0 && p->kind_rotmod != 2)
+        return fail(c, TRANS_EUNSUPPORTED, "Kind_RotMod must be 0 or 2 (mode 1 reads an unset index)");
     if (p->cfl_sub_step < 1) return fail(c, TRANS_EINVAL, "CFL_Sub_step must be >= 1");
     if (p->if_rk != 0 && p->if_rk != 1) return fail(c, TRANS_EINVAL, "IF_RK must be 0 or 1");
     if (p->kind_spatial < 1 || p->kind_spatial > 4)
