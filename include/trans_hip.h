@@ -55,7 +55,7 @@ typedef struct trans_params {
     int kind_subhy;      /* with Kind_Hybrid 3: 0 DES, 1/3 DDES with Menter's F1, 2 DDES with F2, 5 IDDES */
     int if_comppress;    /* SST compressibility correction */
     int kind_cc;
-    int kind_rotmod;     /* must be 0 (RC_Correction.F90:107 reads an unset index otherwise) */
+    int kind_rotmod;     /* 0 or 2 (mode 2 only writes f_r2, which nothing reads; mode 1 reads an unset index) */
     int if_precondition; /* must be 0 */
     int cfl_sub_step;    /* >= 1 (SpectralRadius.F90:159 divides by it) */
     int ki_ini;
@@ -242,6 +242,40 @@ int trans_hip_profile(trans_ctx* ctx, int on);
 int trans_hip_profile_count(trans_ctx* ctx);
 int trans_hip_profile_get(trans_ctx* ctx, int idx, char* name, int name_len, double* total_ms,
                           long* launches);
+
+/* ---- Restart records and time statistics ------------------------------------------------
+ * Files are Fortran unformatted sequential, as the reference writes them: every record framed
+ * by its byte count as a 4-byte integer before and after (records below 2 GiB). */
+
+/* OUTPUT_SaveFields for block blk (Output/OUTPUT_SaveField.F90:14-26): records KI (int32),
+ * V(1:7), Rmiu, then Wt0 (Kind_dual 1) or Wt0 and Wt1 (Kind_dual 2), each over the interior
+ * cells with K fastest, then J, I and the component outermost, as float64.  ki is the KI the
+ * record carries (the reference writes the iteration just completed). */
+int trans_hip_write_restart(trans_ctx* ctx, int blk, int ki, const char* path);
+/* Reads such a record into block blk and recomputes p and T from V (Input/Init.F90:22-46,
+ * 146-160); *ki (may be NULL) receives the stored KI (the reference continues at KI + 1).
+ * The Kind_Hybrid history reset of Input/Init.F90:54-71 is the host's. */
+int trans_hip_read_restart(trans_ctx* ctx, int blk, const char* path, int* ki);
+
+/* Time statistics of Average_Fields (Output/Average_Field.F90:46-263), TRANS_NSTAT
+ * components per cell in the order of its FLfld record: U V W R M Mu k P0 P T uu vv ww uv uw vw
+ * (running means from KI_aver on; the last six are the modelled Reynolds stresses) and
+ * uurms vvrms wwrms uvrms vwrms uwrms pprms (from KI_statis on).  stats_enable allocates
+ * them (zero) and makes trans_hip_iterate update them after every outer iteration with
+ * KI >= ki_aver, as main.F90:185-188 does; ki_aver < 0 switches that off.
+ * trans_hip_average_fields is the routine-level call for hosts that drive the loop
+ * routine by routine (not to be combined with trans_hip_iterate's own update). */
+#define TRANS_NSTAT 23
+int trans_hip_stats_enable(trans_ctx* ctx, int ki_aver, int ki_statis);
+int trans_hip_average_fields(trans_ctx* ctx);
+int trans_hip_get_stats(trans_ctx* ctx, int blk, const trans_fview* out);
+int trans_hip_set_stats(trans_ctx* ctx, int blk, const trans_fview* in);
+/* FLfld record file (Output/Average_Field.F90:264-281; read back at Input/Init.F90:77-91): one
+ * record per interior cell, I fastest, of Xc Yc Zc (from the caller's cell-centre views) and
+ * the TRANS_NSTAT statistics. */
+int trans_hip_write_stats(trans_ctx* ctx, int blk, const trans_fview* xc, const trans_fview* yc,
+                          const trans_fview* zc, const char* path);
+int trans_hip_read_stats(trans_ctx* ctx, int blk, const char* path);
 
 /* Device bytes a context for these inputs would allocate (trans_hip_create's own sizing,
  * run without allocating or touching the device), and the bytes a context holds.  No

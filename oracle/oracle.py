@@ -62,6 +62,8 @@ def lib():
         L.ora_fsst.argtypes = [PP] + [C.c_double] * 6
         L.ora_fsst.restype = C.c_double
         L.ora_sst_sources.argtypes = [PP, PB]
+        L.ora_average_fields.argtypes = [PP, C.c_int, C.c_int, C.c_int, PB, DP]
+        L.ora_pp_t_from_v.argtypes = [PP, PB]
         L.ora_iddes_fd.argtypes = [C.c_double] * 4 + [DP, C.c_double]
         L.ora_iddes_fd.restype = C.c_double
         L.ora_iddes_fe.argtypes = [C.c_double] * 5 + [DP, C.c_double]

@@ -54,6 +54,8 @@ int ora_if_turb(const trans_params* p, int ki);
 double ora_fsst(const trans_params* p, double Den, double vK, double vOmega, double Dist, double RmiuL,
                 double CDiff);
 void ora_sst_sources(const trans_params* p, ora_block* b);
+void ora_average_fields(const trans_params* p, int ki, int ki_aver, int ki_statis, ora_block* b, double* S);
+void ora_pp_t_from_v(const trans_params* p, ora_block* b);
 double ora_iddes_fd(double Den1, double RmiuT1, double aDst1, double aLmax1, const double g[9], double Ref);
 double ora_iddes_fe(double Den2, double RmiuT2, double RMIUL2, double aDst2, double aLmax2, const double g[9],
                     double Ref);

@@ -893,3 +893,87 @@ void ora_turb_lusgs_sst(const trans_params* p, ora_block* b)
                 }
     }
 }
+
+/* ------------------------------------------------------------------------------------ */
+/* Output/Average_Field.F90:46-263 (Average_Fields at KI, one block).  S(23, cells) in the
+ * FLfld record order: U V W R M Mu k P0 P T uu vv ww uv uw vw, uurms vvrms wwrms uvrms vwrms
+ * uwrms pprms; component index fastest, like the other oracle arrays. */
+void ora_average_fields(const trans_params* p, int ki, int ki_aver, int ki_statis, ora_block* b, double* S)
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    const double Ref = p->ref;
+#define STAT(q, i, j, k) S[(q) + 23 * IX(i, j, k)]
+    for (int J = 1; J <= NJ1; J++)
+        for (int K = 1; K <= NK1; K++)
+            for (int I = 1; I <= NI1; I++) {
+                double Den = V(1, I, J, K), RmiuT = RMIU(I, J, K), vK = V(6, I, J, K) / Den;
+                double vab[16];
+                vab[0] = V(2, I, J, K) / Den;
+                vab[1] = V(3, I, J, K) / Den;
+                vab[2] = V(4, I, J, K) / Den;
+                vab[3] = Den;
+                double QQ = sqrt(vab[0] * vab[0] + vab[1] * vab[1] + vab[2] * vab[2]);
+                double AA = sqrt(1.4 * PP(I, J, K) / Den);
+                vab[4] = QQ / AA;
+                vab[5] = RMIU(I, J, K);
+                vab[6] = V(6, I, J, K) / Den;
+                vab[7] = PP(I, J, K) * pow(1.0 + 0.2 * vab[4] * vab[4], 3.5) / p->ppf;
+                vab[8] = PP(I, J, K);
+                vab[9] = T(I, J, K);
+                double dudx = DQ(1, I, J, K), dudy = DQ(2, I, J, K), dudz = DQ(3, I, J, K);
+                double dvdx = DQ(4, I, J, K), dvdy = DQ(5, I, J, K), dvdz = DQ(6, I, J, K);
+                double dwdx = DQ(7, I, J, K), dwdy = DQ(8, I, J, K), dwdz = DQ(9, I, J, K);
+                double S0 = -2.0 / 3.0 * (dudx + dvdy + dwdz);
+                double upup = RmiuT / Ref * (S0 + 2.0 * dudx) - 2.0 / 3.0 * Den * vK;
+                double vpvp = RmiuT / Ref * (S0 + 2.0 * dvdy) - 2.0 / 3.0 * Den * vK;
+                double wpwp = RmiuT / Ref * (S0 + 2.0 * dwdz) - 2.0 / 3.0 * Den * vK;
+                double upvp = RmiuT / Ref * (dudy + dvdx);
+                double upwp = RmiuT / Ref * (dudz + dwdx);
+                double vpwp = RmiuT / Ref * (dvdz + dwdy);
+                vab[10] = -upup;
+                vab[11] = -vpvp;
+                vab[12] = -wpwp;
+                vab[13] = -upvp;
+                vab[14] = -upwp;
+                vab[15] = -vpwp;
+                if (ki == ki_aver) {
+                    for (int q = 0; q < 16; q++) STAT(q, I, J, K) = vab[q];
+                } else {
+                    double Time = (double)(ki - ki_aver);
+                    for (int q = 0; q < 16; q++) STAT(q, I, J, K) = (STAT(q, I, J, K) * Time + vab[q]) / (Time + 1.);
+                }
+            }
+    if (ki >= ki_statis) {
+        double Time = (double)(ki - ki_statis);
+        for (int K = 1; K <= NK1; K++)
+            for (int J = 1; J <= NJ1; J++)
+                for (int I = 1; I <= NI1; I++) {
+                    double Den = V(1, I, J, K);
+                    double v1 = V(2, I, J, K) / Den, v2 = V(3, I, J, K) / Den, v3 = V(4, I, J, K) / Den;
+                    double v5 = PP(I, J, K);
+                    double Ub = STAT(0, I, J, K), Vb = STAT(1, I, J, K), Wb = STAT(2, I, J, K), Pb = STAT(8, I, J, K);
+                    STAT(16, I, J, K) = (STAT(16, I, J, K) * Time + (v1 - Ub) * (v1 - Ub)) / (Time + 1.);
+                    STAT(17, I, J, K) = (STAT(17, I, J, K) * Time + (v2 - Vb) * (v2 - Vb)) / (Time + 1.);
+                    STAT(18, I, J, K) = (STAT(18, I, J, K) * Time + (v3 - Wb) * (v3 - Wb)) / (Time + 1.);
+                    STAT(19, I, J, K) = (STAT(19, I, J, K) * Time + (v1 - Ub) * (v2 - Vb)) / (Time + 1.);
+                    STAT(21, I, J, K) = (STAT(21, I, J, K) * Time + (v1 - Ub) * (v3 - Wb)) / (Time + 1.);
+                    STAT(20, I, J, K) = (STAT(20, I, J, K) * Time + (v2 - Vb) * (v3 - Wb)) / (Time + 1.);
+                    STAT(22, I, J, K) = (STAT(22, I, J, K) * Time + (v5 - Pb) * (v5 - Pb)) / (Time + 1.);
+                }
+    }
+#undef STAT
+}
+
+/* Input/Init.F90:146-160: p and T from V after a restart read */
+void ora_pp_t_from_v(const trans_params* p, ora_block* b)
+{
+    const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
+    for (int K = 1; K <= NK1; K++)
+        for (int J = 1; J <= NJ1; J++)
+            for (int I = 1; I <= NI1; I++) {
+                double VV = V(2, I, J, K) * V(2, I, J, K) + V(3, I, J, K) * V(3, I, J, K) + V(4, I, J, K) * V(4, I, J, K);
+                double wr = p->omega1 * b->rad[IX(I, J, K)];
+                PP(I, J, K) = (1.4 - 1.0) * V(5, I, J, K) - 0.5 * (1.4 - 1.0) * (VV / V(1, I, J, K) + -(wr * wr) * V(1, I, J, K));
+                T(I, J, K) = p->rxm2 * PP(I, J, K) / V(1, I, J, K);
+            }
+}

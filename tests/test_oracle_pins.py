@@ -1395,3 +1395,74 @@ def test_iddes_rans_and_les_limits():
                 want = (-np.sqrt(rk) / (cdes * 1.0) + prm.beta_star * om * prm.ref) * V[(5,) + c]
                 got = d5[(5,) + c] - d0[(5,) + c]
                 assert abs(got - want) <= 1e-9 * abs(want), (i, j, k, got, want)
+
+
+# ---------------------------------------------------------------------------------------
+# Average_Fields (Output/Average_Field.F90:46-263)
+
+def _stats_box():
+    b, prm, edges = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1)
+    oc = OracleCase([b], prm)
+    S = np.zeros((23,) + b.shape(), order="F")
+    return b, prm, oc, S
+
+
+def _avg(oc, prm, S, ki, ki_aver, ki_statis):
+    lib().ora_average_fields(C.byref(prm), ki, ki_aver, ki_statis, C.byref(oc.ob[0]),
+                             S.ctypes.data_as(C.POINTER(C.c_double)))
+
+
+def test_average_fields_closed_forms():
+    """A velocity u_n = a + b n (samples n = 0..N-1 from KI_aver = KI_statis) gives the mean
+    a + b (N-1)/2 and, about the running means a + b n/2, the mean square deviation
+    b^2 (N-1)(2N-1)/24; the other velocity components stay at their constant values with zero
+    deviation.  A uniform shear du/dy = g gives the modelled stresses uv = -mu_t g / Re and
+    uu = vv = ww = 2/3 rho k (the eddy-viscosity closure used for the statistics)."""
+    b, prm, oc, S = _stats_box()
+    A = oc[0]
+    V, D = A["v"], A["DQdxyz"]
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    a, bb, g, N = 0.3, 0.02, 0.7, 9
+    rho = V[0].copy()
+    D[...] = 0.0
+    D[1] = g
+    for n in range(N):
+        V[1] = rho * (a + bb * n)
+        V[2] = rho * 0.1
+        V[3] = 0.0
+        _avg(oc, prm, S, 10 + n, 10, 10)
+    np.testing.assert_allclose(S[0][I, J, K], a + bb * (N - 1) / 2, rtol=1e-13)
+    np.testing.assert_allclose(S[1][I, J, K], 0.1, rtol=1e-14)
+    np.testing.assert_allclose(S[16][I, J, K], bb * bb * (N - 1) * (2 * N - 1) / 24, rtol=1e-12)
+    # (v/rho of a constant carries one rounding, so its deviation is of order 1e-17)
+    assert np.max(np.abs(S[17][I, J, K])) < 1e-30 and np.max(np.abs(S[19][I, J, K])) < 1e-16
+    mut, k = A["rmiu"], V[5] / V[0]
+    np.testing.assert_allclose(S[13][I, J, K], (-mut * g / prm.ref)[I, J, K], rtol=1e-13)
+    for q in (10, 11, 12):
+        np.testing.assert_allclose(S[q][I, J, K], (2.0 / 3.0 * V[0] * k)[I, J, K], rtol=1e-13)
+
+
+def test_average_fields_window_starts():
+    """Before KI_statis the deviations are not accumulated; the first averaging call (KI =
+    KI_aver) overwrites whatever the means held."""
+    b, prm, oc, S = _stats_box()
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    S[...] = 99.0
+    _avg(oc, prm, S, 5, 5, 8)
+    V = oc[0]["v"]
+    np.testing.assert_array_equal(S[0][I, J, K], (V[1] / V[0])[I, J, K])
+    np.testing.assert_array_equal(S[16][I, J, K], 99.0)
+    np.testing.assert_array_equal(S[8][I, J, K], oc[0]["pp"][I, J, K])
+
+
+def test_restart_pressure_formula():
+    """Input/Init.F90:146-160 on a state built from p and T: p recovered to rounding, and
+    T = gamma M^2 p / rho."""
+    b, prm, oc, S = _stats_box()
+    A = oc[0]
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    p0 = A["pp"].copy()
+    A["pp"][...] = 0.0
+    lib().ora_pp_t_from_v(C.byref(prm), C.byref(oc.ob[0]))
+    np.testing.assert_allclose(A["pp"][I, J, K], p0[I, J, K], rtol=1e-13)
+    np.testing.assert_allclose(A["t"][I, J, K], (prm.rxm2 * p0 / A["v"][0])[I, J, K], rtol=1e-13)

@@ -124,6 +124,10 @@ class TransSrvStatus(C.Structure):
 
 SRV_QUEUED, SRV_RUNNING, SRV_CONVERGED, SRV_MAX_ITER, SRV_CANCELLED, SRV_FAILED = range(6)
 
+NSTAT = 23
+STAT_NAMES = ("U", "V", "W", "R", "M", "Mu", "k", "P0", "P", "T", "uu", "vv", "ww", "uv", "uw", "vw",
+              "uurms", "vvrms", "wwrms", "uvrms", "vwrms", "uwrms", "pprms")
+
 EXPORTS = [
     "trans_hip_create", "trans_hip_destroy", "trans_hip_last_error", "trans_hip_set_params",
     "trans_hip_set_iteration", "trans_hip_set_geometry", "trans_hip_set_state",
@@ -132,6 +136,8 @@ EXPORTS = [
     "trans_hip_update_u",
     "trans_hip_iterate", "trans_hip_plane_integrals", "trans_hip_sync", "trans_hip_profile", "trans_hip_profile_count",
     "trans_hip_profile_get", "trans_hip_footprint", "trans_hip_device_bytes",
+    "trans_hip_write_restart", "trans_hip_read_restart", "trans_hip_stats_enable", "trans_hip_average_fields",
+    "trans_hip_get_stats", "trans_hip_set_stats", "trans_hip_write_stats", "trans_hip_read_stats",
     "trans_srv_create", "trans_srv_submit", "trans_srv_progress", "trans_srv_poll", "trans_srv_cancel",
     "trans_srv_result", "trans_srv_release", "trans_srv_destroy", "trans_srv_last_error",
 ]
@@ -192,6 +198,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_profile_get.argtypes = [C.c_void_p, C.c_int, C.c_char_p, C.c_int,
                                           C.POINTER(C.c_double), C.POINTER(C.c_long)]
     lib.trans_hip_footprint.argtypes = [P(TransParams), C.c_int, P(TransBlockDesc), P(C.c_size_t)]
+    lib.trans_hip_write_restart.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_char_p]
+    lib.trans_hip_read_restart.argtypes = [C.c_void_p, C.c_int, C.c_char_p, P(C.c_int)]
+    lib.trans_hip_stats_enable.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.trans_hip_average_fields.argtypes = [C.c_void_p]
+    lib.trans_hip_get_stats.argtypes = [C.c_void_p, C.c_int, P(TransFView)]
+    lib.trans_hip_set_stats.argtypes = [C.c_void_p, C.c_int, P(TransFView)]
+    lib.trans_hip_write_stats.argtypes = [C.c_void_p, C.c_int, P(TransFView), P(TransFView), P(TransFView),
+                                          C.c_char_p]
+    lib.trans_hip_read_stats.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
     lib.trans_hip_device_bytes.argtypes = [C.c_void_p, P(C.c_size_t)]
     lib.trans_srv_create.argtypes = [P(TransSrvConfig), P(C.c_void_p)]
     lib.trans_srv_submit.argtypes = [C.c_void_p, P(TransSrvRequest), P(C.c_int)]

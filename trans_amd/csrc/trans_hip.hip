@@ -14,6 +14,7 @@
 
 #include "dev_common.h"
 #include "k_bc.h"
+#include "k_io.h"
 #include "k_monitor.h"
 #include "k_space.h"
 #include "k_rk.h"
@@ -85,6 +86,10 @@ struct trans_ctx {
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
     std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
     std::vector<void*> allocs;
+    // time statistics (Average_Fields): TRANS_NSTAT components, allocated when enabled
+    double* d_stats = nullptr;
+    bool stats_on = false;
+    int ki_aver = 0, ki_statis = 0;
     size_t bytes = 0;  // device bytes allocated by the context
     bool dry = false;  // sizing run of trans_hip_footprint: no HIP call, no allocation
     std::string err;
@@ -946,6 +951,8 @@ int trans_hip_update_u(trans_ctx* c)
 }
 
 // The device work of one outer iteration (main.F90:65-197 without I/O), queued only.
+static int average_fields_async(trans_ctx* c);  // io.h
+
 static int iteration_body(trans_ctx* c)
 {
     int rc;
@@ -1007,6 +1014,7 @@ int trans_hip_iterate(trans_ctx* c, int n_iter, trans_resid* per_block)
             drop_graph(c);
             if ((rc = iteration_body(c))) return rc;
         }
+        if (c->stats_on && (rc = average_fields_async(c))) return rc;  // main.F90:185-188
         c->ki += 1;
     }
     return per_block ? fetch_resid(c, per_block) : 0;
@@ -1076,3 +1084,4 @@ int trans_hip_sync(trans_ctx* c)
 }  // extern "C"
 
 #include "serve.h"
+#include "io.h"

@@ -12,6 +12,7 @@ module trans_hip_mod
 
     integer(c_int), parameter, public :: TRANS_OK = 0, TRANS_EINVAL = -1, TRANS_EHIP = -2, &
         TRANS_ENOMEM = -3, TRANS_EUNSUPPORTED = -4, TRANS_ENONFINITE = 1
+    integer(c_int), parameter, public :: TRANS_NSTAT = 23
     integer(c_int), parameter, public :: TRANS_FIELD_D = 0, TRANS_FIELD_RDS = 1, TRANS_FIELD_DTM = 2, &
         TRANS_FIELD_DQDXYZ = 3, TRANS_FIELD_SHOCK = 4, TRANS_FIELD_FSCHEME = 5
 
@@ -197,6 +198,54 @@ module trans_hip_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx
         end function
+        ! path arguments: null-terminated, e.g. trim(ThisBlock%FLDAT)//c_null_char
+        integer(c_int) function trans_hip_write_restart(ctx, blk, ki, path) bind(C, name="trans_hip_write_restart")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk, ki
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+        integer(c_int) function trans_hip_read_restart(ctx, blk, path, ki) bind(C, name="trans_hip_read_restart")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk
+            character(kind=c_char), intent(in) :: path(*)
+            integer(c_int), intent(out) :: ki
+        end function
+        integer(c_int) function trans_hip_stats_enable(ctx, ki_aver, ki_statis) bind(C, name="trans_hip_stats_enable")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: ki_aver, ki_statis
+        end function
+        integer(c_int) function trans_hip_average_fields(ctx) bind(C, name="trans_hip_average_fields")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+        end function
+        integer(c_int) function trans_hip_get_stats(ctx, blk, out) bind(C, name="trans_hip_get_stats")
+            import :: c_int, c_ptr, trans_fview
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk
+            type(trans_fview), intent(in) :: out
+        end function
+        integer(c_int) function trans_hip_set_stats(ctx, blk, in) bind(C, name="trans_hip_set_stats")
+            import :: c_int, c_ptr, trans_fview
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk
+            type(trans_fview), intent(in) :: in
+        end function
+        integer(c_int) function trans_hip_write_stats(ctx, blk, xc, yc, zc, path) bind(C, name="trans_hip_write_stats")
+            import :: c_int, c_ptr, c_char, trans_fview
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk
+            type(trans_fview), intent(in) :: xc, yc, zc
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+        integer(c_int) function trans_hip_read_stats(ctx, blk, path) bind(C, name="trans_hip_read_stats")
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk
+            character(kind=c_char), intent(in) :: path(*)
+        end function
         integer(c_int) function trans_hip_footprint(params, nblk, blocks, bytes) bind(C, name="trans_hip_footprint")
             import :: c_int, c_size_t, trans_params, trans_block_desc
             type(trans_params), intent(in) :: params
@@ -253,6 +302,8 @@ module trans_hip_mod
         trans_hip_get_field, trans_hip_spectral_radius, trans_hip_bc, trans_hip_space, &
         trans_hip_residual, trans_hip_lusgs_update, trans_hip_rk_update, trans_hip_update_u, trans_hip_iterate, &
         trans_hip_plane_integrals, trans_hip_sync, trans_hip_footprint, trans_srv_create, trans_srv_submit, &
+        trans_hip_write_restart, trans_hip_read_restart, trans_hip_stats_enable, trans_hip_average_fields, &
+        trans_hip_get_stats, trans_hip_set_stats, trans_hip_write_stats, trans_hip_read_stats, &
         trans_srv_progress, trans_srv_poll, trans_srv_cancel, trans_srv_result, trans_srv_release, trans_srv_destroy
 
 contains

@@ -126,6 +126,40 @@ class Solver:
     def sync(self):
         self._check(self.lib.trans_hip_sync(self.ctx))
 
+    # ---- restart records and time statistics
+    def write_restart(self, blk, ki, path):
+        self._check(self.lib.trans_hip_write_restart(self.ctx, blk, ki, str(path).encode()))
+
+    def read_restart(self, blk, path):
+        ki = C.c_int()
+        self._check(self.lib.trans_hip_read_restart(self.ctx, blk, str(path).encode(), C.byref(ki)))
+        return ki.value
+
+    def stats_enable(self, ki_aver, ki_statis):
+        self._check(self.lib.trans_hip_stats_enable(self.ctx, ki_aver, ki_statis))
+
+    def average_fields(self):
+        self._check(self.lib.trans_hip_average_fields(self.ctx))
+
+    def get_stats(self, blk):
+        a = np.zeros((capi.NSTAT,) + self.blocks[blk].shape(), order="F")
+        v = capi.fview(a)
+        self._check(self.lib.trans_hip_get_stats(self.ctx, blk, C.byref(v)))
+        return a
+
+    def set_stats(self, blk, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        v = capi.fview(a)
+        self._check(self.lib.trans_hip_set_stats(self.ctx, blk, C.byref(v)))
+
+    def write_stats(self, blk, xc, yc, zc, path):
+        keep = [np.asfortranarray(x, dtype=np.float64) for x in (xc, yc, zc)]
+        views = [capi.fview(x) for x in keep]
+        self._check(self.lib.trans_hip_write_stats(self.ctx, blk, *[C.byref(v) for v in views], str(path).encode()))
+
+    def read_stats(self, blk, path):
+        self._check(self.lib.trans_hip_read_stats(self.ctx, blk, str(path).encode()))
+
     # ---- data out
     def set_state(self, blk, arrays):
         """Upload v, pp, t, rmiu, wt0, wt1 and wt2 of block blk from `arrays` (reference
