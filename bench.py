@@ -189,13 +189,21 @@ def measured_traffic(workload, kernel):
     return None
 
 
-def timed_run(blocks, prm, steps, warmup, prof_steps):
-    """Warm up, time `steps` iterations without instrumentation, then profile `prof_steps`
-    more with HIP events on the library's stream."""
+def timed_run(blocks, prm, steps, warmup, prof_steps, tune=0):
+    """Warm up, let the library's autotuner pick the launch configuration (`tune` timed
+    iterations per candidate, state restored afterwards; 0 keeps the default), time `steps`
+    iterations without instrumentation, then profile `prof_steps` more with HIP events on the
+    library's stream."""
     from trans_amd.solver import Solver
     sv = Solver(blocks, prm, ki=1)
     sv.iterate(warmup, want_resid=False)
     sv.sync()
+    timed_run.tuning = None
+    if tune > 0:
+        chosen, table = sv.autotune(iters=tune)
+        timed_run.tuning = {"chosen": {k: chosen[k] for k in ("sweep", "streams", "graph")},
+                            "ms_per_iter": {f"sweep{r['sweep']}/streams{r['streams']}/graph{r['graph']}":
+                                            round(r["ms_per_iter"], 4) for r in table}}
     t0 = time.perf_counter()
     res = sv.iterate(steps, want_resid=True)
     sv.sync()
@@ -250,6 +258,8 @@ def main():
                     help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serving-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-implicit leg")
+    ap.add_argument("--tune", type=int, default=10, metavar="ITERS",
+                    help="autotuner iterations per launch configuration before timing (0: default configuration)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup+steps of the FP64 path only and print the kernel table")
     ap.add_argument("--implicit-fp32", action="store_true",
@@ -286,7 +296,8 @@ def main():
     neq = 7 if (prm.if_euler == 0 and prm.if_turb_org == 1) else 5
     print(f"[bench] {desc}: {cells} cells, warmup {args.warmup}, steps {args.steps}", flush=True)
     prof_steps = min(args.steps, 50)
-    dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps)
+    dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps, args.tune)
+    tuning = timed_run.tuning
     tag = "FP32 implicit" if prm.implicit_fp32 else "FP64"
     print(f"[bench] {tag}: {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)
     if args.profile_only:
@@ -315,10 +326,11 @@ def main():
     fp32 = None
     if not args.no_fp32:
         b32, p32, _ = build_workload(args.workload, implicit_fp32=1)
-        dt32, res32, _ = timed_run(b32, p32, args.steps, args.warmup, 0)
+        dt32, res32, _ = timed_run(b32, p32, args.steps, args.warmup, 0, args.tune)
         fp32 = {"value": round(cells * args.steps / dt32 / 1e6, 3),
                 "ms_per_step": round(dt32 / args.steps * 1e3, 4),
-                "drsa_block1": res32[0]["drsa"], "drsa_block1_fp64": res[0]["drsa"]}
+                "drsa_block1": res32[0]["drsa"], "drsa_block1_fp64": res[0]["drsa"],
+                "tuning": (timed_run.tuning or {}).get("chosen")}
         print(f"[bench] FP32 implicit: {args.steps} steps in {dt32:.3f} s", flush=True)
     cpu = None if args.no_cpu else cpu_baseline(blocks, prm)
     serve = None
@@ -346,7 +358,7 @@ def main():
         "data": "synthetic (generated grid, uniform start)",
         "config": {"workload": desc, "cells": cells, "blocks": len(blocks), "implicit": "fp64"},
         "roofline": roof, "iteration_roofline": it_roof, "cpu_baseline": cpu, "fp32_implicit": fp32,
-        "kernels": kern, "serving": serve,
+        "kernels": kern, "tuning": tuning, "serving": serve,
     }
     print(json.dumps(out))
 

@@ -277,6 +277,26 @@ int trans_hip_write_stats(trans_ctx* ctx, int blk, const trans_fview* xc, const 
                           const trans_fview* zc, const char* path);
 int trans_hip_read_stats(trans_ctx* ctx, int blk, const char* path);
 
+/* ---- Launch autotuner (no counterpart in the reference) ------------------------------------
+ * Times `iters` iterations of every launch configuration of the context from the same saved
+ * state and keeps the fastest: the LU-SGS sweep (sweep = 0, 1, 2: line march along i, j or k,
+ * where the cross sections fit one workgroup; -1: hyperplane kernel), helper streams for the
+ * face kernels (streams = 4) or one stream (1), HIP-graph replay (graph = 1) or eager launches.
+ * Every configuration computes bitwise the same iteration.  The state and KI are restored;
+ * out (max_out entries, may be NULL) receives the timings, *n_out their count.  The sweep
+ * buffer is reallocated for the chosen axis, so the context's device bytes may then differ
+ * from trans_hip_footprint's.  get_tuning reports the configuration in use. */
+typedef struct trans_tune_entry {
+    int sweep;
+    int streams;
+    int graph;
+    double ms_per_iter;
+} trans_tune_entry;
+int trans_hip_autotune(trans_ctx* ctx, int iters, trans_tune_entry* out, int max_out, int* n_out);
+int trans_hip_get_tuning(const trans_ctx* ctx, trans_tune_entry* out);
+/* Applies a configuration (ms_per_iter ignored), e.g. one found by an earlier autotune. */
+int trans_hip_set_tuning(trans_ctx* ctx, const trans_tune_entry* t);
+
 /* Device bytes a context for these inputs would allocate (trans_hip_create's own sizing,
  * run without allocating or touching the device), and the bytes a context holds.  No
  * counterpart in the reference; used to admit solves by HBM footprint. */

@@ -1,0 +1,62 @@
+"""GPU tests of the launch autotuner: every configuration it tries (sweep variant, helper
+streams, HIP graph) computes bitwise the same iterations, it restores the state and KI, and the
+configuration it keeps is its fastest."""
+import numpy as np
+import pytest
+
+from trans_amd import cases, solver
+from trans_amd.solver import Solver
+
+from helpers import interior, perturb
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    if name == "plate":
+        blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
+    else:
+        blocks, org, glob = cases.bump_channel(33, 17, 5)
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=5, turb=name == "plate")
+    return blocks, prm
+
+
+@pytest.mark.parametrize("name", ["plate", "bump"])
+def test_autotune_keeps_results(name):
+    print(f"case: {name}, autotune then 6 iterations vs the default configuration", flush=True)
+    blocks, prm = _case(name)
+    a = Solver(blocks, prm, ki=1)
+    a.iterate(2, want_resid=False)
+    before = [a.get_state(n)["v"].copy() for n in range(len(blocks))]
+    chosen, table = a.autotune(iters=3)
+    print("chosen", chosen, "of", len(table), flush=True)
+    for row in table:
+        print(row, flush=True)
+    # the tried set covers both sweep kernels, both stream modes and both graph modes
+    assert {r["sweep"] >= 0 for r in table} == {True, False}
+    assert {r["streams"] for r in table} == {1, 4} and {r["graph"] for r in table} == {0, 1}
+    best = min(table, key=lambda r: r["ms_per_iter"])
+    assert (chosen["sweep"], chosen["streams"], chosen["graph"]) == (best["sweep"], best["streams"], best["graph"])
+    # the state and KI are restored
+    for n in range(len(blocks)):
+        assert np.array_equal(a.get_state(n)["v"], before[n])
+    b = Solver(blocks, prm, ki=1)
+    b.iterate(2, want_resid=False)
+    ra = a.iterate(6)
+    rb = b.iterate(6)
+    for n, blk in enumerate(blocks):
+        assert np.array_equal(interior(a.get_state(n)["v"], blk), interior(b.get_state(n)["v"], blk)), n
+        assert ra[n]["drsa"] == rb[n]["drsa"]
+    # every configuration it tried, applied to a fresh context, gives the same iterations
+    for row in table:
+        c = Solver(blocks, prm, ki=1)
+        c.iterate(2, want_resid=False)
+        c.set_tuning(row["sweep"], row["streams"], row["graph"])
+        rc = c.iterate(6)
+        for n, blk in enumerate(blocks):
+            assert np.array_equal(interior(c.get_state(n)["v"], blk), interior(b.get_state(n)["v"], blk)), (row, n)
+            assert rc[n]["drsa"] == rb[n]["drsa"], row
+        c.close()
+    a.close()
+    b.close()

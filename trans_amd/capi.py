@@ -124,6 +124,13 @@ class TransSrvStatus(C.Structure):
 
 SRV_QUEUED, SRV_RUNNING, SRV_CONVERGED, SRV_MAX_ITER, SRV_CANCELLED, SRV_FAILED = range(6)
 
+class TransTuneEntry(C.Structure):
+    _fields_ = [("sweep", C.c_int), ("streams", C.c_int), ("graph", C.c_int), ("ms_per_iter", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 NSTAT = 23
 STAT_NAMES = ("U", "V", "W", "R", "M", "Mu", "k", "P0", "P", "T", "uu", "vv", "ww", "uv", "uw", "vw",
               "uurms", "vvrms", "wwrms", "uvrms", "vwrms", "uwrms", "pprms")
@@ -138,6 +145,7 @@ EXPORTS = [
     "trans_hip_profile_get", "trans_hip_footprint", "trans_hip_device_bytes",
     "trans_hip_write_restart", "trans_hip_read_restart", "trans_hip_stats_enable", "trans_hip_average_fields",
     "trans_hip_get_stats", "trans_hip_set_stats", "trans_hip_write_stats", "trans_hip_read_stats",
+    "trans_hip_autotune", "trans_hip_get_tuning", "trans_hip_set_tuning",
     "trans_srv_create", "trans_srv_submit", "trans_srv_progress", "trans_srv_poll", "trans_srv_cancel",
     "trans_srv_result", "trans_srv_release", "trans_srv_destroy", "trans_srv_last_error",
 ]
@@ -207,6 +215,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_write_stats.argtypes = [C.c_void_p, C.c_int, P(TransFView), P(TransFView), P(TransFView),
                                           C.c_char_p]
     lib.trans_hip_read_stats.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+    lib.trans_hip_autotune.argtypes = [C.c_void_p, C.c_int, P(TransTuneEntry), C.c_int, P(C.c_int)]
+    lib.trans_hip_get_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]
+    lib.trans_hip_set_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]
     lib.trans_hip_device_bytes.argtypes = [C.c_void_p, P(C.c_size_t)]
     lib.trans_srv_create.argtypes = [P(TransSrvConfig), P(C.c_void_p)]
     lib.trans_srv_submit.argtypes = [C.c_void_p, P(TransSrvRequest), P(C.c_int)]

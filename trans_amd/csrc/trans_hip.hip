@@ -75,6 +75,7 @@ struct trans_ctx {
     // helper streams for independent kernels of one call (forked from and joined back
     // into `stream` inside the call, so that calls stay ordered on `stream`)
     hipStream_t aux[3] = {nullptr, nullptr, nullptr};
+    hipStream_t aux_own[3] = {nullptr, nullptr, nullptr};  // created helper streams (null: single-stream context)
     hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
     // one outer iteration captured as a HIP graph and replayed while the iteration's kernel
     // arguments stay the same (set_params and profiling drop it)
@@ -84,6 +85,8 @@ struct trans_ctx {
     int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
     int max_lines = 0;     // largest line count over the blocks for that axis
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
+    void* wbuf = nullptr;  // the sweep buffer W / Wf (reallocated when the march axis changes)
+    size_t wbytes = 0;
     std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
     std::vector<void*> allocs;
     // time statistics (Average_Fields): TRANS_NSTAT components, allocated when enabled
@@ -294,6 +297,97 @@ static int check_params(trans_ctx* c, const trans_params* p)
     return 0;
 }
 
+// Whether the line-march sweep can run along axis m (0 i, 1 j, 2 k): every block's cross
+// section (the lines) fits one workgroup and the LDS rows of the march.  *mx: the largest.
+static bool march_fits(const trans_ctx* c, int m, int* mx_out)
+{
+    const size_t esz = c->p.implicit_fp32 ? sizeof(float) : sizeof(double);
+    const size_t neq = (c->p.if_euler == 0 && c->p.if_turb_org == 1) ? 7 : 5;
+    int mx = 0;
+    for (const HostBlock& h : c->hb) {
+        const int n[3] = {h.ni - 1, h.nj - 1, h.nk - 1};
+        mx = std::max(mx, n[(m + 1) % 3] * n[(m + 2) % 3]);
+    }
+    if (mx_out) *mx_out = mx;
+    return mx <= 1024 && 4 * std::min(neq, (size_t)5) * (mx + 2) * esz <= 128 * 1024;
+}
+
+// Per-block descriptors (with each block's first slot in the sweep buffer) to the device.
+static int upload_blk_table(trans_ctx* c)
+{
+    const int nblk = (int)c->hb.size();
+    std::vector<BlkInfo> bi(nblk);
+    long wt = 0;
+    for (int b = 0; b < nblk; b++) {
+        const HostBlock& h = c->hb[b];
+        const int n[3] = {h.ni - 1, h.nj - 1, h.nk - 1};
+        bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b],
+                        c->mwoff[b], c->line_threads ? wt : 0};
+        wt += (long)(n[0] + n[1] + n[2] - 2) * n[(c->march + 1) % 3] * n[(c->march + 2) % 3];
+    }
+    if (!c->d_blk) return 0;  // set before the table exists: create uploads it
+    return h2d(c, c->d_blk, bi.data(), nblk * sizeof(BlkInfo));
+}
+
+// Selects the LU-SGS sweep: m = 0, 1, 2 the line march along that axis (hyperplane-major
+// buffer W or Wf, reallocated here), m = -1 the hyperplane kernel on the natural layout.
+static int set_sweep(trans_ctx* c, int m, bool upload)
+{
+    int mx = 0;
+    if (m >= 0 && !march_fits(c, m, &mx)) return fail(c, TRANS_EINVAL, "the line sweep does not fit this axis");
+    if (c->wbuf) {
+        if (!c->dry) (void)hipFree(c->wbuf);
+        c->wbuf = nullptr;
+    }
+    c->bytes -= c->wbytes;
+    c->wbytes = 0;
+    Fields& f = c->f;
+    f.W = nullptr;
+    f.Wf = nullptr;
+    c->line_threads = 0;
+    if (m >= 0) {
+        c->march = m;
+        c->max_lines = mx;
+        c->line_threads = (mx + 63) / 64 * 64;
+        // hyperplane-major sweep buffer: nplanes x nlines slots per block, then per
+        // component a sink of 1024 slots for each of the 2 workgroups of every block
+        long wt = 0;
+        for (const HostBlock& h : c->hb) {
+            const int n[3] = {h.ni - 1, h.nj - 1, h.nk - 1};
+            wt += (long)(n[0] + n[1] + n[2] - 2) * n[(m + 1) % 3] * n[(m + 2) % 3];
+        }
+        f.wsink = wt;
+        wt += (long)c->hb.size() * 2 * 1024;
+        // the sweep addresses a component with 32-bit byte offsets
+        if (wt * (long)sizeof(double) >= (1L << 32)) {
+            c->line_threads = 0;
+            return fail(c, TRANS_EUNSUPPORTED, "too many cells for the line-march sweep buffer");
+        }
+        f.wtot = wt;
+        f.march = m;
+        const size_t bytes = (size_t)W_NCOMP * wt * (c->p.implicit_fp32 ? sizeof(float) : sizeof(double));
+        if (!c->dry) {
+            hipError_t e = hipMalloc(&c->wbuf, bytes);
+            if (e != hipSuccess) return fail(c, TRANS_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+            HIPCHK(c, hipMemsetAsync(c->wbuf, 0, bytes, c->stream));
+        }
+        c->wbytes = bytes;
+        c->bytes += bytes;
+        if (c->p.implicit_fp32)
+            f.Wf = (float*)c->wbuf;
+        else
+            f.W = (double*)c->wbuf;
+    }
+    drop_graph(c);
+    if (!upload) return 0;
+    int rc = upload_blk_table(c);
+    if (rc || m < 0 || c->dry) return rc;
+    // the geometry part of the new buffer (face means W_SDM, written once per geometry upload)
+    hipLaunchKernelGGL(k_face_means, dim3(c->seg_cell.n), dim3(WG), 0, c->stream, c->f, c->d_blk, c->seg_cell.d);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
 // dry: sizing run only.  single_stream: the helper streams are the context's own stream, so
 // that every launch of the context is ordered on one stream (the serving layer's contexts:
 // one stream each spreads the resident solves over the device's hardware queues).
@@ -337,7 +431,8 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
                 c->aux[q] = c->stream;
                 continue;
             }
-            HIPCHK(c, hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
+            HIPCHK(c, hipStreamCreateWithFlags(&c->aux_own[q], hipStreamNonBlocking));
+            c->aux[q] = c->aux_own[q];
             HIPCHK(c, hipEventCreateWithFlags(&c->ev_join[q], hipEventDisableTiming));
         }
         if (!single_stream) HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
@@ -423,53 +518,13 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         const char* env = std::getenv("TRANS_HIP_LINE_SWEEP");
         const bool lines_ok = !(env && env[0] == '0');
         const int order[3] = {1, 2, 0};
-        const size_t esz = c->p.implicit_fp32 ? sizeof(float) : sizeof(double);
-        const size_t neq = (c->p.if_euler == 0 && c->p.if_turb_org == 1) ? 7 : 5;
-        for (int q = 0; q < 3 && lines_ok && !c->line_threads; q++) {
-            const int m = order[q];
-            int mx = 0;
-            for (int b = 0; b < nblk; b++) {
-                const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
-                mx = std::max(mx, n[(m + 1) % 3] * n[(m + 2) % 3]);
-            }
-            if (mx <= 1024 && 4 * std::min(neq, (size_t)5) * (mx + 2) * esz <= 128 * 1024) {
-                c->march = m;
-                c->max_lines = mx;
-                c->line_threads = (mx + 63) / 64 * 64;
-            }
-        }
-    }
-    std::vector<long> woff(nblk, 0);
-    if (c->line_threads) {
-        // hyperplane-major sweep buffer: nplanes x nlines slots per block, then per
-        // component a sink of 1024 slots for each of the 2 workgroups of every block
-        long wt = 0;
-        for (int b = 0; b < nblk; b++) {
-            const int n[3] = {c->hb[b].ni - 1, c->hb[b].nj - 1, c->hb[b].nk - 1};
-            woff[b] = wt;
-            wt += (long)(n[0] + n[1] + n[2] - 2) * n[(c->march + 1) % 3] * n[(c->march + 2) % 3];
-        }
-        f.wsink = wt;
-        wt += (long)nblk * 2 * 1024;
-        // the sweep addresses a component with 32-bit byte offsets
-        if (wt * (long)sizeof(double) >= (1L << 32))
-            return fail(c, TRANS_EUNSUPPORTED, "too many cells for the line-march sweep buffer");
-        f.wtot = wt;
-        f.march = c->march;
-        if (c->p.implicit_fp32) {
-            if ((rc = dalloc(c, &f.Wf, (size_t)W_NCOMP * wt))) return rc;
-        } else if ((rc = dalloc(c, &f.W, (size_t)W_NCOMP * wt))) {
-            return rc;
-        }
-    }
-    std::vector<BlkInfo> bi(nblk);
-    for (int b = 0; b < nblk; b++) {
-        const HostBlock& h = c->hb[b];
-        bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b],
-                        c->mwoff[b], woff[b]};
+        int m = -1;
+        for (int q = 0; q < 3 && lines_ok && m < 0; q++)
+            if (march_fits(c, order[q], nullptr)) m = order[q];
+        if ((rc = set_sweep(c, m, false))) return rc;
     }
     if ((rc = dalloc(c, &c->d_blk, nblk))) return rc;
-    if ((rc = h2d(c, c->d_blk, bi.data(), nblk * sizeof(BlkInfo)))) return rc;
+    if ((rc = upload_blk_table(c))) return rc;
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
@@ -514,10 +569,11 @@ int trans_hip_destroy(trans_ctx* c)
     }
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->wbuf) (void)hipFree(c->wbuf);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (int q = 0; q < 3; q++) {
-        if (c->aux[q] && c->aux[q] != c->stream) (void)hipStreamDestroy(c->aux[q]);
+        if (c->aux_own[q]) (void)hipStreamDestroy(c->aux_own[q]);
         if (c->ev_join[q]) (void)hipEventDestroy(c->ev_join[q]);
     }
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -1085,3 +1141,4 @@ int trans_hip_sync(trans_ctx* c)
 
 #include "serve.h"
 #include "io.h"
+#include "tune.h"

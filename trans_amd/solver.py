@@ -126,6 +126,23 @@ class Solver:
     def sync(self):
         self._check(self.lib.trans_hip_sync(self.ctx))
 
+    def autotune(self, iters=10):
+        """Times every launch configuration and keeps the fastest; returns
+        (chosen configuration, list of all timings)."""
+        out = (capi.TransTuneEntry * 32)()
+        n = C.c_int()
+        self._check(self.lib.trans_hip_autotune(self.ctx, iters, out, 32, C.byref(n)))
+        return self.tuning(), [out[q].as_dict() for q in range(min(n.value, 32))]
+
+    def set_tuning(self, sweep, streams, graph):
+        t = capi.TransTuneEntry(sweep, streams, graph, 0.0)
+        self._check(self.lib.trans_hip_set_tuning(self.ctx, C.byref(t)))
+
+    def tuning(self):
+        t = capi.TransTuneEntry()
+        self._check(self.lib.trans_hip_get_tuning(self.ctx, C.byref(t)))
+        return t.as_dict()
+
     # ---- restart records and time statistics
     def write_restart(self, blk, ki, path):
         self._check(self.lib.trans_hip_write_restart(self.ctx, blk, ki, str(path).encode()))
