@@ -297,6 +297,13 @@ int trans_hip_get_tuning(const trans_ctx* ctx, trans_tune_entry* out);
 /* Applies a configuration (ms_per_iter ignored), e.g. one found by an earlier autotune. */
 int trans_hip_set_tuning(trans_ctx* ctx, const trans_tune_entry* t);
 
+/* ---- Geometry pre-processing on the device -------------------------------------------------
+ * Wall distance of Geom/CalcDst.F90:121-158 (IF_dst = 0): for each of ncell cell centres
+ * (cells: x[ncell], then y, then z) the minimum over npanel wall-panel centres (panels: same
+ * layout) of the squared distance, dst = sqrt(max(min, 1e-20)); 1e3 everywhere without
+ * panels.  Host pointers; no context needed.  Bitwise the same as the host computation. */
+int trans_geom_wall_distance(const double* cells, long ncell, const double* panels, long npanel, double* dst);
+
 /* Device bytes a context for these inputs would allocate (trans_hip_create's own sizing,
  * run without allocating or touching the device), and the bytes a context holds.  No
  * counterpart in the reference; used to admit solves by HBM footprint. */

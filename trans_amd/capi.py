@@ -148,6 +148,7 @@ EXPORTS = [
     "trans_hip_autotune", "trans_hip_get_tuning", "trans_hip_set_tuning",
     "trans_srv_create", "trans_srv_submit", "trans_srv_progress", "trans_srv_poll", "trans_srv_cancel",
     "trans_srv_result", "trans_srv_release", "trans_srv_destroy", "trans_srv_last_error",
+    "trans_geom_wall_distance",
 ]
 
 
@@ -219,6 +220,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_get_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]
     lib.trans_hip_set_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]
     lib.trans_hip_device_bytes.argtypes = [C.c_void_p, P(C.c_size_t)]
+    lib.trans_geom_wall_distance.argtypes = [P(C.c_double), C.c_long, P(C.c_double), C.c_long, P(C.c_double)]
     lib.trans_srv_create.argtypes = [P(TransSrvConfig), P(C.c_void_p)]
     lib.trans_srv_submit.argtypes = [C.c_void_p, P(TransSrvRequest), P(C.c_int)]
     lib.trans_srv_progress.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]

@@ -14,6 +14,7 @@
 
 #include "dev_common.h"
 #include "k_bc.h"
+#include "k_geom.h"
 #include "k_io.h"
 #include "k_monitor.h"
 #include "k_space.h"
@@ -1142,3 +1143,4 @@ int trans_hip_sync(trans_ctx* c)
 #include "serve.h"
 #include "io.h"
 #include "tune.h"
+#include "geom.h"

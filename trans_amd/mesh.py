@@ -193,8 +193,11 @@ def patch_structs(b: Block):
 # --------------------------------------------------------------------------------------
 # metrics (Geom/CalcDst.F90:28-719)
 
-def compute_geometry(blocks: list, tiny: float = 1e-20):
+def compute_geometry(blocks: list, tiny: float = 1e-20, device: bool = False):
+    """Metrics of every block; device=True computes the wall distances with the HIP
+    library's brute-force kernel (trans_geom_wall_distance), bitwise the same as here."""
     for b in blocks:
+        b.wall_distance_on_device = device
         _metrics(b, tiny)
     for b in blocks:
         _ghost_geometry_physical(b)
@@ -364,6 +367,9 @@ def _wall_distance(b: Block):
         dst[I, J, K] = 1.0e3  # no wall anywhere: distance is never used
         return
     pts = np.stack([b.arrays[n][I, J, K].ravel(order="F") for n in ("xc", "yc", "zc")], axis=1)
+    if getattr(b, "wall_distance_on_device", False):
+        dst[I, J, K] = wall_distance_device(pts, panels).reshape((NI1, NJ1, NK1), order="F")
+        return
     best = np.full(pts.shape[0], 1.0e20)
     chunk = 4096
     for s in range(0, pts.shape[0], chunk):
@@ -373,6 +379,22 @@ def _wall_distance(b: Block):
             d2 = ((p[:, None, :] - pan[None, :, :]) ** 2).sum(axis=2).min(axis=1)
             best[s:s + chunk] = np.minimum(best[s:s + chunk], d2)
     dst[I, J, K] = np.sqrt(np.maximum(best, 1e-20)).reshape((NI1, NJ1, NK1), order="F")
+
+
+def wall_distance_device(cells: np.ndarray, panels: np.ndarray) -> np.ndarray:
+    """cells (n, 3), panels (m, 3) -> distance of each cell to the nearest panel centre,
+    computed by the HIP library (raises if it is not built or the device fails)."""
+    import ctypes as C
+    lib = capi.load_library()
+    c = np.ascontiguousarray(np.asarray(cells, dtype=np.float64).T)
+    p = np.ascontiguousarray(np.asarray(panels, dtype=np.float64).T)
+    out = np.empty(c.shape[1])
+    DP = C.POINTER(C.c_double)
+    rc = lib.trans_geom_wall_distance(c.ctypes.data_as(DP), c.shape[1], p.ctypes.data_as(DP), p.shape[1],
+                                      out.ctypes.data_as(DP))
+    if rc != 0:
+        raise RuntimeError(f"trans_geom_wall_distance failed ({rc})")
+    return out
 
 
 def _layers(c):

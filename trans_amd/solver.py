@@ -211,12 +211,13 @@ class Solver:
         return a
 
 
-def prepare_case(blocks, org, glob, implicit_fp32=0):
-    """Connectivity, geometry and initial field for a generated case."""
+def prepare_case(blocks, org, glob, implicit_fp32=0, device_geometry=False):
+    """Connectivity, geometry and initial field for a generated case (device_geometry: the
+    wall distances by the HIP library's kernel)."""
     from .params import make_params
     prm = make_params(org, glob, implicit_fp32=implicit_fp32)
     mesh.build_connectivity(blocks, nblade=glob.nblade, npassage=glob.Npassage)
     mesh.attach_wall_panels(blocks)
-    mesh.compute_geometry(blocks)
+    mesh.compute_geometry(blocks, device=device_geometry)
     mesh.initial_state(blocks, prm, q_dst=org.Q_Dst, if_ns_uniform=org.IF_NS_Uniform)
     return prm
