@@ -76,9 +76,14 @@ def build_workload(name, implicit_fp32=0):
     elif name == "flatplate":
         blocks, org, glob = cases.flat_plate()
         desc = "flat plate 257x129x5 in 8 uneven blocks, RANS-SST, AUSMPW+ + MDCD, LU-SGS CFL 5"
+    elif name == "plate8m":
+        # the same deck at the 8M-cell scale: 1025x513x17 points in 16 uneven blocks
+        # (wall-normal stretching 1.0157 so that 512 cells from y+ = 1 reach the plate length)
+        blocks, org, glob = cases.flat_plate(nblk=16, ni_total=1025, nj=513, nk=17, lo=33, hi=97, ratio=1.0157)
+        desc = "flat plate 1025x513x17 (8.4M cells) in 16 uneven blocks, RANS-SST, AUSMPW+ + MDCD, LU-SGS CFL 5"
     else:
         raise SystemExit(f"unknown workload {name}")
-    prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32)
+    prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32, device_geometry=name == "plate8m")
     return blocks, prm, desc
 
 

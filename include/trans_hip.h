@@ -280,7 +280,8 @@ int trans_hip_read_stats(trans_ctx* ctx, int blk, const char* path);
 /* ---- Launch autotuner (no counterpart in the reference) ------------------------------------
  * Times `iters` iterations of every launch configuration of the context from the same saved
  * state and keeps the fastest: the LU-SGS sweep (sweep = 0, 1, 2: line march along i, j or k,
- * where the cross sections fit one workgroup; -1: hyperplane kernel), helper streams for the
+ * where the cross sections fit one workgroup; -2: one launch per hyperplane over all blocks;
+ * -1: one workgroup per block walking its hyperplanes), helper streams for the
  * face kernels (streams = 4) or one stream (1), HIP-graph replay (graph = 1) or eager launches.
  * Every configuration computes bitwise the same iteration.  The state and KI are restored;
  * out (max_out entries, may be NULL) receives the timings, *n_out their count.  The sweep

@@ -166,3 +166,26 @@ def test_hyperplane_sweep(monkeypatch, rk):
     _compare_state(sv, oc, blocks, TOL)
     _compare_norms(gres, ores, TOL)
     sv.close()
+
+
+@pytest.mark.parametrize("rk", [0, 1])
+def test_plane_launch_sweep(rk):
+    """The sweep as one launch per hyperplane (the choice for blocks whose cross sections
+    exceed a line-march workgroup), selected through trans_hip_set_tuning: one LU-SGS update,
+    or one RK stage with its k-omega sweeps, from identical inputs at 1e-12."""
+    print(f"case: flat plate, per-plane sweep launches, IF_RK={rk}", flush=True)
+    blocks, prm = _plate()
+    prm.if_rk = rk
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    sv.set_tuning(-2, 4, 0)
+    assert sv.tuning()["sweep"] == -2
+    oc.residual(1)
+    sv.residual()
+    if rk:
+        ores, gres = oc.rk_update(1, 1), sv.rk_update(1)
+    else:
+        ores, gres = oc.lusgs_update(1), sv.lusgs_update()
+    _compare_state(sv, oc, blocks, TOL)
+    _compare_norms(gres, ores, TOL)
+    sv.close()

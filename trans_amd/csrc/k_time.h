@@ -192,6 +192,58 @@ __device__ __forceinline__ void turb_nb(const Fields& f, const R* __restrict__ D
 // :442-675 backward; NEQ = 7 adds Time/TURBTWO_TIME_LUSGS_SST.F90:550-766).
 // R = double or float (FP32 implicit operator); Dq is D or its FP32 copy.  NEQ = 2 runs the
 // k-omega sweeps alone (the Runge-Kutta path, main.F90:160-163).
+// One cell (i, j, k) of the hyperplane sweeps: its increment from the neighbours' on the
+// previous hyperplane (natural layout).
+template <typename R, bool FORWARD, int NEQ>
+__device__ __forceinline__ void plane_cell(const trans_params& p, const Fields& f, R* Dq, const BlkInfo& b, int i,
+                                           int j, int k)
+{
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const long tot = f.tot;
+    const long st3[3] = {1, (long)b.nx, (long)b.nx * b.ny};
+    const double dtm = p.dt_lusgs_main;
+    const long c = cidx(b, i, j, k);
+    const double Dt = f.Dtm[c];
+    const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
+    const double cvs = stept / f.Vol[c];
+    R dn[3][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+    R tn[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    const int pos[3] = {i, j, k};
+    const int nmax[3] = {NI1, NJ1, NK1};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const bool edge = FORWARD ? pos[d] == 1 : pos[d] == nmax[d];
+        if (edge) continue;
+        const long cn = FORWARD ? c - st3[d] : c + st3[d];
+        if (NEQ != 2) lusgs_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), dn[d]);
+        if (NEQ != 5) turb_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), tn[d]);
+    }
+    if (NEQ != 2) {
+        const R f1 = R(f.F1[c]);
+#pragma unroll
+        for (int L = 0; L < 5; L++) {
+            const long e = (long)L * tot + c;
+            if (FORWARD)
+                Dq[e] = (Dq[e] * R(stept) + (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs)) / f1;
+            else
+                Dq[e] = Dq[e] - (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs) / f1;
+        }
+    }
+    if (NEQ != 5) {
+        const R f1t = R(f.F1t[c]);
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const long e = (long)(5 + q) * tot + c;
+            if (FORWARD)
+                Dq[e] = (Dq[e] * R(stept) + (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs)) / f1t;
+            else
+                Dq[e] = Dq[e] - (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs) / f1t;
+        }
+    }
+}
+
+// Whole sweep of a block in one workgroup, hyperplane after hyperplane (the fallback when
+// nothing else fits; one workgroup per block).
 template <typename R, bool FORWARD, int NEQ>
 __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, R* Dq,
                                                       const BlkInfo* __restrict__ blks)
@@ -200,53 +252,13 @@ __global__ void __launch_bounds__(1024) k_lusgs_sweep(trans_params p, Fields f, 
     const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
     const int nplanes = NI1 + NJ1 + NK1 - 2;
     const int nlines = NJ1 * NK1;
-    const long tot = f.tot;
-    const long st3[3] = {1, (long)b.nx, (long)b.nx * b.ny};
-    const double dtm = p.dt_lusgs_main;
     for (int s0 = 0; s0 < nplanes; s0++) {
         const int s = FORWARD ? s0 : nplanes - 1 - s0;
         for (int ln = threadIdx.x; ln < nlines; ln += blockDim.x) {
             const int j = ln % NJ1 + 1, k = ln / NJ1 + 1;
             const int i = s - (j - 1) - (k - 1) + 1;
             if (i < 1 || i > NI1) continue;
-            const long c = cidx(b, i, j, k);
-            const double Dt = f.Dtm[c];
-            const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
-            const double cvs = stept / f.Vol[c];
-            R dn[3][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
-            R tn[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-            const int pos[3] = {i, j, k};
-            const int nmax[3] = {NI1, NJ1, NK1};
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const bool edge = FORWARD ? pos[d] == 1 : pos[d] == nmax[d];
-                if (edge) continue;
-                const long cn = FORWARD ? c - st3[d] : c + st3[d];
-                if (NEQ != 2) lusgs_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), dn[d]);
-                if (NEQ != 5) turb_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), tn[d]);
-            }
-            if (NEQ != 2) {
-                const R f1 = R(f.F1[c]);
-#pragma unroll
-                for (int L = 0; L < 5; L++) {
-                    const long e = (long)L * tot + c;
-                    if (FORWARD)
-                        Dq[e] = (Dq[e] * R(stept) + (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs)) / f1;
-                    else
-                        Dq[e] = Dq[e] - (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs) / f1;
-                }
-            }
-            if (NEQ != 5) {
-                const R f1t = R(f.F1t[c]);
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    const long e = (long)(5 + q) * tot + c;
-                    if (FORWARD)
-                        Dq[e] = (Dq[e] * R(stept) + (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs)) / f1t;
-                    else
-                        Dq[e] = Dq[e] - (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs) / f1t;
-                }
-            }
+            plane_cell<R, FORWARD, NEQ>(p, f, Dq, b, i, j, k);
         }
         __syncthreads();
     }
@@ -496,6 +508,123 @@ __global__ void __launch_bounds__(MAXT) k_lusgs_line(Fields f, R* Dq, const BlkI
             line_sweep<R, FORWARD, 2, MARCH>(f, Dq, b, sink, buf);
     } else {
         line_sweep<R, FORWARD, NEQ, MARCH>(f, Dq, b, sink, buf);
+    }
+}
+
+// One hyperplane of every block per launch, on the hyperplane-major buffer of the line march
+// (march axis j): the host launches the planes in order, so the launch boundary is the
+// barrier, and workgroups cover every block's lines, so there is no limit on a block's cross
+// section.  A cell forms the terms of its three upstream neighbours itself from their slots on
+// the previous plane (the same expressions, in the same order, as the line march evaluates
+// them on the neighbour's side), sums them in i, j, k order and stores its increment in place.
+// Consecutive lines hold consecutive slots, so every load is coalesced.
+struct PlaneBlk {
+    int nplanes;  // NI1 + NJ1 + NK1 - 2
+    int na, nb;   // lines: i (fastest) by k
+};
+
+template <typename R, int NEQ>
+__device__ __forceinline__ void wplane_term(const Fields& f, const R* __restrict__ W, size_t wt, unsigned wn, int d,
+                                            R sgn, R o[NEQ])
+{
+    constexpr bool MF = NEQ != 2;
+    constexpr int C0 = sweep_c0(NEQ);
+    R dq[NEQ];
+#pragma unroll
+    for (int L = 0; L < NEQ; L++) dq[L] = ldg(W + (C0 + L) * wt, wn);
+    const R v0 = ldg(W + W_V * wt, wn);
+    const double roinv = 1.0 / double(v0);
+    const R pVx = R(ldg(W + (W_V + 1) * wt, wn) * roinv), pVy = R(ldg(W + (W_V + 2) * wt, wn) * roinv),
+            pVz = R(ldg(W + (W_V + 3) * wt, wn) * roinv);
+    const R sx = ldg(W + (W_SDM + d) * wt, wn), sy = ldg(W + (W_SDM + d + 3) * wt, wn),
+            sz = ldg(W + (W_SDM + d + 6) * wt, wn);
+    const R rd = ldg(W + (W_RDS + d) * wt, wn);
+    if constexpr (NEQ == 7) {
+        R dif[5];
+        lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, ldg(W + W_PP * wt, wn), R(roinv), dq, dif);
+#pragma unroll
+        for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
+        const R VP = pVx * sx + pVy * sy + pVz * sz;
+#pragma unroll
+        for (int L = 5; L < 7; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
+    } else if constexpr (MF) {
+        R dif[5];
+        lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, ldg(W + W_PP * wt, wn), R(roinv), dq, dif);
+#pragma unroll
+        for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
+    } else {
+        const R VP = pVx * sx + pVy * sy + pVz * sz;
+#pragma unroll
+        for (int L = 0; L < NEQ; L++) o[L] = R(0.5) * (dq[L] * VP + sgn * rd * dq[L]);
+    }
+}
+
+template <typename R, bool FORWARD, int NEQ>
+__global__ void __launch_bounds__(256) k_lusgs_wplane(Fields f, R* __restrict__ W, const BlkInfo* __restrict__ blks,
+                                                      const Seg* __restrict__ seg, const PlaneBlk* __restrict__ pb,
+                                                      int s0)
+{
+    const Seg sg = seg[blockIdx.x];
+    const PlaneBlk q = pb[sg.blk];
+    if (s0 >= q.nplanes) return;
+    const int ln = sg.base + threadIdx.x;
+    const int nlines = q.na * q.nb;
+    if (ln >= nlines) return;
+    const BlkInfo b = blks[sg.blk];
+    const int s = FORWARD ? s0 : q.nplanes - 1 - s0;
+    const int i = ln % q.na + 1, k = ln / q.na + 1;
+    const int j = s - (i - 1) - (k - 1) + 1;
+    const int NJ1 = b.nj - 1;
+    if (j < 1 || j > NJ1) return;
+    constexpr bool MF = NEQ != 2;
+    constexpr int C0 = sweep_c0(NEQ);
+    const size_t wt = (size_t)f.wtot;
+    const unsigned w = (unsigned)(b.woff + (long)s * nlines + ln);
+    const unsigned wp = FORWARD ? w - (unsigned)nlines : w + (unsigned)nlines;  // same line, previous plane
+    const R sgn = FORWARD ? R(1) : R(-1);
+    const int pos[3] = {i, j, k};
+    const int nmax[3] = {q.na, NJ1, q.nb};
+    // upstream neighbours on the previous plane: i -/+ 1 (line -/+ 1), j -/+ 1 (same line),
+    // k -/+ 1 (line -/+ na)
+    const int dl[3] = {FORWARD ? -1 : 1, 0, FORWARD ? -q.na : q.na};
+    R sum[NEQ];
+#pragma unroll
+    for (int L = 0; L < NEQ; L++) sum[L] = R(0);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const bool edge = FORWARD ? pos[d] == 1 : pos[d] == nmax[d];
+        R o[NEQ];
+        if (edge) {
+#pragma unroll
+            for (int L = 0; L < NEQ; L++) o[L] = R(0);
+        } else {
+            wplane_term<R, NEQ>(f, W, wt, wp + (unsigned)dl[d], d, sgn, o);
+        }
+#pragma unroll
+        for (int L = 0; L < NEQ; L++) sum[L] = d == 0 ? o[L] : sum[L] + o[L];
+    }
+    const double stept = ldg(W + W_STEPT * wt, w);
+    const double cvs = ldg(W + W_CVS * wt, w);
+    if constexpr (MF) {
+        // F1 as Time/TIME_LUSGS.F90:92-108 forms it (k_lusgs_rhs evaluates the same expression)
+        const double r0 = ldg(W + W_RDS * wt, w), r1 = ldg(W + (W_RDS + 1) * wt, w), r2 = ldg(W + (W_RDS + 2) * wt, w);
+        const R f1 = R(1.0 + (double(r0) + double(r1) + double(r2)) * cvs);
+#pragma unroll
+        for (int L = 0; L < 5; L++) {
+            R* a = W + (C0 + L) * wt;
+            const R dv = ldg(a, w);
+            stg(a, w, FORWARD ? (dv * R(stept) + sum[L] * R(cvs)) / f1 : dv - sum[L] * R(cvs) / f1);
+        }
+    }
+    if constexpr (NEQ != 5) {
+        constexpr int T0 = NEQ == 7 ? 5 : 0;
+        const R f1t = ldg(W + W_F1T * wt, w);
+#pragma unroll
+        for (int L = T0; L < NEQ; L++) {
+            R* a = W + (C0 + L) * wt;
+            const R dv = ldg(a, w);
+            stg(a, w, FORWARD ? (dv * R(stept) + sum[L] * R(cvs)) / f1t : dv - sum[L] * R(cvs) / f1t);
+        }
     }
 }
 

@@ -86,6 +86,10 @@ struct trans_ctx {
     int march = 1;         // march axis of the line sweep (0 i, 1 j, 2 k)
     int max_lines = 0;     // largest line count over the blocks for that axis
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
+    bool plane_launch = false;  // hyperplane sweep as one launch per plane (else one workgroup per block)
+    SegTable seg_plane;         // workgroups of the per-plane launches: (block, first line)
+    PlaneBlk* d_pblk = nullptr;
+    int max_planes = 0;
     void* wbuf = nullptr;  // the sweep buffer W / Wf (reallocated when the march axis changes)
     size_t wbytes = 0;
     std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
@@ -323,7 +327,7 @@ static int upload_blk_table(trans_ctx* c)
         const HostBlock& h = c->hb[b];
         const int n[3] = {h.ni - 1, h.nj - 1, h.nk - 1};
         bi[b] = BlkInfo{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, c->patch0[b + 1] - c->patch0[b], c->patch0[b],
-                        c->mwoff[b], c->line_threads ? wt : 0};
+                        c->mwoff[b], c->wbytes ? wt : 0};
         wt += (long)(n[0] + n[1] + n[2] - 2) * n[(c->march + 1) % 3] * n[(c->march + 2) % 3];
     }
     if (!c->d_blk) return 0;  // set before the table exists: create uploads it
@@ -335,7 +339,10 @@ static int upload_blk_table(trans_ctx* c)
 static int set_sweep(trans_ctx* c, int m, bool upload)
 {
     int mx = 0;
-    if (m >= 0 && !march_fits(c, m, &mx)) return fail(c, TRANS_EINVAL, "the line sweep does not fit this axis");
+    const bool planes = m == -2;  // per-plane launches on the hyperplane-major buffer, march axis j
+    if (planes) m = 1;
+    if (!planes && m >= 0 && !march_fits(c, m, &mx)) return fail(c, TRANS_EINVAL, "the line sweep does not fit this axis");
+    c->plane_launch = planes;
     if (c->wbuf) {
         if (!c->dry) (void)hipFree(c->wbuf);
         c->wbuf = nullptr;
@@ -349,7 +356,7 @@ static int set_sweep(trans_ctx* c, int m, bool upload)
     if (m >= 0) {
         c->march = m;
         c->max_lines = mx;
-        c->line_threads = (mx + 63) / 64 * 64;
+        c->line_threads = planes ? 0 : (mx + 63) / 64 * 64;
         // hyperplane-major sweep buffer: nplanes x nlines slots per block, then per
         // component a sink of 1024 slots for each of the 2 workgroups of every block
         long wt = 0;
@@ -362,6 +369,7 @@ static int set_sweep(trans_ctx* c, int m, bool upload)
         // the sweep addresses a component with 32-bit byte offsets
         if (wt * (long)sizeof(double) >= (1L << 32)) {
             c->line_threads = 0;
+            c->plane_launch = false;
             return fail(c, TRANS_EUNSUPPORTED, "too many cells for the line-march sweep buffer");
         }
         f.wtot = wt;
@@ -519,7 +527,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         const char* env = std::getenv("TRANS_HIP_LINE_SWEEP");
         const bool lines_ok = !(env && env[0] == '0');
         const int order[3] = {1, 2, 0};
-        int m = -1;
+        int m = lines_ok ? -2 : -1;  // blocks too large for a line-march workgroup: one launch per plane
         for (int q = 0; q < 3 && lines_ok && m < 0; q++)
             if (march_fits(c, order[q], nullptr)) m = order[q];
         if ((rc = set_sweep(c, m, false))) return rc;
@@ -528,6 +536,24 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     if ((rc = upload_blk_table(c))) return rc;
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
+    {
+        // per-plane sweep launches: lines (i, k) of every block, march axis j
+        std::vector<PlaneBlk> pb;
+        std::vector<Seg> ps;
+        for (int b = 0; b < nblk; b++) {
+            const HostBlock& h = c->hb[b];
+            const int n[3] = {h.ni - 1, h.nj - 1, h.nk - 1};
+            PlaneBlk q{n[0] + n[1] + n[2] - 2, n[0], n[2]};
+            pb.push_back(q);
+            c->max_planes = std::max(c->max_planes, q.nplanes);
+            for (int base = 0; base < q.na * q.nb; base += WG) ps.push_back(Seg{b, base});
+        }
+        c->seg_plane.n = (int)ps.size();
+        if ((rc = dalloc(c, &c->seg_plane.d, ps.size())) || (rc = dalloc(c, &c->d_pblk, pb.size()))) return rc;
+        if ((rc = h2d(c, c->seg_plane.d, ps.data(), ps.size() * sizeof(Seg))) ||
+            (rc = h2d(c, c->d_pblk, pb.data(), pb.size() * sizeof(PlaneBlk))))
+            return rc;
+    }
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
     if ((rc = h2d(c, c->d_wrange, c->seg_cell.wrange.data(), (nblk + 1) * sizeof(int)))) return rc;
     if ((rc = dalloc(c, &c->d_part, c->seg_cell.n)) || (rc = dalloc(c, &c->d_resid, nblk)) ||
@@ -877,6 +903,14 @@ static void launch_sweeps(trans_ctx* c, R* dq)
             launch_lines<R, NEQ, 512>(c, dq);
         else
             launch_lines<R, NEQ, 1024>(c, dq);
+    } else if (c->plane_launch) {
+        const bool kw = NEQ == 2;
+        for (int s0 = 0; s0 < c->max_planes; s0++)
+            KLAUNCH(c, kw ? "k_turb_wplane<fwd>" : "k_lusgs_wplane<fwd>", (k_lusgs_wplane<R, true, NEQ>),
+                    dim3(c->seg_plane.n), dim3(WG), c->f, dq, c->d_blk, c->seg_plane.d, c->d_pblk, s0);
+        for (int s0 = 0; s0 < c->max_planes; s0++)
+            KLAUNCH(c, kw ? "k_turb_wplane<bwd>" : "k_lusgs_wplane<bwd>", (k_lusgs_wplane<R, false, NEQ>),
+                    dim3(c->seg_plane.n), dim3(WG), c->f, dq, c->d_blk, c->seg_plane.d, c->d_pblk, s0);
     } else {
         const bool kw = NEQ == 2;
         KLAUNCH(c, kw ? "k_turb_sweep<fwd>" : "k_lusgs_sweep<fwd>", (k_lusgs_sweep<R, true, NEQ>), dim3(nblk),

@@ -3,7 +3,7 @@
 // same dependencies and performs the same operations per cell, so results are bitwise the
 // same):
 //   - the LU-SGS sweep: the line march along each axis whose cross sections fit one
-//     workgroup (i, j or k), or the hyperplane kernel;
+//     workgroup (i, j or k), one launch per hyperplane, or the one-workgroup hyperplane kernel;
 //   - the helper streams (face kernels of the three directions concurrently) or one stream;
 //   - replaying the iteration as a HIP graph or launching it eagerly.
 // Each candidate runs `iters` timed iterations from the same saved state (after one untimed
@@ -45,7 +45,7 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
     const int ki0 = c->ki;
     const bool stats0 = c->stats_on;
     const bool graphs0 = c->graphs;
-    const int sweep0 = c->line_threads ? c->march : -1;
+    const int sweep0 = c->line_threads ? c->march : (c->plane_launch ? -2 : -1);
     const bool single0 = c->aux[0] == c->stream;
     c->stats_on = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -53,8 +53,8 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = fail(c, TRANS_EHIP, "hipEventCreate");
     std::vector<trans_tune_entry> res;
-    const int sweeps[4] = {1, 2, 0, -1};
-    for (int si = 0; si < 4 && !rc; si++) {
+    const int sweeps[5] = {1, 2, 0, -2, -1};
+    for (int si = 0; si < 5 && !rc; si++) {
         const int sw = sweeps[si];
         if (sw >= 0 && !march_fits(c, sw, nullptr)) continue;
         if ((rc = set_sweep(c, sw, true))) break;
@@ -115,11 +115,11 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
 int trans_hip_set_tuning(trans_ctx* c, const trans_tune_entry* t)
 {
     if (!c || !t) return TRANS_EINVAL;
-    if (t->sweep < -1 || t->sweep > 2) return fail(c, TRANS_EINVAL, "sweep must be -1, 0, 1 or 2");
+    if (t->sweep < -2 || t->sweep > 2) return fail(c, TRANS_EINVAL, "sweep must be -2, -1, 0, 1 or 2");
     if (t->streams != 1 && t->streams != 4) return fail(c, TRANS_EINVAL, "streams must be 1 or 4");
     if (t->streams == 4 && !c->aux_own[0]) return fail(c, TRANS_EINVAL, "this context has one stream");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int cur = c->line_threads ? c->march : -1;
+    const int cur = c->line_threads ? c->march : (c->plane_launch ? -2 : -1);
     if (t->sweep != cur) {
         int rc = set_sweep(c, t->sweep, true);
         if (rc) return rc;
@@ -133,7 +133,7 @@ int trans_hip_set_tuning(trans_ctx* c, const trans_tune_entry* t)
 int trans_hip_get_tuning(const trans_ctx* c, trans_tune_entry* out)
 {
     if (!c || !out) return TRANS_EINVAL;
-    out->sweep = c->line_threads ? c->march : -1;
+    out->sweep = c->line_threads ? c->march : (c->plane_launch ? -2 : -1);
     out->streams = c->aux[0] == c->stream ? 1 : 4;
     out->graph = c->graphs ? 1 : 0;
     out->ms_per_iter = 0.0;

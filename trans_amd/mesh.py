@@ -112,13 +112,9 @@ def build_connectivity(blocks: list, nblade: int = 1, npassage: int = 1):
             c["rotate_angle"] = 0.0
             if r.pertype != 0:
                 c["rotate_angle"] = float(npassage * r.pertype) * 2 * math.pi / float(nblade)
-            if Imin == Imax:
-                d = 1
-            elif Jmin == Jmax:
-                d = 2
-            elif Kmin == Kmax:
-                d = 3
-            else:
+            # the face number fixes the direction (a one-cell-wide patch has two constant indices)
+            d = (r.face + 1) // 2
+            if [Imin, Jmin, Kmin][d - 1] != [Imax, Jmax, Kmax][d - 1]:
                 raise ValueError("patch must be a constant-index face")
             c["ijkdirect"] = d
             mn = [Imin, Jmin, Kmin][d - 1]
