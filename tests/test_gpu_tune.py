@@ -1,13 +1,15 @@
 """GPU tests of the launch autotuner: every configuration it tries (sweep variant, helper
-streams, HIP graph) computes bitwise the same iterations, it restores the state and KI, and the
-configuration it keeps is its fastest."""
+streams, HIP graph) computes the same iteration (one update from the same state within 1e-12
+relative: the sweep variants contract multiply-adds into FMAs in different places, so they may
+differ in the last bits; six iterations within 1e-6, the bar of the other multi-iteration
+tests), it restores the state and KI bitwise, and the configuration it keeps is its fastest."""
 import numpy as np
 import pytest
 
 from trans_amd import cases, solver
 from trans_amd.solver import Solver
 
-from helpers import interior, perturb
+from helpers import interior, perturb, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -43,20 +45,28 @@ def test_autotune_keeps_results(name):
         assert np.array_equal(a.get_state(n)["v"], before[n])
     b = Solver(blocks, prm, ki=1)
     b.iterate(2, want_resid=False)
+    b1 = b.iterate(1)
+    v1 = [b.get_state(n)["v"].copy() for n in range(len(blocks))]
+    rb = b.iterate(5)
     ra = a.iterate(6)
-    rb = b.iterate(6)
     for n, blk in enumerate(blocks):
-        assert np.array_equal(interior(a.get_state(n)["v"], blk), interior(b.get_state(n)["v"], blk)), n
-        assert ra[n]["drsa"] == rb[n]["drsa"]
+        assert relerr(interior(a.get_state(n)["v"], blk), interior(b.get_state(n)["v"], blk)).max() < 1e-6, n
+        assert abs(ra[n]["drsa"] - rb[n]["drsa"]) <= 1e-6 * rb[n]["drsa"]
     # every configuration it tried, applied to a fresh context, gives the same iterations
     for row in table:
         c = Solver(blocks, prm, ki=1)
         c.iterate(2, want_resid=False)
         c.set_tuning(row["sweep"], row["streams"], row["graph"])
-        rc = c.iterate(6)
+        c1 = c.iterate(1)
         for n, blk in enumerate(blocks):
-            assert np.array_equal(interior(c.get_state(n)["v"], blk), interior(b.get_state(n)["v"], blk)), (row, n)
-            assert rc[n]["drsa"] == rb[n]["drsa"], row
+            e = relerr(interior(c.get_state(n)["v"], blk), interior(v1[n], blk))
+            assert np.all(e < 1e-12), (row, n, e)
+            assert abs(c1[n]["drsa"] - b1[n]["drsa"]) <= 1e-12 * b1[n]["drsa"], row
+        rc = c.iterate(5)
+        for n, blk in enumerate(blocks):
+            e = relerr(interior(c.get_state(n)["v"], blk), interior(b.get_state(n)["v"], blk))
+            assert np.all(e < 1e-6), (row, n, e)
+            assert abs(rc[n]["drsa"] - rb[n]["drsa"]) <= 1e-6 * rb[n]["drsa"], row
         c.close()
     a.close()
     b.close()

@@ -4,6 +4,11 @@
 #pragma once
 #include "dev_common.h"
 
+// The implicit operator (LU-SGS sweeps) may contract multiply-adds into FMAs: its result
+// only has to reach the update within the parity bar (it also runs in FP32), while the
+// residual kernels keep the oracle's exact operation sequence (-ffp-contract=off).
+#define TRANS_SWEEP_CONTRACT _Pragma("clang fp contract(fast)")
+
 // ---------------------------------------------------------------------------------------
 // Time/TIME_LUSGS.F90:60-108: BDF2 right-hand side and the scalar diagonal F1.  With the
 // line-march sweep the right-hand side and the state the sweep needs go to the
@@ -105,6 +110,7 @@ template <typename R>
 __device__ __forceinline__ void lusgs_adq(R pVx, R pVy, R pVz, R aSDx, R aSDy, R aSDz, R pPP,
                                           R roinv, const R ain[5], R dif[5])
 {
+    TRANS_SWEEP_CONTRACT
     const R pGam = R(1.4);
     const R AinRo = ain[0], AinRx = ain[1], AinRy = ain[2], AinRz = ain[3], AinRe = ain[4];
     const R VP = pVx * aSDx + pVy * aSDy + pVz * aSDz + R(0);
@@ -137,6 +143,7 @@ template <typename R>
 __device__ __forceinline__ void lusgs_nb(const Fields& f, const R* __restrict__ Dq, int d,
                                          long cn, long st, R sgn, R acc[5])
 {
+    TRANS_SWEEP_CONTRACT
     const long tot = f.tot;
     R ain[5];
 #pragma unroll
@@ -166,6 +173,7 @@ template <typename R>
 __device__ __forceinline__ void turb_nb(const Fields& f, const R* __restrict__ Dq, int d, long cn, long st,
                                         R sgn, R acc[2])
 {
+    TRANS_SWEEP_CONTRACT
     const long tot = f.tot;
     const double roinv = 1.0 / f.V[cn];
     const R pVx = R(f.V[tot + cn] * roinv), pVy = R(f.V[2 * tot + cn] * roinv), pVz = R(f.V[3 * tot + cn] * roinv);
@@ -198,6 +206,7 @@ template <typename R, bool FORWARD, int NEQ>
 __device__ __forceinline__ void plane_cell(const trans_params& p, const Fields& f, R* Dq, const BlkInfo& b, int i,
                                            int j, int k)
 {
+    TRANS_SWEEP_CONTRACT
     const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
     const long tot = f.tot;
     const long st3[3] = {1, (long)b.nx, (long)b.nx * b.ny};
@@ -388,6 +397,7 @@ template <typename R, bool FORWARD, int NEQ, int MARCH>
 __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, const BlkInfo& b, unsigned sink,
                                            R* __restrict__ buf)
 {
+    TRANS_SWEEP_CONTRACT
     static_assert(NEQ == 5 || NEQ == 2, "mean flow or k-omega");
     constexpr bool MF = NEQ == 5;
     const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
@@ -527,6 +537,7 @@ template <typename R, int NEQ>
 __device__ __forceinline__ void wplane_term(const Fields& f, const R* __restrict__ W, size_t wt, unsigned wn, int d,
                                             R sgn, R o[NEQ])
 {
+    TRANS_SWEEP_CONTRACT
     constexpr bool MF = NEQ != 2;
     constexpr int C0 = sweep_c0(NEQ);
     R dq[NEQ];
@@ -564,6 +575,7 @@ __global__ void __launch_bounds__(256) k_lusgs_wplane(Fields f, R* __restrict__ 
                                                       const Seg* __restrict__ seg, const PlaneBlk* __restrict__ pb,
                                                       int s0)
 {
+    TRANS_SWEEP_CONTRACT
     const Seg sg = seg[blockIdx.x];
     const PlaneBlk q = pb[sg.blk];
     if (s0 >= q.nplanes) return;
