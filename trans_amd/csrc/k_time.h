@@ -129,8 +129,9 @@ __device__ __forceinline__ void lusgs_adq(R pVx, R pVy, R pVz, R aSDx, R aSDy, R
     dif[3] = AinRo * (aSDz * gUt2 - pVz * VP) + AinRx * (aSDx * pVz - aSDz * Gm1 * pVx) +
              AinRy * (aSDy * pVz - aSDz * Gm1 * pVy) + AinRz * (aSDt + VP - aSDz * Gm2 * pVz) +
              AinRe * (aSDz * Gm1);
-    const R GEt1 = R(0.5) * Gm2 * Ut2 - AA2 / Gm1;
-    const R GEt2 = R(0.5) * Ut2 + AA2 / Gm1;
+    const R AA2g = AA2 * (R(1) / Gm1);  // one reciprocal of the constant instead of two divisions
+    const R GEt1 = R(0.5) * Gm2 * Ut2 - AA2g;
+    const R GEt2 = R(0.5) * Ut2 + AA2g;
     dif[4] = AinRo * VP * GEt1 + AinRx * (aSDx * GEt2 - Gm1 * pVx * VP) +
              AinRy * (aSDy * GEt2 - Gm1 * pVy * VP) + AinRz * (aSDz * GEt2 - Gm1 * pVz * VP) +
              AinRe * (aSDt + pGam * VP);
@@ -228,25 +229,25 @@ __device__ __forceinline__ void plane_cell(const trans_params& p, const Fields& 
         if (NEQ != 5) turb_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), tn[d]);
     }
     if (NEQ != 2) {
-        const R f1 = R(f.F1[c]);
+        const R rf1 = R(1) / R(f.F1[c]);  // one division per cell, five multiplications
 #pragma unroll
         for (int L = 0; L < 5; L++) {
             const long e = (long)L * tot + c;
             if (FORWARD)
-                Dq[e] = (Dq[e] * R(stept) + (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs)) / f1;
+                Dq[e] = (Dq[e] * R(stept) + (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs)) * rf1;
             else
-                Dq[e] = Dq[e] - (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs) / f1;
+                Dq[e] = Dq[e] - (dn[0][L] + dn[1][L] + dn[2][L]) * R(cvs) * rf1;
         }
     }
     if (NEQ != 5) {
-        const R f1t = R(f.F1t[c]);
+        const R rf1t = R(1) / R(f.F1t[c]);
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const long e = (long)(5 + q) * tot + c;
             if (FORWARD)
-                Dq[e] = (Dq[e] * R(stept) + (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs)) / f1t;
+                Dq[e] = (Dq[e] * R(stept) + (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs)) * rf1t;
             else
-                Dq[e] = Dq[e] - (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs) / f1t;
+                Dq[e] = Dq[e] - (tn[0][q] + tn[1][q] + tn[2][q]) * R(cvs) * rf1t;
         }
     }
 }
@@ -443,6 +444,7 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
         // F1 as Time/TIME_LUSGS.F90:92-108 forms it (k_lusgs_rhs evaluates the same expression)
         const R f1 = MF ? R(1.0 + (double(cur.rds[0]) + double(cur.rds[1]) + double(cur.rds[2])) * cvs)
                         : R(cur.f1);
+        const R rf1 = R(1) / f1;
         R dq[NEQ];
 #pragma unroll
         for (int L = 0; L < NEQ; L++) {
@@ -458,9 +460,9 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
             else
                 sum = sa + sb + sm;
             if (FORWARD)
-                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) / f1;
+                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) * rf1;
             else
-                dq[L] = R(cur.d[L]) - sum * R(cvs) / f1;
+                dq[L] = R(cur.d[L]) - sum * R(cvs) * rf1;
             stg(src.d[L], wst, dq[L]);
         }
         const double roinv = 1.0 / double(cur.v[0]);
@@ -620,22 +622,22 @@ __global__ void __launch_bounds__(256) k_lusgs_wplane(Fields f, R* __restrict__ 
     if constexpr (MF) {
         // F1 as Time/TIME_LUSGS.F90:92-108 forms it (k_lusgs_rhs evaluates the same expression)
         const double r0 = ldg(W + W_RDS * wt, w), r1 = ldg(W + (W_RDS + 1) * wt, w), r2 = ldg(W + (W_RDS + 2) * wt, w);
-        const R f1 = R(1.0 + (double(r0) + double(r1) + double(r2)) * cvs);
+        const R rf1 = R(1) / R(1.0 + (double(r0) + double(r1) + double(r2)) * cvs);
 #pragma unroll
         for (int L = 0; L < 5; L++) {
             R* a = W + (C0 + L) * wt;
             const R dv = ldg(a, w);
-            stg(a, w, FORWARD ? (dv * R(stept) + sum[L] * R(cvs)) / f1 : dv - sum[L] * R(cvs) / f1);
+            stg(a, w, FORWARD ? (dv * R(stept) + sum[L] * R(cvs)) * rf1 : dv - sum[L] * R(cvs) * rf1);
         }
     }
     if constexpr (NEQ != 5) {
         constexpr int T0 = NEQ == 7 ? 5 : 0;
-        const R f1t = ldg(W + W_F1T * wt, w);
+        const R rf1t = R(1) / ldg(W + W_F1T * wt, w);
 #pragma unroll
         for (int L = T0; L < NEQ; L++) {
             R* a = W + (C0 + L) * wt;
             const R dv = ldg(a, w);
-            stg(a, w, FORWARD ? (dv * R(stept) + sum[L] * R(cvs)) / f1t : dv - sum[L] * R(cvs) / f1t);
+            stg(a, w, FORWARD ? (dv * R(stept) + sum[L] * R(cvs)) * rf1t : dv - sum[L] * R(cvs) * rf1t);
         }
     }
 }
