@@ -301,7 +301,9 @@ def main():
     neq = 7 if (prm.if_euler == 0 and prm.if_turb_org == 1) else 5
     print(f"[bench] {desc}: {cells} cells, warmup {args.warmup}, steps {args.steps}", flush=True)
     prof_steps = min(args.steps, 50)
-    dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps, args.tune)
+    # a profiled run (--profile-only) keeps the default launch configuration: the one the
+    # tuner picks for the flat plate, without its graph captures under the profiler
+    dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps, 0 if args.profile_only else args.tune)
     tuning = timed_run.tuning
     tag = "FP32 implicit" if prm.implicit_fp32 else "FP64"
     print(f"[bench] {tag}: {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)
