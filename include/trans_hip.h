@@ -56,7 +56,7 @@ typedef struct trans_params {
     int if_comppress;    /* SST compressibility correction */
     int kind_cc;
     int kind_rotmod;     /* 0 or 2 (mode 2 only writes f_r2, which nothing reads; mode 1 reads an unset index) */
-    int if_precondition; /* must be 0 */
+    int if_precondition; /* 0, or 1: low-Mach preconditioning (hyperplane LU-SGS sweep); fixed at creation */
     int cfl_sub_step;    /* >= 1 (SpectralRadius.F90:159 divides by it) */
     int ki_ini;
     int implicit_fp32;   /* 1: LU-SGS operator and Rds diagonals in FP32 */

@@ -94,6 +94,11 @@ def lib():
         L.ora_alimiter.argtypes = [C.c_double, C.c_double]
         L.ora_alimiter.restype = C.c_double
         L.ora_roe_common.argtypes = [PP] + [C.c_double] * 16 + [DP]
+        L.ora_roe_common_pc.argtypes = [PP] + [C.c_double] * 18 + [DP]
+        L.ora_roe_rotated_face_pc.argtypes = [PP] + [C.c_double] * 19 + [DP]
+        L.ora_precond_para.argtypes = [PP] + [C.c_double] * 6
+        L.ora_precond_para.restype = C.c_double
+        L.ora_precond_dw.argtypes = [PP, DP] + [C.c_double] * 9
         L.ora_lusgs_adq.argtypes = [C.c_double] * 8 + [DP, DP]
         L.ora_ijk_start_end.argtypes = [C.c_int] * 3 + [C.POINTER(C.c_int)] * 2 + [C.c_int,
                                                                                    C.POINTER(C.c_int)]

@@ -70,11 +70,23 @@ void ora_spectral_radius(const trans_params* p, int ki, ora_block* b)
                 double VV = Vxi * ARX2 + Vyi * ARY2 + Vzi * ARZ2 + 0.0;
                 double WW = Vxi * ARX3 + Vyi * ARY3 + Vzi * ARZ3 + 0.0;
                 double vol = VOL(I, J, K);
-                RDS(1, I, J, K) = fabs(UU) * (1.0 - 0.0) + sqrt(ST1sq) * AAc +
+                double alf[3] = {0.0, 0.0, 0.0}, AAcd[3] = {AAc, AAc, AAc};
+                if (p->if_precondition) { /* :91-120 */
+                    double UVW2 = Vxi * Vxi + Vyi * Vyi + Vzi * Vzi;
+                    double ub = ora_precond_para(p, UVW2, AAc * AAc, Den, 0.0, 0.0, 1.0);
+                    double RmiuAll = RmiuL + RmiuTL;
+                    for (int d = 0; d < 3; d++) {
+                        double r = RmiuAll / Den / p->ref / ALAGM(d + 1, I, J, K) / AAc;
+                        double ubd = fmax(ub, r * r);
+                        alf[d] = (1.0 - ubd) / 2.0;
+                        AAcd[d] = sqrt(alf[d] * alf[d] * UVW2 + ubd * AAc * AAc);
+                    }
+                }
+                RDS(1, I, J, K) = fabs(UU) * (1.0 - alf[0]) + sqrt(ST1sq) * AAcd[0] +
                                   2.0 * ST1sq / vol * pruvmu / p->ref;
-                RDS(2, I, J, K) = fabs(VV) * (1.0 - 0.0) + sqrt(ST2sq) * AAc +
+                RDS(2, I, J, K) = fabs(VV) * (1.0 - alf[1]) + sqrt(ST2sq) * AAcd[1] +
                                   2.0 * ST2sq / vol * pruvmu / p->ref;
-                RDS(3, I, J, K) = fabs(WW) * (1.0 - 0.0) + sqrt(ST3sq) * AAc +
+                RDS(3, I, J, K) = fabs(WW) * (1.0 - alf[2]) + sqrt(ST3sq) * AAcd[2] +
                                   2.0 * ST3sq / vol * pruvmu / p->ref;
                 double Dtm_local = vol / (RDS(1, I, J, K) + RDS(2, I, J, K) + RDS(3, I, J, K));
                 /* IF_local_Main == 0 branch */
@@ -700,11 +712,72 @@ void ora_convect_muscl2(const trans_params* p, ora_block* b)
     convect_muscl_lim(b, ora_muscl2);
 }
 
-/* Space/CONVECT_Roe_Original.F90:242-407 (CONVECT_Roe_Common, IF_PRECONDITION = 0) */
+/* Common/UTIL_Precond_Funcs.F90:6-30 (getPrecondPara): (U_r/c)^2 of the low-Mach
+ * preconditioning, K = 3, Kc = 1, with the viscous limit. */
+double ora_precond_para(const trans_params* p, double VM2, double AM2, double RR, double TT, double RMIUT,
+                        double ALAGMR)
+{
+    const double Kp = 3.0, Kc = 1.0;
+    double ubetac2 = fmin(fmax(VM2 / AM2, Kp * p->xm * p->xm), 1.0);
+    double RmiuL = (1.0 + p->csthlnd) / (TT * p->csth1 + p->csthlnd) * pow(p->csth1 * TT, 1.5);
+    double RmiuAll = RmiuL + RMIUT;
+    double r = RmiuAll / RR / p->ref / ALAGMR;
+    ubetac2 = fmax(ubetac2, Kc * (r * r) / AM2);
+    return ubetac2;
+}
+
+/* Common/UTIL_Precond_Funcs.F90:86-130 (PrecondDW): the conservative increment a[5] through
+ * the preconditioning transform (momentum components unchanged). */
+void ora_precond_dw(const trans_params* p, double a[5], double pRo, double pVx, double pVy, double pVz, double pE,
+                    double AAc, double ubetac2, double pPP, double pTT)
+{
+    (void)pPP;
+    double pU2 = pVx * pVx + pVy * pVy + pVz * pVz;
+    double pUAU = pVx * a[1] + pVy * a[2] + pVz * a[3];
+    a[4] = a[4] - (pUAU) - (pE / 1.4 - 1.2 * (pU2)) * a[0];
+    double rCp = 1.4 / 0.4 * p->ppf * pRo;
+    double a11 = AAc * AAc * ubetac2;
+    double a15 = -0.4 * ubetac2 / rCp;
+    double b11 = 1.0 * a11 / rCp;
+    double b15 = (1.0 + 0.4 * ubetac2) / rCp;
+    double AinP = a11 * a[0] + a15 * a[4];
+    double AinT = b11 * a[0] + b15 * a[4];
+    a[0] = AinP / p->ppf / pTT - pRo * AinT / pTT;
+    a[4] = AinP / 0.4 + (pUAU);
+}
+
+/* The preconditioning transform of a cell's increment with the cell's own state
+ * (Time/TIME_LUSGS.F90:113-160 and the neighbour terms :235-246 and mirrors). */
+static void precond_cell(const trans_params* p, ora_block* b, int i, int j, int k, double a[5])
+{
+    double pRo = V(1, i, j, k), roinv = 1.0 / pRo;
+    double pVx = V(2, i, j, k) * roinv, pVy = V(3, i, j, k) * roinv, pVz = V(4, i, j, k) * roinv;
+    double pE = V(5, i, j, k) * roinv;
+    double pPP = PP(i, j, k), pTT = T(i, j, k);
+    double AAc = sqrt(1.4 * pPP * roinv);
+    double UVW2 = pVx * pVx + pVy * pVy + pVz * pVz;
+    double ALL = fmin(fmin(ALAGM(1, i, j, k), ALAGM(2, i, j, k)), ALAGM(3, i, j, k));
+    double ubetac2 = ora_precond_para(p, UVW2, AAc * AAc, pRo, pTT, RMIU(i, j, k), ALL);
+    ora_precond_dw(p, a, pRo, pVx, pVy, pVz, pE, AAc, ubetac2, pPP, pTT);
+}
+
+/* Space/CONVECT_Roe_Original.F90:242-407 (CONVECT_Roe_Common) without the face eddy
+ * viscosity and Alagm, which only its preconditioned branch reads. */
 void ora_roe_common(const trans_params* p, double PLo, double PRo, double TLo, double TRo,
                     double ULo, double URo, double VLo, double VRo, double WLo, double WRo,
                     double sav1, double sav2, double sav3, double sav, double rLo, double rRo,
                     double out[5])
+{
+    ora_roe_common_pc(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, sav1, sav2, sav3, sav, rLo, rRo, 0.0,
+                      1.0, out);
+}
+
+/* Space/CONVECT_Roe_Original.F90:242-407 (CONVECT_Roe_Common) with the face eddy viscosity
+ * RmiuT = VL(d,8) and AlagmR = Alagm(d) that the preconditioned branch (:319-332,
+ * IF_PRECONDITION = 1, no entropy fix) reads. */
+void ora_roe_common_pc(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
+                       double URo, double VLo, double VRo, double WLo, double WRo, double sav1, double sav2,
+                       double sav3, double sav, double rLo, double rRo, double RmiuT, double AlagmR, double out[5])
 {
     const double rcpcv = p->ppf;
     double DENLo = PLo / (rcpcv * TLo);
@@ -731,14 +804,27 @@ void ora_roe_common(const trans_params* p, double PLo, double PRo, double TLo, d
     double unormaln = (sav1n * um + sav2n * vm + sav3n * wm);
     double anormaln = am;
     (void)acl; (void)acr;
-    double eig1 = fabs(unormaln);
-    double eig2 = fabs(unormaln + anormaln);
-    double eig3 = fabs(unormaln - anormaln);
-    if (eig1 < 0.1) eig1 = (eig1 * eig1 + 0.1 * 0.1) / 0.2;
-    if (eig2 < 0.1) eig2 = (eig2 * eig2 + 0.1 * 0.1) / 0.2;
-    if (eig3 < 0.1) eig3 = (eig3 * eig3 + 0.1 * 0.1) / 0.2;
-    const double alf = 0.0, ubetac2 = 1.0;
-    double amrev = am;
+    double eig1, eig2, eig3, alf, ubetac2, amrev;
+    if (p->if_precondition) { /* :319-332 */
+        double TTL = am2 / ga / p->ppf;
+        ubetac2 = ora_precond_para(p, vm2, am2, rm, TTL, RmiuT, AlagmR);
+        alf = (1.0 - ubetac2) / 2.0;
+        double unrev = unormaln * (1.0 - alf);
+        amrev = sqrt((alf * unormaln) * (alf * unormaln) + ubetac2 * am2);
+        eig1 = fabs(unormaln);
+        eig2 = fabs(unrev + amrev);
+        eig3 = fabs(unrev - amrev);
+    } else {
+        eig1 = fabs(unormaln);
+        eig2 = fabs(unormaln + anormaln);
+        eig3 = fabs(unormaln - anormaln);
+        if (eig1 < 0.1) eig1 = (eig1 * eig1 + 0.1 * 0.1) / 0.2;
+        if (eig2 < 0.1) eig2 = (eig2 * eig2 + 0.1 * 0.1) / 0.2;
+        if (eig3 < 0.1) eig3 = (eig3 * eig3 + 0.1 * 0.1) / 0.2;
+        alf = 0.0;
+        ubetac2 = 1.0;
+        amrev = am;
+    }
     double dunormaln = (sav1n * (URo - ULo) + sav2n * (VRo - VLo) + sav3n * (WRo - WLo));
     double astar = (eig2 + eig3) / 2.0;
     double Mstar = (eig2 - eig3) / 2.0 / amrev;
@@ -782,12 +868,12 @@ void ora_convect_roe(const trans_params* p, ora_block* b)
                 for (int i = 1; i <= ihi; i++) {
                     double f[5];
                     double sav = GRAD(d, i, j, k);
-                    ora_roe_common(p, VL(d, 4, i, j, k), VR(d, 4, i, j, k), VL(d, 5, i, j, k),
-                                   VR(d, 5, i, j, k), VL(d, 1, i, j, k), VR(d, 1, i, j, k),
-                                   VL(d, 2, i, j, k), VR(d, 2, i, j, k), VL(d, 3, i, j, k),
-                                   VR(d, 3, i, j, k), SD(d, 1, i, j, k), SD(d, 2, i, j, k),
-                                   SD(d, 3, i, j, k), sav, RCCL(d, i, j, k) * p->omega1,
-                                   RCCR(d, i, j, k) * p->omega1, f);
+                    ora_roe_common_pc(p, VL(d, 4, i, j, k), VR(d, 4, i, j, k), VL(d, 5, i, j, k),
+                                      VR(d, 5, i, j, k), VL(d, 1, i, j, k), VR(d, 1, i, j, k),
+                                      VL(d, 2, i, j, k), VR(d, 2, i, j, k), VL(d, 3, i, j, k),
+                                      VR(d, 3, i, j, k), SD(d, 1, i, j, k), SD(d, 2, i, j, k),
+                                      SD(d, 3, i, j, k), sav, RCCL(d, i, j, k) * p->omega1,
+                                      RCCR(d, i, j, k) * p->omega1, VL(d, 8, i, j, k), ALAGM(d, i, j, k), f);
                     for (int L = 1; L <= 5; L++) {
                         D(L, i, j, k) = D(L, i, j, k) - f[L - 1] * sav;
                         D(L, i - di, j - dj, k - dk) = D(L, i - di, j - dj, k - dk) + f[L - 1] * sav;
@@ -796,12 +882,13 @@ void ora_convect_roe(const trans_params* p, ora_block* b)
     }
 }
 
-/* Space/CONVECT_Roe_Rotated.F90:597-752 (its own Roe_Common, IF_PRECONDITION = 0): adds the
- * Roe dissipation through the sub-face (sav1, sav2, sav3), |.| = sav, weighted by Funscheme,
- * to the area-weighted flux acc[5]. */
+/* Space/CONVECT_Roe_Rotated.F90:597-752 (its own Roe_Common; the preconditioned eigenvalues
+ * of :669-680 read RmiuT and AlagmR): adds the Roe dissipation through the sub-face (sav1,
+ * sav2, sav3), |.| = sav, weighted by Funscheme, to the area-weighted flux acc[5]. */
 static void roe_rot_common(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
                            double URo, double VLo, double VRo, double WLo, double WRo, double sav1, double sav2,
-                           double sav3, double sav, double Funscheme, double rLo, double rRo, double acc[5])
+                           double sav3, double sav, double Funscheme, double rLo, double rRo, double RmiuT,
+                           double AlagmR, double acc[5])
 {
     double rcpcv = p->ppf;
     double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
@@ -820,8 +907,24 @@ static void roe_rot_common(const trans_params* p, double PLo, double PRo, double
     double sav1n = sav1 / sav, sav2n = sav2 / sav, sav3n = sav3 / sav;
     double unormaln = (sav1n * um + sav2n * vm + sav3n * wm);
     double anormaln = am;
-    double eig1 = fabs(unormaln), eig2 = fabs(unormaln + anormaln), eig3 = fabs(unormaln - anormaln);
-    double alf = 0.0, ubetac2 = 1.0, amrev = am;
+    double eig1, eig2, eig3, alf, ubetac2, amrev;
+    if (p->if_precondition) {
+        double TTL = am2 / ga / p->ppf;
+        ubetac2 = ora_precond_para(p, vm2, am2, rm, TTL, RmiuT, AlagmR);
+        alf = (1.0 - ubetac2) / 2.0;
+        double unrev = unormaln * (1.0 - alf);
+        amrev = sqrt((alf * unormaln) * (alf * unormaln) + ubetac2 * am2);
+        eig1 = fabs(unormaln);
+        eig2 = fabs(unrev + amrev);
+        eig3 = fabs(unrev - amrev);
+    } else {
+        eig1 = fabs(unormaln);
+        eig2 = fabs(unormaln + anormaln);
+        eig3 = fabs(unormaln - anormaln);
+        alf = 0.0;
+        ubetac2 = 1.0;
+        amrev = am;
+    }
     double dunormaln = (sav1n * (URo - ULo) + sav2n * (VRo - VLo) + sav3n * (WRo - WLo));
     double astar = (eig2 + eig3) / 2.0;
     double Mstar = (eig2 - eig3) / 2.0 / amrev;
@@ -848,6 +951,17 @@ void ora_roe_rotated_face(const trans_params* p, double PLo, double PRo, double 
                           double URo, double VLo, double VRo, double WLo, double WRo, double s1, double s2,
                           double s3, double grad, double Funscheme, double rLo, double rRo, double out[5])
 {
+    ora_roe_rotated_face_pc(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, s1, s2, s3, grad, Funscheme, rLo,
+                            rRo, 0.0, 1.0, out);
+}
+
+/* As ora_roe_rotated_face, with the face eddy viscosity RmiuT = VL(d,8) and AlagmR = Alagm(d)
+ * of the preconditioned branch (Space/CONVECT_Roe_Rotated.F90:112-113). */
+void ora_roe_rotated_face_pc(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
+                             double URo, double VLo, double VRo, double WLo, double WRo, double s1, double s2,
+                             double s3, double grad, double Funscheme, double rLo, double rRo, double RmiuT,
+                             double AlagmR, double out[5])
+{
     double rcpcv = p->ppf;
     double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
     double ga = 1.4, ga1 = 0.4;
@@ -868,7 +982,7 @@ void ora_roe_rotated_face(const trans_params* p, double PLo, double PRo, double 
     double DeltaU = sqrt((URo - ULo) * (URo - ULo) + (VRo - VLo) * (VRo - VLo) + (WRo - WLo) * (WRo - WLo));
     if (DeltaU <= p->tiny) {
         roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, s1, s2, s3, grad, Funscheme, rLo, rRo,
-                       out);
+                       RmiuT, AlagmR, out);
     } else {
         sav1n = (URo - ULo) / DeltaU;
         sav2n = (VRo - VLo) / DeltaU;
@@ -882,10 +996,10 @@ void ora_roe_rotated_face(const trans_params* p, double PLo, double PRo, double 
         double b1 = s1 - a1, b2 = s2 - a2, b3 = s3 - a3;
         sav = fmax(sqrt(a1 * a1 + a2 * a2 + a3 * a3), p->tiny);
         roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, a1, a2, a3, sav, Funscheme, rLo, rRo,
-                       out);
+                       RmiuT, AlagmR, out);
         sav = fmax(sqrt(b1 * b1 + b2 * b2 + b3 * b3), p->tiny);
         roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, b1, b2, b3, sav, Funscheme, rLo, rRo,
-                       out);
+                       RmiuT, AlagmR, out);
     }
 }
 
@@ -903,12 +1017,13 @@ void ora_convect_roe_rotated(const trans_params* p, ora_block* b)
                 for (int i = 1; i <= ihi; i++) {
                     double f[5];
                     double Funscheme = (FSCHEME(i - di, j - dj, k - dk) + FSCHEME(i, j, k)) / 2.;
-                    ora_roe_rotated_face(p, VL(d, 4, i, j, k), VR(d, 4, i, j, k), VL(d, 5, i, j, k),
-                                         VR(d, 5, i, j, k), VL(d, 1, i, j, k), VR(d, 1, i, j, k),
-                                         VL(d, 2, i, j, k), VR(d, 2, i, j, k), VL(d, 3, i, j, k),
-                                         VR(d, 3, i, j, k), SD(d, 1, i, j, k), SD(d, 2, i, j, k),
-                                         SD(d, 3, i, j, k), GRAD(d, i, j, k), Funscheme,
-                                         RCCL(d, i, j, k) * p->omega1, RCCR(d, i, j, k) * p->omega1, f);
+                    ora_roe_rotated_face_pc(p, VL(d, 4, i, j, k), VR(d, 4, i, j, k), VL(d, 5, i, j, k),
+                                            VR(d, 5, i, j, k), VL(d, 1, i, j, k), VR(d, 1, i, j, k),
+                                            VL(d, 2, i, j, k), VR(d, 2, i, j, k), VL(d, 3, i, j, k),
+                                            VR(d, 3, i, j, k), SD(d, 1, i, j, k), SD(d, 2, i, j, k),
+                                            SD(d, 3, i, j, k), GRAD(d, i, j, k), Funscheme,
+                                            RCCL(d, i, j, k) * p->omega1, RCCR(d, i, j, k) * p->omega1,
+                                            VL(d, 8, i, j, k), ALAGM(d, i, j, k), f);
                     for (int L = 1; L <= 5; L++) {
                         D(L, i, j, k) = D(L, i, j, k) - f[L - 1];
                         D(L, i - di, j - dj, k - dk) = D(L, i - di, j - dj, k - dk) + f[L - 1];
@@ -950,7 +1065,7 @@ void ora_lusgs_adq(double pVx, double pVy, double pVz, double aSDx, double aSDy,
 /* Neighbour term of the sweeps (Time/TIME_LUSGS.F90:190-256 and mirrors):
  * 0.5*(A_nb dQ_nb + sgn*Rds_nb dQ_nb), A built from the neighbour's state and the mean of
  * its two faces in direction d. */
-static void lusgs_nb(ora_block* b, int d, int in, int jn, int kn, int sgn, double out[5])
+static void lusgs_nb(const trans_params* p, ora_block* b, int d, int in, int jn, int kn, int sgn, double out[5])
 {
     const int di = d == 1, dj = d == 2, dk = d == 3;
     double ain[5];
@@ -966,11 +1081,12 @@ static void lusgs_nb(ora_block* b, int d, int in, int jn, int kn, int sgn, doubl
     double aSDz = 0.5 * (SD(d, 3, in + di, jn + dj, kn + dk) + SD(d, 3, in, jn, kn));
     double dif[5];
     ora_lusgs_adq(pVx, pVy, pVz, aSDx, aSDy, aSDz, PP(in, jn, kn), roinv, ain, dif);
+    if (p->if_precondition) precond_cell(p, b, in, jn, kn, dif); /* :235-246 and mirrors */
     double RDSt = RDS(d, in, jn, kn);
     for (int L = 0; L < 5; L++) out[L] = 0.5 * (dif[L] + (double)sgn * RDSt * ain[L]);
 }
 
-/* Time/TIME_LUSGS.F90:2-688 (IF_PRECONDITION = 0) */
+/* Time/TIME_LUSGS.F90:2-688 (with the preconditioning transforms when IF_PRECONDITION = 1) */
 void ora_lusgs(const trans_params* p, ora_block* b)
 {
     const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
@@ -982,6 +1098,15 @@ void ora_lusgs(const trans_params* p, ora_block* b)
                     D(L, I, J, K) = D(L, I, J, K) / VOL(I, J, K) -
                                     (3.0 * V(L, I, J, K) - 4.0 * WT0(L, I, J, K) + WT1(L, I, J, K)) *
                                         0.5 / dtm;
+    if (p->if_precondition) /* :113-160 */
+        for (int K = 1; K <= NK1; K++)
+            for (int J = 1; J <= NJ1; J++)
+                for (int I = 1; I <= NI1; I++) {
+                    double a[5];
+                    for (int L = 0; L < 5; L++) a[L] = D(L + 1, I, J, K);
+                    precond_cell(p, b, I, J, K, a);
+                    for (int L = 0; L < 5; L++) D(L + 1, I, J, K) = a[L];
+                }
     for (int K = 1; K <= NK1; K++)
         for (int J = 1; J <= NJ1; J++)
             for (int I = 1; I <= NI1; I++) {
@@ -996,9 +1121,9 @@ void ora_lusgs(const trans_params* p, ora_block* b)
                 double stept = 2.0 * DTM(I, J, K) / (3.0 * DTM(I, J, K) / dtm + 2.0);
                 double cvs = stept / VOL(I, J, K);
                 double dI[5] = {0, 0, 0, 0, 0}, dJ[5] = {0, 0, 0, 0, 0}, dK[5] = {0, 0, 0, 0, 0};
-                if (I != 1) lusgs_nb(b, 1, I - 1, J, K, +1, dI);
-                if (J != 1) lusgs_nb(b, 2, I, J - 1, K, +1, dJ);
-                if (K != 1) lusgs_nb(b, 3, I, J, K - 1, +1, dK);
+                if (I != 1) lusgs_nb(p, b, 1, I - 1, J, K, +1, dI);
+                if (J != 1) lusgs_nb(p, b, 2, I, J - 1, K, +1, dJ);
+                if (K != 1) lusgs_nb(p, b, 3, I, J, K - 1, +1, dK);
                 double f1 = F(1, I, J, K);
                 for (int L = 1; L <= 5; L++)
                     D(L, I, J, K) = (D(L, I, J, K) * stept + (dI[L - 1] + dJ[L - 1] + dK[L - 1]) * cvs) / f1;
@@ -1010,9 +1135,9 @@ void ora_lusgs(const trans_params* p, ora_block* b)
                 double stept = 2.0 * DTM(I, J, K) / (3.0 * DTM(I, J, K) / dtm + 2.0);
                 double cvs = stept / VOL(I, J, K);
                 double dI[5] = {0, 0, 0, 0, 0}, dJ[5] = {0, 0, 0, 0, 0}, dK[5] = {0, 0, 0, 0, 0};
-                if (I != NI1) lusgs_nb(b, 1, I + 1, J, K, -1, dI);
-                if (J != NJ1) lusgs_nb(b, 2, I, J + 1, K, -1, dJ);
-                if (K != NK1) lusgs_nb(b, 3, I, J, K + 1, -1, dK);
+                if (I != NI1) lusgs_nb(p, b, 1, I + 1, J, K, -1, dI);
+                if (J != NJ1) lusgs_nb(p, b, 2, I, J + 1, K, -1, dJ);
+                if (K != NK1) lusgs_nb(p, b, 3, I, J, K + 1, -1, dK);
                 double f1 = F(1, I, J, K);
                 for (int L = 1; L <= 5; L++)
                     D(L, I, J, K) = D(L, I, J, K) - (dI[L - 1] + dJ[L - 1] + dK[L - 1]) * cvs / f1;
@@ -1085,6 +1210,29 @@ void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b)
                 double AinRz = D(4, I, J, K), AinRe = D(5, I, J, K);
                 double pRo = V(1, I, J, K);
                 double roinv = 1.0 / pRo;
+                if (p->if_precondition) { /* Time/TIME_Update_V.F90:522-606 */
+                    double pRx = V(2, I, J, K) + AinRx, pRy = V(3, I, J, K) + AinRy;
+                    double pRz = V(4, I, J, K) + AinRz;
+                    double pRe = fabs(V(5, I, J, K) + AinRe);
+                    pRo = fabs(pRo + AinRo);
+                    /* the kinetic energy divides by the density before the update */
+                    double DV = 0.5 * (pRx * pRx + pRy * pRy + pRz * pRz) * roinv;
+                    double pPP = (pRe - DV) * (1.4 - 1.0);
+                    if (pPP <= 0.0) {
+                        pPP = fabs(pPP);
+                        pRe = pPP / (1.4 - 1.0) + DV;
+                    }
+                    /* T from the pressure before the update */
+                    double pTT = p->rxm2 * PP(I, J, K) / pRo;
+                    V(1, I, J, K) = pRo;
+                    V(2, I, J, K) = pRx;
+                    V(3, I, J, K) = pRy;
+                    V(4, I, J, K) = pRz;
+                    V(5, I, J, K) = pRe;
+                    PP(I, J, K) = pPP;
+                    T(I, J, K) = pTT;
+                    goto turb;
+                }
                 double pVx = V(2, I, J, K) * roinv;
                 double pVy = V(3, I, J, K) * roinv;
                 double pVz = V(4, I, J, K) * roinv;
@@ -1115,6 +1263,7 @@ void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b)
                 V(5, I, J, K) = PP(I, J, K) / (1.4 - 1.0) + DV * pRo -
                                 0.5 * sqr(om * RAD(I, J, K)) * pRo;
 
+            turb:
                 if (!(p->if_euler == 0 && if_turb == 1 && ki >= p->kitm)) continue;
 
                 double AinRTk = D(6, I, J, K), AinRTO = D(7, I, J, K);

@@ -15,8 +15,10 @@
  * brute force, except for these parts, whose parity is unpinned (checked GPU-vs-oracle
  * only): the crossed/reversed orientations of the block-interface copy (ora_bc, lcross = 1
  * or revMatrix != 0), the Cmua / rho*omega floor / rho*k cap branch of ora_update_v, the SST
- * cross-diffusion and blending at F1 != 0, and the length-scale blend coefficients of
- * Fscheme inside its blended range (its limits and monotonicity are pinned).
+ * cross-diffusion and blending at F1 != 0, the length-scale blend coefficients of
+ * Fscheme inside its blended range (its limits and monotonicity are pinned), and the
+ * density and energy rows of ora_precond_dw (PrecondDW; only its structure is pinned:
+ * momentum passes, linear in the increment, affine in (U_r/c)^2).
  */
 #ifndef TRANS_ORACLE_H
 #define TRANS_ORACLE_H
@@ -71,6 +73,10 @@ void ora_convect_roe_rotated(const trans_params* p, ora_block* b);
 void ora_roe_rotated_face(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
                           double URo, double VLo, double VRo, double WLo, double WRo, double s1, double s2,
                           double s3, double grad, double Funscheme, double rLo, double rRo, double out[5]);
+void ora_roe_rotated_face_pc(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
+                             double URo, double VLo, double VRo, double WLo, double WRo, double s1, double s2,
+                             double s3, double grad, double Funscheme, double rLo, double rRo, double RmiuT,
+                             double AlagmR, double out[5]);
 void ora_lusgs(const trans_params* p, ora_block* b);
 void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b);
 void ora_update_u(const trans_params* p, ora_block* b);
@@ -112,6 +118,13 @@ void ora_roe_common(const trans_params* p, double PLo, double PRo, double TLo, d
                     double ULo, double URo, double VLo, double VRo, double WLo, double WRo,
                     double sav1, double sav2, double sav3, double sav, double rLo, double rRo,
                     double out[5]);
+void ora_roe_common_pc(const trans_params* p, double PLo, double PRo, double TLo, double TRo, double ULo,
+                       double URo, double VLo, double VRo, double WLo, double WRo, double sav1, double sav2,
+                       double sav3, double sav, double rLo, double rRo, double RmiuT, double AlagmR, double out[5]);
+double ora_precond_para(const trans_params* p, double VM2, double AM2, double RR, double TT, double RMIUT,
+                        double ALAGMR);
+void ora_precond_dw(const trans_params* p, double a[5], double pRo, double pVx, double pVy, double pVz, double pE,
+                    double AAc, double ubetac2, double pPP, double pTT);
 void ora_lusgs_adq(double pVx, double pVy, double pVz, double aSDx, double aSDy,
                    double aSDz, double pPP, double roinv, const double ain[5], double dif[5]);
 void ora_ijk_start_end(int i, int j, int k, const int tl[3], const int tt[3], int lcross,

@@ -69,14 +69,15 @@ def test_footprint_without_a_device():
 def test_deck_checks_without_a_device():
     """The library's deck checks run in the sizing call: RCCorrection mode 2 (which only
     writes f_r2, read by no routine) is accepted, mode 1 (unset index) is rejected, and
-    IDDES / Suzen are accepted for the SST model."""
+    IDDES / Suzen and low-Mach preconditioning are accepted for the SST model."""
     from trans_amd import cases, serving, solver
     from trans_amd.solver import TransError
     import pytest
     blocks, org, glob = cases.bump_channel(33, 17, 5)
     org.IF_Euler, org.IF_Turb_Org = 0, 1
     prm = solver.prepare_case(blocks, org, glob)
-    for field, value, ok in (("kind_rotmod", 2, True), ("kind_rotmod", 1, False), ("if_precondition", 1, False),
+    for field, value, ok in (("kind_rotmod", 2, True), ("kind_rotmod", 1, False), ("if_precondition", 1, True),
+                             ("if_precondition", 2, False),
                              ("kind_subhy", 5, True), ("kind_subhy", 4, False), ("if_comppress", 1, True)):
         q = capi.TransParams.from_buffer_copy(prm)
         q.kind_hybrid = 3

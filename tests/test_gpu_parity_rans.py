@@ -317,3 +317,84 @@ def test_suzen_compressibility(hybrid, sub):
         for key in ("ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo"):
             assert gres[n][key] == ores[n][key], (n, key)
     sv.close()
+
+
+@pytest.mark.parametrize("spatial,roe,rk,fp32,nit", [(2, 1, 0, 0, 1), (2, 2, 0, 0, 1), (3, 3, 0, 0, 4),
+                                                     (2, 1, 1, 0, 0), (2, 1, 0, 1, 0)])
+def test_low_mach_preconditioning(spatial, roe, rk, fp32, nit):
+    """IF_PRECONDITION = 1 at M = 0.1, CFL 0.3: the preconditioned spectral radius and Roe
+    eigenvalues (original and rotated), the transforms of the LU-SGS right-hand side and
+    neighbour terms (hyperplane sweep) and the conservative update.  Residual, Rds and Dtm, and
+    one LU-SGS update from identical inputs at 1e-12 (FP32 sweeps: the increment within 1e-5);
+    then `nit` whole iterations from the start within 1e-8.  The reference's preconditioned
+    LU-SGS diverges on this case after 3 (Roe) or more (AUSMPW+) iterations -- its PrecondDW
+    is not the identity at (U_r/c)^2 = 1 -- and explicit RK with the preconditioned time step
+    diverges within its first iteration, so the RK case stops at the residual.  On the way
+    the Roe cases amplify rounding ~1e4-1e6 per iteration (the oracle built with FMA
+    contraction differs from itself by 1e-7 after 2 iterations, and the rotated flux's
+    velocity-jump switch flips), so they are iterated once; AUSMPW+ stays within 1e-11
+    over 4 iterations."""
+    print(f"case: flat plate M=0.1, IF_Precondition=1, Kind_Spatial={spatial}, Kind_Roe={roe}, IF_RK={rk}, "
+          f"fp32={fp32}", flush=True)
+
+    def case():
+        blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25, mach=0.1)
+        org.IF_Precondition, org.Kind_Spatial, org.Kind_Roe = 1, spatial, roe
+        org.CFL_Sub_start = org.CFL_Sub_end = 0.3
+        if rk:
+            org.IF_RK, org.CFL_Sub_start, org.CFL_Sub_end = 1, 0.8, 0.8
+        prm = solver.prepare_case(blocks, org, glob)
+        perturb(blocks, prm, seed=20261202, turb=True)
+        p_dev = capi.TransParams.from_buffer_copy(prm)
+        p_dev.implicit_fp32 = fp32
+        return blocks, prm, p_dev
+
+    blocks, prm, p_dev = case()
+    oc = OracleCase(blocks, prm)
+    v0 = [b["v"].copy() for b in blocks]
+    sv = Solver(blocks, p_dev, ki=1)
+    oc.residual(1)
+    sv.residual()
+    for n, b in enumerate(blocks):
+        for nm, fld in (("D", capi.FIELD_D), ("Rds", capi.FIELD_RDS), ("Dtm", capi.FIELD_DTM)):
+            e = np.asarray(relerr(interior(sv.get_field(n, fld), b), interior(oc[n][nm], b)))
+            print(n, nm, e)
+            # D(6:7) (not preconditioned) in the first cells off the inlet is a small difference
+            # of large k-omega fluxes through ghosts set with pow/exp: last-bit differences there
+            # reach ~1e-11 of max|D| at M = 0.1
+            bar = np.array([TOL] * 5 + [1e-9] * 2) if nm == "D" else TOL
+            assert np.all(e < bar), (n, nm, e)
+    if rk:
+        sv.close()
+        return
+    ores, gres = oc.lusgs_update(1), sv.lusgs_update()
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        if fp32:
+            dv_o = interior(oc[n]["v"], b) - interior(v0[n], b)
+            e = relerr(interior(st["v"], b) - interior(v0[n], b), dv_o)
+            print(n, "FP32 increment relerr", e)
+            assert np.all(e < 1e-5), (n, e)
+            continue
+        # the k-omega part inherits the inlet-cell D(6:7) differences (bar 1e-9, as above)
+        for nm, bar in (("v", np.array([TOL] * 5 + [1e-9] * 2)), ("pp", TOL), ("t", TOL), ("rmiu", 1e-9)):
+            e = np.asarray(relerr(interior(st[nm], b), interior(oc[n][nm], b)))
+            print(n, nm, e)
+            assert np.all(e < bar), (n, nm, e)
+        for key, bar in (("drsa", TOL), ("dua", TOL), ("drsk", 1e-9), ("duk", 1e-9), ("drso", 1e-9), ("duo", 1e-9)):
+            assert abs(gres[n][key] - ores[n][key]) <= bar * abs(ores[n][key]), (n, key)
+    sv.close()
+    if not nit:
+        return
+    blocks, prm, p_dev = case()
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, p_dev, ki=1)
+    oc.iterate(1, nit)
+    gres = sv.iterate(nit)
+    ores = oc.resid()
+    for n, b in enumerate(blocks):
+        e = relerr(interior(sv.get_state(n)["v"], b), interior(oc[n]["v"], b))
+        print(n, f"state relerr after {nit} iterations", e)
+        assert np.all(e < 1e-8), (n, e)
+        assert abs(gres[n]["drsa"] - ores[n]["drsa"]) <= 1e-8 * ores[n]["drsa"], n
+    sv.close()

@@ -375,14 +375,38 @@ def test_interface_halo_matches_global_grid():
 # ---------------------------------------------------------------------------------------
 # LU-SGS (Time/TIME_LUSGS.F90) against a dense solve of the same factorisation
 
-def test_lusgs_equals_dense_factorised_solve():
-    b, prm, edges = _box(ni=5, nj=4, nk=4, ratio=1.3)
+def _precond_matrix(prm, b, c):
+    """5x5 matrix of the preconditioning transform with the state of cell c, column by column
+    through the oracle's PrecondDW and getPrecondPara."""
+    V = b["v"]
+    rho = V[0][c]
+    vel = [V[q][c] / rho for q in (1, 2, 3)]
+    e = V[4][c] / rho
+    pp, tt = b["pp"][c], b["t"][c]
+    aac = np.sqrt(1.4 * pp / rho)
+    alls = min(b["alagm"][0][c], b["alagm"][1][c], b["alagm"][2][c])
+    ub = lib().ora_precond_para(C.byref(prm), sum(x * x for x in vel), aac * aac, rho, tt, b["rmiu"][c], alls)
+    M = np.zeros((5, 5))
+    for q in range(5):
+        a = (C.c_double * 5)(*np.eye(5)[q])
+        lib().ora_precond_dw(C.byref(prm), a, rho, *vel, e, aac, ub, pp, tt)
+        M[:, q] = a[:]
+    return M
+
+
+@pytest.mark.parametrize("precond", [0, 1])
+def test_lusgs_equals_dense_factorised_solve(precond):
+    """With preconditioning (M = 0.1, XM = 0.1, so (U_r/c)^2 = 3 XM^2 < 1) the right-hand side
+    passes through the cell's own transform P and each neighbour's flux-Jacobian term through
+    the neighbour's P, while the Rds term is untransformed (Time/TIME_LUSGS.F90:113-160,
+    :235-246): (F1 I - cvs L_P) x* = stept P b."""
+    b, prm, edges = _box(ni=5, nj=4, nk=4, ratio=1.3, **(dict(IF_Precondition=1, XM=0.1) if precond else {}))
     rng = np.random.Generator(np.random.MT19937(11))
     shp = b["pp"].shape
     V = b["v"]
     rho = 1.0 + 0.05 * rng.uniform(-1, 1, shp)
     u, v, w = 1.0 + 0.1 * rng.uniform(-1, 1, shp), 0.1 * rng.uniform(-1, 1, shp), 0.1 * rng.uniform(-1, 1, shp)
-    p = (1 / (1.4 * 0.25)) * (1 + 0.05 * rng.uniform(-1, 1, shp))
+    p = (1 / (1.4 * (0.01 if precond else 0.25))) * (1 + 0.05 * rng.uniform(-1, 1, shp))
     V[0], V[1], V[2], V[3] = rho, rho * u, rho * v, rho * w
     V[4] = p / 0.4 + 0.5 * rho * (u * u + v * v + w * w)
     b["pp"][...] = p
@@ -430,12 +454,15 @@ def test_lusgs_equals_dense_factorised_solve():
                         vel = [V[q][c] / rho_n for q in (1, 2, 3)]
                         sm = [0.5 * (SD[d + 3 * x][O + ii + di, O + jj + dj, O + kk + dk] + SD[d + 3 * x][c])
                               for x in range(3)]
-                        blk = np.zeros((5, 5))
+                        adq = np.zeros((5, 5))
                         for q in range(5):
                             e = (C.c_double * 5)(*np.eye(5)[q])
                             o = (C.c_double * 5)()
                             lib().ora_lusgs_adq(*vel, *sm, b["pp"][c], 1.0 / rho_n, e, o)
-                            blk[:, q] = 0.5 * (np.array(o[:]) - sgn * Rds[d][c] * np.eye(5)[q])
+                            adq[:, q] = o[:]
+                        if precond:
+                            adq = _precond_matrix(prm, b, c) @ adq
+                        blk = 0.5 * (adq - sgn * Rds[d][c] * np.eye(5))
                         rn = idx(ii, jj, kk)
                         if sgn < 0:   # lower neighbour: -cvs * 0.5 (A + Rds)
                             M[5 * r:5 * r + 5, 5 * rn:5 * rn + 5] -= cvs * blk
@@ -446,9 +473,12 @@ def test_lusgs_equals_dense_factorised_solve():
         for j in range(1, n2 + 1):
             for i in range(1, n1 + 1):
                 r = idx(i, j, k)
-                bvec[5 * r:5 * r + 5] = st[r] * (rhs[:, i - 1, j - 1, k - 1] - (3 * V[:5, O + i, O + j, O + k] -
-                                                 4 * b["wt0"][:5, O + i, O + j, O + k] +
-                                                 b["wt1"][:5, O + i, O + j, O + k]) * 0.5 / dtm)
+                rr = rhs[:, i - 1, j - 1, k - 1] - (3 * V[:5, O + i, O + j, O + k] -
+                                                    4 * b["wt0"][:5, O + i, O + j, O + k] +
+                                                    b["wt1"][:5, O + i, O + j, O + k]) * 0.5 / dtm
+                if precond:
+                    rr = _precond_matrix(prm, b, (O + i, O + j, O + k)) @ rr
+                bvec[5 * r:5 * r + 5] = st[r] * rr
     x = np.linalg.solve(B, np.linalg.solve(A, bvec))
     want = np.zeros_like(got)
     for k in range(1, n3 + 1):
@@ -1466,3 +1496,188 @@ def test_restart_pressure_formula():
     lib().ora_pp_t_from_v(C.byref(prm), C.byref(oc.ob[0]))
     np.testing.assert_allclose(A["pp"][I, J, K], p0[I, J, K], rtol=1e-13)
     np.testing.assert_allclose(A["t"][I, J, K], (prm.rxm2 * p0 / A["v"][0])[I, J, K], rtol=1e-13)
+
+
+# ---------------------------------------------------------------------------------------
+# Low-Mach preconditioning (IF_PRECONDITION = 1): getPrecondPara
+# (Common/UTIL_Precond_Funcs.F90:6-30), the preconditioned Roe eigenvalues
+# (Space/CONVECT_Roe_Original.F90:319-332, Space/CONVECT_Roe_Rotated.F90:669-680), the
+# preconditioned spectral radius (Time/SpectralRadius.F90:91-120), the conservative update
+# (Time/TIME_Update_V.F90:522-606).  PrecondDW itself is pinned only structurally (momentum
+# passes, linear, affine in (U_r/c)^2); its placement in LU-SGS is pinned by the dense solve.
+
+def _pc_prm(ref=None, **kw):
+    prm = _prm(**{**dict(XM=0.1, IF_Precondition=1), **kw})
+    if ref is not None:
+        prm.ref = ref  # Reynolds number per unit length, set directly
+    return prm
+
+
+def test_precond_para_regimes():
+    """min(max(M^2, 3 XM^2), 1) at negligible viscosity, then the viscous floor
+    (mu_all / rho / Re / Alagm)^2 / c^2 (Sutherland mu at T plus mu_t)."""
+    L = lib()
+    prm = _pc_prm(ref=1e12)
+    am2 = 4.0
+    for m2, want in ((0.25, 0.25), (1e-4, 3 * 0.01), (2.0, 1.0), (0.03, 0.03)):
+        got = L.ora_precond_para(C.byref(prm), m2 * am2, am2, 1.0, 1.0, 0.0, 0.1)
+        assert got == pytest.approx(want, rel=1e-12)
+    prm = _pc_prm(ref=10.0)
+    T, mut, rho, alagm = 1.2, 0.3, 0.9, 0.01
+    mu = (1 + prm.csthlnd) / (T * prm.csth1 + prm.csthlnd) * (prm.csth1 * T) ** 1.5
+    want = ((mu + mut) / rho / prm.ref / alagm) ** 2 / am2
+    assert want > 1.0  # the floor wins here
+    got = L.ora_precond_para(C.byref(prm), 1e-4 * am2, am2, rho, T, mut, alagm)
+    assert got == pytest.approx(want, rel=1e-12)
+
+
+def test_precond_dw_structure():
+    """Momentum passes unchanged, the map is linear in the increment and affine in ub."""
+    L = lib()
+    prm = _pc_prm()
+    rng = np.random.Generator(np.random.MT19937(41))
+    st = (1.1, 0.3, 0.1, -0.05)
+    p = 1 / (1.4 * 0.01)
+    T = p / (prm.ppf * st[0])
+    E = p / 0.4 / st[0] + 0.5 * sum(x * x for x in st[1:])
+    c = np.sqrt(1.4 * p / st[0])
+
+    def f(a, ub):
+        x = (C.c_double * 5)(*a)
+        L.ora_precond_dw(C.byref(prm), x, *st, E, c, ub, p, T)
+        return np.array(x[:])
+
+    a, b = rng.standard_normal(5), rng.standard_normal(5)
+    for ub in (0.03, 0.4, 1.0):
+        ya = f(a, ub)
+        np.testing.assert_array_equal(ya[1:4], a[1:4])
+        np.testing.assert_allclose(f(2.0 * a - 3.0 * b, ub), 2.0 * ya - 3.0 * f(b, ub), rtol=1e-12, atol=1e-12)
+    y0, y1, y2 = f(a, 0.1), f(a, 0.3), f(a, 0.5)
+    np.testing.assert_allclose(y0 - 2 * y1 + y2, 0.0, atol=1e-11 * np.abs(y1).max())
+
+
+def _roe_pc(prm, L_, R_, S, mut=0.0, alagm=1.0):
+    TL, TR = L_[4] / (prm.ppf * L_[0]), R_[4] / (prm.ppf * R_[0])
+    out = (C.c_double * 5)()
+    sav = float(np.sqrt((S ** 2).sum()))
+    lib().ora_roe_common_pc(C.byref(prm), L_[4], R_[4], TL, TR, L_[1], R_[1], L_[2], R_[2], L_[3], R_[3],
+                            *S, sav, 0.0, 0.0, mut, alagm, out)
+    return np.array(out[:])
+
+
+@pytest.mark.parametrize("st", STATES)
+@pytest.mark.parametrize("S", FACES)
+def test_precond_roe_consistency(st, S):
+    """Equal states: the physical flux, whatever the preconditioning."""
+    np.testing.assert_allclose(_roe_pc(_pc_prm(), st, st, S), -_euler_flux(*st, S), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("S", FACES)
+def test_precond_roe_supersonic_is_plain_roe(S):
+    """At a Roe-mean Mach number above 1 (and negligible viscosity) (U_r/c)^2 = 1: the
+    preconditioned flux is the plain Roe flux, whose entropy fix is idle here (all
+    eigenvalues above 0.1)."""
+    n = S / np.sqrt((S ** 2).sum())
+    p = 1 / (1.4 * 0.25)
+    L_ = (1.0, *(2.4 * n + np.array([0.1, -0.2, 0.05])), p)
+    R_ = (0.95, *(2.3 * n + np.array([0.12, -0.18, 0.05])), 1.05 * p)
+    pc = _roe_pc(_pc_prm(ref=1e12), L_, R_, S)
+    plain = _roe_orig(_prm(XM=0.1), L_, R_, S)
+    np.testing.assert_allclose(pc, plain, rtol=1e-13, atol=1e-14)
+
+
+def test_precond_roe_low_mach_dissipation_scaling():
+    """Low Mach (M = 0.05): the preconditioned acoustic speeds are O(u), so the dissipation of a
+    normal-velocity jump in the normal momentum is O(rho u du) instead of the plain Roe's
+    O(rho c du), and that of a pressure jump in the mass flux O(dp / u) instead of O(dp / c)."""
+    S = np.array([0.3, 0.0, 0.0])
+    p = 1 / (1.4 * 0.0025)  # c = 20, u = 1
+    prm_pc, prm = _pc_prm(ref=1e12, XM=0.05), _prm(XM=0.05)
+
+    def diss(f, L_, R_):
+        cen = -0.5 * (_euler_flux(*L_, S) + _euler_flux(*R_, S))
+        return f(L_, R_) - cen
+
+    pc = lambda L_, R_: _roe_pc(prm_pc, L_, R_, S)
+    plain = lambda L_, R_: _roe_orig(prm, L_, R_, S)
+    L_, R_ = (1.0, 1.0, 0.0, 0.0, p), (1.0, 1.0 - 1e-3, 0.0, 0.0, p)
+    assert abs(diss(pc, L_, R_)[1]) < 0.2 * abs(diss(plain, L_, R_)[1])
+    L_, R_ = (1.0, 1.0, 0.0, 0.0, p), (1.0, 1.0, 0.0, 0.0, p * (1 + 1e-4))
+    assert abs(diss(pc, L_, R_)[0]) > 5.0 * abs(diss(plain, L_, R_)[0])
+
+
+@pytest.mark.parametrize("S", FACES)
+def test_precond_rotated_roe_normal_jump_is_precond_roe(S):
+    """The rotated flux hands RmiuT and AlagmR to each sub-face: a normal velocity jump with
+    Funscheme = 1 gives the preconditioned original Roe flux times the area."""
+    prm = _pc_prm(ref=50.0)
+    n = S / np.sqrt((S ** 2).sum())
+    L_ = (1.0, 0.5, 0.2, -0.1, 1 / (1.4 * 0.01))
+    uR = np.array(L_[1:4]) - 0.07 * n
+    R_ = (0.98, *uR, 1.01 * L_[4])
+    sav = float(np.sqrt((S ** 2).sum()))
+    TL, TR = L_[4] / (prm.ppf * L_[0]), R_[4] / (prm.ppf * R_[0])
+    out = (C.c_double * 5)()
+    lib().ora_roe_rotated_face_pc(C.byref(prm), L_[4], R_[4], TL, TR, L_[1], R_[1], L_[2], R_[2], L_[3], R_[3],
+                                  *S, sav, 1.0, 0.0, 0.0, 0.2, 0.003, out)
+    np.testing.assert_allclose(np.array(out[:]), _roe_pc(prm, L_, R_, S, 0.2, 0.003) * sav,
+                               rtol=1e-12, atol=1e-13 * sav)
+
+
+def test_precond_spectral_radius_closed_form():
+    """Rds_d = |u.S_d| (1 - alf_d) + |S_d| sqrt(alf_d^2 |u|^2 + ub_d c^2) + viscous term, ub =
+    min(max(M^2, 3 XM^2), 1) raised to ((mu + mu_t)/rho/Re/Alagm_d/c)^2, alf_d = (1 - ub_d)/2."""
+    b, prm, edges = _box(IF_Euler=0, CFL_Sub_start=3.0, CFL_Sub_end=3.0, XM=0.1, IF_Precondition=1)
+    rng = np.random.Generator(np.random.MT19937(3))
+    V = b["v"]
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    V[0] = 1.0 + 0.1 * rng.uniform(-1, 1, V[0].shape)
+    V[1] = V[0] * 0.2 * rng.uniform(-1, 1, V[0].shape)
+    V[2] = V[0] * 0.05 * rng.uniform(-1, 1, V[0].shape)
+    V[3] = 0.0
+    b["pp"][...] = 1 / (1.4 * 0.01)
+    b["t"][...] = prm.rxm2 * b["pp"] / V[0]
+    b["rmiu"][...] = 1e-3 * rng.uniform(0, 1, V[0].shape)
+    oc = OracleCase([b], prm)
+    oc.spectral_radius(1)
+    x1, y1, z1 = edges
+    dx = np.diff(x1)[:, None, None]
+    dy = np.diff(y1)[None, :, None]
+    dz = np.diff(z1)[None, None, :]
+    rho = V[0][I, J, K]
+    u, v = V[1][I, J, K] / rho, V[2][I, J, K] / rho
+    c2 = 1.4 * b["pp"][I, J, K] / rho
+    q2 = u * u + v * v
+    ub0 = np.minimum(np.maximum(q2 / c2, 3 * prm.xm ** 2), 1.0)
+    tt = b["t"][I, J, K]
+    mu = (1 + prm.csthlnd) / (tt * prm.csth1 + prm.csthlnd) * (prm.csth1 * tt) ** 1.5
+    mt = b["rmiu"][I, J, K]
+    nu = np.maximum(4.0 / 3.0 * (mu + mt), 1.4 * (mu / 0.72 + mt / 0.9)) / rho
+    vol = dx * dy * dz
+    for d, (S, un) in enumerate(((dy * dz, u), (dx * dz, v), (dx * dy, 0.0 * u))):
+        S = np.broadcast_to(S, rho.shape)
+        ubd = np.maximum(ub0, ((mu + mt) / rho / prm.ref / b["alagm"][d][I, J, K]) ** 2 / c2)
+        alf = (1 - ubd) / 2
+        want = np.abs(un * S) * (1 - alf) + S * np.sqrt(alf * alf * q2 + ubd * c2) + 2 * S * S / vol * nu / prm.ref
+        np.testing.assert_allclose(oc[0]["Rds"][d][I, J, K], want, rtol=1e-12)
+
+
+def test_precond_update_v_is_conservative():
+    """The increment is added to the conservative variables (|rho|, |rho E|); p follows from the
+    new energy with the kinetic energy over the density before the update, and T from the
+    pressure before the update (Time/TIME_Update_V.F90:566-606)."""
+    b, prm, edges = _box(IF_Precondition=1, XM=0.1)
+    rng = np.random.Generator(np.random.MT19937(5))
+    oc = OracleCase([b], prm)
+    D = oc[0]["D"]
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    v0 = oc[0]["v"].copy()
+    p0 = oc[0]["pp"].copy()
+    D[:5, I, J, K] = 1e-3 * rng.uniform(-1, 1, D[:5, I, J, K].shape)
+    d = D.copy()
+    oc.call("ora_update_v", 0, 1, 0)
+    V, P, T = oc[0]["v"], oc[0]["pp"], oc[0]["t"]
+    np.testing.assert_allclose(V[:5][:, I, J, K], (v0[:5] + d[:5])[:, I, J, K], rtol=1e-15, atol=1e-15)
+    ke_old_rho = 0.5 * ((v0[1] + d[1]) ** 2 + (v0[2] + d[2]) ** 2 + (v0[3] + d[3]) ** 2)[I, J, K] / v0[0][I, J, K]
+    np.testing.assert_allclose(P[I, J, K], 0.4 * ((v0[4] + d[4])[I, J, K] - ke_old_rho), rtol=1e-13)
+    np.testing.assert_allclose(T[I, J, K], prm.rxm2 * p0[I, J, K] / V[0][I, J, K], rtol=1e-15)

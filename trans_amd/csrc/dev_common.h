@@ -135,3 +135,53 @@ __device__ __forceinline__ double sutherland(const trans_params& p, double tt)
 }
 
 __device__ __forceinline__ double sqr(double x) { return x * x; }
+
+// Common/UTIL_Precond_Funcs.F90:6-30 (getPrecondPara): (U_r/c)^2 of the low-Mach
+// preconditioning, min(max(M^2, 3 M_inf^2), 1), raised to the viscous limit
+// ((mu + mu_t) / rho / Re / Alagm)^2 / c^2.
+__device__ __forceinline__ double precond_para(const trans_params& p, double VM2, double AM2, double RR, double TT,
+                                               double RMIUT, double ALAGMR)
+{
+    const double ub = fmin(fmax(VM2 / AM2, 3.0 * p.xm * p.xm), 1.0);
+    const double RmiuAll = sutherland(p, TT) + RMIUT;
+    const double r = RmiuAll / RR / p.ref / ALAGMR;
+    return fmax(ub, 1.0 * (r * r) / AM2);
+}
+
+// Common/UTIL_Precond_Funcs.F90:86-130 (PrecondDW): a conservative increment a[5] through
+// the preconditioning transform; the momentum components pass unchanged.
+template <typename R>
+__device__ __forceinline__ void precond_dw(const trans_params& p, R a[5], R pRo, R pVx, R pVy, R pVz, R pE, R AAc,
+                                           R ubetac2, R pTT)
+{
+    const R pU2 = pVx * pVx + pVy * pVy + pVz * pVz;
+    const R pUAU = pVx * a[1] + pVy * a[2] + pVz * a[3];
+    a[4] = a[4] - (pUAU) - (pE / R(1.4) - R(1.2) * (pU2)) * a[0];
+    const R rCp = R(1.4 / 0.4) * R(p.ppf) * pRo;
+    const R a11 = AAc * AAc * ubetac2;
+    const R a15 = R(-0.4) * ubetac2 / rCp;
+    const R b11 = R(1) * a11 / rCp;
+    const R b15 = (R(1) + R(0.4) * ubetac2) / rCp;
+    const R AinP = a11 * a[0] + a15 * a[4];
+    const R AinT = b11 * a[0] + b15 * a[4];
+    a[0] = AinP / R(p.ppf) / pTT - pRo * AinT / pTT;
+    a[4] = AinP / R(0.4) + (pUAU);
+}
+
+// The transform with the state of cell c (Time/TIME_LUSGS.F90:113-160 for the right-hand
+// side, :235-246 and the mirrored blocks for the neighbour terms): own velocity, energy,
+// p, T, eddy viscosity and the smallest of the cell's three Alagm.
+template <typename R>
+__device__ __forceinline__ void precond_cell(const trans_params& p, const Fields& f, long c, R a[5])
+{
+    const long tot = f.tot;
+    const double pRo = f.V[c], roinv = 1.0 / pRo;
+    const double pVx = f.V[tot + c] * roinv, pVy = f.V[2 * tot + c] * roinv, pVz = f.V[3 * tot + c] * roinv;
+    const double pE = f.V[4 * tot + c] * roinv;
+    const double pPP = f.PP[c], pTT = f.T[c];
+    const double AAc = sqrt(1.4 * pPP * roinv);
+    const double UVW2 = pVx * pVx + pVy * pVy + pVz * pVz;
+    const double ALL = fmin(fmin(f.Alagm[c], f.Alagm[tot + c]), f.Alagm[2 * tot + c]);
+    const double ub = precond_para(p, UVW2, AAc * AAc, pRo, pTT, f.Rmiu[c], ALL);
+    precond_dw<R>(p, a, R(pRo), R(pVx), R(pVy), R(pVz), R(pE), R(AAc), R(ub), R(pTT));
+}

@@ -5,8 +5,10 @@
 
 // ---------------------------------------------------------------------------------------
 // Time/SpectralRadius.F90:47-180 -- one thread per interior cell.
-// Rds(3) and the local pseudo time step Dtm; preconditioning off, grid velocity zero,
-// IF_Local_Main = 0 and IF_Local_Sub = 1 (Input/ReadOrg.F90:155-156).
+// Rds(3) and the local pseudo time step Dtm; grid velocity zero, IF_Local_Main = 0 and
+// IF_Local_Sub = 1 (Input/ReadOrg.F90:155-156).  With preconditioning (:91-120) the acoustic
+// part of each direction uses the preconditioned speed sqrt(alf^2 |u|^2 + ub c^2) and the
+// convective part |u.S| (1 - alf), ub raised to that direction's viscous limit.
 __global__ void __launch_bounds__(256) k_spectral_radius(trans_params p, int ki, Fields f,
                                                          const BlkInfo* __restrict__ blks,
                                                          const Seg* __restrict__ seg)
@@ -34,6 +36,19 @@ __global__ void __launch_bounds__(256) k_spectral_radius(trans_params p, int ki,
     const double vol = f.Vol[c];
     double rds[3];
     const long step[3] = {sx, sy, sz};
+    double alf[3] = {0.0, 0.0, 0.0}, AAcd[3] = {AAc, AAc, AAc};
+    if (p.if_precondition) {
+        const double UVW2 = Vxi * Vxi + Vyi * Vyi + Vzi * Vzi;
+        const double ub = precond_para(p, UVW2, AAc * AAc, Den, 0.0, 0.0, 1.0);
+        const double RmiuAll = RmiuL + RmiuTL;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double r = RmiuAll / Den / p.ref / f.Alagm[(long)d * tot + c] / AAc;
+            const double ubd = fmax(ub, r * r);
+            alf[d] = (1.0 - ubd) / 2.0;
+            AAcd[d] = sqrt(alf[d] * alf[d] * UVW2 + ubd * AAc * AAc);
+        }
+    }
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         const double* S = f.SD + (long)d * tot;  // component (d, x) at S[3*x*tot]
@@ -42,7 +57,7 @@ __global__ void __launch_bounds__(256) k_spectral_radius(trans_params p, int ki,
         double az = 0.5 * (S[6 * tot + c] + S[6 * tot + c + step[d]]);
         double st = ax * ax + ay * ay + az * az;
         double un = Vxi * ax + Vyi * ay + Vzi * az + 0.0;
-        rds[d] = fabs(un) * (1.0 - 0.0) + sqrt(st) * AAc + 2.0 * st / vol * pruvmu / p.ref;
+        rds[d] = fabs(un) * (1.0 - alf[d]) + sqrt(st) * AAcd[d] + 2.0 * st / vol * pruvmu / p.ref;
         f.Rds[(long)d * tot + c] = rds[d];
     }
     if (f.Rdsf) {
@@ -76,14 +91,33 @@ __device__ __forceinline__ double muscl2(double var1, double var2)
     return ((1. + smooth) * var1 + (1. - smooth) * var2) * smooth / 2.;
 }
 
-// Space/CONVECT_Roe_Rotated.F90:597-752 (that file's own Roe_Common, IF_PRECONDITION = 0):
-// adds the Roe dissipation through the sub-face (sav1, sav2, sav3), |.| = sav, weighted by
-// Funscheme, to the area-weighted flux acc.
+// Preconditioned Roe eigenvalues (Space/CONVECT_Roe_Original.F90:319-332 and
+// Space/CONVECT_Roe_Rotated.F90:669-680): ub from the Roe-mean state at the Roe-mean
+// temperature with the face eddy viscosity RmiuT and AlagmR, alf = (1 - ub)/2, the acoustic
+// pair |un (1 - alf) +/- sqrt((alf un)^2 + ub c^2)|, no entropy fix.
+__device__ __forceinline__ void roe_precond_eigs(const trans_params& p, double vm2, double am2, double rm, double un,
+                                                 double RmiuT, double AlagmR, double& eig1, double& eig2,
+                                                 double& eig3, double& alf, double& ubetac2, double& amrev)
+{
+    const double TTL = am2 / 1.4 / p.ppf;
+    ubetac2 = precond_para(p, vm2, am2, rm, TTL, RmiuT, AlagmR);
+    alf = (1.0 - ubetac2) / 2.0;
+    const double unrev = un * (1.0 - alf);
+    amrev = sqrt((alf * un) * (alf * un) + ubetac2 * am2);
+    eig1 = fabs(un);
+    eig2 = fabs(unrev + amrev);
+    eig3 = fabs(unrev - amrev);
+}
+
+// Space/CONVECT_Roe_Rotated.F90:597-752 (that file's own Roe_Common): adds the Roe
+// dissipation through the sub-face (sav1, sav2, sav3), |.| = sav, weighted by Funscheme, to
+// the area-weighted flux acc.  RmiuT, AlagmR: face eddy viscosity and Alagm of the
+// preconditioned eigenvalues.
 __device__ __forceinline__ void roe_rot_common(const trans_params& p, double PLo, double PRo, double TLo,
                                                double TRo, double ULo, double URo, double VLo, double VRo,
                                                double WLo, double WRo, double sav1, double sav2, double sav3,
                                                double sav, double Funscheme, double rLo, double rRo,
-                                               double acc[5])
+                                               double RmiuT, double AlagmR, double acc[5])
 {
     const double rcpcv = p.ppf;
     const double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
@@ -102,8 +136,10 @@ __device__ __forceinline__ void roe_rot_common(const trans_params& p, double PLo
     const double sav1n = sav1 / sav, sav2n = sav2 / sav, sav3n = sav3 / sav;
     const double unormaln = (sav1n * um + sav2n * vm + sav3n * wm);
     const double anormaln = am;
-    const double eig1 = fabs(unormaln), eig2 = fabs(unormaln + anormaln), eig3 = fabs(unormaln - anormaln);
-    const double alf = 0.0, ubetac2 = 1.0, amrev = am;
+    double eig1 = fabs(unormaln), eig2 = fabs(unormaln + anormaln), eig3 = fabs(unormaln - anormaln);
+    double alf = 0.0, ubetac2 = 1.0, amrev = am;
+    if (p.if_precondition)
+        roe_precond_eigs(p, vm2, am2, rm, unormaln, RmiuT, AlagmR, eig1, eig2, eig3, alf, ubetac2, amrev);
     const double dunormaln = (sav1n * (URo - ULo) + sav2n * (VRo - VLo) + sav3n * (WRo - WLo));
     const double astar = (eig2 + eig3) / 2.0;
     const double Mstar = (eig2 - eig3) / 2.0 / amrev;
@@ -128,7 +164,7 @@ __device__ __forceinline__ void roe_rotated_face(const trans_params& p, double P
                                                  double TRo, double ULo, double URo, double VLo, double VRo,
                                                  double WLo, double WRo, double s1, double s2, double s3,
                                                  double grad, double Funscheme, double rLo, double rRo,
-                                                 double out[5])
+                                                 double RmiuT, double AlagmR, double out[5])
 {
     const double rcpcv = p.ppf;
     const double DENLo = PLo / (rcpcv * TLo), DENRo = PRo / (rcpcv * TRo);
@@ -150,7 +186,7 @@ __device__ __forceinline__ void roe_rotated_face(const trans_params& p, double P
     const double DeltaU = sqrt((URo - ULo) * (URo - ULo) + (VRo - VLo) * (VRo - VLo) + (WRo - WLo) * (WRo - WLo));
     if (DeltaU <= p.tiny) {
         roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, s1, s2, s3, grad, Funscheme, rLo, rRo,
-                       out);
+                       RmiuT, AlagmR, out);
     } else {
         sav1n = (URo - ULo) / DeltaU;
         sav2n = (VRo - VLo) / DeltaU;
@@ -163,19 +199,19 @@ __device__ __forceinline__ void roe_rotated_face(const trans_params& p, double P
         const double a1 = sav1n * sav, a2 = sav2n * sav, a3 = sav3n * sav;
         const double b1 = s1 - a1, b2 = s2 - a2, b3 = s3 - a3;
         roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, a1, a2, a3,
-                       fmax(sqrt(a1 * a1 + a2 * a2 + a3 * a3), p.tiny), Funscheme, rLo, rRo, out);
+                       fmax(sqrt(a1 * a1 + a2 * a2 + a3 * a3), p.tiny), Funscheme, rLo, rRo, RmiuT, AlagmR, out);
         roe_rot_common(p, PLo, PRo, TLo, TRo, ULo, URo, VLo, VRo, WLo, WRo, b1, b2, b3,
-                       fmax(sqrt(b1 * b1 + b2 * b2 + b3 * b3), p.tiny), Funscheme, rLo, rRo, out);
+                       fmax(sqrt(b1 * b1 + b2 * b2 + b3 * b3), p.tiny), Funscheme, rLo, rRo, RmiuT, AlagmR, out);
     }
 }
 
-// Space/CONVECT_Roe_Original.F90:242-407, IF_PRECONDITION = 0.  Returns the flux per unit
-// area (droR..dreR).
+// Space/CONVECT_Roe_Original.F90:242-407.  Returns the flux per unit area (droR..dreR).
+// RmiuT, AlagmR: face eddy viscosity and Alagm of the preconditioned eigenvalues.
 __device__ __forceinline__ void roe_common(const trans_params& p, double PLo, double PRo,
                                            double TLo, double TRo, double ULo, double URo,
                                            double VLo, double VRo, double WLo, double WRo,
                                            double sav1, double sav2, double sav3, double sav,
-                                           double rLo, double rRo, double out[5])
+                                           double rLo, double rRo, double RmiuT, double AlagmR, double out[5])
 {
     const double rcpcv = p.ppf;
     const double DENLo = PLo / (rcpcv * TLo);
@@ -198,15 +234,20 @@ __device__ __forceinline__ void roe_common(const trans_params& p, double PLo, do
     const double am = sqrt(am2);
     const double s1 = sav1 / sav, s2 = sav2 / sav, s3 = sav3 / sav;
     const double un = (s1 * um + s2 * vm + s3 * wm);
-    double eig1 = fabs(un);
-    double eig2 = fabs(un + am);
-    double eig3 = fabs(un - am);
-    if (eig1 < 0.1) eig1 = (eig1 * eig1 + 0.1 * 0.1) / 0.2;
-    if (eig2 < 0.1) eig2 = (eig2 * eig2 + 0.1 * 0.1) / 0.2;
-    if (eig3 < 0.1) eig3 = (eig3 * eig3 + 0.1 * 0.1) / 0.2;
+    double eig1, eig2, eig3, alf = 0.0, ubetac2 = 1.0, amrev = am;
+    if (p.if_precondition) {
+        roe_precond_eigs(p, vm2, am2, rm, un, RmiuT, AlagmR, eig1, eig2, eig3, alf, ubetac2, amrev);
+    } else {
+        eig1 = fabs(un);
+        eig2 = fabs(un + am);
+        eig3 = fabs(un - am);
+        if (eig1 < 0.1) eig1 = (eig1 * eig1 + 0.1 * 0.1) / 0.2;
+        if (eig2 < 0.1) eig2 = (eig2 * eig2 + 0.1 * 0.1) / 0.2;
+        if (eig3 < 0.1) eig3 = (eig3 * eig3 + 0.1 * 0.1) / 0.2;
+    }
     const double dun = (s1 * (URo - ULo) + s2 * (VRo - VLo) + s3 * (WRo - WLo));
     const double astar = (eig2 + eig3) / 2.0;
-    const double Mstar = (eig2 - eig3) / 2.0 / am;
+    const double Mstar = (eig2 - eig3) / 2.0 / amrev;
     const double dW1 = DENRo - DENLo;
     const double dU1 = URo - ULo, dU2 = VRo - VLo, dU3 = WRo - WLo;
     const double dP1 = PRo - PLo;
@@ -214,8 +255,14 @@ __device__ __forceinline__ void roe_common(const trans_params& p, double PLo, do
     const double dW3 = rm * dU2 + dW1 * vm;
     const double dW4 = rm * dU3 + dW1 * wm;
     const double dW5 = dP1 / ga1 + 0.5 * vm2 * dW1 + rm * um * dU1 + rm * vm * dU2 + rm * wm * dU3;
-    const double dUroe = Mstar * dun + (astar - 1.0 * eig1) * (PRo - PLo) / rm / (1.0 * am2);
-    const double dProe = Mstar * (PRo - PLo) + (astar - eig1) * rm * dun;
+    double dUroe, dProe;
+    if (p.if_precondition) {
+        dUroe = Mstar * dun + (astar - (1 - 2.0 * alf) * eig1 - alf * un * Mstar) * (PRo - PLo) / rm / (ubetac2 * am2);
+        dProe = Mstar * (PRo - PLo) + (astar - eig1 + alf * un * Mstar) * rm * dun;
+    } else {
+        dUroe = Mstar * dun + (astar - 1.0 * eig1) * (PRo - PLo) / rm / (1.0 * am2);
+        dProe = Mstar * (PRo - PLo) + (astar - eig1) * rm * dun;
+    }
     const double droR1 = eig1 * dW1 + dUroe * rm;
     const double drxR1 = eig1 * dW2 + dUroe * rm * um + dProe * s1;
     const double dryR1 = eig1 * dW3 + dUroe * rm * vm + dProe * s2;

@@ -10,7 +10,8 @@
 #define TRANS_SWEEP_CONTRACT _Pragma("clang fp contract(fast)")
 
 // ---------------------------------------------------------------------------------------
-// Time/TIME_LUSGS.F90:60-108: BDF2 right-hand side and the scalar diagonal F1.  With the
+// Time/TIME_LUSGS.F90:60-108: BDF2 right-hand side and the scalar diagonal F1, then with
+// preconditioning the right-hand side through the cell's transform (:113-160).  With the
 // line-march sweep the right-hand side and the state the sweep needs go to the
 // hyperplane-major buffer (the sweep re-forms F1 from Rds and cvs); otherwise D, F1 and Stp.
 __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
@@ -34,6 +35,7 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
         const long e = (long)L * tot + c;
         rhs[L] = f.D[e] / vol - (3.0 * f.V[e] - 4.0 * f.Wt0[e] + f.Wt1[e]) * 0.5 / dtm;
     }
+    if (p.if_precondition) precond_cell<double>(p, f, c, rhs);
     const double Dt = f.Dtm[c];
     const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
     const double cvs = stept / vol;
@@ -139,9 +141,10 @@ __device__ __forceinline__ void lusgs_adq(R pVx, R pVy, R pVz, R aSDx, R aSDy, R
 
 // Neighbour contribution 0.5*(A_nb dQ_nb +/- Rds_nb dQ_nb) of the sweeps
 // (Time/TIME_LUSGS.F90:190-256 and the mirrored blocks).  `cn` is the neighbour cell and
-// `st` the stride of direction d; A uses the mean of the neighbour's two d-faces.
+// `st` the stride of direction d; A uses the mean of the neighbour's two d-faces.  With
+// preconditioning A_nb dQ_nb passes through the neighbour's transform (:235-246).
 template <typename R>
-__device__ __forceinline__ void lusgs_nb(const Fields& f, const R* __restrict__ Dq, int d,
+__device__ __forceinline__ void lusgs_nb(const trans_params& p, const Fields& f, const R* __restrict__ Dq, int d,
                                          long cn, long st, R sgn, R acc[5])
 {
     TRANS_SWEEP_CONTRACT
@@ -160,6 +163,7 @@ __device__ __forceinline__ void lusgs_nb(const Fields& f, const R* __restrict__ 
     const double aSDz = 0.5 * (S[6 * tot + cn + st] + S[6 * tot + cn]);
     R dif[5];
     lusgs_adq<R>(R(pVx), R(pVy), R(pVz), R(aSDx), R(aSDy), R(aSDz), R(f.PP[cn]), R(roinv), ain, dif);
+    if (p.if_precondition) precond_cell<R>(p, f, cn, dif);
     R rdst;
     if constexpr (sizeof(R) == 4)
         rdst = f.Rdsf[(long)d * tot + cn];
@@ -225,7 +229,7 @@ __device__ __forceinline__ void plane_cell(const trans_params& p, const Fields& 
         const bool edge = FORWARD ? pos[d] == 1 : pos[d] == nmax[d];
         if (edge) continue;
         const long cn = FORWARD ? c - st3[d] : c + st3[d];
-        if (NEQ != 2) lusgs_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), dn[d]);
+        if (NEQ != 2) lusgs_nb<R>(p, f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), dn[d]);
         if (NEQ != 5) turb_nb<R>(f, Dq, d, cn, st3[d], FORWARD ? R(1) : R(-1), tn[d]);
     }
     if (NEQ != 2) {
@@ -800,42 +804,66 @@ __global__ void __launch_bounds__(256) k_update_v(trans_params p, int ki, int if
         const double AinRz = f.D[3 * tot + c], AinRe = f.D[4 * tot + c];
         double pRo = v1;
         const double roinv = 1.0 / pRo;
-        double pVx = f.V[tot + c] * roinv;
-        double pVy = f.V[2 * tot + c] * roinv;
-        double pVz = f.V[3 * tot + c] * roinv;
-        const double AinX = (AinRx - pVx * AinRo) * roinv;
-        const double AinY = (AinRy - pVy * AinRo) * roinv;
-        const double AinZ = (AinRz - pVz * AinRo) * roinv;
-        const double AinP = (1.4 - 1.0) * (AinRe - 0.5 * AinRo * (pVx * pVx + pVy * pVy + pVz * pVz) -
-                                           pRo * (AinX * pVx + AinY * pVy + AinZ * pVz));
-        if (AinRo >= 0.)
-            pRo = pRo + AinRo;
-        else
-            pRo = pRo * exp(AinRo / (pRo + 1.e-10));
-        pRo = fabs(pRo);
-        double pp = f.PP[c];
-        if (AinP >= 0.)
-            pp = pp + AinP;
-        else
-            pp = pp * exp(AinP / (pp + 1.e-10));
-        pp = fabs(pp);
-        f.PP[c] = pp;
-        f.T[c] = p.rxm2 * pp / pRo;
-        pVx = pVx + AinX;
-        pVy = pVy + AinY;
-        pVz = pVz + AinZ;
         const double rad = f.rad[c];
-        f.V[c] = pRo;
-        f.V[tot + c] = pRo * pVx;
-        f.V[2 * tot + c] = pRo * pVy;
-        f.V[3 * tot + c] = pRo * pVz;
-        const double DV = 0.5 * (pVx * pVx + pVy * pVy + pVz * pVz);
-        const double v5 = pp / (1.4 - 1.0) + DV * pRo - 0.5 * sqr(p.omega1 * rad) * pRo;
-        f.V[4 * tot + c] = v5;
+        double vn[5];
+        if (p.if_precondition) {
+            // Time/TIME_Update_V.F90:522-606: the increment added to the conservative state; the
+            // kinetic energy divides by the density before the update, and T is formed from the
+            // pressure before the update
+            const double pRx = f.V[tot + c] + AinRx, pRy = f.V[2 * tot + c] + AinRy, pRz = f.V[3 * tot + c] + AinRz;
+            double pRe = fabs(f.V[4 * tot + c] + AinRe);
+            pRo = fabs(pRo + AinRo);
+            const double DV = 0.5 * (pRx * pRx + pRy * pRy + pRz * pRz) * roinv;
+            double pPP = (pRe - DV) * (1.4 - 1.0);
+            if (pPP <= 0.0) {
+                pPP = fabs(pPP);
+                pRe = pPP / (1.4 - 1.0) + DV;
+            }
+            f.T[c] = p.rxm2 * f.PP[c] / pRo;
+            f.PP[c] = pPP;
+            vn[0] = pRo;
+            vn[1] = pRx;
+            vn[2] = pRy;
+            vn[3] = pRz;
+            vn[4] = pRe;
+        } else {
+            double pVx = f.V[tot + c] * roinv;
+            double pVy = f.V[2 * tot + c] * roinv;
+            double pVz = f.V[3 * tot + c] * roinv;
+            const double AinX = (AinRx - pVx * AinRo) * roinv;
+            const double AinY = (AinRy - pVy * AinRo) * roinv;
+            const double AinZ = (AinRz - pVz * AinRo) * roinv;
+            const double AinP = (1.4 - 1.0) * (AinRe - 0.5 * AinRo * (pVx * pVx + pVy * pVy + pVz * pVz) -
+                                               pRo * (AinX * pVx + AinY * pVy + AinZ * pVz));
+            if (AinRo >= 0.)
+                pRo = pRo + AinRo;
+            else
+                pRo = pRo * exp(AinRo / (pRo + 1.e-10));
+            pRo = fabs(pRo);
+            double pp = f.PP[c];
+            if (AinP >= 0.)
+                pp = pp + AinP;
+            else
+                pp = pp * exp(AinP / (pp + 1.e-10));
+            pp = fabs(pp);
+            f.PP[c] = pp;
+            f.T[c] = p.rxm2 * pp / pRo;
+            pVx = pVx + AinX;
+            pVy = pVy + AinY;
+            pVz = pVz + AinZ;
+            const double DV = 0.5 * (pVx * pVx + pVy * pVy + pVz * pVz);
+            vn[0] = pRo;
+            vn[1] = pRo * pVx;
+            vn[2] = pRo * pVy;
+            vn[3] = pRo * pVz;
+            vn[4] = pp / (1.4 - 1.0) + DV * pRo - 0.5 * sqr(p.omega1 * rad) * pRo;
+        }
+#pragma unroll
+        for (int L = 0; L < 5; L++) f.V[(long)L * tot + c] = vn[L];
 
         if (p.if_euler == 0 && if_turb == 1 && ki >= p.kitm) {
-            const double VV = sqr(pRo * pVx) + sqr(pRo * pVy) + sqr(pRo * pVz);
-            const double PPP = (1.4 - 1.0) * v5 - 0.5 * (1.4 - 1.0) * (VV / pRo - sqr(p.omega1 * rad) * pRo);
+            const double VV = sqr(vn[1]) + sqr(vn[2]) + sqr(vn[3]);
+            const double PPP = (1.4 - 1.0) * vn[4] - 0.5 * (1.4 - 1.0) * (VV / pRo - sqr(p.omega1 * rad) * pRo);
             const double TT = p.rxm2 * PPP / pRo;
             sst_kw_update(p, f, c, pRo, sutherland(p, TT));
         }

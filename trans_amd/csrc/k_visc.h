@@ -479,6 +479,10 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
     }
     const double* S = f.SD + (long)d * tot;
     const double sav = f.Grad[(long)d * tot + c0];
+    // face eddy viscosity (Space/SPACEINTERP_GetSurfCenterVars.F90, VL(d,8)) and Alagm of the
+    // preconditioned Roe eigenvalues
+    const double RmiuT = p.if_precondition && turb ? face_avg(Al[c0 - st], Al[c0], f.Rmiu[c0 - st], f.Rmiu[c0]) : 0.0;
+    const double AlagmR = Al[c0];
     double fl[5];
     if (ROE == 2) {  // Space/CONVECT_Roe_Rotated.F90: area-weighted flux, scattered as it is
         // Funscheme from the two cells; the first ghost layer of Fscheme is the nearest
@@ -490,13 +494,14 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
         const long cb = cidx(b, clampi(i, 1, n3[0]), clampi(j, 1, n3[1]), clampi(k, 1, n3[2]));
         const double Funscheme = (f.Fscheme[ca] + f.Fscheme[cb]) / 2.;
         roe_rotated_face(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
-                         S[3 * tot + c0], S[6 * tot + c0], sav, Funscheme, qL[5] * p.omega1, qR[5] * p.omega1, fl);
+                         S[3 * tot + c0], S[6 * tot + c0], sav, Funscheme, qL[5] * p.omega1, qR[5] * p.omega1, RmiuT,
+                         AlagmR, fl);
         double* G = f.G[d];
 #pragma unroll
         for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L];
     } else if (ROE == 1)
         roe_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0], S[3 * tot + c0],
-                   S[6 * tot + c0], sav, qL[5] * p.omega1, qR[5] * p.omega1, fl);
+                   S[6 * tot + c0], sav, qL[5] * p.omega1, qR[5] * p.omega1, RmiuT, AlagmR, fl);
     else
         ausmpw_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
                       S[3 * tot + c0], S[6 * tot + c0], sav, fl);

@@ -279,7 +279,7 @@ const char* trans_hip_last_error(const trans_ctx* ctx) { return ctx ? ctx->err.c
 
 static int check_params(trans_ctx* c, const trans_params* p)
 {
-    if (p->if_precondition != 0) return fail(c, TRANS_EUNSUPPORTED, "IF_PRECONDITION must be 0");
+    if (p->if_precondition != 0 && p->if_precondition != 1) return fail(c, TRANS_EINVAL, "IF_PRECONDITION must be 0 or 1");
     // RCCorrection (Space/RC_Correction.F90): mode 2 only writes f_r2, which no routine reads,
     // so it leaves the state as mode 0 does; mode 1 reads omega(M) with M unset (:107)
     if (p->kind_rotmod != 0 && p->kind_rotmod != 2)
@@ -339,6 +339,9 @@ static int upload_blk_table(trans_ctx* c)
 static int set_sweep(trans_ctx* c, int m, bool upload)
 {
     int mx = 0;
+    // the low-Mach preconditioning transforms live in the natural-layout hyperplane sweep
+    if (c->p.if_precondition && m != -1)
+        return fail(c, TRANS_EUNSUPPORTED, "with IF_PRECONDITION = 1 only the hyperplane sweep (-1) runs");
     const bool planes = m == -2;  // per-plane launches on the hyperplane-major buffer, march axis j
     if (planes) m = 1;
     if (!planes && m >= 0 && !march_fits(c, m, &mx)) return fail(c, TRANS_EINVAL, "the line sweep does not fit this axis");
@@ -523,9 +526,10 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
 
     {
         // march along j (lanes on i: coalesced) unless the (i,k) sections do not fit a workgroup;
-        // TRANS_HIP_LINE_SWEEP=0 in the environment selects the hyperplane sweep instead
+        // TRANS_HIP_LINE_SWEEP=0 in the environment, or preconditioning, selects the hyperplane
+        // sweep instead
         const char* env = std::getenv("TRANS_HIP_LINE_SWEEP");
-        const bool lines_ok = !(env && env[0] == '0');
+        const bool lines_ok = !(env && env[0] == '0') && !c->p.if_precondition;
         const int order[3] = {1, 2, 0};
         int m = lines_ok ? -2 : -1;  // blocks too large for a line-march workgroup: one launch per plane
         for (int q = 0; q < 3 && lines_ok && m < 0; q++)
@@ -618,6 +622,8 @@ int trans_hip_set_params(trans_ctx* c, const trans_params* p)
         return fail(c, TRANS_EINVAL, "implicit_fp32 is fixed at context creation");
     if (p->if_euler != c->p.if_euler || p->if_turb_org != c->p.if_turb_org)
         return fail(c, TRANS_EINVAL, "IF_Euler and IF_Turb_Org are fixed at context creation");
+    if (p->if_precondition != c->p.if_precondition)
+        return fail(c, TRANS_EINVAL, "IF_PRECONDITION is fixed at context creation (it selects the sweep)");
     c->p = *p;
     drop_graph(c);
     return 0;

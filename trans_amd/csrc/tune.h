@@ -57,6 +57,7 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
     for (int si = 0; si < 5 && !rc; si++) {
         const int sw = sweeps[si];
         if (sw >= 0 && !march_fits(c, sw, nullptr)) continue;
+        if (c->p.if_precondition && sw != -1) continue;  // the transform lives in the hyperplane sweep
         if ((rc = set_sweep(c, sw, true))) break;
         for (int st = 0; st < 2 && !rc; st++) {
             if (st == 0 && !c->aux_own[0]) continue;  // single-stream context
