@@ -77,7 +77,7 @@ constexpr int W_D = 0;      // 7: right-hand side, then the increment
 constexpr int W_STEPT = 7;  // stept
 constexpr int W_CVS = 8;    // stept / Vol
 constexpr int W_F1T = 9;    // k-omega diagonal
-constexpr int W_V = 10;     // 4: rho, rho*u, rho*v, rho*w
+constexpr int W_V = 10;     // 4: 1/rho, u, v, w (u = (rho u) * (1/rho), as the sweep forms them)
 constexpr int W_PP = 14;    // pressure
 constexpr int W_RDS = 15;   // 3: spectral radii
 constexpr int W_SDM = 18;   // 9: face-mean vectors (geometry)

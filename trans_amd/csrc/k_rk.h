@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(256) k_rk_stage(trans_params p, int stage, int
             wput(f, W_STEPT, w, stept);
             wput(f, W_CVS, w, stept / vol);
 #pragma unroll
-            for (int L = 0; L < 4; L++) wput(f, W_V + L, w, vn[L]);
+            for (int L = 0; L < 4; L++) wput(f, W_V + L, w, L == 0 ? 1.0 / vn[0] : vn[L] * (1.0 / vn[0]));
 #pragma unroll
             for (int d = 0; d < 3; d++) wput(f, W_RDS + d, w, f.Rds[(long)d * tot + c]);
         }

@@ -46,8 +46,10 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
         for (int L = 0; L < 5; L++) wput(f, W_D + L, w, rhs[L]);
         wput(f, W_STEPT, w, stept);
         wput(f, W_CVS, w, cvs);
+        const double roinv = 1.0 / f.V[c];
+        wput(f, W_V, w, roinv);
 #pragma unroll
-        for (int L = 0; L < 4; L++) wput(f, W_V + L, w, f.V[(long)L * tot + c]);
+        for (int L = 1; L < 4; L++) wput(f, W_V + L, w, f.V[(long)L * tot + c] * roinv);
         wput(f, W_PP, w, f.PP[c]);
 #pragma unroll
         for (int d = 0; d < 3; d++) wput(f, W_RDS + d, w, f.Rds[(long)d * tot + c]);
@@ -114,24 +116,57 @@ __global__ void __launch_bounds__(256) k_unskew(int c0, int neq, Fields f, const
 //   dif0 = S.dm,  dif_i = u_i S.dm + VP (dm_i - u_i drho) + S_i dp,
 //   dif4 = H S.dm + VP (dE + dp - H drho),  H = c^2/(gamma-1) + |u|^2/2,
 // term by term equal to the reference's coefficients, in about half the operations.
+// The direction-independent part of the product for one cell and increment, then the
+// product through one face vector S.
+template <typename R>
+struct AdqPrep {
+    R u[3], dm[3];
+    R dp;   // pressure increment
+    R H;    // total enthalpy
+    R w[3]; // dm_i - u_i drho
+    R q4;   // dE + dp - H drho
+};
+
+template <typename R>
+__device__ __forceinline__ AdqPrep<R> adq_prep(R pVx, R pVy, R pVz, R pPP, R roinv, const R ain[5])
+{
+    TRANS_SWEEP_CONTRACT
+    const R Gm1 = R(1.4 - 1.0);
+    AdqPrep<R> a;
+    a.u[0] = pVx;
+    a.u[1] = pVy;
+    a.u[2] = pVz;
+    a.dm[0] = ain[1];
+    a.dm[1] = ain[2];
+    a.dm[2] = ain[3];
+    const R uDm = pVx * ain[1] + pVy * ain[2] + pVz * ain[3];
+    const R hU2 = R(0.5) * (pVx * pVx + pVy * pVy + pVz * pVz);
+    a.dp = Gm1 * (ain[4] - uDm + hU2 * ain[0]);
+    a.H = R(1.4) * pPP * roinv * (R(1) / Gm1) + hU2;
+#pragma unroll
+    for (int q = 0; q < 3; q++) a.w[q] = a.dm[q] - a.u[q] * ain[0];
+    a.q4 = ain[4] + a.dp - a.H * ain[0];
+    return a;
+}
+
+template <typename R>
+__device__ __forceinline__ void adq_dir(const AdqPrep<R>& a, R aSDx, R aSDy, R aSDz, R dif[5])
+{
+    TRANS_SWEEP_CONTRACT
+    const R VP = a.u[0] * aSDx + a.u[1] * aSDy + a.u[2] * aSDz;
+    const R Sm = a.dm[0] * aSDx + a.dm[1] * aSDy + a.dm[2] * aSDz;
+    dif[0] = Sm;
+    dif[1] = a.u[0] * Sm + VP * a.w[0] + aSDx * a.dp;
+    dif[2] = a.u[1] * Sm + VP * a.w[1] + aSDy * a.dp;
+    dif[3] = a.u[2] * Sm + VP * a.w[2] + aSDz * a.dp;
+    dif[4] = a.H * Sm + VP * a.q4;
+}
+
 template <typename R>
 __device__ __forceinline__ void lusgs_adq(R pVx, R pVy, R pVz, R aSDx, R aSDy, R aSDz, R pPP,
                                           R roinv, const R ain[5], R dif[5])
 {
-    TRANS_SWEEP_CONTRACT
-    const R Gm1 = R(1.4 - 1.0);
-    const R AinRo = ain[0], AinRx = ain[1], AinRy = ain[2], AinRz = ain[3], AinRe = ain[4];
-    const R VP = pVx * aSDx + pVy * aSDy + pVz * aSDz;
-    const R Sm = AinRx * aSDx + AinRy * aSDy + AinRz * aSDz;
-    const R uDm = pVx * AinRx + pVy * AinRy + pVz * AinRz;
-    const R hU2 = R(0.5) * (pVx * pVx + pVy * pVy + pVz * pVz);
-    const R dp = Gm1 * (AinRe - uDm + hU2 * AinRo);
-    const R H = R(1.4) * pPP * roinv * (R(1) / Gm1) + hU2;
-    dif[0] = Sm;
-    dif[1] = pVx * Sm + VP * (AinRx - pVx * AinRo) + aSDx * dp;
-    dif[2] = pVy * Sm + VP * (AinRy - pVy * AinRo) + aSDy * dp;
-    dif[3] = pVz * Sm + VP * (AinRz - pVz * AinRo) + aSDz * dp;
-    dif[4] = H * Sm + VP * (AinRe + dp - H * AinRo);
+    adq_dir<R>(adq_prep<R>(pVx, pVy, pVz, pPP, roinv, ain), aSDx, aSDy, aSDz, dif);
 }
 
 // Neighbour contribution 0.5*(A_nb dQ_nb +/- Rds_nb dQ_nb) of the sweeps
@@ -464,9 +499,11 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
                 dq[L] = R(cur.d[L]) - sum * R(cvs) * rf1;
             stg(src.d[L], wst, dq[L]);
         }
-        const double roinv = 1.0 / double(cur.v[0]);
-        const R pVx = R(cur.v[1] * roinv), pVy = R(cur.v[2] * roinv), pVz = R(cur.v[3] * roinv);
+        const double roinv = double(cur.v[0]);
+        const R pVx = R(cur.v[1]), pVy = R(cur.v[2]), pVz = R(cur.v[3]);
         const int npar = par ^ 1;
+        AdqPrep<R> prep;
+        if constexpr (MF) prep = adq_prep<R>(pVx, pVy, pVz, R(cur.pp), R(roinv), dq);
 #pragma unroll
         for (int d = 0; d < 3; d++) {
             R o[NEQ];
@@ -474,7 +511,7 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
             const R rd = R(cur.rds[d]);
             if constexpr (MF) {
                 R dif[5];
-                lusgs_adq<R>(pVx, pVy, pVz, sx, sy, sz, R(cur.pp), R(roinv), dq, dif);
+                adq_dir<R>(prep, sx, sy, sz, dif);
 #pragma unroll
                 for (int L = 0; L < 5; L++) o[L] = R(0.5) * (dif[L] + sgn * rd * dq[L]);
             } else {
@@ -544,10 +581,8 @@ __device__ __forceinline__ void wplane_term(const Fields& f, const R* __restrict
     R dq[NEQ];
 #pragma unroll
     for (int L = 0; L < NEQ; L++) dq[L] = ldg(W + (C0 + L) * wt, wn);
-    const R v0 = ldg(W + W_V * wt, wn);
-    const double roinv = 1.0 / double(v0);
-    const R pVx = R(ldg(W + (W_V + 1) * wt, wn) * roinv), pVy = R(ldg(W + (W_V + 2) * wt, wn) * roinv),
-            pVz = R(ldg(W + (W_V + 3) * wt, wn) * roinv);
+    const double roinv = double(ldg(W + W_V * wt, wn));
+    const R pVx = ldg(W + (W_V + 1) * wt, wn), pVy = ldg(W + (W_V + 2) * wt, wn), pVz = ldg(W + (W_V + 3) * wt, wn);
     const R sx = ldg(W + (W_SDM + d) * wt, wn), sy = ldg(W + (W_SDM + d + 3) * wt, wn),
             sz = ldg(W + (W_SDM + d + 6) * wt, wn);
     const R rd = ldg(W + (W_RDS + d) * wt, wn);
