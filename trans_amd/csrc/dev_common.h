@@ -71,8 +71,9 @@ struct Fields {
     int march;        // march axis of the line sweep (0 i, 1 j, 2 k)
 };
 
-// components of Fields::W / Fields::Wf (the mean-flow diagonal F1 = 1 + sum(Rds)*cvs is
-// re-formed by the sweep from W_RDS and W_CVS)
+// components of Fields::W / Fields::Wf (the line-march mean-flow sweep reads the step sizes
+// already divided by its diagonal F1 = 1 + sum(Rds)*cvs; the per-plane sweep re-forms F1 from
+// W_RDS and W_CVS)
 constexpr int W_D = 0;      // 7: right-hand side, then the increment
 constexpr int W_STEPT = 7;  // stept
 constexpr int W_CVS = 8;    // stept / Vol
@@ -81,7 +82,9 @@ constexpr int W_V = 10;     // 4: 1/rho, u, v, w (u = (rho u) * (1/rho), as the 
 constexpr int W_PP = 14;    // pressure
 constexpr int W_RDS = 15;   // 3: spectral radii
 constexpr int W_SDM = 18;   // 9: face-mean vectors (geometry)
-constexpr int W_NCOMP = 27;
+constexpr int W_SA = 27;    // stept / F1 (mean flow)
+constexpr int W_SB = 28;    // cvs / F1 (mean flow)
+constexpr int W_NCOMP = 29;
 
 // Whether the sweep inputs live in the hyperplane-major buffer.
 __device__ __forceinline__ bool has_w(const Fields& f) { return f.W != nullptr || f.Wf != nullptr; }

@@ -46,6 +46,9 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
         for (int L = 0; L < 5; L++) wput(f, W_D + L, w, rhs[L]);
         wput(f, W_STEPT, w, stept);
         wput(f, W_CVS, w, cvs);
+        const double rf1 = 1.0 / f1;
+        wput(f, W_SA, w, stept * rf1);
+        wput(f, W_SB, w, cvs * rf1);
         const double roinv = 1.0 / f.V[c];
         wput(f, W_V, w, roinv);
 #pragma unroll
@@ -375,8 +378,9 @@ struct LineSrc {
         const R* W = Dq;  // the increments share the buffer with the other inputs
 #pragma unroll
         for (int L = 0; L < NEQ; L++) d[L] = sbase(Dq + (C0 + L) * wt);
-        stept = sbase(W + W_STEPT * wt);
-        cvs = sbase(W + W_CVS * wt);
+        // the mean flow reads stept / F1 and cvs / F1 in their places
+        stept = sbase(W + (C0 == 0 ? W_SA : W_STEPT) * wt);
+        cvs = sbase(W + (C0 == 0 ? W_SB : W_CVS) * wt);
         f1 = sbase(W + W_F1T * wt);
 #pragma unroll
         for (int L = 0; L < 4; L++) v[L] = sbase(W + (W_V + L) * wt);
@@ -475,10 +479,9 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
         const unsigned wst = act ? lw + (unsigned)s * (unsigned)nlines : my_sink;
         const double stept = cur.stept;
         const double cvs = cur.cvs;
-        // F1 as Time/TIME_LUSGS.F90:92-108 forms it (k_lusgs_rhs evaluates the same expression)
-        const R f1 = MF ? R(1.0 + (double(cur.rds[0]) + double(cur.rds[1]) + double(cur.rds[2])) * cvs)
-                        : R(cur.f1);
-        const R rf1 = R(1) / f1;
+        // mean flow: stept and cvs arrive divided by F1 (Time/TIME_LUSGS.F90:92-108, formed in
+        // k_lusgs_rhs); k-omega: its own diagonal F1t
+        const R rf1 = MF ? R(1) : R(1) / R(cur.f1);
         R dq[NEQ];
 #pragma unroll
         for (int L = 0; L < NEQ; L++) {
@@ -493,10 +496,17 @@ __device__ __forceinline__ void line_sweep(const Fields& f, R* __restrict__ Dq, 
                 sum = sa + sm + sb;
             else
                 sum = sa + sb + sm;
-            if (FORWARD)
-                dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) * rf1;
-            else
-                dq[L] = R(cur.d[L]) - sum * R(cvs) * rf1;
+            if constexpr (MF) {
+                if (FORWARD)
+                    dq[L] = R(cur.d[L]) * R(stept) + sum * R(cvs);
+                else
+                    dq[L] = R(cur.d[L]) - sum * R(cvs);
+            } else {
+                if (FORWARD)
+                    dq[L] = (R(cur.d[L]) * R(stept) + sum * R(cvs)) * rf1;
+                else
+                    dq[L] = R(cur.d[L]) - sum * R(cvs) * rf1;
+            }
             stg(src.d[L], wst, dq[L]);
         }
         const double roinv = double(cur.v[0]);
