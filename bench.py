@@ -53,10 +53,11 @@ def alg_bytes(name, neq, fp32):
         "k_lusgs_rhs": ("cell", d * (5 + 5 + 5 + 5 + 3 + 1 + 1 + 5 + 1)),
         # D(6:7), V(1,6,7), Wt0, Wt1 (2 each), DQ(15), Rmiu, T, Dst, Dtm, Rds(3), Vol in; D(6:7), F1t out
         "k_sst_rhs": ("cell", d * (2 + 3 + 2 + 2 + 15 + 1 + 1 + 1 + 1 + 3 + 1 + 2 + 1)),
-        # hyperplane-major inputs in the operator's precision: D(neq) in/out, stept, cvs,
-        # (F1t), V(1:4), PP, face means SD(9), Rds(3); F1 is re-formed from Rds and cvs
-        "k_lusgs_line<fwd>": ("cell", r * (2 * neq + 2 + (1 if turb else 0) + 4 + 1 + 9 + 3)),
-        "k_lusgs_line<bwd>": ("cell", r * (2 * neq + 2 + (1 if turb else 0) + 4 + 1 + 9 + 3)),
+        # hyperplane-major inputs in the operator's precision: D(neq) in/out, stept/F1 and
+        # cvs/F1 (mean flow), (stept, cvs, F1t for k-omega), 1/rho and u, v, w, PP, face means
+        # SD(9), Rds(3)
+        "k_lusgs_line<fwd>": ("cell", r * (2 * neq + 2 + (3 if turb else 0) + 4 + 1 + 9 + 3)),
+        "k_lusgs_line<bwd>": ("cell", r * (2 * neq + 2 + (3 if turb else 0) + 4 + 1 + 9 + 3)),
         # D(neq), Dtm, Vol, F1(,F1t), V(1:4), PP, SD(9), Rds(3) in the natural layout
         "k_lusgs_sweep<fwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
         "k_lusgs_sweep<bwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
