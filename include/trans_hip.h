@@ -305,6 +305,28 @@ int trans_hip_set_tuning(trans_ctx* ctx, const trans_tune_entry* t);
  * panels.  Host pointers; no context needed.  Bitwise the same as the host computation. */
 int trans_geom_wall_distance(const double* cells, long ncell, const double* panels, long npanel, double* dst);
 
+/* Outputs of trans_geom_metrics: host arrays of one block in the padded layout (bounds
+ * -2:N+2 on each axis, i fastest, component first for sd (9: direction fastest, then x, y,
+ * z), grad (3) and alagm (3)). */
+typedef struct trans_geom_metrics_out {
+    double *xc, *yc, *zc; /* cell centres; boundary face centres at index 0 and N */
+    double *thtc, *rad;   /* atan2(zc, yc), sqrt(yc^2 + zc^2) */
+    double *aleng, *aseng;/* largest / smallest distance to the six neighbour centres */
+    double *vol;
+    double *sd, *grad, *alagm;
+} trans_geom_metrics_out;
+
+/* The O(N) metrics of Geom/CalcDst.F90:170-632 for one block of ni x nj x nk points (xx, yy,
+ * zz in the padded layout, points at indices 1..N): centres, face area vectors and areas
+ * (plus tiny under the root), face-to-face lengths Alagm, neighbour distances, volumes, and
+ * the first-ghost copies of Vol, Alagm, Grad and the SD extension to faces 0 and N+1.
+ * Entries the routine does not set keep the caller's values; the wall distance is
+ * trans_geom_wall_distance.  Bitwise the same as the reference's arithmetic order (thtc up
+ * to the device atan2).  TRANS_EINVAL on a cell volume below tiny (as CalcDst stops).
+ * Host pointers; no context needed. */
+int trans_geom_metrics(int ni, int nj, int nk, const double* xx, const double* yy, const double* zz, double tiny,
+                       const trans_geom_metrics_out* out);
+
 /* Device bytes a context for these inputs would allocate (trans_hip_create's own sizing,
  * run without allocating or touching the device), and the bytes a context holds.  No
  * counterpart in the reference; used to admit solves by HBM footprint. */

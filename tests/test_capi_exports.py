@@ -32,10 +32,10 @@ def test_struct_sizes_match_header():
 #include <stdio.h>
 #include <stddef.h>
 #include "trans_hip.h"
-int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(trans_params),
+int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(trans_params),
  sizeof(trans_patch), sizeof(trans_block_desc), sizeof(trans_fview), sizeof(trans_geom_view),
  sizeof(trans_state_view), sizeof(trans_resid), sizeof(trans_plane_integrals), sizeof(trans_srv_config),
- sizeof(trans_srv_request), sizeof(trans_srv_status)); return 0;}
+ sizeof(trans_srv_request), sizeof(trans_srv_status), sizeof(trans_geom_metrics_out)); return 0;}
 '''
     with tempfile.TemporaryDirectory() as d:
         c = os.path.join(d, "p.c")
@@ -47,7 +47,7 @@ int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(tr
     want = [ctypes.sizeof(t) for t in (capi.TransParams, capi.TransPatch, capi.TransBlockDesc,
                                        capi.TransFView, capi.TransGeomView, capi.TransStateView,
                                        capi.TransResid, capi.TransPlaneIntegrals, capi.TransSrvConfig,
-                                       capi.TransSrvRequest, capi.TransSrvStatus)]
+                                       capi.TransSrvRequest, capi.TransSrvStatus, capi.TransGeomMetricsOut)]
     assert got == want
 
 

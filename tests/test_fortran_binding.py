@@ -34,7 +34,7 @@ def test_bind_c_sizes_match_c_structs():
             "trans_geom_view": capi.TransGeomView, "trans_state_view": capi.TransStateView,
             "trans_resid": capi.TransResid, "trans_plane_integrals": capi.TransPlaneIntegrals,
             "trans_srv_config": capi.TransSrvConfig, "trans_srv_request": capi.TransSrvRequest,
-            "trans_srv_status": capi.TransSrvStatus}
+            "trans_srv_status": capi.TransSrvStatus, "trans_geom_metrics_out": capi.TransGeomMetricsOut}
     for name, ty in want.items():
         assert sizes[name] == C.sizeof(ty), (name, sizes[name], C.sizeof(ty))
 

@@ -1,7 +1,9 @@
 """GPU test of the device geometry pre-processing: the brute-force wall distance
 (trans_geom_wall_distance, Geom/CalcDst.F90:121-158) equals the host computation bitwise,
 on the flat plate's blocks (panels from every wall patch) and on a random cloud of cells and
-panels larger than one LDS tile."""
+panels larger than one LDS tile; the O(N) metrics (trans_geom_metrics, Geom/CalcDst.F90:170-632)
+give every geometry array of the prepared case bitwise (thtc, from the device atan2, within
+one rounding)."""
 import time
 
 import numpy as np
@@ -47,3 +49,42 @@ def test_wall_distance_flat_plate_blocks():
 def test_no_panels():
     got = mesh.wall_distance_device(np.zeros((5, 3)), np.zeros((0, 3)))
     assert np.all(got == 1.0e3)
+
+
+@pytest.mark.parametrize("case", ["plate", "bump", "thin"])
+def test_metrics_equal_host(case):
+    """Every array the case set-up produces (centres, SD, Grad, Alagm, aleng, aseng, Vol, rad,
+    ghost copies, wall distance, and the ghost geometry built on top of them) is the same with
+    the metrics computed on the device.  'thin' has two cells in k, which switches the
+    neighbour-distance candidates (CalcDst.F90:430-476)."""
+    print(f"case: {case}, host vs device metrics", flush=True)
+
+    def build():
+        if case == "plate":
+            return cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25)
+        if case == "bump":
+            return cases.bump_channel(33, 17, 5)
+        return cases.bump_channel(17, 9, 3)
+
+    host, org, glob = build()
+    solver.prepare_case(host, org, glob)
+    dev, org, glob = build()
+    solver.prepare_case(dev, org, glob, device_geometry=True)
+    for a, b in zip(host, dev):
+        assert set(a.arrays) == set(b.arrays)
+        for nm in a.arrays:
+            x, y = a[nm], b[nm]
+            if nm == "thtc":
+                np.testing.assert_allclose(y, x, rtol=0, atol=4e-15)  # a few ulps of pi
+            else:
+                assert np.array_equal(x, y), nm
+
+
+def test_metrics_reject_negative_volume():
+    """A block whose i direction is reversed against its own orientation check fails as
+    CalcDst stops: no device fault, TRANS_EINVAL mapped to ValueError."""
+    blocks, org, glob = cases.bump_channel(17, 9, 5)
+    b = blocks[0]
+    b.xx[...] = b.xx[::-1, :, :].copy(order="F")
+    with pytest.raises(ValueError):
+        mesh.compute_geometry(blocks, device=True)

@@ -100,6 +100,7 @@ class TransPlaneIntegrals(C.Structure):
 
 
 FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK, FIELD_FSCHEME = 0, 1, 2, 3, 4, 5
+TRANS_EINVAL, TRANS_EHIP, TRANS_ENOMEM, TRANS_EUNSUPPORTED = -1, -2, -3, -4
 
 
 class TransSrvConfig(C.Structure):
@@ -123,6 +124,11 @@ class TransSrvStatus(C.Structure):
 
 
 SRV_QUEUED, SRV_RUNNING, SRV_CONVERGED, SRV_MAX_ITER, SRV_CANCELLED, SRV_FAILED = range(6)
+
+class TransGeomMetricsOut(C.Structure):
+    _fields_ = [(nm, C.POINTER(C.c_double)) for nm in
+                ("xc", "yc", "zc", "thtc", "rad", "aleng", "aseng", "vol", "sd", "grad", "alagm")]
+
 
 class TransTuneEntry(C.Structure):
     _fields_ = [("sweep", C.c_int), ("streams", C.c_int), ("graph", C.c_int), ("ms_per_iter", C.c_double)]
@@ -148,7 +154,7 @@ EXPORTS = [
     "trans_hip_autotune", "trans_hip_get_tuning", "trans_hip_set_tuning",
     "trans_srv_create", "trans_srv_submit", "trans_srv_progress", "trans_srv_poll", "trans_srv_cancel",
     "trans_srv_result", "trans_srv_release", "trans_srv_destroy", "trans_srv_last_error",
-    "trans_geom_wall_distance",
+    "trans_geom_wall_distance", "trans_geom_metrics",
 ]
 
 
@@ -221,6 +227,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_set_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]
     lib.trans_hip_device_bytes.argtypes = [C.c_void_p, P(C.c_size_t)]
     lib.trans_geom_wall_distance.argtypes = [P(C.c_double), C.c_long, P(C.c_double), C.c_long, P(C.c_double)]
+    lib.trans_geom_metrics.argtypes = [C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
+                                       C.c_double, P(TransGeomMetricsOut)]
     lib.trans_srv_create.argtypes = [P(TransSrvConfig), P(C.c_void_p)]
     lib.trans_srv_submit.argtypes = [C.c_void_p, P(TransSrvRequest), P(C.c_int)]
     lib.trans_srv_progress.argtypes = [C.c_void_p, C.c_int, P(C.c_int)]

@@ -36,4 +36,70 @@ int trans_geom_wall_distance(const double* cells, long ncell, const double* pane
     return 0;
 }
 
+int trans_geom_metrics(int ni, int nj, int nk, const double* xx, const double* yy, const double* zz, double tiny,
+                       const trans_geom_metrics_out* out)
+{
+    if (ni < 3 || nj < 3 || nk < 3 || !xx || !yy || !zz || !out) return TRANS_EINVAL;
+    double* const host[11] = {out->xc,   out->yc,    out->zc, out->thtc, out->rad, out->aleng,
+                              out->aseng, out->vol, out->sd,  out->grad, out->alagm};
+    const int ncomp[11] = {1, 1, 1, 1, 1, 1, 1, 1, 9, 3, 3};
+    for (double* h : host)
+        if (!h) return TRANS_EINVAL;
+    const GeomGrid g{ni, nj, nk, (long)ni + 5, (long)nj + 5};
+    const size_t n = (size_t)(ni + 5) * (nj + 5) * (nk + 5);
+    hipStream_t s = nullptr;
+    double* dx = nullptr;  // points (3n), then the outputs in the order of `host`
+    int rc = 0;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && !rc) rc = TRANS_EHIP;
+        return e == hipSuccess;
+    };
+    size_t total = 3 * n;
+    for (int q = 0; q < 11; q++) total += (size_t)ncomp[q] * n;
+    if (ok(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) && ok(hipMalloc(&dx, total * sizeof(double)))) {
+        double* dev[11];
+        size_t off = 3 * n;
+        for (int q = 0; q < 11; q++) {
+            dev[q] = dx + off;
+            off += (size_t)ncomp[q] * n;
+        }
+        // outputs start as the caller's arrays, so entries no routine sets keep their values
+        bool up = ok(hipMemcpyAsync(dx, xx, n * sizeof(double), hipMemcpyHostToDevice, s)) &&
+                  ok(hipMemcpyAsync(dx + n, yy, n * sizeof(double), hipMemcpyHostToDevice, s)) &&
+                  ok(hipMemcpyAsync(dx + 2 * n, zz, n * sizeof(double), hipMemcpyHostToDevice, s));
+        for (int q = 0; q < 11 && up; q++)
+            up = ok(hipMemcpyAsync(dev[q], host[q], (size_t)ncomp[q] * n * sizeof(double), hipMemcpyHostToDevice, s));
+        if (up) {
+            const GeomOut o{dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], dev[6], dev[7], dev[8], dev[9], dev[10]};
+            const double *X = dx, *Y = dx + n, *Z = dx + 2 * n;
+            auto blocks = [](long m) { return dim3((unsigned)((m + 255) / 256)); };
+            hipLaunchKernelGGL(k_geom_centres, blocks((long)(ni + 1) * (nj + 1) * (nk + 1)), dim3(256), 0, s, g, X, Y,
+                               Z, o);
+            hipLaunchKernelGGL(k_geom_faces<1>, blocks((long)ni * (nj - 1) * (nk - 1)), dim3(256), 0, s, g, X, Y, Z, o,
+                               tiny);
+            hipLaunchKernelGGL(k_geom_faces<2>, blocks((long)(ni - 1) * nj * (nk - 1)), dim3(256), 0, s, g, X, Y, Z, o,
+                               tiny);
+            hipLaunchKernelGGL(k_geom_faces<3>, blocks((long)(ni - 1) * (nj - 1) * nk), dim3(256), 0, s, g, X, Y, Z, o,
+                               tiny);
+            hipLaunchKernelGGL(k_geom_cells, blocks((long)(ni - 1) * (nj - 1) * (nk - 1)), dim3(256), 0, s, g, X, Y, Z,
+                               o);
+            hipLaunchKernelGGL(k_geom_ghosts, blocks((long)(ni + 2) * (nj + 2) * (nk + 2)), dim3(256), 0, s, g, o);
+            bool down = ok(hipGetLastError());
+            for (int q = 0; q < 11 && down; q++)
+                down = ok(
+                    hipMemcpyAsync(host[q], dev[q], (size_t)ncomp[q] * n * sizeof(double), hipMemcpyDeviceToHost, s));
+            if (down) ok(hipStreamSynchronize(s));
+        }
+    }
+    if (dx) (void)hipFree(dx);
+    if (s) (void)hipStreamDestroy(s);
+    if (rc) return rc;
+    // Geom/CalcDst.F90:481-515 stops on a non-positive volume
+    for (int k = 1; k <= nk - 1; k++)
+        for (int j = 1; j <= nj - 1; j++)
+            for (int i = 1; i <= ni - 1; i++)
+                if (out->vol[g.at(i, j, k)] < tiny) return TRANS_EINVAL;
+    return 0;
+}
+
 }  // extern "C"

@@ -15,6 +15,7 @@ program abi_sizes
     type(trans_srv_config) :: q1
     type(trans_srv_request) :: q2
     type(trans_srv_status) :: q3
+    type(trans_geom_metrics_out) :: q4
     print '(a, i0)', "trans_params ", c_sizeof(a)
     print '(a, i0)', "trans_patch ", c_sizeof(b)
     print '(a, i0)', "trans_block_desc ", c_sizeof(c)
@@ -26,4 +27,5 @@ program abi_sizes
     print '(a, i0)', "trans_srv_config ", c_sizeof(q1)
     print '(a, i0)', "trans_srv_request ", c_sizeof(q2)
     print '(a, i0)', "trans_srv_status ", c_sizeof(q3)
+    print '(a, i0)', "trans_geom_metrics_out ", c_sizeof(q4)
 end program abi_sizes

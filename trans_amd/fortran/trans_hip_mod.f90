@@ -97,6 +97,11 @@ module trans_hip_mod
         integer(c_size_t) :: footprint
     end type trans_srv_status
 
+    ! outputs of trans_geom_metrics (c_loc of each -2:N+2 array; sd, grad, alagm component first)
+    type, bind(C), public :: trans_geom_metrics_out
+        type(c_ptr) :: xc, yc, zc, thtc, rad, aleng, aseng, vol, sd, grad, alagm
+    end type trans_geom_metrics_out
+
     public :: trans_view, trans_error_text
 
     interface
@@ -296,6 +301,21 @@ module trans_hip_mod
             import :: c_int, c_ptr
             type(c_ptr), value :: srv
         end function
+        integer(c_int) function trans_geom_wall_distance(cells, ncell, panels, npanel, dst) &
+                bind(C, name="trans_geom_wall_distance")
+            import :: c_int, c_long, c_double
+            real(c_double), intent(in) :: cells(*), panels(*)
+            integer(c_long), value :: ncell, npanel
+            real(c_double), intent(out) :: dst(*)
+        end function
+        integer(c_int) function trans_geom_metrics(ni, nj, nk, xx, yy, zz, tiny, out) &
+                bind(C, name="trans_geom_metrics")
+            import :: c_int, c_double, trans_geom_metrics_out
+            integer(c_int), value :: ni, nj, nk
+            real(c_double), intent(in) :: xx(*), yy(*), zz(*)
+            real(c_double), value :: tiny
+            type(trans_geom_metrics_out), intent(in) :: out
+        end function
     end interface
     public :: trans_hip_create, trans_hip_destroy, trans_hip_last_error, trans_hip_set_params, &
         trans_hip_set_iteration, trans_hip_set_geometry, trans_hip_set_state, trans_hip_get_state, &
@@ -304,7 +324,8 @@ module trans_hip_mod
         trans_hip_plane_integrals, trans_hip_sync, trans_hip_footprint, trans_srv_create, trans_srv_submit, &
         trans_hip_write_restart, trans_hip_read_restart, trans_hip_stats_enable, trans_hip_average_fields, &
         trans_hip_get_stats, trans_hip_set_stats, trans_hip_write_stats, trans_hip_read_stats, &
-        trans_srv_progress, trans_srv_poll, trans_srv_cancel, trans_srv_result, trans_srv_release, trans_srv_destroy
+        trans_srv_progress, trans_srv_poll, trans_srv_cancel, trans_srv_result, trans_srv_release, trans_srv_destroy, &
+        trans_geom_wall_distance, trans_geom_metrics
 
 contains
 

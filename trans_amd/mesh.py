@@ -190,11 +190,15 @@ def patch_structs(b: Block):
 # metrics (Geom/CalcDst.F90:28-719)
 
 def compute_geometry(blocks: list, tiny: float = 1e-20, device: bool = False):
-    """Metrics of every block; device=True computes the wall distances with the HIP
-    library's brute-force kernel (trans_geom_wall_distance), bitwise the same as here."""
+    """Metrics of every block; device=True computes the O(N) metrics (trans_geom_metrics)
+    and the wall distances (trans_geom_wall_distance) with the HIP library, bitwise the same
+    as here (thtc up to the device atan2)."""
     for b in blocks:
         b.wall_distance_on_device = device
-        _metrics(b, tiny)
+        if device:
+            _metrics_device(b, tiny)
+        else:
+            _metrics(b, tiny)
     for b in blocks:
         _ghost_geometry_physical(b)
     _ghost_geometry_interfaces(blocks)
@@ -325,6 +329,32 @@ def _metrics(b: Block, tiny: float):
         sd[1 + 3 * x][sl(1, NI), O + NJ + 1, sl(1, NK)] = sd[1 + 3 * x][sl(1, NI), O + NJ1, sl(1, NK)]
         sd[2 + 3 * x][sl(1, NI), sl(1, NJ), O + 0] = sd[2 + 3 * x][sl(1, NI), sl(1, NJ), O + 2]
         sd[2 + 3 * x][sl(1, NI), sl(1, NJ), O + NK + 1] = sd[2 + 3 * x][sl(1, NI), sl(1, NJ), O + NK1]
+
+
+def _metrics_device(b: Block, tiny: float):
+    """_metrics with the per-cell work on the device; the wall distance and its ghost copies
+    follow as in _metrics."""
+    import ctypes as C
+    NI, NJ, NK = b.ni, b.nj, b.nk
+    NI1, NJ1, NK1 = NI - 1, NJ - 1, NK - 1
+    names = ("xc", "yc", "zc", "thtc", "rad", "aleng", "aseng", "vol", "sd", "grad", "alagm")
+    ncomp = {"sd": 9, "grad": 3, "alagm": 3}
+    arrs = {nm: b.alloc(nm, ncomp.get(nm, 1)) for nm in names}
+    pts = [np.asfortranarray(a, dtype=np.float64) for a in (b.xx, b.yy, b.zz)]
+    DP = C.POINTER(C.c_double)
+    out = capi.TransGeomMetricsOut(**{nm: arrs[nm].ctypes.data_as(DP) for nm in names})
+    lib = capi.load_library()
+    rc = lib.trans_geom_metrics(NI, NJ, NK, *(p.ctypes.data_as(DP) for p in pts), tiny, C.byref(out))
+    if rc == capi.TRANS_EINVAL:
+        raise ValueError("negative cell volume")
+    if rc != 0:
+        raise RuntimeError(f"trans_geom_metrics failed ({rc})")
+    dst = b.alloc("dst", fill=0.0)
+    _wall_distance(b)
+    J, K, I = sl(1, NJ1), sl(1, NK1), sl(1, NI1)
+    dst[O + 0, J, K] = dst[O + 1, J, K]; dst[O + NI, J, K] = dst[O + NI1, J, K]
+    dst[I, O + 0, K] = dst[I, O + 1, K]; dst[I, O + NJ, K] = dst[I, O + NJ1, K]
+    dst[I, J, O + 0] = dst[I, J, O + 1]; dst[I, J, O + NK] = dst[I, J, O + NK1]
 
 
 def _wall_panels(b: Block):
