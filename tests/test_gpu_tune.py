@@ -70,3 +70,30 @@ def test_autotune_keeps_results(name):
         c.close()
     a.close()
     b.close()
+
+
+def test_autotune_with_preconditioning():
+    """With IF_PRECONDITION = 1 only the hyperplane sweep carries the transforms: the tuner
+    tries it alone (streams and graphs), keeps it, the tuned context iterates as an untuned
+    one, and asking for another sweep fails."""
+    print("case: flat plate M=0.1, IF_Precondition=1, autotune", flush=True)
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=4, lo=9, hi=25, mach=0.1)
+    org.IF_Precondition = 1
+    org.CFL_Sub_start = org.CFL_Sub_end = 0.3
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=5, turb=True)
+    a = Solver(blocks, prm, ki=1)
+    assert a.tuning()["sweep"] == -1
+    chosen, table = a.autotune(iters=1)
+    assert {r["sweep"] for r in table} == {-1} and chosen["sweep"] == -1
+    assert {r["streams"] for r in table} == {1, 4} and {r["graph"] for r in table} == {0, 1}
+    with pytest.raises(Exception):
+        a.set_tuning(1, 4, 0)
+    b = Solver(blocks, prm, ki=1)
+    ra, rb = a.iterate(1), b.iterate(1)
+    for n, blk in enumerate(blocks):
+        e = relerr(interior(a.get_state(n)["v"], blk), interior(b.get_state(n)["v"], blk))
+        assert np.all(e < 1e-12), (n, e)
+        assert abs(ra[n]["drsa"] - rb[n]["drsa"]) <= 1e-12 * rb[n]["drsa"]
+    a.close()
+    b.close()
