@@ -819,9 +819,11 @@ __device__ __forceinline__ void norm_reduce(const double dr[3], int arg, NormPar
     }
 }
 
+// shift: 0, or the Kind_Dual of a fused dual-time history shift (TIME_Update_U, as
+// k_update_u) of the updated state, for the last sub-iteration of an outer iteration.
 __global__ void __launch_bounds__(256) k_update_v(trans_params p, int ki, int if_turb, Fields f,
                                                   const BlkInfo* __restrict__ blks,
-                                                  const Seg* __restrict__ seg, NormPart* part)
+                                                  const Seg* __restrict__ seg, NormPart* part, int shift)
 {
     const Seg s = seg[blockIdx.x];
     const BlkInfo b = blks[s.blk];
@@ -906,6 +908,14 @@ __global__ void __launch_bounds__(256) k_update_v(trans_params p, int ki, int if
             const double PPP = (1.4 - 1.0) * vn[4] - 0.5 * (1.4 - 1.0) * (VV / pRo - sqr(p.omega1 * rad) * pRo);
             const double TT = p.rxm2 * PPP / pRo;
             sst_kw_update(p, f, c, pRo, sutherland(p, TT));
+        }
+        if (shift) {
+#pragma unroll
+            for (int L = 0; L < TRANS_ML; L++) {
+                const long e = (long)L * tot + c;
+                if (shift == 2) f.Wt1[e] = f.Wt0[e];
+                f.Wt0[e] = f.V[e];
+            }
         }
     }
     norm_reduce(dr, arg, part, 0);

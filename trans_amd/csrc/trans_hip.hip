@@ -929,7 +929,7 @@ static void launch_sweeps(trans_ctx* c, R* dq)
 extern "C" {
 
 // LU-SGS (Time/TIME_LUSGS.F90, Time/TURBTWO_TIME_LUSGS_SST.F90) and Update_V (main.F90:135-150)
-static int lusgs_update_async(trans_ctx* c)
+static int lusgs_update_async(trans_ctx* c, int shift = 0)
 {
     const int nblk = (int)c->hb.size();
     const int if_turb = if_turb_now(c);
@@ -970,7 +970,7 @@ static int lusgs_update_async(trans_ctx* c)
             launch_sweeps<double, 5>(c, c->f.D);
     }
     KLAUNCH(c, "k_update_v", k_update_v, dim3(c->seg_cell.n), dim3(WG), c->p, c->ki, if_turb, c->f, c->d_blk,
-            c->seg_cell.d, c->d_part);
+            c->seg_cell.d, c->d_part, shift);
     KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk, c->d_resid);
     return launch_check(c, "lusgs_update");
 }
@@ -1054,11 +1054,15 @@ static int iteration_body(trans_ctx* c)
 {
     int rc;
     const int nsub = c->p.if_rk == 1 ? 3 : (c->p.kind_dual != 0 ? c->p.kchld : 1);
+    // with LU-SGS the dual-time shift of the last sub-iteration is fused into its update
+    const bool fuse = c->p.if_rk != 1 && (c->p.kind_dual == 1 || c->p.kind_dual == 2);
     for (int s = 0; s < nsub; s++) {
         if ((rc = trans_hip_residual(c))) return rc;
-        if ((rc = c->p.if_rk == 1 ? rk_update_async(c, s + 1) : lusgs_update_async(c))) return rc;
+        if ((rc = c->p.if_rk == 1 ? rk_update_async(c, s + 1)
+                                  : lusgs_update_async(c, fuse && s == nsub - 1 ? c->p.kind_dual : 0)))
+            return rc;
     }
-    if (c->p.kind_dual != 0)
+    if (c->p.kind_dual != 0 && !fuse)
         if ((rc = trans_hip_update_u(c))) return rc;
     return 0;
 }
