@@ -1681,3 +1681,56 @@ def test_precond_update_v_is_conservative():
     ke_old_rho = 0.5 * ((v0[1] + d[1]) ** 2 + (v0[2] + d[2]) ** 2 + (v0[3] + d[3]) ** 2)[I, J, K] / v0[0][I, J, K]
     np.testing.assert_allclose(P[I, J, K], 0.4 * ((v0[4] + d[4])[I, J, K] - ke_old_rho), rtol=1e-13)
     np.testing.assert_allclose(T[I, J, K], prm.rxm2 * p0[I, J, K] / V[0][I, J, K], rtol=1e-15)
+
+
+# ---------------------------------------------------------------------------------------
+# SST branch of Update_V (Time/TIME_Update_V.F90:640-690): the eddy-viscosity limiter
+
+def _sst_update_case(G, rho_k, rho_w, dst):
+    b, prm, edges = _box(IF_Euler=0, IF_Turb_Org=1, KItm=1, Kind_Model=1)
+    b["dst"][...] = dst
+    oc = OracleCase([b], prm)
+    V = oc[0]["v"]
+    V[0] = 1.3  # rho != 1, so that a misplaced density shows
+    V[1:4] = 0.0
+    V[4] = oc[0]["pp"] / 0.4
+    oc[0]["t"][...] = prm.rxm2 * oc[0]["pp"] / V[0]
+    V[5] = rho_k
+    V[6] = rho_w
+    oc[0]["D"][...] = 0.0
+    oc[0]["DQdxyz"][...] = 0.0
+    if callable(G):
+        G = G(prm)
+    oc[0]["DQdxyz"][1] = G  # du/dy: pure shear, S = |G|, production G^2
+    oc.call("ora_update_v", 0, 1, 1)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    return prm, oc, (I, J, K), G
+
+
+def test_sst_limiter_bradshaw_in_strong_shear():
+    """Near a wall (F2 = 1) and in shear strong enough that a1 omega < S F2, the SST limiter
+    gives Bradshaw's relation: the shear stress mu_t S / Re equals a1 rho k (Menter 2003,
+    mu_t = a1 rho k / max(a1 omega, S F2)), while omega and k are left as they were (the
+    production floor and the k cap are not active here)."""
+    rho_w = 1e-3
+    # Cmua = 1/2; a1 < prod_lmt beta* keeps the omega floor below omega at this shear
+    prm, oc, (I, J, K), G = _sst_update_case(lambda p: 2.0 * p.a1 * rho_w * p.ref, 1e-4, rho_w, 1e-6)
+    assert prm.a1 < prm.prod_lmt * prm.beta_star
+    V = oc[0]["v"]
+    rho = V[0][I, J, K]
+    np.testing.assert_allclose(V[6][I, J, K], rho_w, rtol=1e-15)
+    np.testing.assert_allclose(V[5][I, J, K], 1e-4, rtol=1e-15)
+    np.testing.assert_allclose(oc[0]["rmiu"][I, J, K] * G / prm.ref, prm.a1 * V[5][I, J, K], rtol=1e-12)
+    assert np.allclose(rho, 1.3, rtol=1e-15)
+
+
+def test_sst_limiter_mild_shear_and_cap():
+    """Far from walls (F2 -> 0) the limiter is idle: mu_t = rho k / omega.  With a huge k the
+    cap k <= 1e5 mu_L omega / (rho Cmua) makes mu_t exactly 1e5 mu_L."""
+    prm, oc, (I, J, K), _ = _sst_update_case(1.0, 1e-4, 1e-2, 1e3)
+    V = oc[0]["v"]
+    np.testing.assert_allclose(oc[0]["rmiu"][I, J, K], (V[5] / V[6] * V[0])[I, J, K], rtol=1e-12)
+    prm, oc, (I, J, K), _ = _sst_update_case(1.0, 1e6, 1e-2, 1e3)
+    tt = oc[0]["t"][I, J, K]
+    mul = (1 + prm.csthlnd) / (tt * prm.csth1 + prm.csthlnd) * (prm.csth1 * tt) ** 1.5
+    np.testing.assert_allclose(oc[0]["rmiu"][I, J, K], 1e5 * mul, rtol=1e-12)
