@@ -1734,3 +1734,14 @@ def test_sst_limiter_mild_shear_and_cap():
     tt = oc[0]["t"][I, J, K]
     mul = (1 + prm.csthlnd) / (tt * prm.csth1 + prm.csthlnd) * (prm.csth1 * tt) ** 1.5
     np.testing.assert_allclose(oc[0]["rmiu"][I, J, K], 1e5 * mul, rtol=1e-12)
+
+
+def test_sst_omega_floor_balances_production():
+    """With omega far below the production limit, the floor raises it until the modelled
+    production mu_t S^2 equals prod_lmt times the destruction beta* rho k omega (in this
+    nondimensionalisation: Rmiu * prod = prod_lmt beta* (rho k)(rho omega) Re^2)."""
+    prm, oc, (I, J, K), G = _sst_update_case(1.0e3, 1e-4, 1e-8, 1e3)
+    V = oc[0]["v"]
+    assert np.all(V[6][I, J, K] > 1e-6)
+    np.testing.assert_allclose(oc[0]["rmiu"][I, J, K] * G * G,
+                               prm.prod_lmt * prm.beta_star * (V[5] * V[6])[I, J, K] * prm.ref ** 2, rtol=1e-12)
