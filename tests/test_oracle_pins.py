@@ -1745,3 +1745,30 @@ def test_sst_omega_floor_balances_production():
     assert np.all(V[6][I, J, K] > 1e-6)
     np.testing.assert_allclose(oc[0]["rmiu"][I, J, K] * G * G,
                                prm.prod_lmt * prm.beta_star * (V[5] * V[6])[I, J, K] * prm.ref ** 2, rtol=1e-12)
+
+
+def test_sst_sources_at_zero_strain_near_a_wall():
+    """As the zero-strain test, but a vanishing wall distance drives F1 to 1 exactly: the
+    blended coefficients become the inner (k-omega) set, so omega is destroyed with beta_1
+    instead of beta_2 (Time/TURBTWO_TIME_LUSGS_SST.F90:139-415)."""
+    b, prm, edges = _box(ni=5, nj=5, nk=5, IF_Euler=0, IF_Turb_Org=1)
+    b["dst"][...] = 1e-7
+    oc = OracleCase([b], prm)
+    V = oc[0]["v"]
+    V[1:4] = 0.0
+    oc[0]["wt0"][...] = V
+    oc[0]["wt1"][...] = V
+    oc[0]["Rds"][...] = 0.0
+    oc[0]["Dtm"][...] = 0.7
+    oc[0]["D"][...] = 0.0
+    oc[0]["DQdxyz"][...] = 0.0
+    oc.call("ora_turb_lusgs_sst", 0)
+    rho, rk, rw = V[0], V[5], V[6]
+    om = rw / rho
+    Re = prm.ref
+    assert prm.beta1 != prm.beta2
+    stept = 2.0 * 0.7 / (3.0 * 0.7 / prm.dt_lusgs_main + 2.0)
+    f = 1.0 + (prm.beta_star * om * Re + 2.0 * prm.beta1 * om * Re) * stept
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    np.testing.assert_allclose(oc[0]["D"][5][I, J, K], (-prm.beta_star * om * Re * rk * stept / f)[I, J, K], rtol=1e-12)
+    np.testing.assert_allclose(oc[0]["D"][6][I, J, K], (-prm.beta1 * om * Re * rw * stept / f)[I, J, K], rtol=1e-12)
