@@ -1772,3 +1772,38 @@ def test_sst_sources_at_zero_strain_near_a_wall():
     I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
     np.testing.assert_allclose(oc[0]["D"][5][I, J, K], (-prm.beta_star * om * Re * rk * stept / f)[I, J, K], rtol=1e-12)
     np.testing.assert_allclose(oc[0]["D"][6][I, J, K], (-prm.beta1 * om * Re * rw * stept / f)[I, J, K], rtol=1e-12)
+
+
+def test_sst_cross_diffusion_far_from_walls():
+    """Far from walls (F1 = 0) the omega equation gains the cross-diffusion term
+    2 (1 - F1) rho sigma_w2 (grad k . grad omega) / omega / Re (below its clip at half the
+    production here): the omega increment grows by exactly that term over the implicit
+    diagonal, while the k increment does not change."""
+    def run(cdif):
+        b, prm, edges = _box(ni=5, nj=5, nk=5, IF_Euler=0, IF_Turb_Org=1)
+        oc = OracleCase([b], prm)
+        V = oc[0]["v"]
+        V[1:4] = 0.0
+        oc[0]["wt0"][...] = V
+        oc[0]["wt1"][...] = V
+        oc[0]["Rds"][...] = 0.0
+        oc[0]["Dtm"][...] = 0.7
+        oc[0]["D"][...] = 0.0
+        Q = oc[0]["DQdxyz"]
+        Q[...] = 0.0
+        Q[1] = 50.0  # du/dy: production
+        Q[12] = 1.0  # d(k)/dx
+        Q[15] = cdif  # d(omega)/dx
+        oc.call("ora_turb_lusgs_sst", 0)
+        return b, prm, V, oc[0]["D"][5].copy(), oc[0]["D"][6].copy()
+
+    b, prm, V, k0, w0 = run(0.0)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    rho, om, Re = V[0], V[6] / V[0], prm.ref
+    cdif = 0.1 * prm.gama2 * 50.0 ** 2 * float(om[I, J, K].min()) / prm.sigo2
+    b, prm, V, k1, w1 = run(cdif)
+    stept = 2.0 * 0.7 / (3.0 * 0.7 / prm.dt_lusgs_main + 2.0)
+    f = 1.0 + (prm.beta_star * om * Re + 2.0 * prm.beta2 * om * Re) * stept
+    cross = 2.0 * rho * prm.sigo2 / om * cdif / Re
+    np.testing.assert_array_equal(k1[I, J, K], k0[I, J, K])
+    np.testing.assert_allclose((w1 - w0)[I, J, K], (cross * stept / f)[I, J, K], rtol=1e-9)
