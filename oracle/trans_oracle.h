@@ -14,7 +14,7 @@
  * Every routine is pinned by tests/test_oracle_pins.py against closed forms, invariants or
  * brute force, except for these parts, whose parity is unpinned (checked GPU-vs-oracle
  * only): the crossed/reversed orientations of the block-interface copy (ora_bc, lcross = 1
- * or revMatrix != 0), the SST blend for 0 < F1 < 1 and cross-diffusion clip, the length-scale
+ * or revMatrix != 0), the SST blend for 0 < F1 < 1 (both ends pinned), the length-scale
  * blend coefficients of Fscheme inside its blended range (its limits and monotonicity are
  * pinned), and the density and energy rows of ora_precond_dw (PrecondDW; only its
  * structure is pinned: momentum passes, linear in the increment, affine in (U_r/c)^2).

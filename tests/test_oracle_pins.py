@@ -1776,9 +1776,9 @@ def test_sst_sources_at_zero_strain_near_a_wall():
 
 def test_sst_cross_diffusion_far_from_walls():
     """Far from walls (F1 = 0) the omega equation gains the cross-diffusion term
-    2 (1 - F1) rho sigma_w2 (grad k . grad omega) / omega / Re (below its clip at half the
-    production here): the omega increment grows by exactly that term over the implicit
-    diagonal, while the k increment does not change."""
+    2 (1 - F1) rho sigma_w2 (grad k . grad omega) / omega / Re: the omega increment grows by
+    exactly that term over the implicit diagonal, while the k increment does not change; a
+    term beyond half the omega production is clipped to it."""
     def run(cdif):
         b, prm, edges = _box(ni=5, nj=5, nk=5, IF_Euler=0, IF_Turb_Org=1)
         oc = OracleCase([b], prm)
@@ -1807,3 +1807,7 @@ def test_sst_cross_diffusion_far_from_walls():
     cross = 2.0 * rho * prm.sigo2 / om * cdif / Re
     np.testing.assert_array_equal(k1[I, J, K], k0[I, J, K])
     np.testing.assert_allclose((w1 - w0)[I, J, K], (cross * stept / f)[I, J, K], rtol=1e-9)
+    # a cross term beyond half the omega production (gamma_2 rho S^2 / Re) is clipped there
+    b, prm, V, k2, w2 = run(1e3 * cdif)
+    clip = 0.5 * prm.gama2 * rho * 50.0 ** 2 / Re
+    np.testing.assert_allclose((w2 - w0)[I, J, K], (clip * stept / f)[I, J, K], rtol=1e-9)
