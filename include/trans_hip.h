@@ -33,6 +33,7 @@ extern "C" {
 #define TRANS_EHIP (-2)
 #define TRANS_ENOMEM (-3)
 #define TRANS_EUNSUPPORTED (-4)
+#define TRANS_ESCRATCH (-5) /* input larger than a static staging array (trans_geom_wall_distance_static) */
 #define TRANS_ENONFINITE 1 /* a residual norm came out NaN (Time/TIME_Update_V.F90:124-151) */
 
 /*
@@ -304,6 +305,18 @@ int trans_hip_set_tuning(trans_ctx* ctx, const trans_tune_entry* t);
  * layout) of the squared distance, dst = sqrt(max(min, 1e-20)); 1e3 everywhere without
  * panels.  Host pointers; no context needed.  Bitwise the same as the host computation. */
 int trans_geom_wall_distance(const double* cells, long ncell, const double* panels, long npanel, double* dst);
+
+/* trans_geom_wall_distance without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving loops
+ * that must not call hipMalloc, harnesses that forbid it).  Same arguments and bitwise the same
+ * result as trans_geom_wall_distance, but cells, panels and the squared minima are staged in a
+ * static device array of TRANS_WD_STATIC_DOUBLES doubles (32 MiB; one copy per GPU, held until
+ * the process ends), and the work runs on the null stream of the current device.  Requires
+ * 4 ncell + 3 npanel <= TRANS_WD_STATIC_DOUBLES, else returns TRANS_ESCRATCH without touching
+ * the device.  Calls that overlap on one GPU (from several host threads) share the array and
+ * race; serialise them. */
+#define TRANS_WD_STATIC_DOUBLES (4L << 20)
+int trans_geom_wall_distance_static(const double* cells, long ncell, const double* panels, long npanel, double* dst);
 
 /* Outputs of trans_geom_metrics: host arrays of one block in the padded layout (bounds
  * -2:N+2 on each axis, i fastest, component first for sd (9: direction fastest, then x, y,

@@ -100,7 +100,8 @@ class TransPlaneIntegrals(C.Structure):
 
 
 FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK, FIELD_FSCHEME = 0, 1, 2, 3, 4, 5
-TRANS_EINVAL, TRANS_EHIP, TRANS_ENOMEM, TRANS_EUNSUPPORTED = -1, -2, -3, -4
+TRANS_EINVAL, TRANS_EHIP, TRANS_ENOMEM, TRANS_EUNSUPPORTED, TRANS_ESCRATCH = -1, -2, -3, -4, -5
+TRANS_WD_STATIC_DOUBLES = 4 << 20
 
 
 class TransSrvConfig(C.Structure):
@@ -154,7 +155,7 @@ EXPORTS = [
     "trans_hip_autotune", "trans_hip_get_tuning", "trans_hip_set_tuning",
     "trans_srv_create", "trans_srv_submit", "trans_srv_progress", "trans_srv_poll", "trans_srv_cancel",
     "trans_srv_result", "trans_srv_release", "trans_srv_destroy", "trans_srv_last_error",
-    "trans_geom_wall_distance", "trans_geom_metrics",
+    "trans_geom_wall_distance", "trans_geom_metrics", "trans_geom_wall_distance_static",
 ]
 
 
@@ -227,6 +228,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_set_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]
     lib.trans_hip_device_bytes.argtypes = [C.c_void_p, P(C.c_size_t)]
     lib.trans_geom_wall_distance.argtypes = [P(C.c_double), C.c_long, P(C.c_double), C.c_long, P(C.c_double)]
+    lib.trans_geom_wall_distance_static.argtypes = [P(C.c_double), C.c_long, P(C.c_double), C.c_long,
+                                                    P(C.c_double)]
     lib.trans_geom_metrics.argtypes = [C.c_int, C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double),
                                        C.c_double, P(TransGeomMetricsOut)]
     lib.trans_srv_create.argtypes = [P(TransSrvConfig), P(C.c_void_p)]

@@ -1,5 +1,71 @@
 // Host side of the device geometry pre-processing (k_geom.h).
+#include <mutex>
+
+// Staging memory of trans_geom_wall_distance_static: the cell centres (3 ncell), the panel
+// centres (3 npanel) and the squared minima (ncell) of one call, in that order.  Each GPU the
+// library runs on holds its own copy until the process ends.  Host code never passes the
+// name `scratch` to a kernel or a copy (on the host it is only a stand-in for the symbol):
+// it uses the per-device address from hipGetSymbolAddress below.
+static __device__ __align__(256) double scratch[TRANS_WD_STATIC_DOUBLES];
+
+namespace {
+constexpr int WD_MAX_DEVICES = 64;
+std::mutex wd_scratch_mu;
+double* wd_scratch_ptr[WD_MAX_DEVICES] = {};
+
+// Address of the current device's copy of `scratch`, looked up once per device with that
+// device current.  nullptr on failure.
+double* wd_scratch_for_current_device()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= WD_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(wd_scratch_mu);
+    if (!wd_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        wd_scratch_ptr[dev] = (double*)p;
+    }
+    return wd_scratch_ptr[dev];
+}
+}  // namespace
+
 extern "C" {
+
+int trans_geom_wall_distance_static(const double* cells, long ncell, const double* panels, long npanel, double* dst)
+{
+    if (!cells || !dst || ncell < 0 || npanel < 0 || (npanel > 0 && !panels)) return TRANS_EINVAL;
+    // 4 ncell + 3 npanel doubles must fit the static staging array
+    if (ncell > TRANS_WD_STATIC_DOUBLES || npanel > TRANS_WD_STATIC_DOUBLES ||
+        4 * ncell + 3 * npanel > TRANS_WD_STATIC_DOUBLES)
+        return TRANS_ESCRATCH;
+    if (ncell == 0) return 0;
+    if (npanel == 0) {  // no wall anywhere: the distance is never used
+        for (long c = 0; c < ncell; c++) dst[c] = 1.0e3;
+        return 0;
+    }
+    double* base = wd_scratch_for_current_device();
+    if (!base) return TRANS_EHIP;
+    double* dc = base;                // 3 ncell
+    double* dp = base + 3 * ncell;    // 3 npanel
+    double* dd = dp + 3 * npanel;     // ncell, every entry written by the kernel
+    // the null stream: no stream is created (creating one allocates)
+    const hipStream_t s = 0;
+    if (hipMemcpyAsync(dc, cells, 3 * ncell * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dp, panels, 3 * npanel * sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess)
+        return TRANS_EHIP;
+    const long per = 256L * WD_CPT;
+    hipLaunchKernelGGL(k_wall_distance, dim3((unsigned)((ncell + per - 1) / per)), dim3(256), 0, s, dc, ncell, dp,
+                       npanel, dd);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(dst, dd, ncell * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return TRANS_EHIP;
+    for (long c = 0; c < ncell; c++) dst[c] = std::sqrt(std::max(dst[c], 1.0e-20));
+    return 0;
+}
 
 int trans_geom_wall_distance(const double* cells, long ncell, const double* panels, long npanel, double* dst)
 {
