@@ -82,9 +82,14 @@ def build_workload(name, implicit_fp32=0):
         # (wall-normal stretching 1.0157 so that 512 cells from y+ = 1 reach the plate length)
         blocks, org, glob = cases.flat_plate(nblk=16, ni_total=1025, nj=513, nk=17, lo=33, hi=97, ratio=1.0157)
         desc = "flat plate 1025x513x17 (8.4M cells) in 16 uneven blocks, RANS-SST, AUSMPW+ + MDCD, LU-SGS CFL 5"
+    elif name == "compressor":
+        blocks, org, glob = cases.compressor_sector()
+        desc = ("compressor-rotor sector, 16 blocks (8.13M cells), 1200 rad/s, periodic 10 deg, RANS-SST, "
+                "Roe + MUSCL, LU-SGS CFL 5")
     else:
         raise SystemExit(f"unknown workload {name}")
-    prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32, device_geometry=name == "plate8m")
+    prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32,
+                              device_geometry=name in ("plate8m", "compressor"))
     return blocks, prm, desc
 
 
@@ -195,13 +200,15 @@ def measured_traffic(workload, kernel):
     return None
 
 
-def timed_run(blocks, prm, steps, warmup, prof_steps, tune=0):
+def timed_run(blocks, prm, steps, warmup, prof_steps, tune=0, force=None):
     """Warm up, let the library's autotuner pick the launch configuration (`tune` timed
     iterations per candidate, state restored afterwards; 0 keeps the default), time `steps`
     iterations without instrumentation, then profile `prof_steps` more with HIP events on the
     library's stream."""
     from trans_amd.solver import Solver
     sv = Solver(blocks, prm, ki=1)
+    if force is not None:
+        sv.set_tuning(*force)
     sv.iterate(warmup, want_resid=False)
     sv.sync()
     timed_run.tuning = None
@@ -268,6 +275,8 @@ def main():
                     help="autotuner iterations per launch configuration before timing (0: default configuration)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup+steps of the FP64 path only and print the kernel table")
+    ap.add_argument("--sweep", default=None, metavar="S[,STREAMS,GRAPH,TA,TB,CHUNK]",
+                    help="with --profile-only: force a launch configuration (trans_hip_set_tuning)")
     ap.add_argument("--implicit-fp32", action="store_true",
                     help="with --profile-only: profile the FP32-implicit path instead")
     ap.add_argument("--converge", type=int, default=0,
@@ -304,7 +313,12 @@ def main():
     prof_steps = min(args.steps, 50)
     # a profiled run (--profile-only) keeps the default launch configuration: the one the
     # tuner picks for the flat plate, without its graph captures under the profiler
-    dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps, 0 if args.profile_only else args.tune)
+    force = None
+    if args.sweep:
+        v = [int(x) for x in args.sweep.split(",")]
+        force = tuple(v + [4, 1, 16, 16, 8][len(v) - 1:])[:6]
+    dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps,
+                               0 if args.profile_only else args.tune, force)
     tuning = timed_run.tuning
     tag = "FP32 implicit" if prm.implicit_fp32 else "FP64"
     print(f"[bench] {tag}: {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)

@@ -34,6 +34,7 @@ extern "C" {
 #define TRANS_ENOMEM (-3)
 #define TRANS_EUNSUPPORTED (-4)
 #define TRANS_ESCRATCH (-5) /* input larger than a static staging array (trans_geom_wall_distance_static) */
+#define TRANS_ETIMEOUT (-6) /* a bounded wait of the tile-wavefront sweep expired (results invalid) */
 #define TRANS_ENONFINITE 1 /* a residual norm came out NaN (Time/TIME_Update_V.F90:124-151) */
 
 /*
@@ -281,18 +282,23 @@ int trans_hip_read_stats(trans_ctx* ctx, int blk, const char* path);
 /* ---- Launch autotuner (no counterpart in the reference) ------------------------------------
  * Times `iters` iterations of every launch configuration of the context from the same saved
  * state and keeps the fastest: the LU-SGS sweep (sweep = 0, 1, 2: line march along i, j or k,
- * where the cross sections fit one workgroup; -2: one launch per hyperplane over all blocks;
- * -1: one workgroup per block walking its hyperplanes), helper streams for the
+ * where the cross sections fit one workgroup; 3: tile wavefront over many CUs with the tile
+ * shapes tile_a x tile_b and progress chunks below; -2: one launch per hyperplane over all
+ * blocks; -1: one workgroup per block walking its hyperplanes), helper streams for the
  * face kernels (streams = 4) or one stream (1), HIP-graph replay (graph = 1) or eager launches.
- * Every configuration computes bitwise the same iteration.  The state and KI are restored;
+ * Every configuration computes the same iteration (the sweep variants contract multiply-adds
+ * in different places, so they agree within rounding, not bitwise).  The state and KI are restored;
  * out (max_out entries, may be NULL) receives the timings, *n_out their count.  The sweep
  * buffer is reallocated for the chosen axis, so the context's device bytes may then differ
  * from trans_hip_footprint's.  get_tuning reports the configuration in use. */
 typedef struct trans_tune_entry {
-    int sweep;
+    int sweep;     /* 0, 1, 2 line march along i, j, k; 3 tile wavefront; -2 one launch per hyperplane;
+                      -1 one workgroup per block */
     int streams;
     int graph;
     double ms_per_iter;
+    int tile_a, tile_b; /* sweep 3: lines of a tile along i and along k (tile_a * tile_b <= 256) */
+    int chunk;          /* sweep 3: steps between a tile's progress stores */
 } trans_tune_entry;
 int trans_hip_autotune(trans_ctx* ctx, int iters, trans_tune_entry* out, int max_out, int* n_out);
 int trans_hip_get_tuning(const trans_ctx* ctx, trans_tune_entry* out);

@@ -1,0 +1,186 @@
+"""GPU parity of the tile-wavefront LU-SGS sweep (sweep mode 3) against the oracle.
+
+The tile wavefront cuts every block's lines along j into tiles of tile_a x tile_b lines, one
+workgroup each, handing the terms that cross a tile edge through global memory; the tests use
+tile shapes that leave partial tiles at the block edges and several tiles along both axes, and
+progress chunks of 1, 3 and 8 steps.  Bars: one LU-SGS + update within 1e-12 with the norms'
+argmax locations exact (mean flow and k-omega, FP64 operator), the FP32 operator within 1e-5
+of the FP64 increment, ten iterations within 1e-6; the Runge-Kutta path's k-omega sweeps
+alone (NEQ = 2) within 1e-12 for one stage.  Single updates start from the oracle's BC(-1)
+ghosts (the device residual then skips its own BC(-1), whose inlet pow may differ in the last
+bit and switch an ALIMITER branch); the tile result is also held to 1e-12 of the device's
+line-march sweep from the same device residual.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleCase
+from trans_amd import capi, cases, solver
+from trans_amd.solver import Solver
+
+from helpers import interior, perturb, relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+SHAPES = [(4, 2, 1), (5, 3, 3), (16, 16, 8), (64, 4, 4), (3, 1, 8)]
+
+
+def _post_bc(blocks, prm):
+    """The oracle's BC(-1) ghosts written into the blocks, so that device and oracle start the
+    residual from identical bits (a last-bit difference of a device pow in an inlet ghost can
+    switch an ALIMITER branch of the k-omega reconstruction)."""
+    oc = OracleCase(blocks, prm)
+    oc.spectral_radius(1)
+    oc.bc(1, -1)
+    for n, b in enumerate(blocks):
+        for nm in ("v", "pp", "t", "rmiu"):
+            b[nm][...] = oc[n][nm]
+    return blocks, prm
+
+
+def _plate(**deck):
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=6, lo=9, hi=25)
+    for k, v in deck.items():
+        setattr(org, k, v)
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=13, turb=True)
+    return _post_bc(blocks, prm)
+
+
+def _sector(seed=17):
+    blocks, org, glob = cases.compressor_sector(scale=1 / 16, crossed=True)
+    prm = solver.prepare_case(blocks, org, glob)
+    perturb(blocks, prm, seed=seed, turb=True)
+    return _post_bc(blocks, prm)
+
+
+def _check_update(blocks, sv, oc, gres, ores, tol=TOL):
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        for nm in ("v", "pp", "t", "rmiu"):
+            e = relerr(interior(st[nm], b), interior(oc[n][nm], b))
+            assert np.all(np.asarray(e) < tol), (n, nm, e)
+        for key in ("drsa", "dua", "drsk", "duk", "drso", "duo"):
+            assert abs(gres[n][key] - ores[n][key]) <= tol * abs(ores[n][key]), (n, key)
+        for key in ("ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo"):
+            assert gres[n][key] == ores[n][key], (n, key)
+
+
+@pytest.mark.parametrize("ta,tb,chunk", SHAPES)
+@pytest.mark.parametrize("case", ["plate", "sector"])
+def test_tile_sweep_one_update(case, ta, tb, chunk):
+    print(f"case: {case}, tile wavefront {ta}x{tb} chunk {chunk}, one LU-SGS update", flush=True)
+    blocks, prm = _plate() if case == "plate" else _sector()
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    sv.set_tuning(3, 4, 0, ta, tb, chunk)
+    assert sv.tuning()["sweep"] == 3
+    oc.residual(1)  # its BC(-1) is idempotent on the filled halo
+    sv.spectral_radius()
+    sv.space()
+    ores = oc.lusgs_update(1)
+    gres = sv.lusgs_update()
+    sv.sync()
+    _check_update(blocks, sv, oc, gres, ores)
+    # and against the line-march sweep on the device
+    ln = Solver(blocks, prm, ki=1)
+    ln.set_tuning(1, 4, 0)
+    ln.spectral_radius()
+    ln.space()
+    lres = ln.lusgs_update()
+    for n, b in enumerate(blocks):
+        e = relerr(interior(sv.get_state(n)["v"], b), interior(ln.get_state(n)["v"], b))
+        assert np.all(e < TOL), (n, "tile vs line", e)
+        assert abs(gres[n]["drsa"] - lres[n]["drsa"]) <= TOL * lres[n]["drsa"]
+    ln.close()
+    sv.close()
+
+
+def test_tile_sweep_fp32_and_ten_iterations():
+    print("case: sector, tile wavefront 8x4, FP32 operator and 10 iterations", flush=True)
+    blocks, prm = _sector()
+    oc = OracleCase(blocks, prm)
+    v0 = [b["v"].copy() for b in blocks]
+    p32 = capi.TransParams.from_buffer_copy(prm)
+    p32.implicit_fp32 = 1
+    sv = Solver(blocks, p32, ki=1)
+    sv.set_tuning(3, 4, 0, 8, 4, 4)
+    oc.residual(1)
+    sv.spectral_radius()
+    sv.space()
+    oc.lusgs_update(1)
+    sv.lusgs_update()
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        e = relerr(interior(st["v"], b) - interior(v0[n], b), interior(oc[n]["v"], b) - interior(v0[n], b))
+        assert np.all(e < 1e-5), (n, e)
+    sv.close()
+    # ten iterations (BC included): the seed of the rotating-sector tests, whose line-march run
+    # holds 1e-6 against the oracle; and the line march on the device from the same inputs
+    blocks, prm = _sector(seed=20261202)
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    sv.set_tuning(3, 4, 1, 8, 4, 2)
+    ln = Solver(blocks, prm, ki=1)
+    ln.set_tuning(1, 4, 1)
+    oc.iterate(1, 10)
+    gres = sv.iterate(10)
+    lres = ln.iterate(10)
+    ores = oc.resid()
+    for n, b in enumerate(blocks):
+        e = relerr(interior(sv.get_state(n)["v"], b), interior(oc[n]["v"], b))
+        assert np.all(e < 1e-6), (n, e)
+        e = relerr(interior(sv.get_state(n)["v"], b), interior(ln.get_state(n)["v"], b))
+        assert np.all(e < 1e-9), (n, "tile vs line", e)
+        for key in ("drsa", "drsk", "drso"):
+            assert abs(gres[n][key] - ores[n][key]) <= 1e-6 * ores[n][key], (n, key)
+            assert abs(gres[n][key] - lres[n][key]) <= 1e-9 * lres[n][key], (n, key)
+    sv.close()
+    ln.close()
+
+
+def test_tile_sweep_rk_kw_alone():
+    """IF_RK = 1: the k-omega sweeps run alone (NEQ = 2 table) inside each stage."""
+    print("case: plate, RK3 stage with the tile wavefront k-omega sweeps", flush=True)
+    blocks, prm = _plate(IF_RK=1, CFL_Sub_start=0.8, CFL_Sub_end=0.8)
+    oc = OracleCase(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    sv.set_tuning(3, 4, 0, 5, 3, 2)
+    oc.residual(1)
+    sv.spectral_radius()
+    sv.space()
+    ores = oc.rk_update(1, 1)
+    gres = sv.rk_update(1)
+    for n, b in enumerate(blocks):
+        st = sv.get_state(n)
+        for nm in ("v", "pp", "t", "rmiu"):
+            e = relerr(interior(st[nm], b), interior(oc[n][nm], b))
+            assert np.all(np.asarray(e) < TOL), (n, nm, e)
+        for key in ("drsa", "drsk", "drso"):
+            assert abs(gres[n][key] - ores[n][key]) <= TOL * abs(ores[n][key]), (n, key)
+    sv.close()
+
+
+def test_tile_sweep_is_default_for_large_cross_sections():
+    """Blocks whose cross sections along every axis exceed one line-march workgroup (1024
+    lines) take the tile wavefront."""
+    print("case: plate 80x40x40 cells in 2 blocks, default sweep", flush=True)
+    blocks, org, glob = cases.flat_plate(nblk=2, ni_total=81, nj=41, nk=41, lo=35, hi=45)
+    prm = solver.prepare_case(blocks, org, glob)
+    for b in blocks:
+        n = (b.ni - 1, b.nj - 1, b.nk - 1)
+        assert min(n[0] * n[1], n[1] * n[2], n[0] * n[2]) > 1024
+    # perturbed, so that no two cells tie for the largest increment
+    perturb(blocks, prm, seed=19, turb=True)
+    _post_bc(blocks, prm)
+    sv = Solver(blocks, prm, ki=1)
+    assert sv.tuning()["sweep"] == 3
+    oc = OracleCase(blocks, prm)
+    oc.residual(1)
+    sv.spectral_radius()
+    sv.space()
+    ores = oc.lusgs_update(1)
+    gres = sv.lusgs_update()
+    _check_update(blocks, sv, oc, gres, ores)
+    sv.close()

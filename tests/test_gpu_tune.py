@@ -56,7 +56,7 @@ def test_autotune_keeps_results(name):
     for row in table:
         c = Solver(blocks, prm, ki=1)
         c.iterate(2, want_resid=False)
-        c.set_tuning(row["sweep"], row["streams"], row["graph"])
+        c.set_tuning(row["sweep"], row["streams"], row["graph"], row["tile_a"], row["tile_b"], row["chunk"])
         c1 = c.iterate(1)
         for n, blk in enumerate(blocks):
             e = relerr(interior(c.get_state(n)["v"], blk), interior(v1[n], blk))

@@ -100,7 +100,7 @@ class TransPlaneIntegrals(C.Structure):
 
 
 FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK, FIELD_FSCHEME = 0, 1, 2, 3, 4, 5
-TRANS_EINVAL, TRANS_EHIP, TRANS_ENOMEM, TRANS_EUNSUPPORTED, TRANS_ESCRATCH = -1, -2, -3, -4, -5
+TRANS_EINVAL, TRANS_EHIP, TRANS_ENOMEM, TRANS_EUNSUPPORTED, TRANS_ESCRATCH, TRANS_ETIMEOUT = -1, -2, -3, -4, -5, -6
 TRANS_WD_STATIC_DOUBLES = 4 << 20
 
 
@@ -132,7 +132,8 @@ class TransGeomMetricsOut(C.Structure):
 
 
 class TransTuneEntry(C.Structure):
-    _fields_ = [("sweep", C.c_int), ("streams", C.c_int), ("graph", C.c_int), ("ms_per_iter", C.c_double)]
+    _fields_ = [("sweep", C.c_int), ("streams", C.c_int), ("graph", C.c_int), ("ms_per_iter", C.c_double),
+                ("tile_a", C.c_int), ("tile_b", C.c_int), ("chunk", C.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -328,13 +328,17 @@ def compressor_sector(scale=1.0, nblade=36, rpm=1200.0 * 60.0 / (2.0 * math.pi),
     pressure surfaces (walls rotating with the frame, SubBCType 0); the hub rotates (SubBCType
     0), the casing is stationary (SubBCType 1: ghost velocity omega x r in the relative frame,
     BC/BC_Wall.F90:41-90).  Inlet at U's imin (BC 5), outlet at D2's imax (BC 7) spanning hub to
-    tip so that its radial-equilibrium march covers the full span.
+    tip so that its radial-equilibrium march covers the full span.  The deck ramps the CFL
+    number from 1 to 5 over the first 50 iterations (Time/SpectralRadius.F90:159-160): from the
+    uniform start the full-scale grid diverges within ten iterations at CFL 5, and runs at CFL
+    2 or with the ramp.
 
     Cells at scale 1: segments 48, 128, 128, 128, 64 along x, 128 radial, 128 circumferential:
     blocks of 49x129x65 (2), 129x65x65 (12) and 65x129x65 (2) points, 8,126,464 cells.
     `crossed` stores the tip-side pitch-max block of B2 with its i and k axes exchanged and k
     reversed, so that its interfaces need lcross and lrev.  Returns (blocks, OrgDeck,
-    GlobalInfo): Roe + MUSCL, RANS-SST, LU-SGS at CFL 5, dual time with one sub-iteration."""
+    GlobalInfo): Roe + MUSCL, RANS-SST, LU-SGS at CFL 5 (after the ramp), dual time with one
+    sub-iteration."""
     from .params import GlobalInfo, OrgDeck
     if seg is None:
         seg = [max(3, int(round(n * scale))) for n in (48, 128, 128, 128, 64)]
@@ -433,8 +437,8 @@ def compressor_sector(scale=1.0, nblade=36, rpm=1200.0 * 60.0 / (2.0 * math.pi),
 
     blocks = assemble_blocks(specs, point_fn, (NI, nj, nk), physical_bc, periodic_axis=2, periodic_ok=periodic_ok)
     org = OrgDeck(XM=mach, Tinf=288.0, Kind_Spatial=2, Kind_Roe=1, IF_Euler=0, IF_Turb_Org=1, Kind_Model=1,
-                  IF_RK=0, Kind_Dual=2, Kchld=1, DT_LUSGS_Main=1.0e10, CFL_Sub_start=5.0, CFL_Sub_end=5.0,
-                  CFL_Sub_step=1, KItm=1, Prod_Lmt=20.0, VisRatio=1.0, FSTI=0.01, IF_NS_Uniform=1, MA=20000)
+                  IF_RK=0, Kind_Dual=2, Kchld=1, DT_LUSGS_Main=1.0e10, CFL_Sub_start=1.0, CFL_Sub_end=5.0,
+                  CFL_Sub_step=50, KItm=1, Prod_Lmt=20.0, VisRatio=1.0, FSTI=0.01, IF_NS_Uniform=1, MA=20000)
     glob = GlobalInfo(nblocks=len(blocks), P0in=101325.0, T0in=288.0, Pouthub=back_pressure * 101325.0,
                       nblade=nblade, Npassage=1, omega_rpm=(rpm, 0.0, 0.0))
     return blocks, org, glob

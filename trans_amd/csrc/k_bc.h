@@ -323,6 +323,9 @@ __global__ void __launch_bounds__(128) k_bc(trans_params p, int if_turb, int swe
                             0.5 * sqr(p.omega1 * f.rad[il]) * Roin;
         f.PP[il] = PsIn;
         f.T[il] = TsIn;
+        for (int d = 0; d < 3; d++) f.Rds[(long)d * tot + il] = f.Rds[(long)d * tot + is];
+        if (f.Rdsf)
+            for (int d = 0; d < 3; d++) f.Rdsf[(long)d * tot + il] = f.Rdsf[(long)d * tot + is];
         for (int lay = 1; lay < 3; lay++) {
             const int* ea = lay_dst(c, lay);
             const long d2 = gidx(b, i + ea[0], j + ea[1], k + ea[2]);

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
+#include <tuple>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -20,6 +22,7 @@
 #include "k_space.h"
 #include "k_rk.h"
 #include "k_time.h"
+#include "k_tile.h"
 #include "k_visc.h"
 
 namespace {
@@ -92,6 +95,17 @@ struct trans_ctx {
     int max_planes = 0;
     void* wbuf = nullptr;  // the sweep buffer W / Wf (reallocated when the march axis changes)
     size_t wbytes = 0;
+    // tile wavefront of the sweeps (sweep mode 3, march axis j): tile tables for NEQ = 7, 5
+    // and 2 (mean flow and k-omega, mean flow alone, k-omega alone), the tiles' edge buffers,
+    // the item and progress counters of both sweeps, and the error flag of the bounded waits
+    bool tile_mode = false;
+    int tile_a = 16, tile_b = 16, tile_chunk = 8, tile_grid = 0;
+    SweepTile* d_tiles[3] = {nullptr, nullptr, nullptr};
+    int ntiles[3] = {0, 0, 0};
+    void* tile_edge = nullptr;
+    int* d_tctr = nullptr;
+    size_t tile_bytes = 0;
+    int* d_err = nullptr;
     std::vector<long> mwoff;  // per-block offset of the wall marks (size nblk+1)
     std::vector<void*> allocs;
     // time statistics (Average_Fields): TRANS_NSTAT components, allocated when enabled
@@ -334,18 +348,147 @@ static int upload_blk_table(trans_ctx* c)
     return h2d(c, c->d_blk, bi.data(), nblk * sizeof(BlkInfo));
 }
 
+// The sweep in use: 0, 1, 2 line march along that axis, 3 tile wavefront, -2 one launch per
+// hyperplane, -1 one workgroup per block.
+static int cur_sweep(const trans_ctx* c)
+{
+    if (c->tile_mode) return 3;
+    return c->line_threads ? c->march : (c->plane_launch ? -2 : -1);
+}
+
+static void free_tiles(trans_ctx* c)
+{
+    if (!c->dry) {
+        for (SweepTile*& t : c->d_tiles)
+            if (t) (void)hipFree(t);
+        if (c->tile_edge) (void)hipFree(c->tile_edge);
+        if (c->d_tctr) (void)hipFree(c->d_tctr);
+    }
+    for (SweepTile*& t : c->d_tiles) t = nullptr;
+    for (int& n : c->ntiles) n = 0;
+    c->tile_edge = nullptr;
+    c->d_tctr = nullptr;
+    c->bytes -= c->tile_bytes;
+    c->tile_bytes = 0;
+}
+
+// Tile tables of the wavefront sweep for the current tile shape (tile_a x tile_b lines):
+// table 0 holds the mean-flow and k-omega tiles, 1 the mean-flow ones, 2 the k-omega ones,
+// each in the order of the tiles' diagonals (upstream tiles first).  Edge buffers are laid
+// out once for table 0; the other tables use the same offsets.
+static int build_tiles(trans_ctx* c)
+{
+    const int TA = c->tile_a, TB = c->tile_b;
+    if (TA < 1 || TB < 1 || TA * TB > TILE_MAXT) return fail(c, TRANS_EINVAL, "tile shape must have 1..256 lines");
+    const size_t esz = c->p.implicit_fp32 ? sizeof(float) : sizeof(double);
+    struct Item {
+        int diag, blk, neq, ta, tb;
+    };
+    std::vector<Item> all;
+    for (int b = 0; b < (int)c->hb.size(); b++) {
+        const HostBlock& h = c->hb[b];
+        const int nta = (h.ni - 1 + TA - 1) / TA, ntb = (h.nk - 1 + TB - 1) / TB;
+        for (int tb = 0; tb < ntb; tb++)
+            for (int ta = 0; ta < nta; ta++)
+                for (int neq : {5, 2}) all.push_back(Item{ta + tb, b, neq, ta, tb});
+    }
+    std::stable_sort(all.begin(), all.end(), [](const Item& x, const Item& y) {
+        if (x.diag != y.diag) return x.diag < y.diag;
+        if (x.blk != y.blk) return x.blk < y.blk;
+        return x.neq > y.neq;
+    });
+    // edge offsets (elements), per (block, kind, ta, tb), from the full table
+    std::map<std::tuple<int, int, int, int>, std::pair<long, long>> eoff;
+    long etot = 0;
+    for (const Item& it : all) {
+        const HostBlock& h = c->hb[it.blk];
+        const int na = std::min(TA, h.ni - 1 - it.ta * TA), nb = std::min(TB, h.nk - 1 - it.tb * TB);
+        const long ea = etot;
+        etot += (long)it.neq * (h.nj - 1) * nb;
+        const long eb = etot;
+        etot += (long)it.neq * (h.nj - 1) * na;
+        eoff[std::make_tuple(it.blk, it.neq, it.ta, it.tb)] = std::make_pair(ea, eb);
+    }
+    int ntmax = 0;
+    std::vector<SweepTile> tab[3];
+    for (int w = 0; w < 3; w++) {
+        std::map<std::tuple<int, int, int, int>, int> index;
+        for (const Item& it : all) {
+            if ((w == 1 && it.neq != 5) || (w == 2 && it.neq != 2)) continue;
+            const HostBlock& h = c->hb[it.blk];
+            SweepTile t{};
+            t.blk = it.blk;
+            t.neq = it.neq;
+            t.a0 = it.ta * TA;
+            t.b0 = it.tb * TB;
+            t.na = std::min(TA, h.ni - 1 - t.a0);
+            t.nb = std::min(TB, h.nk - 1 - t.b0);
+            const auto& e = eoff[std::make_tuple(it.blk, it.neq, it.ta, it.tb)];
+            t.ea = e.first;
+            t.eb = e.second;
+            index[std::make_tuple(it.blk, it.neq, it.ta, it.tb)] = (int)tab[w].size();
+            tab[w].push_back(t);
+        }
+        for (SweepTile& t : tab[w]) {
+            const int ta = t.a0 / TA, tb = t.b0 / TB;
+            auto find = [&](int a, int b) {
+                auto q = index.find(std::make_tuple(t.blk, t.neq, a, b));
+                return q == index.end() ? -1 : q->second;
+            };
+            t.lo_a = find(ta - 1, tb);
+            t.hi_a = find(ta + 1, tb);
+            t.lo_b = find(ta, tb - 1);
+            t.hi_b = find(ta, tb + 1);
+        }
+        c->ntiles[w] = (int)tab[w].size();
+        ntmax = std::max(ntmax, c->ntiles[w]);
+    }
+    const size_t ctr_n = 2 + 2 * (size_t)ntmax;
+    const size_t bytes = 3 * (size_t)ntmax * sizeof(SweepTile) + (size_t)std::max(etot, 1L) * esz + ctr_n * sizeof(int);
+    c->tile_bytes = bytes;
+    c->bytes += bytes;
+    if (c->dry) return 0;
+    for (int w = 0; w < 3; w++) {
+        HIPCHK(c, hipMalloc(&c->d_tiles[w], std::max<size_t>(1, tab[w].size()) * sizeof(SweepTile)));
+        if (!tab[w].empty()) {
+            int rc = h2d(c, c->d_tiles[w], tab[w].data(), tab[w].size() * sizeof(SweepTile));
+            if (rc) return rc;
+        }
+    }
+    HIPCHK(c, hipMalloc(&c->tile_edge, (size_t)std::max(etot, 1L) * esz));
+    HIPCHK(c, hipMemsetAsync(c->tile_edge, 0, (size_t)std::max(etot, 1L) * esz, c->stream));
+    HIPCHK(c, hipMalloc(&c->d_tctr, ctr_n * sizeof(int)));
+    HIPCHK(c, hipMemsetAsync(c->d_tctr, 0, ctr_n * sizeof(int), c->stream));
+    // persistent grid: as many tile workgroups as the device holds at once
+    int dev = 0, cus = 0, occ = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (c->p.implicit_fp32)
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lusgs_tile<float, true>, TILE_MAXT, 0));
+    else
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lusgs_tile<double, true>, TILE_MAXT, 0));
+    c->tile_grid = std::max(1, std::min(TILE_GRID_MAX, std::max(1, occ) * std::max(1, cus)));
+    return 0;
+}
+
 // Selects the LU-SGS sweep: m = 0, 1, 2 the line march along that axis (hyperplane-major
-// buffer W or Wf, reallocated here), m = -1 the hyperplane kernel on the natural layout.
+// buffer W or Wf, reallocated here), m = 3 the tile wavefront (march axis j), m = -2 one
+// launch per hyperplane, m = -1 the hyperplane kernel on the natural layout.
 static int set_sweep(trans_ctx* c, int m, bool upload)
 {
     int mx = 0;
     // the low-Mach preconditioning transforms live in the natural-layout hyperplane sweep
     if (c->p.if_precondition && m != -1)
         return fail(c, TRANS_EUNSUPPORTED, "with IF_PRECONDITION = 1 only the hyperplane sweep (-1) runs");
+    if (m < -2 || m > 3) return fail(c, TRANS_EINVAL, "sweep must be -2, -1, 0, 1, 2 or 3");
     const bool planes = m == -2;  // per-plane launches on the hyperplane-major buffer, march axis j
-    if (planes) m = 1;
-    if (!planes && m >= 0 && !march_fits(c, m, &mx)) return fail(c, TRANS_EINVAL, "the line sweep does not fit this axis");
+    const bool tiles = m == 3;    // tile wavefront on the hyperplane-major buffer, march axis j
+    if (planes || tiles) m = 1;
+    if (!planes && !tiles && m >= 0 && !march_fits(c, m, &mx))
+        return fail(c, TRANS_EINVAL, "the line sweep does not fit this axis");
     c->plane_launch = planes;
+    c->tile_mode = tiles;
+    free_tiles(c);
     if (c->wbuf) {
         if (!c->dry) (void)hipFree(c->wbuf);
         c->wbuf = nullptr;
@@ -359,7 +502,7 @@ static int set_sweep(trans_ctx* c, int m, bool upload)
     if (m >= 0) {
         c->march = m;
         c->max_lines = mx;
-        c->line_threads = planes ? 0 : (mx + 63) / 64 * 64;
+        c->line_threads = (planes || tiles) ? 0 : (mx + 63) / 64 * 64;
         // hyperplane-major sweep buffer: nplanes x nlines slots per block, then per
         // component a sink of 1024 slots for each of the 2 workgroups of every block
         long wt = 0;
@@ -368,11 +511,14 @@ static int set_sweep(trans_ctx* c, int m, bool upload)
             wt += (long)(n[0] + n[1] + n[2] - 2) * n[(m + 1) % 3] * n[(m + 2) % 3];
         }
         f.wsink = wt;
-        wt += (long)c->hb.size() * 2 * 1024;
+        // (the tile wavefront's persistent workgroups have TILE_MAXT sink slots each)
+        wt += tiles ? std::max((long)c->hb.size() * 2 * 1024, (long)TILE_GRID_MAX * TILE_MAXT)
+                    : (long)c->hb.size() * 2 * 1024;
         // the sweep addresses a component with 32-bit byte offsets
         if (wt * (long)sizeof(double) >= (1L << 32)) {
             c->line_threads = 0;
             c->plane_launch = false;
+            c->tile_mode = false;
             return fail(c, TRANS_EUNSUPPORTED, "too many cells for the line-march sweep buffer");
         }
         f.wtot = wt;
@@ -389,6 +535,10 @@ static int set_sweep(trans_ctx* c, int m, bool upload)
             f.Wf = (float*)c->wbuf;
         else
             f.W = (double*)c->wbuf;
+    }
+    if (tiles) {
+        int rc = build_tiles(c);
+        if (rc) return rc;
     }
     drop_graph(c);
     if (!upload) return 0;
@@ -531,7 +681,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         const char* env = std::getenv("TRANS_HIP_LINE_SWEEP");
         const bool lines_ok = !(env && env[0] == '0') && !c->p.if_precondition;
         const int order[3] = {1, 2, 0};
-        int m = lines_ok ? -2 : -1;  // blocks too large for a line-march workgroup: one launch per plane
+        int m = lines_ok ? 3 : -1;  // blocks too large for a line-march workgroup: the tile wavefront
         for (int q = 0; q < 3 && lines_ok && m < 0; q++)
             if (march_fits(c, order[q], nullptr)) m = order[q];
         if ((rc = set_sweep(c, m, false))) return rc;
@@ -561,7 +711,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
     if ((rc = h2d(c, c->d_wrange, c->seg_cell.wrange.data(), (nblk + 1) * sizeof(int)))) return rc;
     if ((rc = dalloc(c, &c->d_part, c->seg_cell.n)) || (rc = dalloc(c, &c->d_resid, nblk)) ||
-        (rc = dalloc(c, &c->d_mon, 4)))
+        (rc = dalloc(c, &c->d_mon, 4)) || (rc = dalloc(c, &c->d_err, 1)))
         return rc;
     if (!dry) HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -601,6 +751,7 @@ int trans_hip_destroy(trans_ctx* c)
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->wbuf) (void)hipFree(c->wbuf);
+    free_tiles(c);
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     for (int q = 0; q < 3; q++) {
@@ -909,6 +1060,19 @@ static void launch_sweeps(trans_ctx* c, R* dq)
             launch_lines<R, NEQ, 512>(c, dq);
         else
             launch_lines<R, NEQ, 1024>(c, dq);
+    } else if (c->tile_mode) {
+        const int w = NEQ == 7 ? 0 : (NEQ == 5 ? 1 : 2);
+        const int nt = c->ntiles[w];
+        const int ntmax = std::max(c->ntiles[0], std::max(c->ntiles[1], c->ntiles[2]));
+        const int nctr = 2 + 2 * ntmax;
+        const int grid = std::max(1, std::min(c->tile_grid, nt));
+        R* edge = (R*)c->tile_edge;
+        KLAUNCH(c, "k_tile_reset", k_tile_reset, dim3((nctr + 255) / 256), dim3(256), c->d_tctr, nctr);
+        KLAUNCH(c, "k_lusgs_tile<fwd>", (k_lusgs_tile<R, true>), dim3(grid), dim3(TILE_MAXT), c->f, dq, c->d_blk,
+                c->d_tiles[w], nt, c->d_tctr, c->d_tctr + 2, edge, c->d_err, c->tile_chunk, TILE_SPIN_LIMIT);
+        KLAUNCH(c, "k_lusgs_tile<bwd>", (k_lusgs_tile<R, false>), dim3(grid), dim3(TILE_MAXT), c->f, dq, c->d_blk,
+                c->d_tiles[w], nt, c->d_tctr + 1, c->d_tctr + 2 + ntmax, edge, c->d_err, c->tile_chunk,
+                TILE_SPIN_LIMIT);
     } else if (c->plane_launch) {
         const bool kw = NEQ == 2;
         for (int s0 = 0; s0 < c->max_planes; s0++)
@@ -1011,11 +1175,25 @@ static int rk_update_async(trans_ctx* c, int stage)
     return launch_check(c, "rk_update");
 }
 
+// The error flag of the tile wavefront's bounded waits (read once the stream is idle).
+static int check_sweep_flag(trans_ctx* c)
+{
+    if (!c->d_err) return 0;
+    int e = 0;
+    HIPCHK(c, hipMemcpyAsync(&e, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!e) return 0;
+    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return fail(c, TRANS_ETIMEOUT, "a bounded wait of the tile-wavefront LU-SGS sweep expired; results are invalid");
+}
+
 static int fetch_resid(trans_ctx* c, trans_resid* per_block)
 {
     HIPCHK(c, hipMemcpyAsync(per_block, c->d_resid, c->hb.size() * sizeof(trans_resid), hipMemcpyDeviceToHost,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (int rc = check_sweep_flag(c)) return rc;
     for (size_t b = 0; b < c->hb.size(); b++)
         if (per_block[b].drsa != per_block[b].drsa)
             return fail(c, TRANS_ENONFINITE, "residual of block " + std::to_string(b + 1) + " is NaN");
@@ -1179,7 +1357,7 @@ int trans_hip_sync(trans_ctx* c)
 {
     if (!c) return TRANS_EINVAL;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return 0;
+    return check_sweep_flag(c);
 }
 
 }  // extern "C"

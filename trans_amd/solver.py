@@ -134,8 +134,8 @@ class Solver:
         self._check(self.lib.trans_hip_autotune(self.ctx, iters, out, 32, C.byref(n)))
         return self.tuning(), [out[q].as_dict() for q in range(min(n.value, 32))]
 
-    def set_tuning(self, sweep, streams, graph):
-        t = capi.TransTuneEntry(sweep, streams, graph, 0.0)
+    def set_tuning(self, sweep, streams, graph, tile_a=16, tile_b=16, chunk=8):
+        t = capi.TransTuneEntry(sweep, streams, graph, 0.0, tile_a, tile_b, chunk)
         self._check(self.lib.trans_hip_set_tuning(self.ctx, C.byref(t)))
 
     def tuning(self):
