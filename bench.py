@@ -58,6 +58,14 @@ def alg_bytes(name, neq, fp32):
         # SD(9), Rds(3)
         "k_lusgs_line<fwd>": ("cell", r * (2 * neq + 2 + (3 if turb else 0) + 4 + 1 + 9 + 3)),
         "k_lusgs_line<bwd>": ("cell", r * (2 * neq + 2 + (3 if turb else 0) + 4 + 1 + 9 + 3)),
+        # tile wavefront: the same inputs and increments as the line march, every input read
+        # once (the mean-flow and k-omega tiles share 1/rho, u, v, w, SD means and Rds)
+        "k_lusgs_tile<fwd>": ("cell", r * (2 * neq + (5 if turb else 2) + 4 + 1 + 9 + 3)),
+        "k_lusgs_tile<bwd>": ("cell", r * (2 * neq + (5 if turb else 2) + 4 + 1 + 9 + 3)),
+        # D(5), V(5), Wt0(5), Wt1(5), Rds(3), Dtm, Vol in; 17 sweep inputs out in the operator's precision
+        "k_lusgs_rhs_w": ("cell", d * (5 + 5 + 5 + 5 + 3 + 1 + 1) + r * 17),
+        "k_sst_rhs_w": ("cell", d * (2 + 3 + 2 + 2 + 15 + 1 + 1 + 1 + 1 + 3 + 1) + r * 3),
+        "k_unskew_w": ("cell", (r + d) * neq),
         # D(neq), Dtm, Vol, F1(,F1t), V(1:4), PP, SD(9), Rds(3) in the natural layout
         "k_lusgs_sweep<fwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
         "k_lusgs_sweep<bwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
@@ -200,6 +208,11 @@ def measured_traffic(workload, kernel):
     return None
 
 
+def _cfg_name(r):
+    s = f"sweep{r['sweep']}/streams{r['streams']}/graph{r['graph']}"
+    return s + (f"/tile{r['tile_a']}x{r['tile_b']}/chunk{r['chunk']}" if r["sweep"] == 3 else "")
+
+
 def timed_run(blocks, prm, steps, warmup, prof_steps, tune=0, force=None):
     """Warm up, let the library's autotuner pick the launch configuration (`tune` timed
     iterations per candidate, state restored afterwards; 0 keeps the default), time `steps`
@@ -214,15 +227,23 @@ def timed_run(blocks, prm, steps, warmup, prof_steps, tune=0, force=None):
     timed_run.tuning = None
     if tune > 0:
         chosen, table = sv.autotune(iters=tune)
-        timed_run.tuning = {"chosen": {k: chosen[k] for k in ("sweep", "streams", "graph")},
-                            "ms_per_iter": {f"sweep{r['sweep']}/streams{r['streams']}/graph{r['graph']}":
-                                            round(r["ms_per_iter"], 4) for r in table}}
+        timed_run.tuning = {"chosen": {k: chosen[k] for k in ("sweep", "streams", "graph", "tile_a", "tile_b",
+                                                              "chunk")},
+                            "ms_per_iter": {_cfg_name(r): round(r["ms_per_iter"], 4) for r in table}}
+        # one untimed iteration captures the chosen configuration's graph outside the timed region
+        sv.iterate(1, want_resid=False)
+        sv.sync()
     t0 = time.perf_counter()
     res = sv.iterate(steps, want_resid=True)
     sv.sync()
     dt = time.perf_counter() - t0
     stats = {}
     if prof_steps > 0:
+        # per-kernel durations on one stream without graph replay, so that no kernel's
+        # duration includes time it shared the device with the helper streams' kernels
+        t = sv.tuning()
+        if t["sweep"] != -3:
+            sv.set_tuning(t["sweep"], 1, 0, t["tile_a"], t["tile_b"], t["chunk"])
         sv.profile(True)
         sv.iterate(prof_steps, want_resid=False)
         sv.sync()
@@ -260,25 +281,63 @@ def converge(blocks, prm, desc, max_iter, drop=1e-8, chunk=100):
                                    "Mcell_iter_per_s": round(cells * it / dt / 1e6, 3)}}))
 
 
+def physical_cores():
+    """Physical cores of the host (distinct (core, socket) pairs of lscpu), or None."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, check=True).stdout
+        return len({ln for ln in out.splitlines() if ln and not ln.startswith("#")})
+    except (OSError, subprocess.CalledProcessError):
+        return None
+
+
+def roofline_of(stats, blocks, neq, fp32, workload):
+    """Roofline of the dominant kernel (largest device time) from the HIP-event profile."""
+    u = units(blocks)
+    name, (ms, n) = max(stats.items(), key=lambda kv: kv[1][0])
+    avg_s = ms / 1e3 / n
+    unit, per = alg_bytes(name, neq, fp32)
+    if per is None:
+        return None
+    alg = per * u[unit]
+    ach = alg / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": measured_traffic(workload, name),
+            "kernel": name, "avg_launch_us": round(avg_s * 1e6, 3), "alg_bytes_per_launch": alg,
+            "alg_bytes_per_unit": per, "unit_of_work": unit, "units_per_launch": u[unit]}
+
+
+def kernel_table(stats, blocks, neq, fp32, prof_steps):
+    u = units(blocks)
+    kern = {}
+    for k, (kms, kn) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
+        un, pb = alg_bytes(k, neq, fp32)
+        avg = kms / kn / 1e3
+        kern[k] = {"us": round(avg * 1e6, 2), "per_step": round(kn / prof_steps, 3),
+                   "GBs": round(pb * u[un] / avg / 1e9, 1) if pb else None}
+    return kern
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default=os.environ.get("TRANS_BENCH_WORKLOAD", "flatplate"))
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--workload", default=os.environ.get("TRANS_BENCH_WORKLOAD", "compressor"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--serving", type=int, default=16, metavar="N",
                     help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serving-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32-implicit leg")
-    ap.add_argument("--tune", type=int, default=10, metavar="ITERS",
+    ap.add_argument("--no-fp32", action="store_true", help="skip the second-precision leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain drop-in leg")
+    ap.add_argument("--tune", type=int, default=3, metavar="ITERS",
                     help="autotuner iterations per launch configuration before timing (0: default configuration)")
     ap.add_argument("--profile-only", action="store_true",
-                    help="run warmup+steps of the FP64 path only and print the kernel table")
+                    help="run warmup+steps of the primary precision only and print the kernel table")
     ap.add_argument("--sweep", default=None, metavar="S[,STREAMS,GRAPH,TA,TB,CHUNK]",
                     help="with --profile-only: force a launch configuration (trans_hip_set_tuning)")
-    ap.add_argument("--implicit-fp32", action="store_true",
-                    help="with --profile-only: profile the FP32-implicit path instead")
+    ap.add_argument("--implicit-fp32", type=int, default=None, choices=(0, 1),
+                    help="precision of the implicit operator of the primary leg (default: the workload's)")
     ap.add_argument("--converge", type=int, default=0,
                     help="instead of the timed run: iterate up to N times to a 1e-8 drop of the "
                          "global DRSa and print the history")
@@ -303,16 +362,17 @@ def main():
                                    "blocks is a design only", "n_gpus": args.gpus}))
         sys.exit(2)
 
-    blocks, prm, desc = build_workload(args.workload, 1 if (args.profile_only and args.implicit_fp32) else 0)
+    # the compressor sector (8M cells) is quoted with the FP32 implicit operator and the FP64
+    # residual; the flat plate with both in FP64
+    primary32 = args.implicit_fp32 if args.implicit_fp32 is not None else int(args.workload == "compressor")
+    blocks, prm, desc = build_workload(args.workload, primary32)
     cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
     if args.converge:
         converge(blocks, prm, desc, args.converge)
         return
     neq = 7 if (prm.if_euler == 0 and prm.if_turb_org == 1) else 5
     print(f"[bench] {desc}: {cells} cells, warmup {args.warmup}, steps {args.steps}", flush=True)
-    prof_steps = min(args.steps, 50)
-    # a profiled run (--profile-only) keeps the default launch configuration: the one the
-    # tuner picks for the flat plate, without its graph captures under the profiler
+    prof_steps = min(args.steps, 20)
     force = None
     if args.sweep:
         v = [int(x) for x in args.sweep.split(",")]
@@ -320,40 +380,38 @@ def main():
     dt, res, stats = timed_run(blocks, prm, args.steps, args.warmup, prof_steps,
                                0 if args.profile_only else args.tune, force)
     tuning = timed_run.tuning
-    tag = "FP32 implicit" if prm.implicit_fp32 else "FP64"
+    tag = "FP32 implicit" if primary32 else "FP64"
     print(f"[bench] {tag}: {args.steps} steps in {dt:.3f} s, DRSa(block 1) = {res[0]['drsa']:.6e}", flush=True)
     if args.profile_only:
         for k, v in sorted(stats.items(), key=lambda kv: -kv[1][0]):
             print(f"[bench] {k:28s} {v[0] / v[1] * 1e3:9.2f} us x {v[1]}")
         return
-    u = units(blocks)
-    # dominant kernel: largest total device time
-    name, (ms, n) = max(stats.items(), key=lambda kv: kv[1][0])
-    avg_s = ms / 1e3 / n
-    unit, per = alg_bytes(name, neq, False)
-    roof = None
-    if per is not None:
-        alg = per * u[unit]
-        ach = alg / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, name),
-                "kernel": name, "avg_launch_us": round(avg_s * 1e6, 3), "alg_bytes_per_launch": alg}
     value = cells * args.steps / dt / 1e6
-    kern = {}
-    for k, (kms, kn) in sorted(stats.items(), key=lambda kv: -kv[1][0]):
-        un, pb = alg_bytes(k, neq, False)
-        avg = kms / kn / 1e3
-        kern[k] = {"us": round(avg * 1e6, 2), "per_step": round(kn / prof_steps, 3),
-                   "GBs": round(pb * u[un] / avg / 1e9, 1) if pb else None}
-    fp32 = None
+    roof = roofline_of(stats, blocks, neq, bool(primary32), args.workload)
+    kern = kernel_table(stats, blocks, neq, bool(primary32), prof_steps)
+    other = None
     if not args.no_fp32:
-        b32, p32, _ = build_workload(args.workload, implicit_fp32=1)
-        dt32, res32, _ = timed_run(b32, p32, args.steps, args.warmup, 0, args.tune)
-        fp32 = {"value": round(cells * args.steps / dt32 / 1e6, 3),
-                "ms_per_step": round(dt32 / args.steps * 1e3, 4),
-                "drsa_block1": res32[0]["drsa"], "drsa_block1_fp64": res[0]["drsa"],
-                "tuning": (timed_run.tuning or {}).get("chosen")}
-        print(f"[bench] FP32 implicit: {args.steps} steps in {dt32:.3f} s", flush=True)
+        b2, p2, _ = build_workload(args.workload, 1 - primary32)
+        dt2, res2, _ = timed_run(b2, p2, args.steps, args.warmup, 0, args.tune)
+        other = {"implicit": "fp32" if not primary32 else "fp64",
+                 "value": round(cells * args.steps / dt2 / 1e6, 3),
+                 "ms_per_step": round(dt2 / args.steps * 1e3, 4),
+                 "drsa_block1": res2[0]["drsa"], "drsa_block1_primary": res[0]["drsa"],
+                 "tuning": (timed_run.tuning or {}).get("chosen")}
+        print(f"[bench] {other['implicit']} implicit: {args.steps} steps in {dt2:.3f} s", flush=True)
+        del b2
+    plain = None
+    if not args.no_plain and prm.if_rk == 0:
+        # the same workload as a plain drop-in: every routine launched once per block, the sweeps
+        # one launch per hyperplane per block, one stream, no graph (trans_hip_set_tuning -3)
+        ns = min(args.steps, 10)
+        dtp, _, _ = timed_run(blocks, prm, ns, args.warmup, 0, 0, (-3, 1, 0, 16, 16, 8))
+        vp = cells * ns / dtp / 1e6
+        plain = {"value": round(vp, 3), "ms_per_step": round(dtp / ns * 1e3, 3), "steps": ns,
+                 "speedup_of_value": round(value / vp, 2),
+                 "what": "per-block launches of every routine, per-plane LU-SGS launches per block, "
+                         "one stream, no HIP graph; same precision as value"}
+        print(f"[bench] plain drop-in: {ns} steps in {dtp:.3f} s ({vp:.1f} Mcell*iter/s)", flush=True)
     cpu = None if args.no_cpu else cpu_baseline(blocks, prm)
     serve = None
     if args.serving > 0:
@@ -369,7 +427,7 @@ def main():
         print(f"[bench] serving: {serve['requests']} solves in {serve['makespan_s']} s", flush=True)
     # whole-iteration figure: compulsory bytes of one fused iteration per cell (state in and
     # out once, geometry once, one round trip of the increment between the two sweeps)
-    b_it = ITER_BYTES_RANS if neq == 7 else ITER_BYTES_EULER
+    b_it = (ITER_BYTES_RANS - (2 * 4 * 7 if primary32 else 0)) if neq == 7 else ITER_BYTES_EULER
     it_roof = {"alg_bytes_per_cell": b_it, "achieved": round(value * 1e6 * b_it / 1e9, 2),
                "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(value * 1e6 * b_it / 1e9 / HBM_PEAK_GBS, 5)}
@@ -378,9 +436,10 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (generated grid, uniform start)",
-        "config": {"workload": desc, "cells": cells, "blocks": len(blocks), "implicit": "fp64"},
-        "roofline": roof, "iteration_roofline": it_roof, "cpu_baseline": cpu, "fp32_implicit": fp32,
-        "kernels": kern, "tuning": tuning, "serving": serve,
+        "config": {"workload": desc, "cells": cells, "blocks": len(blocks),
+                   "residual": "fp64", "implicit": "fp32" if primary32 else "fp64"},
+        "roofline": roof, "iteration_roofline": it_roof, "cpu_baseline": cpu, "other_precision": other,
+        "plain_dropin": plain, "kernels": kern, "tuning": tuning, "serving": serve,
     }
     print(json.dumps(out))
 

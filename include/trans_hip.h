@@ -293,7 +293,7 @@ int trans_hip_read_stats(trans_ctx* ctx, int blk, const char* path);
  * from trans_hip_footprint's.  get_tuning reports the configuration in use. */
 typedef struct trans_tune_entry {
     int sweep;     /* 0, 1, 2 line march along i, j, k; 3 tile wavefront; -2 one launch per hyperplane;
-                      -1 one workgroup per block */
+                      -1 one workgroup per block; -3 plain drop-in schedule (set_tuning only) */
     int streams;
     int graph;
     double ms_per_iter;
@@ -302,7 +302,15 @@ typedef struct trans_tune_entry {
 } trans_tune_entry;
 int trans_hip_autotune(trans_ctx* ctx, int iters, trans_tune_entry* out, int max_out, int* n_out);
 int trans_hip_get_tuning(const trans_ctx* ctx, trans_tune_entry* out);
-/* Applies a configuration (ms_per_iter ignored), e.g. one found by an earlier autotune. */
+/* Applies a configuration (ms_per_iter ignored), e.g. one found by an earlier autotune.
+ * sweep = -3 selects the plain drop-in schedule (streams, graph and the tile fields ignored):
+ * every routine of trans_hip_iterate is launched once per block, as a straight port of the
+ * reference's per-block calls would run, each LU-SGS sweep one launch per hyperplane per
+ * block, on one stream, without graph replay; the same arithmetic as the batched schedule.
+ * LU-SGS decks without preconditioning only (TRANS_EUNSUPPORTED otherwise).  A baseline for
+ * the batched schedule; autotune leaves it for the batched candidates.
+ * A tile-wavefront sweep whose progress wait exceeds its spin limit (a schedule that cannot
+ * advance) ends and makes the next synchronising call return TRANS_ETIMEOUT. */
 int trans_hip_set_tuning(trans_ctx* ctx, const trans_tune_entry* t);
 
 /* ---- Geometry pre-processing on the device -------------------------------------------------

@@ -116,6 +116,33 @@ __device__ __forceinline__ void decode3(int lin, int ni, int nj, int& i, int& j,
     k += 1;
 }
 
+// Faces of one block in FTA x FTB tiles of the plane spanned by i and the stencil direction
+// DIR (2: the (i, j) plane, then k; 3: the (i, k) plane, then j), 256 per workgroup: the
+// neighbours a face stencil reads along DIR are read by the same workgroup, so they come from
+// the L2 of one XCD instead of being fetched again by workgroups that the dispatcher put on
+// other XCDs, while each wave still reads one row of 64 consecutive i.  Returns false for the
+// padding of partial tiles.
+constexpr int FTA = 64, FTB = 4;
+template <int DIR>
+__device__ __forceinline__ bool decode_face_tile(int lin, int n0, int n1, int n2, int& i, int& j, int& k)
+{
+    const int tile = lin / (FTA * FTB), w = lin % (FTA * FTB);
+    const int ti = w % FTA, tm = w / FTA;
+    const int ntx = (n0 + FTA - 1) / FTA;
+    const int tx = tile % ntx;
+    i = tx * FTA + ti + 1;
+    if (DIR == 2) {
+        const int nty = (n1 + FTB - 1) / FTB;
+        j = (tile / ntx) % nty * FTB + tm + 1;
+        k = tile / (ntx * nty) + 1;
+        return i <= n0 && j <= n1;
+    }
+    const int ntz = (n2 + FTB - 1) / FTB;
+    k = (tile / ntx) % ntz * FTB + tm + 1;
+    j = tile / (ntx * ntz) + 1;
+    return i <= n0 && k <= n2;
+}
+
 // Hyperplane-major index of interior cell (i,j,k) for the line-march sweeps.  Lines run
 // along the march axis m; the two other axes a < b number the lines a-fastest.  Cells of one
 // hyperplane s = (i-1)+(j-1)+(k-1) are stored contiguously by line, so the lanes of a wave,

@@ -14,6 +14,116 @@
 // preconditioning the right-hand side through the cell's transform (:113-160).  With the
 // line-march sweep the right-hand side and the state the sweep needs go to the
 // hyperplane-major buffer (the sweep re-forms F1 from Rds and cvs); otherwise D, F1 and Stp.
+// The hyperplane-major buffer with march axis j, written or read through LDS: a workgroup owns
+// a WT x WT tile of cells (i, j) at one k (schedule entry Seg{block, tile index}).  A diagonal
+// i + j = const of the tile is a run of consecutive slots (consecutive lines of one
+// hyperplane), so the tile moves to or from the buffer diagonal by diagonal in whole cache
+// lines instead of one element per line.
+constexpr int WT = 16;
+
+struct TileCell {
+    int i, j, k;
+    bool ok;
+};
+
+__device__ __forceinline__ TileCell wtile_cell(const Seg& s, const BlkInfo& b, int ti, int tj)
+{
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1;
+    const int ntx = (NI1 + WT - 1) / WT, nty = (NJ1 + WT - 1) / WT;
+    const int t = s.base;
+    const int tx = t % ntx, ty = (t / ntx) % nty;
+    TileCell c{tx * WT + ti + 1, ty * WT + tj + 1, t / (ntx * nty) + 1, false};
+    c.ok = c.i <= NI1 && c.j <= NJ1;
+    return c;
+}
+
+// Tile position of the q-th cell in diagonal order (diagonals ti + tj ascending, then ti).
+__device__ __forceinline__ void wtile_diag(int q, int& ti, int& tj)
+{
+    int dd = 0, c0 = 0;
+    for (; dd < 2 * WT - 2; dd++) {
+        const int n = dd < WT ? dd + 1 : 2 * WT - 1 - dd;
+        if (q < c0 + n) break;
+        c0 += n;
+    }
+    ti = (dd < WT ? 0 : dd - (WT - 1)) + (q - c0);
+    tj = dd - ti;
+}
+
+// Time/TIME_LUSGS.F90:60-108 for cell c: right-hand side (through the preconditioning
+// transform when on), stept, cvs and the diagonal F1.
+__device__ __forceinline__ void lusgs_rhs_cell(const trans_params& p, const Fields& f, long c, double rhs[5],
+                                               double& stept, double& cvs, double& f1)
+{
+    const long tot = f.tot;
+    const double dtm = p.dt_lusgs_main;
+    const double vol = f.Vol[c];
+#pragma unroll
+    for (int L = 0; L < 5; L++) {
+        const long e = (long)L * tot + c;
+        rhs[L] = f.D[e] / vol - (3.0 * f.V[e] - 4.0 * f.Wt0[e] + f.Wt1[e]) * 0.5 / dtm;
+    }
+    if (p.if_precondition) precond_cell<double>(p, f, c, rhs);
+    const double Dt = f.Dtm[c];
+    stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
+    cvs = stept / vol;
+    f1 = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * cvs;
+}
+
+// The 17 values the line-march and tile sweeps read from the hyperplane-major buffer, written
+// by the right-hand side, with their components: D(1:5), stept, cvs, stept/F1, cvs/F1, 1/rho,
+// u, v, w, p, Rds(1:3).
+constexpr int RHS_NW = 17;
+__device__ __forceinline__ int rhs_w_comp(int q)
+{
+    return q < 5 ? W_D + q : (q < 9 ? (q == 5 ? W_STEPT : (q == 6 ? W_CVS : (q == 7 ? W_SA : W_SB)))
+                                    : (q < 13 ? W_V + (q - 9) : (q == 13 ? W_PP : W_RDS + (q - 14))));
+}
+
+__device__ __forceinline__ void lusgs_rhs_wvals(const trans_params& p, const Fields& f, long c, double v[RHS_NW])
+{
+    const long tot = f.tot;
+    double stept, cvs, f1;
+    lusgs_rhs_cell(p, f, c, v, stept, cvs, f1);
+    v[5] = stept;
+    v[6] = cvs;
+    const double rf1 = 1.0 / f1;
+    v[7] = stept * rf1;
+    v[8] = cvs * rf1;
+    const double roinv = 1.0 / f.V[c];
+    v[9] = roinv;
+#pragma unroll
+    for (int L = 1; L < 4; L++) v[9 + L] = f.V[(long)L * tot + c] * roinv;
+    v[13] = f.PP[c];
+#pragma unroll
+    for (int d = 0; d < 3; d++) v[14 + d] = f.Rds[(long)d * tot + c];
+}
+
+// The right-hand side into the hyperplane-major buffer (march axis j), tile by tile.
+__global__ void __launch_bounds__(256) k_lusgs_rhs_w(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
+                                                     const Seg* __restrict__ seg)
+{
+    __shared__ double sv[RHS_NW][WT * WT];
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int ti = threadIdx.x % WT, tj = threadIdx.x / WT;
+    const TileCell tc = wtile_cell(s, b, ti, tj);
+    if (tc.ok) {
+        double v[RHS_NW];
+        lusgs_rhs_wvals(p, f, cidx(b, tc.i, tc.j, tc.k), v);
+#pragma unroll
+        for (int q = 0; q < RHS_NW; q++) sv[q][threadIdx.x] = v[q];
+    }
+    __syncthreads();
+    int di, dj;
+    wtile_diag(threadIdx.x, di, dj);
+    const TileCell oc = wtile_cell(s, b, di, dj);
+    if (!oc.ok) return;
+    const long w = widx(b, 1, oc.i, oc.j, oc.k);
+#pragma unroll
+    for (int q = 0; q < RHS_NW; q++) wput(f, rhs_w_comp(q), w, sv[q][dj * WT + di]);
+}
+
 __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
                                                    const BlkInfo* __restrict__ blks,
                                                    const Seg* __restrict__ seg)
@@ -27,36 +137,15 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,
     decode3(lin, NI1, NJ1, i, j, k);
     const long c = cidx(b, i, j, k);
     const long tot = f.tot;
-    const double dtm = p.dt_lusgs_main;
-    const double vol = f.Vol[c];
-    double rhs[5];
-#pragma unroll
-    for (int L = 0; L < 5; L++) {
-        const long e = (long)L * tot + c;
-        rhs[L] = f.D[e] / vol - (3.0 * f.V[e] - 4.0 * f.Wt0[e] + f.Wt1[e]) * 0.5 / dtm;
-    }
-    if (p.if_precondition) precond_cell<double>(p, f, c, rhs);
-    const double Dt = f.Dtm[c];
-    const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
-    const double cvs = stept / vol;
-    const double f1 = 1.0 + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * cvs;
     if (has_w(f)) {
+        double v[RHS_NW];
+        lusgs_rhs_wvals(p, f, c, v);
         const long w = widx(b, f.march, i, j, k);
 #pragma unroll
-        for (int L = 0; L < 5; L++) wput(f, W_D + L, w, rhs[L]);
-        wput(f, W_STEPT, w, stept);
-        wput(f, W_CVS, w, cvs);
-        const double rf1 = 1.0 / f1;
-        wput(f, W_SA, w, stept * rf1);
-        wput(f, W_SB, w, cvs * rf1);
-        const double roinv = 1.0 / f.V[c];
-        wput(f, W_V, w, roinv);
-#pragma unroll
-        for (int L = 1; L < 4; L++) wput(f, W_V + L, w, f.V[(long)L * tot + c] * roinv);
-        wput(f, W_PP, w, f.PP[c]);
-#pragma unroll
-        for (int d = 0; d < 3; d++) wput(f, W_RDS + d, w, f.Rds[(long)d * tot + c]);
+        for (int q = 0; q < RHS_NW; q++) wput(f, rhs_w_comp(q), w, v[q]);
     } else {
+        double rhs[5], stept, cvs, f1;
+        lusgs_rhs_cell(p, f, c, rhs, stept, cvs, f1);
 #pragma unroll
         for (int L = 0; L < 5; L++) f.D[(long)L * tot + c] = rhs[L];
         f.F1[c] = f1;
@@ -91,6 +180,30 @@ __global__ void __launch_bounds__(256) k_face_means(Fields f, const BlkInfo* __r
             f.SDm[o + c] = m;
             if (has_w(f)) wput(f, W_SDM + d + 3 * q, w, m);
         }
+}
+
+// Increments D(c0+1..c0+neq) from the hyperplane-major buffer (march axis j) back to D, tile
+// by tile: read diagonal by diagonal, written in the natural layout.
+template <typename R>
+__global__ void __launch_bounds__(256) k_unskew_w(int c0, int neq, Fields f, const R* __restrict__ Wd,
+                                                  const BlkInfo* __restrict__ blks, const Seg* __restrict__ seg)
+{
+    __shared__ R sv[7][WT * WT];
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    int di, dj;
+    wtile_diag(threadIdx.x, di, dj);
+    const TileCell ic = wtile_cell(s, b, di, dj);
+    if (ic.ok) {
+        const long w = widx(b, 1, ic.i, ic.j, ic.k);
+        for (int L = 0; L < neq; L++) sv[L][dj * WT + di] = Wd[(long)(c0 + L) * f.wtot + w];
+    }
+    __syncthreads();
+    const int ti = threadIdx.x % WT, tj = threadIdx.x / WT;
+    const TileCell oc = wtile_cell(s, b, ti, tj);
+    if (!oc.ok) return;
+    const long c = cidx(b, oc.i, oc.j, oc.k);
+    for (int L = 0; L < neq; L++) f.D[(long)(c0 + L) * f.tot + c] = (double)sv[L][threadIdx.x];
 }
 
 // Increments D(c0+1..c0+neq) of the line-march sweep back to D in the natural layout (the

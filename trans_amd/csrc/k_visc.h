@@ -182,7 +182,7 @@ __device__ __forceinline__ void face_grad(const Fields& f, const double* Sd, lon
 // when turbulent, the k-omega diffusion flux (Space/TURBTWO_DIFFUSION_Centered.F90, SST).
 // GV[DIR-1] receives Fx, Fy, Fz, Fe, Fk, Fo.
 template <int DIR>
-__global__ void __launch_bounds__(256) k_visc_face(trans_params p, int if_turb, Fields f,
+__global__ void __launch_bounds__(256, 3) k_visc_face(trans_params p, int if_turb, Fields f,
                                                    const BlkInfo* __restrict__ blks,
                                                    const Seg* __restrict__ seg)
 {
@@ -192,9 +192,13 @@ __global__ void __launch_bounds__(256) k_visc_face(trans_params p, int if_turb, 
     const int n1 = DIR == 2 ? b.nj : b.nj - 1;
     const int n2 = DIR == 3 ? b.nk : b.nk - 1;
     const int lin = s.base + threadIdx.x;
-    if (lin >= n0 * n1 * n2) return;
     int i, j, k;
-    decode3(lin, n0, n1, i, j, k);
+    if (DIR == 1) {
+        if (lin >= n0 * n1 * n2) return;
+        decode3(lin, n0, n1, i, j, k);
+    } else if (!decode_face_tile<DIR>(lin, n0, n1, n2, i, j, k)) {
+        return;
+    }
     const bool turb = if_turb != 0;
     const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
     const long c0 = cidx(b, i, j, k);  // cell on the high side of face f
@@ -400,9 +404,13 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
     const int n1 = DIR == 2 ? b.nj : b.nj - 1;
     const int n2 = DIR == 3 ? b.nk : b.nk - 1;
     const int lin = s.base + threadIdx.x;
-    if (lin >= n0 * n1 * n2) return;
     int i, j, k;
-    decode3(lin, n0, n1, i, j, k);
+    if (DIR == 1) {
+        if (lin >= n0 * n1 * n2) return;
+        decode3(lin, n0, n1, i, j, k);
+    } else if (!decode_face_tile<DIR>(lin, n0, n1, n2, i, j, k)) {
+        return;
+    }
     const bool turb = if_turb != 0;
     const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
     const long c0 = cidx(b, i, j, k);
@@ -693,17 +701,11 @@ __device__ __forceinline__ double iddes_fe(double Den, double RmiuT, double Rmiu
     return fe2 * fmax((fe1 - 1.), 0.);
 }
 
-__global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
-                                                 const Seg* __restrict__ seg)
+// The k-omega right-hand side d6, d7 and the diagonal F1t of cell (i, j, k) = c.
+__device__ __forceinline__ void sst_rhs_cell(const trans_params& p, const Fields& f, const BlkInfo& b, int i, int j,
+                                             int k, long c, double& d6_out, double& d7_out, double& f1t_out)
 {
-    const Seg s = seg[blockIdx.x];
-    const BlkInfo b = blks[s.blk];
-    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
-    const int lin = s.base + threadIdx.x;
-    if (lin >= NI1 * NJ1 * NK1) return;
-    int i, j, k;
-    decode3(lin, NI1, NJ1, i, j, k);
-    const long c = cidx(b, i, j, k);
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1;
     const long tot = f.tot;
     const double dtm = p.dt_lusgs_main, Ref = p.ref;
     const double vol = f.Vol[c];
@@ -841,6 +843,25 @@ __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const
     const double Dt = f.Dtm[c];
     const double stept = 2.0 * Dt / (3.0 * Dt / dtm + 2.0);
     const double f1t = 1. + (f.Rds[c] + f.Rds[tot + c] + f.Rds[2 * tot + c]) * (stept / vol) + (CnegTk + CnegTO) * stept;
+    d6_out = d6;
+    d7_out = d7;
+    f1t_out = f1t;
+}
+
+__global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
+                                                 const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    const long tot = f.tot;
+    double d6, d7, f1t;
+    sst_rhs_cell(p, f, b, i, j, k, c, d6, d7, f1t);
     if (has_w(f)) {  // hyperplane-major inputs of the line-march sweep
         const long w = widx(b, f.march, i, j, k);
         wput(f, W_F1T, w, f1t);
@@ -851,4 +872,31 @@ __global__ void __launch_bounds__(256) k_sst_rhs(trans_params p, Fields f, const
         f.D[5 * tot + c] = d6;
         f.D[6 * tot + c] = d7;
     }
+}
+
+// The same into the hyperplane-major buffer (march axis j) tile by tile (see k_lusgs_rhs_w).
+__global__ void __launch_bounds__(256) k_sst_rhs_w(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    __shared__ double sv[3][WT * WT];
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int ti = threadIdx.x % WT, tj = threadIdx.x / WT;
+    const TileCell tc = wtile_cell(s, b, ti, tj);
+    if (tc.ok) {
+        double d6, d7, f1t;
+        sst_rhs_cell(p, f, b, tc.i, tc.j, tc.k, cidx(b, tc.i, tc.j, tc.k), d6, d7, f1t);
+        sv[0][threadIdx.x] = f1t;
+        sv[1][threadIdx.x] = d6;
+        sv[2][threadIdx.x] = d7;
+    }
+    __syncthreads();
+    int di, dj;
+    wtile_diag(threadIdx.x, di, dj);
+    const TileCell oc = wtile_cell(s, b, di, dj);
+    if (!oc.ok) return;
+    const long w = widx(b, 1, oc.i, oc.j, oc.k);
+    wput(f, W_F1T, w, sv[0][dj * WT + di]);
+    wput(f, W_D + 5, w, sv[1][dj * WT + di]);
+    wput(f, W_D + 6, w, sv[2][dj * WT + di]);
 }

@@ -71,6 +71,7 @@ struct trans_ctx {
     OutLine* d_outl = nullptr;  // (patch, i, k) lines of the outlet patches
     int n_outl = 0;
     SegTable seg_cell, seg_face[3];
+    SegTable seg_wtile;  // WT x WT (i, j) tiles at each k: the tiled writers of the sweep buffer
     int* d_wrange = nullptr;
     NormPart* d_part = nullptr;
     trans_resid* d_resid = nullptr;
@@ -91,6 +92,10 @@ struct trans_ctx {
     int line_threads = 0;  // workgroup size of the line-march sweep (0: hyperplane fallback)
     bool plane_launch = false;  // hyperplane sweep as one launch per plane (else one workgroup per block)
     SegTable seg_plane;         // workgroups of the per-plane launches: (block, first line)
+    // plain drop-in (sweep -3, a baseline): every routine launched once per block, the sweeps one
+    // launch per hyperplane per block, one stream, no graph
+    bool plain = false;
+    std::vector<int> bc_wrange, outl_wrange;  // per-block ranges of the BC and outlet-line tables
     PlaneBlk* d_pblk = nullptr;
     int max_planes = 0;
     void* wbuf = nullptr;  // the sweep buffer W / Wf (reallocated when the march axis changes)
@@ -278,7 +283,30 @@ static SegTable make_segs(trans_ctx* c, int kind)  // kind: 0 cells, 1..3 faces 
         const HostBlock& h = c->hb[b];
         long n0 = h.ni - 1 + (kind == 1), n1 = h.nj - 1 + (kind == 2), n2 = h.nk - 1 + (kind == 3);
         long cnt = n0 * n1 * n2;
+        // faces of directions j and k run in FTA x FTB tiles of the (i, j) or (i, k) plane
+        // (decode_face_tile): whole tiles, padding included
+        if (kind == 2) cnt = (long)FTA * FTB * ((n0 + FTA - 1) / FTA) * ((n1 + FTB - 1) / FTB) * n2;
+        if (kind == 3) cnt = (long)FTA * FTB * ((n0 + FTA - 1) / FTA) * ((n2 + FTB - 1) / FTB) * n1;
         for (long base = 0; base < cnt; base += WG) segs.push_back(Seg{b, (int)base});
+        t.wrange.push_back((int)segs.size());
+    }
+    t.n = (int)segs.size();
+    if (dalloc(c, &t.d, segs.size()) == 0)
+        (void)h2d(c, t.d, segs.data(), segs.size() * sizeof(Seg));
+    return t;
+}
+
+// Schedule of the tiled sweep-buffer kernels: per block, ceil(NI1/WT) x ceil(NJ1/WT) tiles at
+// each k, entry Seg{block, tile index}.
+static SegTable make_wtile_segs(trans_ctx* c)
+{
+    std::vector<Seg> segs;
+    SegTable t;
+    t.wrange.push_back(0);
+    for (int b = 0; b < (int)c->hb.size(); b++) {
+        const HostBlock& h = c->hb[b];
+        const int nt = ((h.ni - 1 + WT - 1) / WT) * ((h.nj - 1 + WT - 1) / WT) * (h.nk - 1);
+        for (int q = 0; q < nt; q++) segs.push_back(Seg{b, q});
         t.wrange.push_back((int)segs.size());
     }
     t.n = (int)segs.size();
@@ -352,6 +380,7 @@ static int upload_blk_table(trans_ctx* c)
 // hyperplane, -1 one workgroup per block.
 static int cur_sweep(const trans_ctx* c)
 {
+    if (c->plain) return -3;
     if (c->tile_mode) return 3;
     return c->line_threads ? c->march : (c->plane_launch ? -2 : -1);
 }
@@ -664,6 +693,15 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         for (int k = pt.point_start[2]; k <= pt.point_end[2]; k++)
             for (int i = pt.point_start[0]; i <= pt.point_end[0]; i++) ol.push_back(OutLine{q, i, k});
     }
+    // per-block ranges of the BC segments and outlet lines (both are in block order)
+    c->bc_wrange.assign(nblk + 1, 0);
+    c->outl_wrange.assign(nblk + 1, 0);
+    for (const BcSeg& q : bs) c->bc_wrange[dp[q.patch].blk + 1]++;
+    for (const OutLine& q : ol) c->outl_wrange[dp[q.patch].blk + 1]++;
+    for (int b = 0; b < nblk; b++) {
+        c->bc_wrange[b + 1] += c->bc_wrange[b];
+        c->outl_wrange[b + 1] += c->outl_wrange[b];
+    }
     c->n_outl = (int)ol.size();
     if (c->n_outl) {
         if ((rc = dalloc(c, &c->d_outl, ol.size()))) return rc;
@@ -690,10 +728,12 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     if ((rc = upload_blk_table(c))) return rc;
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
+    c->seg_wtile = make_wtile_segs(c);
     {
         // per-plane sweep launches: lines (i, k) of every block, march axis j
         std::vector<PlaneBlk> pb;
         std::vector<Seg> ps;
+        c->seg_plane.wrange.push_back(0);
         for (int b = 0; b < nblk; b++) {
             const HostBlock& h = c->hb[b];
             const int n[3] = {h.ni - 1, h.nj - 1, h.nk - 1};
@@ -701,6 +741,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
             pb.push_back(q);
             c->max_planes = std::max(c->max_planes, q.nplanes);
             for (int base = 0; base < q.na * q.nb; base += WG) ps.push_back(Seg{b, base});
+            c->seg_plane.wrange.push_back((int)ps.size());
         }
         c->seg_plane.n = (int)ps.size();
         if ((rc = dalloc(c, &c->seg_plane.d, ps.size())) || (rc = dalloc(c, &c->d_pblk, pb.size()))) return rc;
@@ -1097,9 +1138,20 @@ static int lusgs_update_async(trans_ctx* c, int shift = 0)
 {
     const int nblk = (int)c->hb.size();
     const int if_turb = if_turb_now(c);
-    KLAUNCH(c, "k_lusgs_rhs", k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
-    if (if_turb)
-        KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+    // the sweep buffer along j moves through LDS tiles (whole cache lines per diagonal)
+    const bool wtiled = (c->f.W || c->f.Wf) && c->f.march == 1;
+    if (wtiled)
+        KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w, dim3(c->seg_wtile.n), dim3(WG), c->p, c->f, c->d_blk,
+                c->seg_wtile.d);
+    else
+        KLAUNCH(c, "k_lusgs_rhs", k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+    if (if_turb) {
+        if (wtiled)
+            KLAUNCH(c, "k_sst_rhs_w", k_sst_rhs_w, dim3(c->seg_wtile.n), dim3(WG), c->p, c->f, c->d_blk,
+                    c->seg_wtile.d);
+        else
+            KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+    }
     const int neq = if_turb ? 7 : 5;
     if (c->f.W || c->f.Wf) {
         // line-march sweeps on the hyperplane-major buffer, increments back to D after
@@ -1108,15 +1160,23 @@ static int lusgs_update_async(trans_ctx* c, int shift = 0)
                 launch_sweeps<float, 7>(c, c->f.Wf);
             else
                 launch_sweeps<float, 5>(c, c->f.Wf);
-            KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), 0, neq, c->f,
-                    (const float*)c->f.Wf, c->d_blk, c->seg_cell.d);
+            if (wtiled)
+                KLAUNCH(c, "k_unskew_w", k_unskew_w<float>, dim3(c->seg_wtile.n), dim3(WG), 0, neq, c->f,
+                        (const float*)c->f.Wf, c->d_blk, c->seg_wtile.d);
+            else
+                KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), 0, neq, c->f,
+                        (const float*)c->f.Wf, c->d_blk, c->seg_cell.d);
         } else {
             if (if_turb)
                 launch_sweeps<double, 7>(c, c->f.W);
             else
                 launch_sweeps<double, 5>(c, c->f.W);
-            KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), 0, neq, c->f,
-                    (const double*)c->f.W, c->d_blk, c->seg_cell.d);
+            if (wtiled)
+                KLAUNCH(c, "k_unskew_w", k_unskew_w<double>, dim3(c->seg_wtile.n), dim3(WG), 0, neq, c->f,
+                        (const double*)c->f.W, c->d_blk, c->seg_wtile.d);
+            else
+                KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), 0, neq, c->f,
+                        (const double*)c->f.W, c->d_blk, c->seg_cell.d);
         }
     } else if (c->p.implicit_fp32) {
         KLAUNCH(c, "k_convert<double,float>", (k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), neq, 0,
@@ -1228,9 +1288,130 @@ int trans_hip_update_u(trans_ctx* c)
 // The device work of one outer iteration (main.F90:65-197 without I/O), queued only.
 static int average_fields_async(trans_ctx* c);  // io.h
 
+}  // extern "C"
+
+// The plain drop-in of main.F90:65-197 (a baseline, selected by set_tuning with sweep -3): each
+// routine is one launch per block, as a straight port of the reference's per-block calls would
+// run, and the LU-SGS sweeps are one launch per hyperplane per block.  LU-SGS decks only.
+template <typename R>
+static void plain_sweeps(trans_ctx* c, R* W, int neq)
+{
+    for (int b = 0; b < (int)c->hb.size(); b++) {
+        const HostBlock& h = c->hb[b];
+        const int np = (h.ni - 1) + (h.nj - 1) + (h.nk - 1) - 2;
+        const int g0 = c->seg_plane.wrange[b], ng = c->seg_plane.wrange[b + 1] - g0;
+        for (int fwd = 1; fwd >= 0; fwd--)
+            for (int s0 = 0; s0 < np; s0++) {
+                if (neq == 7 && fwd)
+                    KLAUNCH(c, "k_lusgs_wplane<fwd>", (k_lusgs_wplane<R, true, 7>), dim3(ng), dim3(WG), c->f, W,
+                            c->d_blk, c->seg_plane.d + g0, c->d_pblk, s0);
+                else if (neq == 7)
+                    KLAUNCH(c, "k_lusgs_wplane<bwd>", (k_lusgs_wplane<R, false, 7>), dim3(ng), dim3(WG), c->f, W,
+                            c->d_blk, c->seg_plane.d + g0, c->d_pblk, s0);
+                else if (fwd)
+                    KLAUNCH(c, "k_lusgs_wplane<fwd>", (k_lusgs_wplane<R, true, 5>), dim3(ng), dim3(WG), c->f, W,
+                            c->d_blk, c->seg_plane.d + g0, c->d_pblk, s0);
+                else
+                    KLAUNCH(c, "k_lusgs_wplane<bwd>", (k_lusgs_wplane<R, false, 5>), dim3(ng), dim3(WG), c->f, W,
+                            c->d_blk, c->seg_plane.d + g0, c->d_pblk, s0);
+            }
+    }
+}
+
+template <int SPATIAL, int ROE>
+static void plain_conv(trans_ctx* c, int if_turb, int b)
+{
+    const SegTable* t = c->seg_face;
+    KLAUNCH(c, "k_conv_face<1>", (k_conv_face<1, SPATIAL, ROE>), dim3(t[0].wrange[b + 1] - t[0].wrange[b]), dim3(WG),
+            c->p, if_turb, c->f, c->d_blk, t[0].d + t[0].wrange[b]);
+    KLAUNCH(c, "k_conv_face<2>", (k_conv_face<2, SPATIAL, ROE>), dim3(t[1].wrange[b + 1] - t[1].wrange[b]), dim3(WG),
+            c->p, if_turb, c->f, c->d_blk, t[1].d + t[1].wrange[b]);
+    KLAUNCH(c, "k_conv_face<3>", (k_conv_face<3, SPATIAL, ROE>), dim3(t[2].wrange[b + 1] - t[2].wrange[b]), dim3(WG),
+            c->p, if_turb, c->f, c->d_blk, t[2].d + t[2].wrange[b]);
+}
+
+extern "C" {
+
+static int plain_body(trans_ctx* c)
+{
+    const int nblk = (int)c->hb.size();
+    const int if_turb = if_turb_now(c);
+    const bool visc = c->p.if_euler == 0;
+    const int neq = if_turb ? 7 : 5;
+    const int nsub = c->p.kind_dual != 0 ? c->p.kchld : 1;
+    auto cells = [&](const SegTable& t, int b) { return dim3(t.wrange[b + 1] - t.wrange[b]); };
+    for (int s = 0; s < nsub; s++) {
+        for (int b = 0; b < nblk; b++)
+            KLAUNCH(c, "k_spectral_radius", k_spectral_radius, cells(c->seg_cell, b), dim3(WG), c->p, c->ki, c->f,
+                    c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
+        for (int b = 0; b < nblk; b++) {
+            if (c->bc_wrange[b + 1] > c->bc_wrange[b])
+                KLAUNCH(c, "k_bc", k_bc, dim3(c->bc_wrange[b + 1] - c->bc_wrange[b]), dim3(128), c->p, if_turb, -1, c->f,
+                        c->d_blk, c->d_patch, c->d_bcseg + c->bc_wrange[b]);
+            if (c->outl_wrange[b + 1] > c->outl_wrange[b])
+                KLAUNCH(c, "k_bc_outlet", k_bc_outlet, dim3(c->outl_wrange[b + 1] - c->outl_wrange[b]), dim3(256), c->p,
+                        if_turb, c->f, c->d_blk, c->d_patch, c->d_outl + c->outl_wrange[b]);
+        }
+        if (visc) {
+            for (int b = 0; b < nblk; b++)
+                KLAUNCH(c, "k_gradients", k_gradients, cells(c->seg_cell, b), dim3(WG), c->p, if_turb, c->f, c->d_blk,
+                        c->seg_cell.d + c->seg_cell.wrange[b]);
+            for (int b = 0; b < nblk; b++)
+                if (c->bc_wrange[b + 1] > c->bc_wrange[b])
+                    KLAUNCH(c, "k_bc", k_bc, dim3(c->bc_wrange[b + 1] - c->bc_wrange[b]), dim3(128), c->p, if_turb, 0,
+                            c->f, c->d_blk, c->d_patch, c->d_bcseg + c->bc_wrange[b]);
+            for (int b = 0; b < nblk; b++) {
+                KLAUNCH(c, "k_visc_face<1>", k_visc_face<1>, cells(c->seg_face[0], b), dim3(WG), c->p, if_turb, c->f,
+                        c->d_blk, c->seg_face[0].d + c->seg_face[0].wrange[b]);
+                KLAUNCH(c, "k_visc_face<2>", k_visc_face<2>, cells(c->seg_face[1], b), dim3(WG), c->p, if_turb, c->f,
+                        c->d_blk, c->seg_face[1].d + c->seg_face[1].wrange[b]);
+                KLAUNCH(c, "k_visc_face<3>", k_visc_face<3>, cells(c->seg_face[2], b), dim3(WG), c->p, if_turb, c->f,
+                        c->d_blk, c->seg_face[2].d + c->seg_face[2].wrange[b]);
+            }
+        }
+        for (int b = 0; b < nblk; b++) {
+            const int sp = c->p.kind_spatial, ro = c->p.kind_roe;
+            if (sp == 2 && ro == 1) plain_conv<2, 1>(c, if_turb, b);
+            else if (sp == 2 && ro == 3) plain_conv<2, 3>(c, if_turb, b);
+            else if (sp == 3 && ro == 1) plain_conv<3, 1>(c, if_turb, b);
+            else if (sp == 3 && ro == 3) plain_conv<3, 3>(c, if_turb, b);
+            else return fail(c, TRANS_EUNSUPPORTED, "the plain drop-in runs MUSCL or MDCD with Roe or AUSMPW+");
+        }
+        for (int b = 0; b < nblk; b++)
+            KLAUNCH(c, "k_accumulate_visc", k_accumulate_visc, cells(c->seg_cell, b), dim3(WG), c->p, c->p.if_euler,
+                    if_turb, c->f, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
+        for (int b = 0; b < nblk; b++) {
+            KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w, cells(c->seg_wtile, b), dim3(WG), c->p, c->f, c->d_blk,
+                    c->seg_wtile.d + c->seg_wtile.wrange[b]);
+            if (if_turb)
+                KLAUNCH(c, "k_sst_rhs_w", k_sst_rhs_w, cells(c->seg_wtile, b), dim3(WG), c->p, c->f, c->d_blk,
+                        c->seg_wtile.d + c->seg_wtile.wrange[b]);
+        }
+        if (c->p.implicit_fp32)
+            plain_sweeps<float>(c, c->f.Wf, neq);
+        else
+            plain_sweeps<double>(c, c->f.W, neq);
+        const bool fuse = c->p.kind_dual == 1 || c->p.kind_dual == 2;
+        for (int b = 0; b < nblk; b++) {
+            if (c->p.implicit_fp32)
+                KLAUNCH(c, "k_unskew_w", k_unskew_w<float>, cells(c->seg_wtile, b), dim3(WG), 0, neq, c->f,
+                        (const float*)c->f.Wf, c->d_blk, c->seg_wtile.d + c->seg_wtile.wrange[b]);
+            else
+                KLAUNCH(c, "k_unskew_w", k_unskew_w<double>, cells(c->seg_wtile, b), dim3(WG), 0, neq, c->f,
+                        (const double*)c->f.W, c->d_blk, c->seg_wtile.d + c->seg_wtile.wrange[b]);
+            KLAUNCH(c, "k_update_v", k_update_v, cells(c->seg_cell, b), dim3(WG), c->p, c->ki, if_turb, c->f, c->d_blk,
+                    c->seg_cell.d + c->seg_cell.wrange[b], c->d_part + c->seg_cell.wrange[b],
+                    fuse && s == nsub - 1 ? c->p.kind_dual : 0);
+        }
+        KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk, c->d_resid);
+    }
+    return launch_check(c, "plain iteration");
+}
+
 static int iteration_body(trans_ctx* c)
 {
     int rc;
+    if (c->plain) return plain_body(c);
     const int nsub = c->p.if_rk == 1 ? 3 : (c->p.kind_dual != 0 ? c->p.kchld : 1);
     // with LU-SGS the dual-time shift of the last sub-iteration is fused into its update
     const bool fuse = c->p.if_rk != 1 && (c->p.kind_dual == 1 || c->p.kind_dual == 2);
@@ -1285,7 +1466,7 @@ int trans_hip_iterate(trans_ctx* c, int n_iter, trans_resid* per_block)
     if (!c || n_iter < 0) return TRANS_EINVAL;
     int rc;
     for (int it = 0; it < n_iter; it++) {
-        const bool replay = c->graphs && !c->prof && iteration_is_static(c);
+        const bool replay = c->graphs && !c->plain && !c->prof && iteration_is_static(c);
         if (replay && !c->gexec && capture_iteration(c) != 0) c->graphs = false;  // eager from now on
         if (replay && c->gexec) {
             HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));

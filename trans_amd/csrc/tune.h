@@ -48,7 +48,10 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
     const int ki0 = c->ki;
     const bool stats0 = c->stats_on;
     const bool graphs0 = c->graphs;
-    const int sweep0 = cur_sweep(c);
+    // tuning leaves the plain drop-in schedule for the batched candidates
+    const bool plain0 = c->plain;
+    c->plain = false;
+    const int sweep0 = plain0 ? -2 : cur_sweep(c);
     const int ta0 = c->tile_a, tb0 = c->tile_b, ch0 = c->tile_chunk;
     const bool single0 = c->aux[0] == c->stream;
     c->stats_on = false;
@@ -120,6 +123,7 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
         rc2 = set_sweep(c, sweep0, true);
         apply_streams(c, single0);
         c->graphs = graphs0;
+        c->plain = plain0;
     }
     if (!rc2) rc2 = copy_state(true);
     if (!rc2) {
@@ -143,7 +147,20 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
 int trans_hip_set_tuning(trans_ctx* c, const trans_tune_entry* t)
 {
     if (!c || !t) return TRANS_EINVAL;
-    if (t->sweep < -2 || t->sweep > 3) return fail(c, TRANS_EINVAL, "sweep must be -2, -1, 0, 1, 2 or 3");
+    if (t->sweep < -3 || t->sweep > 3) return fail(c, TRANS_EINVAL, "sweep must be -3, -2, -1, 0, 1, 2 or 3");
+    if (t->sweep == -3) {  // the plain drop-in: per-block launches, per-plane sweeps, one stream, no graph
+        if (c->p.if_rk != 0 || c->p.if_precondition)
+            return fail(c, TRANS_EUNSUPPORTED, "the plain drop-in runs LU-SGS decks without preconditioning");
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        int rc = set_sweep(c, -2, true);
+        if (rc) return rc;
+        c->plain = true;
+        apply_streams(c, true);
+        c->graphs = false;
+        drop_graph(c);
+        return 0;
+    }
+    c->plain = false;
     if (t->streams != 1 && t->streams != 4) return fail(c, TRANS_EINVAL, "streams must be 1 or 4");
     if (t->streams == 4 && !c->aux_own[0]) return fail(c, TRANS_EINVAL, "this context has one stream");
     if (t->sweep == 3 && (t->tile_a < 1 || t->tile_b < 1 || t->tile_a * t->tile_b > 256 || t->chunk < 1))
