@@ -196,14 +196,26 @@ def cpu_baseline(blocks, prm, budget_s=12.0):
 
 def measured_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (profiles/hbm_traffic.json, counters scaled by a plain-copy calibration), or None."""
+    summary (profiles/hbm_traffic.json, counters scaled by a plain-copy calibration), or None.
+    Names match on the kernel name and on its direction: the sweep direction (fwd = a `true`
+    template argument) or the face direction (the first template argument)."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         tab = json.load(fh).get(workload, {})
+    kb, _, ka = kernel.partition("<")
+    ka = ka.rstrip(">")
     for name, rec in tab.items():
-        if name.startswith(kernel.split("<")[0]) and (("true" in name) == ("fwd" in kernel)):
+        nb, _, na = name.partition("<")
+        args = [a.strip() for a in na.rstrip(">").split(",")] if na else []
+        if nb != kb:
+            continue
+        if ka in ("fwd", "bwd"):
+            ok = ("true" in args) == (ka == "fwd")
+        else:
+            ok = not ka or (bool(args) and args[0] == ka)
+        if ok:
             return rec["bytes"]
     return None
 
