@@ -177,12 +177,18 @@ typedef struct trans_resid {
  * (Output/OUTPUT_Residual_NS.F90:19-32): q_f = (Vol(index-1) q(index) + Vol(index) q(index-1))
  * / (Vol(index-1) + Vol(index)); dm = (rho u)_f . S.  Total pressure and temperature use the
  * reference's output formula p0 = p (1 + 0.2 M^2)^3.5 (Output/Average_Field.F90:66) and
- * T0 = T (1 + 0.2 M^2).  Nondimensional, as the state is. */
+ * T0 = T (1 + 0.2 M^2), with the stored (relative-frame) velocity.  The absolute-frame
+ * averages (no counterpart in the reference; the rotor's stage performance needs them) use
+ * the absolute velocity u, v - omega1 z_f, w + omega1 y_f, where y_f, z_f are the same
+ * weighting of the cell centres; they equal the relative ones when omega1 = 0.
+ * Nondimensional, as the state is. */
 typedef struct trans_plane_integrals {
     double mass_flow;   /* sum of dm (the reference's msum before its Roref*Vref scaling) */
     double area;        /* sum of |S| */
-    double p0_mass_avg; /* sum(p0 dm) / sum(dm) */
-    double t0_mass_avg; /* sum(T0 dm) / sum(dm) */
+    double p0_mass_avg; /* sum(p0 dm) / sum(dm), relative frame */
+    double t0_mass_avg; /* sum(T0 dm) / sum(dm), relative frame */
+    double p0_abs_mass_avg; /* the same with the absolute velocity */
+    double t0_abs_mass_avg;
 } trans_plane_integrals;
 
 typedef struct trans_ctx trans_ctx;
@@ -278,6 +284,27 @@ int trans_hip_set_stats(trans_ctx* ctx, int blk, const trans_fview* in);
 int trans_hip_write_stats(trans_ctx* ctx, int blk, const trans_fview* xc, const trans_fview* yc,
                           const trans_fview* zc, const char* path);
 int trans_hip_read_stats(trans_ctx* ctx, int blk, const char* path);
+
+/* ---- Tecplot ASCII field output ------------------------------------------------------------
+ * Output/OUTPUT_InstField.F90:150-370 (kind_field = 0) writes each block's point coordinates
+ * and 31 cell-centred variables through the closed TecIO library; this writes the same
+ * variables in Tecplot's documented ASCII format: TITLE, VARIABLES = "X" "Y" "Z" density U V
+ * W Uabs Vabs Wabs Ma Mabs Ps P0rel P0 Ts T0rel T0 Ox Oy Oz Om Q Yita4 Rmiu k shock fr uu vv
+ * ww uv uw vw, one ordered zone I=NI, J=NJ, K=NK with DATAPACKING=BLOCK and the 31 field
+ * variables CELLCENTERED, values in i-fastest order, five per line as %.10e.
+ * x, y, z: the block's point coordinates XX, YY, ZZ (views covering 1..NI, 1..NJ, 1..NK);
+ * scales: {Roref, Vref, Pref, Tinf} as the reference multiplies them in (NULL: nondimensional).
+ * The variables follow the reference's formulas, including its absolute-frame quirks (Mabs is
+ * |c| unscaled, P0 = Ps (1 + Mabs^2) * 3.5, T0 = Ts (1 + Mabs^2)); fr is 1 (no rotation-
+ * curvature model); velocity gradients are the DQdxyz of the last residual evaluation. */
+int trans_hip_write_tecplot(trans_ctx* ctx, int blk, const trans_fview* x, const trans_fview* y,
+                            const trans_fview* z, const double* scales, const char* path);
+/* The host-side formatter of the same file, usable without a GPU: point arrays x, y, z of
+ * ni*nj*nk values and ncellvar cell-centred arrays of (ni-1)(nj-1)(nk-1) values (cellvars,
+ * variable-major), all i fastest; names may not contain '"'. */
+int trans_tecplot_write_ascii(const char* path, const char* title, const char* zone, int ni, int nj, int nk,
+                              const double* x, const double* y, const double* z, int ncellvar,
+                              const char* const* cellvar_names, const double* cellvars);
 
 /* ---- Launch autotuner (no counterpart in the reference) ------------------------------------
  * Times `iters` iterations of every launch configuration of the context from the same saved

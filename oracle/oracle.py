@@ -74,7 +74,7 @@ def lib():
         L.ora_residual.argtypes = [PP, C.c_int, PB, C.c_int]
         L.ora_lusgs_update.argtypes = [PP, C.c_int, PB, C.c_int]
         L.ora_iterate.argtypes = [PP, C.POINTER(C.c_int), C.c_int, PB, C.c_int]
-        L.ora_plane_integrals.argtypes = [PB, C.c_int, C.c_int, DP]
+        L.ora_plane_integrals.argtypes = [PB, C.c_int, C.c_int, C.c_double, DP]
         L.ora_rk_save.argtypes = [C.c_int, PB]
         L.ora_rk.argtypes = [C.c_int, PB]
         L.ora_rk_update_v.argtypes = [PP, C.c_int, C.c_int, PB]
@@ -186,9 +186,10 @@ class OracleCase:
         return k.value
 
     def plane_integrals(self, n, direction, index):
-        out = (C.c_double * 4)()
-        lib().ora_plane_integrals(C.byref(self.ob[n]), direction, index, out)
-        return dict(zip(("mass_flow", "area", "p0_mass_avg", "t0_mass_avg"), out[:]))
+        out = (C.c_double * 6)()
+        lib().ora_plane_integrals(C.byref(self.ob[n]), direction, index, C.c_double(self.params.omega1), out)
+        return dict(zip(("mass_flow", "area", "p0_mass_avg", "t0_mass_avg", "p0_abs_mass_avg",
+                         "t0_abs_mass_avg"), out[:]))
 
     def call(self, name, n, *args):
         """Run one oracle routine `name`(params, *args, block n)."""

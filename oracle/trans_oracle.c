@@ -1520,13 +1520,15 @@ void ora_rk_update(const trans_params* p, int ki, int itr_chld, ora_block* block
 /* ------------------------------------------------------------------------------------ */
 /* Plane integrals.  Face state: the volume weighting of Output/OUTPUT_Residual_NS.F90:19-32
  * (k outer, then the other in-plane index); p0 = p (1 + 0.2 M^2)^3.5 as
- * Output/Average_Field.F90:60-66 forms it per cell, T0 = T (1 + 0.2 M^2). */
-void ora_plane_integrals(ora_block* b, int dir, int index, double out[4])
+ * Output/Average_Field.F90:60-66 forms it per cell, T0 = T (1 + 0.2 M^2), with the stored
+ * relative velocity; out[4], out[5] the same with the absolute velocity u, v - omega z_f,
+ * w + omega y_f (face centre by the same weighting; rotation omega about x). */
+void ora_plane_integrals(ora_block* b, int dir, int index, double omega, double out[6])
 {
     const int NI1 = b->ni - 1, NJ1 = b->nj - 1, NK1 = b->nk - 1;
     const int n0 = dir == 1 ? NJ1 : NI1;  /* fast in-plane index */
     const int n1 = dir == 3 ? NJ1 : NK1;  /* slow in-plane index */
-    double msum = 0.0, area = 0.0, psum = 0.0, tsum = 0.0;
+    double msum = 0.0, area = 0.0, psum = 0.0, tsum = 0.0, pasum = 0.0, tasum = 0.0;
     for (int q1 = 1; q1 <= n1; q1++)
         for (int q0 = 1; q0 <= n0; q0++) {
             int i, j, k, im, jm, km;
@@ -1550,11 +1552,19 @@ void ora_plane_integrals(ora_block* b, int dir, int index, double out[4])
             area = area + sqrt(s1 * s1 + s2 * s2 + s3 * s3);
             psum = psum + pf * pow(fac, 3.5) * dm;
             tsum = tsum + tf * fac * dm;
+            double yf = (wei1 * YC(i, j, k) + wei2 * YC(im, jm, km)) / wei;
+            double zf = (wei1 * ZC(i, j, k) + wei2 * ZC(im, jm, km)) / wei;
+            double va = v - omega * zf, wa = w + omega * yf;
+            double fa = 1.0 + 0.2 * ((u * u + va * va + wa * wa) / (1.4 * pf / ro));
+            pasum = pasum + pf * pow(fa, 3.5) * dm;
+            tasum = tasum + tf * fa * dm;
         }
     out[0] = msum;
     out[1] = area;
     out[2] = psum / msum;
     out[3] = tsum / msum;
+    out[4] = pasum / msum;
+    out[5] = tasum / msum;
 }
 
 /* Threads an OpenMP region of the oracle uses. */

@@ -109,7 +109,7 @@ void ora_set_threads(int n);
 void ora_iterate(const trans_params* p, int* ki, int n_iter, ora_block* blocks, int nblk);
 
 /* Plane integrals of include/trans_hip.h: out = {mass_flow, area, p0_mass_avg, t0_mass_avg}. */
-void ora_plane_integrals(ora_block* b, int dir, int index, double out[4]);
+void ora_plane_integrals(ora_block* b, int dir, int index, double omega, double out[6]);
 
 /* Reference functions exposed for unit tests. */
 double ora_alimiter(double var1, double var2);

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     src = open(os.path.join(ROOT, "include", "trans_hip.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(trans_(?:hip|srv|geom)_\w+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(trans_(?:hip|srv|geom|tecplot)_\w+)\s*\(", src)))
 
 
 def test_header_matches_python_binding():

@@ -509,3 +509,47 @@ def test_periodic_gradient_rotation():
                     g = src[nd][9 + 3 * q:12 + 3 * q, i + O, j + O, kd + O]
                     np.testing.assert_allclose(got[9 + 3 * q:12 + 3 * q], R @ g, rtol=1e-13, atol=1e-14)
     assert slip > 1e-3  # the test tells the written entry from the rotated one
+
+
+@pytest.mark.parametrize("direction", [1, 2, 3])
+def test_plane_totals_absolute_frame(direction):
+    """Swirl-free absolute flow (U, 0, 0) seen in the rotating frame, V_rel = (U, omega z,
+    -omega y) from the cell centres: through any plane the absolute-frame totals are those of
+    the axial speed alone, p0 = p (1 + 0.2 U^2 / a^2)^3.5, T0 = T (1 + 0.2 U^2 / a^2), while
+    the relative-frame totals carry the blade speed (omega r)^2 of every face.  The face
+    centre is the same crossed volume weighting as the face state, which reproduces a linear
+    field at the face, so the absolute swirl cancels to rounding.  At omega = 0 both frames
+    give the same bits."""
+    b, prm, edges = _rbox()
+    om = prm.omega1
+    rho, U, p = 1.1, 0.6, 1.0 / (1.4 * 0.25)
+    yc = _edge_centres(edges[1])[None, :, None] * np.ones((b.ni - 1, 1, b.nk - 1))
+    zc = _edge_centres(edges[2])[None, None, :] * np.ones((b.ni - 1, b.nj - 1, 1))
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    V = b["v"]
+    V[...] = 0.0
+    V[0][...] = rho
+    V[0][I, J, K] = rho
+    vr, wr = om * zc, -om * yc
+    V[1][...] = rho * U
+    V[2][I, J, K] = rho * vr
+    V[3][I, J, K] = rho * wr
+    b["pp"][...] = p
+    b["t"][...] = prm.rxm2 * p / rho
+    oc = OracleCase([b], prm)
+    idx = 3
+    r = oc.plane_integrals(0, direction, idx)
+    a2 = 1.4 * p / rho
+    if direction == 1:  # the axial planes carry the mass flow
+        assert r["p0_abs_mass_avg"] == pytest.approx(p * (1 + 0.2 * U * U / a2) ** 3.5, rel=1e-13)
+        assert r["t0_abs_mass_avg"] == pytest.approx(prm.rxm2 * p / rho * (1 + 0.2 * U * U / a2), rel=1e-13)
+        fac = 1 + 0.2 * (U * U + vr[0] ** 2 + wr[0] ** 2) / a2
+        area = np.diff(edges[1])[:, None] * np.diff(edges[2])[None, :]  # dm = rho U dA
+        want = np.sum(p * fac ** 3.5 * area) / np.sum(area)
+        assert r["p0_mass_avg"] == pytest.approx(want, rel=1e-12)
+        assert r["p0_mass_avg"] > r["p0_abs_mass_avg"] * (1 + 1e-3)
+    b0, prm0, _ = _rbox(omega_rpm=0.0)
+    for nm in ("v", "pp", "t"):
+        b0[nm][...] = b[nm]
+    r0 = OracleCase([b0], prm0).plane_integrals(0, direction, idx)
+    assert r0["p0_abs_mass_avg"] == r0["p0_mass_avg"] and r0["t0_abs_mass_avg"] == r0["t0_mass_avg"]

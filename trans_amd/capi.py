@@ -93,7 +93,7 @@ class TransResid(C.Structure):
 
 class TransPlaneIntegrals(C.Structure):
     _fields_ = [("mass_flow", C.c_double), ("area", C.c_double), ("p0_mass_avg", C.c_double),
-                ("t0_mass_avg", C.c_double)]
+                ("t0_mass_avg", C.c_double), ("p0_abs_mass_avg", C.c_double), ("t0_abs_mass_avg", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -157,6 +157,7 @@ EXPORTS = [
     "trans_srv_create", "trans_srv_submit", "trans_srv_progress", "trans_srv_poll", "trans_srv_cancel",
     "trans_srv_result", "trans_srv_release", "trans_srv_destroy", "trans_srv_last_error",
     "trans_geom_wall_distance", "trans_geom_metrics", "trans_geom_wall_distance_static",
+    "trans_hip_write_tecplot", "trans_tecplot_write_ascii",
 ]
 
 
@@ -224,6 +225,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.trans_hip_write_stats.argtypes = [C.c_void_p, C.c_int, P(TransFView), P(TransFView), P(TransFView),
                                           C.c_char_p]
     lib.trans_hip_read_stats.argtypes = [C.c_void_p, C.c_int, C.c_char_p]
+    lib.trans_hip_write_tecplot.argtypes = [C.c_void_p, C.c_int, P(TransFView), P(TransFView), P(TransFView),
+                                            P(C.c_double), C.c_char_p]
+    lib.trans_tecplot_write_ascii.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int,
+                                              P(C.c_double), P(C.c_double), P(C.c_double), C.c_int,
+                                              P(C.c_char_p), P(C.c_double)]
     lib.trans_hip_autotune.argtypes = [C.c_void_p, C.c_int, P(TransTuneEntry), C.c_int, P(C.c_int)]
     lib.trans_hip_get_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]
     lib.trans_hip_set_tuning.argtypes = [C.c_void_p, P(TransTuneEntry)]

@@ -6,15 +6,18 @@
 // One workgroup: each thread sums a strided share of the plane's faces, then a fixed-order
 // tree reduction, so repeated calls give identical bits.  Face state and totals as
 // ora_plane_integrals restates them (Output/OUTPUT_Residual_NS.F90:19-32,
-// Output/Average_Field.F90:60-66).  out = {mass_flow, area, p0_mass_avg, t0_mass_avg}.
-__global__ void __launch_bounds__(256) k_plane_integrals(Fields f, BlkInfo b, int dir, int index, double* out)
+// Output/Average_Field.F90:60-66), and the same totals with the absolute velocity (rotation
+// omega about x).  out = {mass_flow, area, p0_mass_avg, t0_mass_avg, p0_abs, t0_abs}.
+__global__ void __launch_bounds__(256) k_plane_integrals(Fields f, BlkInfo b, int dir, int index, double omega,
+                                                         double* out)
 {
-    __shared__ double sh[4][256];
+    constexpr int NQ = 6;
+    __shared__ double sh[NQ][256];
     const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
     const int n0 = dir == 1 ? NJ1 : NI1;
     const int n1 = dir == 3 ? NJ1 : NK1;
     const long tot = f.tot;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double acc[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int q = threadIdx.x; q < n0 * n1; q += blockDim.x) {
         const int q0 = q % n0 + 1, q1 = q / n0 + 1;
         const int i = dir == 1 ? index : q0;
@@ -39,12 +42,18 @@ __global__ void __launch_bounds__(256) k_plane_integrals(Fields f, BlkInfo b, in
         acc[1] += sqrt(s1 * s1 + s2 * s2 + s3 * s3);
         acc[2] += pf * pow(fac, 3.5) * dm;
         acc[3] += tf * fac * dm;
+        const double yf = (wei1 * f.Yc[c] + wei2 * f.Yc[cm]) / wei;
+        const double zf = (wei1 * f.Zc[c] + wei2 * f.Zc[cm]) / wei;
+        const double va = v - omega * zf, wa = w + omega * yf;
+        const double fa = 1.0 + 0.2 * ((u * u + va * va + wa * wa) / (1.4 * pf / ro));
+        acc[4] += pf * pow(fa, 3.5) * dm;
+        acc[5] += tf * fa * dm;
     }
-    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = acc[q];
+    for (int q = 0; q < NQ; q++) sh[q][threadIdx.x] = acc[q];
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+            for (int q = 0; q < NQ; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -52,5 +61,7 @@ __global__ void __launch_bounds__(256) k_plane_integrals(Fields f, BlkInfo b, in
         out[1] = sh[1][0];
         out[2] = sh[2][0] / sh[0][0];
         out[3] = sh[3][0] / sh[0][0];
+        out[4] = sh[4][0] / sh[0][0];
+        out[5] = sh[5][0] / sh[0][0];
     }
 }

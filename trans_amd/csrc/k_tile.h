@@ -69,6 +69,13 @@ __device__ __forceinline__ void st_edge(R* p, R v)
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The edge terms a lane reads at one step (across its tile's upstream a and b edges).
+template <typename R, int NEQ>
+struct EdgeIn {
+    R a[NEQ];
+    R b[NEQ];
+};
+
 template <typename R, bool FORWARD, int NEQ>
 __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, const BlkInfo& b,
                                            const SweepTile& tl, int me, const SweepTile* __restrict__ tiles,
@@ -124,7 +131,24 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
     R tm[NEQ];
 #pragma unroll
     for (int L = 0; L < NEQ; L++) tm[L] = R(0);
-    auto step = [&](const int t, const LineRaw<R, NEQ>& cur, LineRaw<R, NEQ>& nxt) {
+    // edge terms of step t: every step of a chunk is covered by the chunk's wait, so the terms
+    // of the next step are loaded one step ahead inside a chunk (at a chunk's first step they
+    // are loaded after its wait)
+    auto load_edges = [&](const int t, EdgeIn<R, NEQ>& e) {
+        const int rel = t - start;
+        const bool act = active && rel >= 0 && rel < NJ1;
+        const int jrow = act ? rel : 0;
+        if (edge_in_a && act) {
+#pragma unroll
+            for (int L = 0; L < NEQ; L++) e.a[L] = ld_edge(in_a + ((long)L * NJ1 + jrow) * tl.nb + ib);
+        }
+        if (edge_in_b && act) {
+#pragma unroll
+            for (int L = 0; L < NEQ; L++) e.b[L] = ld_edge(in_b + ((long)L * NJ1 + jrow) * tl.na + ia);
+        }
+    };
+    auto step = [&](const int t, const LineRaw<R, NEQ>& cur, LineRaw<R, NEQ>& nxt, EdgeIn<R, NEQ>& ec,
+                    EdgeIn<R, NEQ>& en) {
         // chunk start: wait until the upstream tiles have published the steps this chunk reads
         if ((t - t0) % chunk == 0) {
             if (threadIdx.x == 0) {
@@ -153,6 +177,7 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
                 }
             }
             __syncthreads();
+            load_edges(t, ec);
         }
         const int par = t & 1;
         const int rel = t - start;
@@ -160,6 +185,7 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
         const int s = FORWARD ? t : nplanes - 1 - t;
         const int sn = FORWARD ? min(s + 1, nplanes - 1) : max(s - 1, 0);
         line_load<R, NEQ>(src, lw + (unsigned)sn * (unsigned)nlines, nxt);
+        if (t + 1 <= t1 && (t + 1 - t0) % chunk != 0) load_edges(t + 1, en);
         const unsigned wst = act ? lw + (unsigned)s * (unsigned)nlines : my_sink;
         const int jrow = act ? rel : 0;  // j - 1 of the lane's cell
         const double stept = cur.stept;
@@ -170,8 +196,8 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
         for (int L = 0; L < NEQ; L++) {
             R sa = buf[((par * 2 + 0) * NEQ + L) * row + ra];
             R sb = buf[((par * 2 + 1) * NEQ + L) * row + rb];
-            if (edge_in_a && act) sa = ld_edge(in_a + ((long)L * NJ1 + jrow) * tl.nb + ib);
-            if (edge_in_b && act) sb = ld_edge(in_b + ((long)L * NJ1 + jrow) * tl.na + ia);
+            if (edge_in_a && act) sa = ec.a[L];
+            if (edge_in_b && act) sb = ec.b[L];
             const R sm = rel == 0 ? R(0) : tm[L];
             const R sum = sa + sm + sb;  // i, j, k order
             if constexpr (MF) {
@@ -234,13 +260,14 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
         }
     };
     LineRaw<R, NEQ> bufA, bufB;
+    EdgeIn<R, NEQ> edA, edB;
     {
         const int s = FORWARD ? t0 : nplanes - 1 - t0;
         line_load<R, NEQ>(src, lw + (unsigned)s * (unsigned)nlines, bufA);
     }
     for (int t = t0; t <= t1; t += 2) {
-        step(t, bufA, bufB);
-        if (t + 1 <= t1) step(t + 1, bufB, bufA);
+        step(t, bufA, bufB, edA, edB);
+        if (t + 1 <= t1) step(t + 1, bufB, bufA, edB, edA);
     }
 }
 

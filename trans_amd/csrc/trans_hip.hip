@@ -75,7 +75,7 @@ struct trans_ctx {
     int* d_wrange = nullptr;
     NormPart* d_part = nullptr;
     trans_resid* d_resid = nullptr;
-    double* d_mon = nullptr;  // 4 doubles: result of k_plane_integrals
+    double* d_mon = nullptr;  // 6 doubles: result of k_plane_integrals
     hipStream_t stream = nullptr;
     // helper streams for independent kernels of one call (forked from and joined back
     // into `stream` inside the call, so that calls stay ordered on `stream`)
@@ -752,7 +752,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     if ((rc = dalloc(c, &c->d_wrange, nblk + 1))) return rc;
     if ((rc = h2d(c, c->d_wrange, c->seg_cell.wrange.data(), (nblk + 1) * sizeof(int)))) return rc;
     if ((rc = dalloc(c, &c->d_part, c->seg_cell.n)) || (rc = dalloc(c, &c->d_resid, nblk)) ||
-        (rc = dalloc(c, &c->d_mon, 4)) || (rc = dalloc(c, &c->d_err, 1)))
+        (rc = dalloc(c, &c->d_mon, 6)) || (rc = dalloc(c, &c->d_err, 1)))
         return rc;
     if (!dry) HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -1523,14 +1523,17 @@ int trans_hip_plane_integrals(trans_ctx* c, int blk, int dir, int index, trans_p
     const int n = dir == 1 ? h.ni : (dir == 2 ? h.nj : h.nk);
     if (index < 1 || index > n) return fail(c, TRANS_EINVAL, "face index out of range");
     BlkInfo b{h.ni, h.nj, h.nk, h.nx, h.ny, h.nz, h.off, 0, 0, 0, 0};
-    KLAUNCH(c, "k_plane_integrals", k_plane_integrals, dim3(1), dim3(256), c->f, b, dir, index, c->d_mon);
-    double r[4];
+    KLAUNCH(c, "k_plane_integrals", k_plane_integrals, dim3(1), dim3(256), c->f, b, dir, index, c->p.omega1,
+            c->d_mon);
+    double r[6];
     HIPCHK(c, hipMemcpyAsync(r, c->d_mon, sizeof(r), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     out->mass_flow = r[0];
     out->area = r[1];
     out->p0_mass_avg = r[2];
     out->t0_mass_avg = r[3];
+    out->p0_abs_mass_avg = r[4];
+    out->t0_abs_mass_avg = r[5];
     return 0;
 }
 
@@ -1547,3 +1550,4 @@ int trans_hip_sync(trans_ctx* c)
 #include "io.h"
 #include "tune.h"
 #include "geom.h"
+#include "tecplot.h"

@@ -58,7 +58,7 @@ module trans_hip_mod
     end type trans_geom_view
 
     type, bind(C), public :: trans_plane_integrals
-        real(c_double) :: mass_flow, area, p0_mass_avg, t0_mass_avg
+        real(c_double) :: mass_flow, area, p0_mass_avg, t0_mass_avg, p0_abs_mass_avg, t0_abs_mass_avg
     end type trans_plane_integrals
 
     type, bind(C), public :: trans_state_view
@@ -245,6 +245,18 @@ module trans_hip_mod
             type(trans_fview), intent(in) :: xc, yc, zc
             character(kind=c_char), intent(in) :: path(*)
         end function
+        ! Tecplot ASCII instantaneous field of block blk (0-based): point coordinates XX, YY, ZZ
+        ! and the 31 cell-centred variables of Output/OUTPUT_InstField.F90; scales = (/Roref, Vref,
+        ! Pref, Tinf/)
+        integer(c_int) function trans_hip_write_tecplot(ctx, blk, x, y, z, scales, path) &
+                bind(C, name="trans_hip_write_tecplot")
+            import :: c_int, c_ptr, c_char, c_double, trans_fview
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: blk
+            type(trans_fview), intent(in) :: x, y, z
+            real(c_double), intent(in) :: scales(4)
+            character(kind=c_char), intent(in) :: path(*)
+        end function
         integer(c_int) function trans_hip_read_stats(ctx, blk, path) bind(C, name="trans_hip_read_stats")
             import :: c_int, c_ptr, c_char
             type(c_ptr), value :: ctx
@@ -323,7 +335,7 @@ module trans_hip_mod
         trans_hip_residual, trans_hip_lusgs_update, trans_hip_rk_update, trans_hip_update_u, trans_hip_iterate, &
         trans_hip_plane_integrals, trans_hip_sync, trans_hip_footprint, trans_srv_create, trans_srv_submit, &
         trans_hip_write_restart, trans_hip_read_restart, trans_hip_stats_enable, trans_hip_average_fields, &
-        trans_hip_get_stats, trans_hip_set_stats, trans_hip_write_stats, trans_hip_read_stats, &
+        trans_hip_get_stats, trans_hip_set_stats, trans_hip_write_stats, trans_hip_read_stats, trans_hip_write_tecplot, &
         trans_srv_progress, trans_srv_poll, trans_srv_cancel, trans_srv_result, trans_srv_release, trans_srv_destroy, &
         trans_geom_wall_distance, trans_geom_metrics
 

@@ -16,10 +16,11 @@ class TransError(RuntimeError):
 
 
 def stage_performance(inlet, outlet, gamma=1.4):
-    """Total-pressure ratio and isentropic efficiency between two plane integrals:
-    pi = p0_out / p0_in, eta = (pi^((gamma-1)/gamma) - 1) / (T0_out / T0_in - 1)."""
-    pi = outlet["p0_mass_avg"] / inlet["p0_mass_avg"]
-    tau = outlet["t0_mass_avg"] / inlet["t0_mass_avg"]
+    """Total-pressure ratio and isentropic efficiency between two plane integrals, from the
+    absolute-frame totals: pi = p0_out / p0_in, eta = (pi^((gamma-1)/gamma) - 1) / (T0_out /
+    T0_in - 1)."""
+    pi = outlet["p0_abs_mass_avg"] / inlet["p0_abs_mass_avg"]
+    tau = outlet["t0_abs_mass_avg"] / inlet["t0_abs_mass_avg"]
     return {"pressure_ratio": pi, "temperature_ratio": tau,
             "efficiency": (pi ** ((gamma - 1.0) / gamma) - 1.0) / (tau - 1.0) if tau != 1.0 else float("nan")}
 
@@ -173,6 +174,17 @@ class Solver:
         keep = [np.asfortranarray(x, dtype=np.float64) for x in (xc, yc, zc)]
         views = [capi.fview(x) for x in keep]
         self._check(self.lib.trans_hip_write_stats(self.ctx, blk, *[C.byref(v) for v in views], str(path).encode()))
+
+    def write_tecplot(self, blk, path, scales=None):
+        """Tecplot ASCII file of block blk's instantaneous field (point coordinates from the
+        block, 31 cell-centred variables computed on the device); scales = (Roref, Vref, Pref,
+        Tinf) or None for nondimensional values."""
+        b = self.blocks[blk]
+        keep = [np.asfortranarray(a, dtype=np.float64) for a in (b.xx, b.yy, b.zz)]
+        views = [capi.fview(a) for a in keep]
+        sc = None if scales is None else (C.c_double * 4)(*scales)
+        self._check(self.lib.trans_hip_write_tecplot(self.ctx, blk, *[C.byref(v) for v in views], sc,
+                                                     str(path).encode()))
 
     def read_stats(self, blk, path):
         self._check(self.lib.trans_hip_read_stats(self.ctx, blk, str(path).encode()))
