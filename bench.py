@@ -28,9 +28,10 @@ ITER_BYTES_RANS = 8 * (10 + 20 + 10) + 2 * 8 * 7    # 432
 ITER_BYTES_EULER = 8 * (7 + 17 + 7) + 2 * 8 * 5    # 328
 
 
-def alg_bytes(name, neq, fp32):
+def alg_bytes(name, neq, fp32, dual_shift=True):
     """Algorithmic bytes per unit of work of a kernel: every input it needs read once and
-    every output written once, in the precision it is stored in.  Returns (unit, bytes)."""
+    every output written once, in the precision it is stored in.  Returns (unit, bytes).
+    dual_shift: the update carries the dual-time history shift (every deck here)."""
     d = 8
     r = 4 if fp32 else 8
     turb = neq == 7
@@ -69,8 +70,10 @@ def alg_bytes(name, neq, fp32):
         # D(neq), Dtm, Vol, F1(,F1t), V(1:4), PP, SD(9), Rds(3) in the natural layout
         "k_lusgs_sweep<fwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
         "k_lusgs_sweep<bwd>": ("cell", 2 * neq * r + d * (1 + 1 + (2 if turb else 1) + 4 + 1 + 9) + 3 * r),
-        # D(7), V(7), PP, rad, DQ(9), Dst in; V(7), PP, T, Rmiu out
-        "k_update_v": ("cell", d * (7 + 7 + 1 + 1 + 9 + 1 + 7 + 1 + 1 + 1)),
+        # D(7), V(7), PP, rad, DQ(9), Dst in; V(7), PP, T, Rmiu out; with the dual-time shift
+        # fused into the last sub-iteration (Kind_Dual 1, 2) also Wt0(7), Wt1(7) in and Wt0, Wt1,
+        # Wt2 out
+        "k_update_v": ("cell", d * (7 + 7 + 1 + 1 + 9 + 1 + 7 + 1 + 1 + 1 + (35 if dual_shift else 0))),
         # V(7), Wt0(7) in; Wt0(7), Wt1(7) out
         "k_update_u": ("cell", d * 28),
     }

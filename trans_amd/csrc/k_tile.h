@@ -184,7 +184,10 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
         const bool act = active && rel >= 0 && rel < NJ1;
         const int s = FORWARD ? t : nplanes - 1 - t;
         const int sn = FORWARD ? min(s + 1, nplanes - 1) : max(s - 1, 0);
-        line_load<R, NEQ>(src, lw + (unsigned)sn * (unsigned)nlines, nxt);
+        // the next step's inputs, for lanes whose line has a cell on that step (a lane off its
+        // line's range computes on stale registers and stores to its sink slot)
+        const int reln = t + 1 - start;
+        if (active && reln >= 0 && reln < NJ1) line_load<R, NEQ>(src, lw + (unsigned)sn * (unsigned)nlines, nxt);
         if (t + 1 <= t1 && (t + 1 - t0) % chunk != 0) load_edges(t + 1, en);
         const unsigned wst = act ? lw + (unsigned)s * (unsigned)nlines : my_sink;
         const int jrow = act ? rel : 0;  // j - 1 of the lane's cell
@@ -264,6 +267,7 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
     {
         const int s = FORWARD ? t0 : nplanes - 1 - t0;
         line_load<R, NEQ>(src, lw + (unsigned)s * (unsigned)nlines, bufA);
+        bufB = bufA;  // defined values for lanes that load nothing at the next step
     }
     for (int t = t0; t <= t1; t += 2) {
         step(t, bufA, bufB, edA, edB);
