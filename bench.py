@@ -26,6 +26,9 @@ UNIT = "Mcell\u00b7iter/s"
 # compulsory HBM bytes per cell of one fused iteration, FP64: 8*(N_in + N_geom + N_out) + 2*8*N_eq
 ITER_BYTES_RANS = 8 * (10 + 20 + 10) + 2 * 8 * 7    # 432
 ITER_BYTES_EULER = 8 * (7 + 17 + 7) + 2 * 8 * 5    # 328
+# one explicit stage with k-omega, FP64: state, p, T, mu_t in and out, geometry, and the stage's
+# saved state Wt2 (7) and the residual D (5) in
+ITER_BYTES_RK_STAGE = 8 * (10 + 20 + 10) + 8 * (7 + 5)  # 416
 
 
 def alg_bytes(name, neq, fp32, dual_shift=True):
@@ -80,7 +83,7 @@ def alg_bytes(name, neq, fp32, dual_shift=True):
     return table.get(name, ("cell", None))
 
 
-def build_workload(name, implicit_fp32=0):
+def build_workload(name, implicit_fp32=0, scale=1.0, back_pressure=1.2):
     from trans_amd import cases, solver
     if name == "bump":
         blocks, org, glob = cases.bump_channel()
@@ -94,39 +97,97 @@ def build_workload(name, implicit_fp32=0):
         blocks, org, glob = cases.flat_plate(nblk=16, ni_total=1025, nj=513, nk=17, lo=33, hi=97, ratio=1.0157)
         desc = "flat plate 1025x513x17 (8.4M cells) in 16 uneven blocks, RANS-SST, AUSMPW+ + MDCD, LU-SGS CFL 5"
     elif name == "compressor":
-        blocks, org, glob = cases.compressor_sector()
-        desc = ("compressor-rotor sector, 16 blocks (8.13M cells), 1200 rad/s, periodic 10 deg, RANS-SST, "
-                "Roe + MUSCL, LU-SGS CFL 5")
+        blocks, org, glob = cases.compressor_sector(scale=scale, back_pressure=back_pressure)
+        ncell = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
+        desc = (f"compressor-rotor sector, 16 blocks ({ncell / 1e6:.2f}M cells), 1200 rad/s, periodic 10 deg, "
+                f"outlet {back_pressure:g} x inlet P0, RANS-SST, Roe + MUSCL, LU-SGS CFL 5")
+    elif name == "cascade":
+        blocks, org, glob = cases.cascade(scale=scale)
+        ncell = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
+        desc = (f"blunt-trailing-edge linear cascade, 12 blocks ({ncell / 1e6:.2f}M cells), exit Mach 0.7, "
+                f"Re_c 1e6, SST-DES, AUSMPW+ + MDCD, RK3 at CFL 0.8 with a global time step")
     else:
         raise SystemExit(f"unknown workload {name}")
     prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32,
-                              device_geometry=name in ("plate8m", "compressor"))
+                              device_geometry=name in ("plate8m", "compressor", "cascade"))
+    if name == "cascade":
+        # Kind_Dual 0: one global step, CFL 0.8 in the cell that limits it
+        prm.dt_lusgs_main = cases.global_time_step(blocks, prm, org.CFL_Sub_end)
     return blocks, prm, desc
 
 
-def serving_leg(n_req=16, seed=20261202, target=1e-3, max_iter=20000, mean_gap_s=0.25,
-                budget_gib=64, max_active=16, chunk=100):
-    """Continuous-batching serving of a Poisson stream of steady bump-channel solves.
-
-    Requests draw a grid (65x33x9, 97x49x9 or 129x65x9 points) and a freestream Mach number
-    in [0.50, 0.52] from the seed and arrive with exponential gaps; each runs to a residual
-    drop of `target` (or `max_iter`).  The server admits them by HBM footprint (budget
-    `budget_gib`) into at most `max_active` resident slots and interleaves their iterations.
-    Latency = completion - submission."""
+def wake_stats(blocks, prm, steps, desc):
+    """Time-averaged mean and RMS statistics (Output/Average_Field.F90, 23 fields) over the
+    second half of `steps` iterations, on the batched path and on the plain drop-in path from
+    the same start; prints their largest relative difference per block and the throughput of
+    both."""
     import numpy as np
-    from trans_amd import cases, serving, solver
+    from trans_amd.mesh import sl
+    from trans_amd.solver import Solver
+    cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
+    ki_aver = steps // 2 + 1
+    out, stats = {}, []
+    for plain in (False, True):
+        sv = Solver(blocks, prm, ki=1)
+        if plain:
+            sv.set_tuning(-3, 1, 0, 16, 16, 8)
+        sv.stats_enable(ki_aver, ki_aver)
+        t0 = time.perf_counter()
+        res = sv.iterate(steps)
+        sv.sync()
+        dt = time.perf_counter() - t0
+        tag = "plain" if plain else "batched"
+        out[tag] = {"seconds": round(dt, 3), "Mcell_iter_per_s": round(cells * steps / dt / 1e6, 3),
+                    "drsa_block1": res[0]["drsa"]}
+        print(f"[wake] {tag}: {steps} steps in {dt:.2f} s", flush=True)
+        st = []
+        for n, b in enumerate(blocks):
+            a = sv.get_stats(n)[:, sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)]
+            st.append(a.astype(np.float64, copy=True))
+        stats.append(st)
+        sv.close()
+    worst = []
+    for a, b in zip(*stats):
+        den = np.maximum(np.max(np.abs(b), axis=(1, 2, 3)), 1e-300)
+        worst.append(float(np.max(np.max(np.abs(a - b), axis=(1, 2, 3)) / den)))
+    out.update({"workload": desc, "steps": steps, "statistics_from_iteration": ki_aver,
+                "stats_relerr_per_block": worst, "stats_relerr_max": max(worst)})
+    print(json.dumps({"wake_stats": out}), flush=True)
+
+
+def serving_leg(n_req=16, seed=20261202, target=0.5, max_iter=4000, mean_gap_s=1.0,
+                budget_gib=96, max_active=8, chunk=50):
+    """Continuous-batching serving of a Poisson stream of steady compressor-sector solves.
+
+    Requests draw a grid scale of the compressor-rotor sector (0.49M, 2.0M or 8.13M cells) and
+    an outlet back-pressure ratio in [0.9, 1.3] (Pouthub, BC/BC_Outlet.F90:49-70) from the
+    seed and arrive with exponential gaps; each runs the sector's RANS-SST deck (Roe + MUSCL,
+    LU-SGS, FP32 implicit operator, CFL ramp 1 -> 5) to a residual drop of `target` (or
+    `max_iter`).  The server admits them by HBM footprint (budget `budget_gib`) into at most
+    `max_active` resident slots and interleaves their iterations.  Latency = completion -
+    submission.  One prepared case per grid size serves every request of that size: a request
+    differs only in its outlet pressure."""
+    import numpy as np
+    from trans_amd import capi, cases, serving, solver
     rng = np.random.Generator(np.random.MT19937(seed))
-    grids = [(65, 33, 9), (97, 49, 9), (129, 65, 9)]
+    scales = [0.397, 0.63, 1.0]
     reqs = []
     t = 0.0
     for _ in range(n_req):
         t += rng.exponential(mean_gap_s)
-        reqs.append((t, grids[rng.integers(0, 3)], float(rng.uniform(0.50, 0.52))))
-    built = []
-    for (_, g, m) in reqs:
-        blocks, org, glob = cases.bump_channel(*g)
-        org.XM = m
-        built.append((blocks, solver.prepare_case(blocks, org, glob)))
+        reqs.append((t, int(rng.integers(0, 3)), float(rng.uniform(0.9, 1.3))))
+    built, ncell = {}, {}
+    for q in sorted({r[1] for r in reqs}):
+        blocks, org, glob = cases.compressor_sector(scale=scales[q], back_pressure=1.0)
+        built[q] = (blocks, solver.prepare_case(blocks, org, glob, implicit_fp32=1, device_geometry=True))
+        ncell[q] = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
+    print(f"[serving] cases prepared: {sorted(ncell.values())} cells", file=sys.stderr, flush=True)
+
+    def params(q, bp):
+        p = capi.TransParams.from_buffer_copy(built[q][1])
+        p.pouthub = built[q][1].pouthub * bp  # built at Pouthub = 1.0 x P0in
+        return p
+
     srv = serving.Server(budget_gib << 30, max_active=max_active, chunk=chunk)
     ids = []
     t0 = time.perf_counter()
@@ -135,7 +196,8 @@ def serving_leg(n_req=16, seed=20261202, target=1e-3, max_iter=20000, mean_gap_s
     while nxt < n_req or unfinished:
         now = time.perf_counter() - t0
         while nxt < n_req and reqs[nxt][0] <= now:
-            ids.append(srv.submit(built[nxt][0], built[nxt][1], target, max_iter))
+            _, q, bp = reqs[nxt]
+            ids.append(srv.submit(built[q][0], params(q, bp), target, max_iter))
             nxt += 1
         due = nxt < n_req and reqs[nxt][0] - now < 0.01
         unfinished = srv.progress(wait=not due)
@@ -145,17 +207,21 @@ def serving_leg(n_req=16, seed=20261202, target=1e-3, max_iter=20000, mean_gap_s
     srv.close()
     lat = np.array([s["t_done"] - s["t_submit"] for s in sts])
     span = max(s["t_done"] for s in sts) - min(s["t_submit"] for s in sts)
-    cell_iters = sum((g[0] - 1) * (g[1] - 1) * (g[2] - 1) * s["iterations"] for (_, g, _), s in zip(reqs, sts))
-    return {"requests": n_req, "converged": sum(s["state"] == 2 for s in sts),
+    cell_iters = sum(ncell[q] * s["iterations"] for (_, q, _), s in zip(reqs, sts))
+    return {"requests": n_req, "converged": sum(s["state"] == 2 for s in sts), "target_drop": target,
             "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
             "solves_per_hr": round(n_req / span * 3600, 1), "p50_s": round(float(np.percentile(lat, 50)), 3),
             "p99_s": round(float(np.percentile(lat, 99)), 3), "makespan_s": round(span, 3),
             "mcell_iter_per_s": round(cell_iters / span / 1e6, 2),
+            "cells": [ncell[q] for (_, q, _) in reqs],
+            "back_pressure": [round(bp, 4) for (_, _, bp) in reqs],
             "iterations": [s["iterations"] for s in sts],
-            "workload": f"Poisson stream (mean gap {mean_gap_s} s, seed {seed}) of bump-channel Euler solves "
-                        f"(MUSCL + Roe, LU-SGS CFL 5), grids 65x33x9 / 97x49x9 / 129x65x9, Mach U[0.50, 0.52], "
-                        f"to a {target:g} residual drop (cap {max_iter}); budget {budget_gib} GiB, "
-                        f"{max_active} slots, {chunk}-iteration chunks"}
+            "iterations_to_target": [s["iterations_to_target"] for s in sts],
+            "workload": f"Poisson stream (mean gap {mean_gap_s} s, seed {seed}) of compressor-rotor sector solves "
+                        f"(RANS-SST, Roe + MUSCL, LU-SGS CFL ramp 1 -> 5, FP64 residual / FP32 implicit), grids "
+                        f"0.49M / 2.0M / 8.13M cells, outlet back pressure U[0.9, 1.3] x inlet P0, to a {target:g} "
+                        f"residual drop (cap {max_iter}); budget {budget_gib} GiB, {max_active} slots, "
+                        f"{chunk}-iteration chunks"}
 
 
 def units(blocks):
@@ -188,13 +254,21 @@ def cpu_baseline(blocks, prm, budget_s=12.0):
         dt = time.perf_counter() - t0
         return cells * n / dt / 1e6, n, dt
 
-    threads = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16")), len(blocks), os.cpu_count() or 1))
+    # the process's CPU share (OMP_NUM_THREADS; 16 on the GPU hosts here), at most one thread per
+    # block: the oracle parallelises over blocks, which the 16 uneven blocks of the compressor
+    # sector balance to within a few per cent (largest block 524K of 8.13M cells)
+    share = int(os.environ.get("OMP_NUM_THREADS", "16"))
+    threads = max(1, min(share, len(blocks), os.cpu_count() or 1))
+    cells_b = [(b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks]
+    balance = sum(cells_b) / threads / max(cells_b) if threads > 1 else 1.0
     v, n, dt = run(threads, budget_s)
     v1, n1, dt1 = run(1, budget_s / 2)
     return {"value": round(v, 4), "unit": UNIT, "cores": threads, "kind": "port",
+            "physical_cores_host": physical_cores(), "cpu_share": share,
             "sample": f"{n} iterations of the same workload after 1 warm-up iteration, "
-                      f"oracle/trans_oracle*.c with OpenMP over blocks on {threads} threads, "
-                      f"{dt:.1f} s; single thread: {v1:.4f} {UNIT} ({n1} iterations, {dt1:.1f} s)"}
+                      f"oracle/trans_oracle*.c with OpenMP over blocks on {threads} threads (CPU share "
+                      f"{share}, block balance {balance:.2f}), {dt:.1f} s; single thread: {v1:.4f} {UNIT} "
+                      f"({n1} iterations, {dt1:.1f} s)"}
 
 
 def measured_traffic(workload, kernel):
@@ -341,7 +415,9 @@ def main():
     ap.add_argument("--workload", default=os.environ.get("TRANS_BENCH_WORKLOAD", "compressor"))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--serving", type=int, default=16, metavar="N",
-                    help="requests of the serving leg (0 skips it)")
+                    help="requests of the serving leg (0 skips it; 64 is the full serving workload)")
+    ap.add_argument("--serving-target", type=float, default=0.5, metavar="DROP",
+                    help="residual drop each served solve runs to")
     ap.add_argument("--serving-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-fp32", action="store_true", help="skip the second-precision leg")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain drop-in leg")
@@ -356,11 +432,18 @@ def main():
     ap.add_argument("--converge", type=int, default=0,
                     help="instead of the timed run: iterate up to N times to a 1e-8 drop of the "
                          "global DRSa and print the history")
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="compressor workload: grid scale (1 = 8.13M cells, 0.63 = 2.0M, 0.397 = 0.49M)")
+    ap.add_argument("--back-pressure", type=float, default=1.2,
+                    help="compressor workload: outlet static pressure / inlet total pressure")
+    ap.add_argument("--wake-stats", type=int, default=0, metavar="N",
+                    help="instead of the timed run: N iterations with mean/RMS statistics over the second "
+                         "half, batched path against the plain drop-in path")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.serving_child:
-        print(json.dumps(serving_leg(args.serving)), flush=True)
+        print(json.dumps(serving_leg(args.serving, target=args.serving_target)), flush=True)
         return
     if args.copy_probe:
         import torch
@@ -380,10 +463,13 @@ def main():
     # the compressor sector (8M cells) is quoted with the FP32 implicit operator and the FP64
     # residual; the flat plate with both in FP64
     primary32 = args.implicit_fp32 if args.implicit_fp32 is not None else int(args.workload == "compressor")
-    blocks, prm, desc = build_workload(args.workload, primary32)
+    blocks, prm, desc = build_workload(args.workload, primary32, args.scale, args.back_pressure)
     cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
     if args.converge:
         converge(blocks, prm, desc, args.converge)
+        return
+    if args.wake_stats:
+        wake_stats(blocks, prm, args.wake_stats, desc)
         return
     neq = 7 if (prm.if_euler == 0 and prm.if_turb_org == 1) else 5
     print(f"[bench] {desc}: {cells} cells, warmup {args.warmup}, steps {args.steps}", flush=True)
@@ -406,7 +492,7 @@ def main():
     kern = kernel_table(stats, blocks, neq, bool(primary32), prof_steps)
     other = None
     if not args.no_fp32:
-        b2, p2, _ = build_workload(args.workload, 1 - primary32)
+        b2, p2, _ = build_workload(args.workload, 1 - primary32, args.scale, args.back_pressure)
         dt2, res2, _ = timed_run(b2, p2, args.steps, args.warmup, 0, args.tune)
         other = {"implicit": "fp32" if not primary32 else "fp64",
                  "value": round(cells * args.steps / dt2 / 1e6, 3),
@@ -416,7 +502,7 @@ def main():
         print(f"[bench] {other['implicit']} implicit: {args.steps} steps in {dt2:.3f} s", flush=True)
         del b2
     plain = None
-    if not args.no_plain and prm.if_rk == 0:
+    if not args.no_plain:
         # the same workload as a plain drop-in: every routine launched once per block, the sweeps
         # one launch per hyperplane per block, one stream, no graph (trans_hip_set_tuning -3)
         ns = min(args.steps, 10)
@@ -435,14 +521,17 @@ def main():
         import subprocess
         env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
         cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--serving-child",
-                             "--serving", str(args.serving)], env=env, capture_output=True, text=True)
+                             "--serving", str(args.serving), "--serving-target", str(args.serving_target)],
+                            env=env, stdout=subprocess.PIPE, text=True)
         if cp.returncode != 0:
-            raise SystemExit(f"serving leg failed ({cp.returncode}): {cp.stderr[-2000:]}")
+            raise SystemExit(f"serving leg failed ({cp.returncode})")
         serve = json.loads(cp.stdout.strip().splitlines()[-1])
         print(f"[bench] serving: {serve['requests']} solves in {serve['makespan_s']} s", flush=True)
     # whole-iteration figure: compulsory bytes of one fused iteration per cell (state in and
     # out once, geometry once, one round trip of the increment between the two sweeps)
     b_it = (ITER_BYTES_RANS - (2 * 4 * 7 if primary32 else 0)) if neq == 7 else ITER_BYTES_EULER
+    if prm.if_rk == 1:
+        b_it = 3 * ITER_BYTES_RK_STAGE  # three Runge-Kutta stages per iteration
     it_roof = {"alg_bytes_per_cell": b_it, "achieved": round(value * 1e6 * b_it / 1e9, 2),
                "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(value * 1e6 * b_it / 1e9 / HBM_PEAK_GBS, 5)}

@@ -121,3 +121,38 @@ def test_footprint_equals_allocation():
         assert sv.lib.trans_hip_device_bytes(sv.ctx, C.byref(got)) == 0
         assert got.value == serving.footprint(blocks, prm)
         sv.close()
+
+
+SECTORS = [(1 / 16, 0.9), (1 / 8, 1.3)]
+S_TARGET, S_MAXIT = 0.5, 240
+
+
+def _sector(scale, bp):
+    blocks, org, glob = cases.compressor_sector(scale=scale, back_pressure=bp)
+    return blocks, solver.prepare_case(blocks, org, glob, implicit_fp32=1)
+
+
+def test_compressor_family_alone_and_in_the_mix():
+    """Two compressor-sector solves of the serving workload's family (RANS-SST, FP32 implicit
+    operator, 2K and 16K cells, outlet back pressure 0.9 and 1.3): final state and iteration
+    counts bitwise the same alone and resident together."""
+    print("case: compressor sectors 1/16 (bp 0.9) and 1/8 (bp 1.3), alone then together", flush=True)
+    alone = []
+    for sc in SECTORS:
+        srv = serving.Server(BIG, max_active=1, chunk=CHUNK)
+        rid = srv.submit(*_sector(*sc), S_TARGET, S_MAXIT)
+        srv.run()
+        alone.append((srv.poll(rid), srv.result(rid)))
+        srv.close()
+        print("alone", sc, alone[-1][0]["state_name"], alone[-1][0]["iterations"], flush=True)
+    srv = serving.Server(BIG, max_active=2, chunk=CHUNK)
+    ids = [srv.submit(*_sector(*sc), S_TARGET, S_MAXIT) for sc in SECTORS]
+    srv.run()
+    sts = [srv.poll(i) for i in ids]
+    assert max(s["t_admit"] for s in sts) < min(s["t_done"] for s in sts)
+    for (st0, res0), st, rid in zip(alone, sts, ids):
+        assert st["state"] in (capi.SRV_CONVERGED, capi.SRV_MAX_ITER)
+        for key in ("state", "iterations", "iterations_to_target", "drsa_first", "drsa_last"):
+            assert st[key] == st0[key], key
+        _same(srv.result(rid), res0)
+    srv.close()

@@ -176,19 +176,33 @@ def _face_rect(spec, m, side):
     return g, c, tang, rect
 
 
-def assemble_blocks(specs, point_fn, dims, physical_bc, periodic_axis=None, periodic_ok=None):
+def assemble_blocks(specs, point_fn, dims, physical_bc, periodic_axis=None, periodic_ok=None,
+                    periodic_turn=True):
     """Blocks and their BC rows (grd/dimension_blkNNN_m.dat, Input/Read_BlockDimensions.F90:61-74)
-    for boxes that tile the global grid of dims = (NI, NJ, NK) cells conformingly.
+    for boxes that tile the global grid of dims = (NI, NJ, NK) cells conformingly, or part of
+    it (cells no box covers are solid).
 
     Faces inside the grid become interface patches (BC 9) against every block sharing the
-    plane, faces on the grid's boundary take physical_bc(axis, index, rect) -> (bctype,
-    subbctype), unless the axis is `periodic_axis` and periodic_ok(rect) holds: then the face is
-    a periodic interface to the blocks on the opposite boundary (PerType +1 at index 0, whose
-    ghosts lie one pitch below and take the donor turned back by the pitch; -1 at the top).
+    plane; a face inside the grid that no other block shares bounds a solid region and takes
+    physical_bc(axis, index, rect) -> (bctype, subbctype), as do faces on the grid's boundary,
+    unless the axis is in `periodic_axis` (one axis or a tuple) and periodic_ok(rect) holds:
+    then the face is a periodic interface to the blocks on the opposite boundary.  With
+    periodic_turn (an annular sector) it is PerType +1 at index 0, whose ghosts lie one pitch
+    below and take the donor turned back by the pitch, -1 at the top; without it (a linear
+    cascade, a spanwise period) PerType 0, a plain copy of the translated donor.
     lcross and lrevA/lrevB are written from the geometry: lcross when the receiver's first
     tangential axis runs along the donor's second, lrev where a receiver axis runs against its
     donor counterpart.  point_fn(I, J, K) gives the global point coordinates."""
     nb = len(specs)
+    if periodic_axis is None:
+        per_axes = ()
+    elif isinstance(periodic_axis, int):
+        per_axes = (periodic_axis,)
+    else:
+        per_axes = tuple(periodic_axis)
+    if periodic_ok is None:
+        def periodic_ok(rect):
+            return True
     faces = []  # per block: list of (m, side, g, c, tang, rect)
     for s in specs:
         fl = []
@@ -206,10 +220,17 @@ def assemble_blocks(specs, point_fn, dims, physical_bc, periodic_axis=None, peri
             out[g] = (a, b)
         return out
 
+    def area(r):
+        a = 1
+        for (lo_, hi_) in r.values():
+            a *= hi_ - lo_
+        return a
+
     patches = [[] for _ in range(nb)]  # (m, side, rect, kind, data)
     for ib, s in enumerate(specs):
         for (m, side, g, c, tang, rect) in faces[ib]:
             if 0 < c < dims[g]:
+                covered = 0
                 for jb, s2 in enumerate(specs):
                     if jb == ib:
                         continue
@@ -218,9 +239,15 @@ def assemble_blocks(specs, point_fn, dims, physical_bc, periodic_axis=None, peri
                             ov = overlap(rect, rect2)
                             if ov and ((s.lo[g] == c) != (s2.lo[g] == c)):
                                 patches[ib].append((m, side, ov, "iface", (jb, m2, side2, 0)))
-            elif periodic_axis is not None and g == periodic_axis and periodic_ok(rect):
+                                covered += area(ov)
+                if covered == 0:
+                    bt, sub = physical_bc(g, c, rect)
+                    patches[ib].append((m, side, rect, "phys", (bt, sub)))
+                elif covered != area(rect):
+                    raise ValueError(f"block {ib} face {(m, side)}: partly covered by other blocks")
+            elif g in per_axes and periodic_ok(rect):
                 opp = dims[g] - c
-                pert = 1 if c == 0 else -1
+                pert = 0 if not periodic_turn else (1 if c == 0 else -1)
                 for jb, s2 in enumerate(specs):
                     for (m2, side2, g2, c2, tang2, rect2) in faces[jb]:
                         if g2 == g and c2 == opp:
@@ -442,3 +469,134 @@ def compressor_sector(scale=1.0, nblade=36, rpm=1200.0 * 60.0 / (2.0 * math.pi),
     glob = GlobalInfo(nblocks=len(blocks), P0in=101325.0, T0in=288.0, Pouthub=back_pressure * 101325.0,
                       nblade=nblade, Npassage=1, omega_rpm=(rpm, 0.0, 0.0))
     return blocks, org, glob
+
+
+def cascade(scale=1.0, nk=None, mach=0.7, re_c=1.0e6, t_max=0.06, t_le=0.005, t_te=0.02, pitch_c=0.8,
+            depth_c=0.2, x_up_c=1.0, x_down_c=2.0, dy_wall_c=1.0e-4, r0_c=100.0, seg=None, cfl=0.8):
+    """Linear cascade of symmetric blades with a blunt trailing edge, extruded in span.
+
+    Section: thickness t(xi) = t_le + (t_te - t_le) xi + NACA-65-style thickness of maximum
+    t_max (fractions of chord c), so the trailing edge is a flat face of height t_te c (and the
+    leading edge one of height t_le c); zero stagger and camber, axial inflow.  Pitch
+    `pitch_c` c, span depth `depth_c` c with a spanwise period, the pitchwise period a
+    translation (PerType 0); the blade row sits at y = r0_c c above the x axis so that the
+    outlet's radial-equilibrium march (BC/BC_Outlet.F90:49-103, along j = pitch) adds only
+    O(pitch / r0) to the outlet pressure.
+
+    H topology on the global cell box NI x NJ x NK, i streamwise, j pitchwise, k spanwise: the
+    pitch is split into the band j < nS, which continues the blade's thickness upstream (height
+    t_le) and downstream (height t_te) and is solid over the blade, and the passage j >= nS from
+    the blade's upper surface to the next blade's lower surface.  The band's upstream end is the
+    leading-edge face, its downstream start the trailing-edge face (walls), the passage's j
+    faces over the blade are the blade surfaces (walls).  Blocks: upstream band and passage (2),
+    blade passage split twice in i and once in j (4), wake band and passage in three streamwise
+    segments (6): 12 blocks of uneven sizes, 19.27M cells at scale 1 (NK = 64).
+
+    Deck: SST-DES (Kind_Hybrid 3, Kind_SubHY 0), AUSMPW+ with MDCD, explicit SSP-RK3 (IF_RK 1)
+    with one global time step (Kind_Dual 0: Dtm = DT_LUSGS_Main, Time/SpectralRadius.F90:146-176)
+    set on the host to `cfl` times the smallest local step of the uniform start (see
+    global_time_step), all FP64.  Inlet total pressure / temperature of the exit Mach number
+    `mach` at rho = 1.2 kg/m^3 and 288 K, outlet static pressure that of the uniform start,
+    Re_c = re_c."""
+    from .params import make_params
+    if seg is None:
+        seg = [max(2, int(round(n * scale))) for n in (64, 96, 96, 96, 128, 160)]
+    nU, nB1, nB2, nW1, nW2, nW3 = seg
+    nS = max(4, 2 * int(round(16 * scale)))
+    nP = max(8, 2 * int(round(224 * scale)))
+    NK = nk or max(4, int(round(64 * scale)))
+    nB, nW = nB1 + nB2, nW1 + nW2 + nW3
+    NI, NJ = nU + nB + nW, nS + nP
+    org = OrgDeck(XM=mach, Tinf=288.0, Kind_Spatial=3, Kind_Roe=3, IF_Euler=0, IF_Turb_Org=1, Kind_Model=1,
+                  IF_RK=1, Kind_Dual=0, Kchld=1, DT_LUSGS_Main=1.0, CFL_Sub_start=cfl, CFL_Sub_end=cfl,
+                  CFL_Sub_step=1, KItm=1, gama_max=0.012, gama_min=0.001, Prod_Lmt=20.0, VisRatio=1.0,
+                  FSTI=0.01, Kind_Hybrid=3, Kind_SubHY=0, IF_NS_Uniform=1, MA=5000)
+    p_inf = 1.2 * 287.0 * org.Tinf
+    stag = 1.0 + 0.2 * mach * mach
+    glob = GlobalInfo(nblocks=12, P0in=p_inf * stag ** 3.5, T0in=org.Tinf * stag, Pouthub=p_inf)
+    c = re_c / make_params(org, glob).ref  # chord in grid units: Re_c = ref * c
+    pitch, depth, r0 = pitch_c * c, depth_c * c, r0_c * c
+
+    def thick(xi):
+        return c * (t_le + (t_te - t_le) * xi + 2.0 * naca65_half_thickness(xi, t_max))
+
+    xs_u = -x_up_c * c * (1.0 - np.sin(np.linspace(0.0, math.pi / 2, nU + 1)))
+    xs_b = c * _two_sided(nB, min(0.5 / nB, 0.004))
+    xs_w = c + x_down_c * c * (1.0 - np.cos(np.linspace(0.0, math.pi / 2, nW + 1)))
+    xs = np.concatenate([xs_u, xs_b[1:], xs_w[1:]])
+    xi = np.clip(xs / c, 0.0, 1.0)
+    tt = thick(xi)
+    eta_s = _two_sided(nS, min(0.5 / nS, dy_wall_c / t_te))
+    eta_p = _two_sided(nP, dy_wall_c / (pitch_c - t_te))
+    zs = np.linspace(0.0, depth, NK + 1)
+    assert len(xs) == NI + 1
+
+    def point_fn(I, J, K):
+        t = tt[I]
+        Jb = np.minimum(J, nS)
+        Jp = np.maximum(J - nS, 0)
+        y = np.where(J <= nS, -0.5 * t + t * eta_s[Jb], 0.5 * t + (pitch - t) * eta_p[Jp])
+        return xs[I], r0 + y, zs[K]
+
+    i1, i2, i3 = nU, nU + nB1, nU + nB
+    w1, w2 = i3 + nW1, i3 + nW1 + nW2
+    jm = nS + nP // 2
+    specs = [BoxSpec((0, 0, 0), (i1, nS, NK)), BoxSpec((0, nS, 0), (i1, NJ, NK))]
+    for (a, b_) in ((i1, i2), (i2, i3)):
+        specs += [BoxSpec((a, nS, 0), (b_, jm, NK)), BoxSpec((a, jm, 0), (b_, NJ, NK))]
+    for (a, b_) in ((i3, w1), (w1, w2), (w2, NI)):
+        specs += [BoxSpec((a, 0, 0), (b_, nS, NK)), BoxSpec((a, nS, 0), (b_, NJ, NK))]
+
+    def physical_bc(g, cc, rect):
+        if g == 0 and cc == 0:
+            return (5, 0)
+        if g == 0 and cc == NI:
+            return (7, 0)
+        return (1, 0)  # leading- and trailing-edge faces, blade surfaces
+
+    def periodic_ok(rect):
+        if 0 in rect and 2 in rect:  # a j face: periodic up- and downstream of the blade
+            i0, i1_ = rect[0]
+            return i1_ <= i1 or i0 >= i3
+        return True  # k faces: the spanwise period
+
+    blocks = assemble_blocks(specs, point_fn, (NI, NJ, NK), physical_bc, periodic_axis=(1, 2),
+                             periodic_ok=periodic_ok, periodic_turn=False)
+    glob.nblocks = len(blocks)
+    return blocks, org, glob
+
+
+def global_time_step(blocks, prm, cfl):
+    """cfl times the smallest local time step vol / (Rds1 + Rds2 + Rds3) of the current state
+    (Time/SpectralRadius.F90:40-138 without preconditioning): the global step of a Kind_Dual 0
+    deck.  Needs the geometry and the initial state of prepare_case."""
+    from .mesh import O
+    dt = math.inf
+    for b in blocks:
+        A = b.arrays
+        I = slice(O + 1, O + b.ni)
+        J = slice(O + 1, O + b.nj)
+        K = slice(O + 1, O + b.nk)
+        I1 = slice(O + 2, O + b.ni + 1)
+        J1 = slice(O + 2, O + b.nj + 1)
+        K1 = slice(O + 2, O + b.nk + 1)
+        v = A["v"]
+        den = v[0][I, J, K]
+        u = [v[m][I, J, K] / den for m in (1, 2, 3)]
+        pp = A["pp"][I, J, K]
+        aa = np.sqrt(1.4 * pp / den)
+        tt = A["t"][I, J, K]
+        rmiul = (1.0 + prm.csthlnd) / (tt * prm.csth1 + prm.csthlnd) * (prm.csth1 * tt) ** 1.5
+        rmiut = A["rmiu"][I, J, K]
+        uvmu = (rmiul + rmiut + 1.0e-20) * 4.0 / 3.0
+        uvmupr = (rmiul / 0.72 + rmiut / 0.9 + 1.0e-20) * 1.4
+        pr = np.maximum(uvmu, uvmupr) / den
+        vol = A["vol"][I, J, K]
+        rsum = 0.0
+        for d, (nxt) in enumerate(((I1, J, K), (I, J1, K), (I, J, K1))):
+            s = [0.5 * (A["sd"][d + 3 * x][I, J, K] + A["sd"][d + 3 * x][nxt]) for x in range(3)]
+            st = s[0] ** 2 + s[1] ** 2 + s[2] ** 2
+            un = u[0] * s[0] + u[1] * s[1] + u[2] * s[2]
+            rsum = rsum + np.abs(un) + np.sqrt(st) * aa + 2.0 * st / vol * pr / prm.ref
+        dt = min(dt, float(np.min(vol / rsum)))
+    return cfl * dt

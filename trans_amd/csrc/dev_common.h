@@ -122,7 +122,11 @@ __device__ __forceinline__ void decode3(int lin, int ni, int nj, int& i, int& j,
 // the L2 of one XCD instead of being fetched again by workgroups that the dispatcher put on
 // other XCDs, while each wave still reads one row of 64 consecutive i.  Returns false for the
 // padding of partial tiles.
-constexpr int FTA = 64, FTB = 4;
+#ifndef TRANS_FTA
+#define TRANS_FTA 64
+#define TRANS_FTB 4
+#endif
+constexpr int FTA = TRANS_FTA, FTB = TRANS_FTB;
 template <int DIR>
 __device__ __forceinline__ bool decode_face_tile(int lin, int n0, int n1, int n2, int& i, int& j, int& k)
 {

@@ -178,28 +178,14 @@ __device__ __forceinline__ void face_grad(const Fields& f, const double* Sd, lon
     }
 }
 
-// One thread per face of direction DIR: viscous flux (Space/DIFFUSION_Centered.F90) and,
-// when turbulent, the k-omega diffusion flux (Space/TURBTWO_DIFFUSION_Centered.F90, SST).
-// GV[DIR-1] receives Fx, Fy, Fz, Fe, Fk, Fo.
+// Viscous flux (Space/DIFFUSION_Centered.F90) and, when turbulent, the k-omega diffusion flux
+// (Space/TURBTWO_DIFFUSION_Centered.F90, SST) through one face of direction DIR.
+// out[0..3]: Fx, Fy, Fz, Fe; out[4..5]: Fk, Fo when turbulent.  (i, j, k): the face's index,
+// the cell on its high side.
 template <int DIR>
-__global__ void __launch_bounds__(256, 3) k_visc_face(trans_params p, int if_turb, Fields f,
-                                                   const BlkInfo* __restrict__ blks,
-                                                   const Seg* __restrict__ seg)
+__device__ __forceinline__ void visc_face_flux(const trans_params& p, bool turb, const Fields& f,
+                                               const BlkInfo& b, int i, int j, int k, double out[6])
 {
-    const Seg s = seg[blockIdx.x];
-    const BlkInfo b = blks[s.blk];
-    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
-    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
-    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
-    const int lin = s.base + threadIdx.x;
-    int i, j, k;
-    if (DIR == 1) {
-        if (lin >= n0 * n1 * n2) return;
-        decode3(lin, n0, n1, i, j, k);
-    } else if (!decode_face_tile<DIR>(lin, n0, n1, n2, i, j, k)) {
-        return;
-    }
-    const bool turb = if_turb != 0;
     const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
     const long c0 = cidx(b, i, j, k);  // cell on the high side of face f
     const long cm = c0 - st;
@@ -263,11 +249,10 @@ __global__ void __launch_bounds__(256, 3) k_visc_face(trans_params p, int if_tur
     const double Phiy = akmu * gt[1] + ucc * Txy + vcc * Tyy + wcc * Tyz;
     const double Phiz = akmu * gt[2] + ucc * Tzx + vcc * Tyz + wcc * Tzz;
     const double Dx = Sd[c0], Dy = Sd[3 * tot + c0], Dz = Sd[6 * tot + c0];
-    double* G = f.GV[d];
-    G[c0] = (Txx * Dx + Txy * Dy + Tzx * Dz) / p.ref;
-    G[tot + c0] = (Txy * Dx + Tyy * Dy + Tyz * Dz) / p.ref;
-    G[2 * tot + c0] = (Tzx * Dx + Tyz * Dy + Tzz * Dz) / p.ref;
-    G[3 * tot + c0] = (Phix * Dx + Phiy * Dy + Phiz * Dz) / p.ref;
+    out[0] = (Txx * Dx + Txy * Dy + Tzx * Dz) / p.ref;
+    out[1] = (Txy * Dx + Tyy * Dy + Tyz * Dz) / p.ref;
+    out[2] = (Tzx * Dx + Tyz * Dy + Tzz * Dz) / p.ref;
+    out[3] = (Phix * Dx + Phiy * Dy + Phiz * Dz) / p.ref;
     if (!turb) return;
     // k-omega diffusion (Space/TURBTWO_DIFFUSION_Centered.F90:40-214), same Alagm weights
     const double TTL = all * f.T[cm] + alr * f.T[c0];
@@ -292,8 +277,39 @@ __global__ void __launch_bounds__(256, 3) k_visc_face(trans_params p, int if_tur
     const double SigoAB = p.kind_model == 2 ? p.sigo1 : fc1c * p.sigo1 + (1. - fc1c) * p.sigo2;
     const double vmute = RmiuLt + RmiuTL * SigkAB;
     const double vmuoq = RmiuLt + RmiuTL * SigoAB;
-    G[4 * tot + c0] = (vmute * (gk[0] * Dx + gk[1] * Dy + gk[2] * Dz)) / p.ref;
-    G[5 * tot + c0] = (vmuoq * (go[0] * Dx + go[1] * Dy + go[2] * Dz)) / p.ref;
+    out[4] = (vmute * (gk[0] * Dx + gk[1] * Dy + gk[2] * Dz)) / p.ref;
+    out[5] = (vmuoq * (go[0] * Dx + go[1] * Dy + go[2] * Dz)) / p.ref;
+}
+
+// One thread per face of direction DIR: GV[DIR-1] receives visc_face_flux.
+template <int DIR>
+__global__ void __launch_bounds__(256, 3) k_visc_face(trans_params p, int if_turb, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
+    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
+    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    int i, j, k;
+    if (DIR == 1) {
+        if (lin >= n0 * n1 * n2) return;
+        decode3(lin, n0, n1, i, j, k);
+    } else if (!decode_face_tile<DIR>(lin, n0, n1, n2, i, j, k)) {
+        return;
+    }
+    double o[6];
+    visc_face_flux<DIR>(p, if_turb != 0, f, b, i, j, k, o);
+    const long c0 = cidx(b, i, j, k), tot = f.tot;
+    double* G = f.GV[DIR - 1];
+#pragma unroll
+    for (int q = 0; q < 4; q++) G[(long)q * tot + c0] = o[q];
+    if (if_turb) {
+        G[4 * tot + c0] = o[4];
+        G[5 * tot + c0] = o[5];
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -388,30 +404,15 @@ __device__ __forceinline__ void ausmpw_common(const trans_params& p, double PLo,
 
 __device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
-// One thread per face of direction DIR: reconstruction (SPATIAL 1 = CONVECT_MUSCL2, 2 = CONVECT_MUSCL,
-// 3 = CONVECT_MDCD, 4 = CONVECT_MDCDHYB), inviscid flux (ROE 1 = CONVECT_Roe_Original,
-// 2 = CONVECT_Roe_Rotated, 3 = CONVECT_AUSMPW_Plus)
-// and, when turbulent, TURBTWO_CONVECT_MUSCL + TURBTWO_CONVECT_Upwind.  G[DIR-1] receives
-// flux*|S| (5 comps) and the area-weighted k, omega fluxes (comps 5, 6).
+// Convective flux through one face of direction DIR: reconstruction (SPATIAL 1 = CONVECT_MUSCL2,
+// 2 = CONVECT_MUSCL, 3 = CONVECT_MDCD, 4 = CONVECT_MDCDHYB), inviscid flux (ROE 1 =
+// CONVECT_Roe_Original, 2 = CONVECT_Roe_Rotated, 3 = CONVECT_AUSMPW_Plus) and, when turbulent,
+// TURBTWO_CONVECT_MUSCL + TURBTWO_CONVECT_Upwind.  out[0..4]: flux*|S| (the rotated flux is
+// area-weighted already), out[5..6]: the area-weighted k, omega fluxes.
 template <int DIR, int SPATIAL, int ROE>
-__global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, Fields f,
-                                                   const BlkInfo* __restrict__ blks,
-                                                   const Seg* __restrict__ seg)
+__device__ __forceinline__ void conv_face_flux(const trans_params& p, bool turb, const Fields& f,
+                                               const BlkInfo& b, int i, int j, int k, double out[7])
 {
-    const Seg s = seg[blockIdx.x];
-    const BlkInfo b = blks[s.blk];
-    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
-    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
-    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
-    const int lin = s.base + threadIdx.x;
-    int i, j, k;
-    if (DIR == 1) {
-        if (lin >= n0 * n1 * n2) return;
-        decode3(lin, n0, n1, i, j, k);
-    } else if (!decode_face_tile<DIR>(lin, n0, n1, n2, i, j, k)) {
-        return;
-    }
-    const bool turb = if_turb != 0;
     const long st = DIR == 1 ? 1 : (DIR == 2 ? (long)b.nx : (long)b.nx * b.ny);
     const long c0 = cidx(b, i, j, k);
     const long tot = f.tot;
@@ -504,19 +505,17 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
         roe_rotated_face(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
                          S[3 * tot + c0], S[6 * tot + c0], sav, Funscheme, qL[5] * p.omega1, qR[5] * p.omega1, RmiuT,
                          AlagmR, fl);
-        double* G = f.G[d];
 #pragma unroll
-        for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L];
+        for (int L = 0; L < 5; L++) out[L] = fl[L];
     } else if (ROE == 1)
         roe_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0], S[3 * tot + c0],
                    S[6 * tot + c0], sav, qL[5] * p.omega1, qR[5] * p.omega1, RmiuT, AlagmR, fl);
     else
         ausmpw_common(p, qL[3], qR[3], qL[4], qR[4], qL[0], qR[0], qL[1], qR[1], qL[2], qR[2], S[c0],
                       S[3 * tot + c0], S[6 * tot + c0], sav, fl);
-    double* G = f.G[d];
     if (ROE != 2)
 #pragma unroll
-        for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = fl[L] * sav;
+        for (int L = 0; L < 5; L++) out[L] = fl[L] * sav;
     if (!turb) return;
     // k/rho, omega/rho: face averages + ALIMITER reconstruction (Space/TURBTWO_CONVECT_MUSCL.F90)
     double tL[2], tR[2];
@@ -550,8 +549,39 @@ __global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, 
     const double dTK = -0.5 * (rulnormal * TKL + rurnormal * TKR - eigTKO * (DENR * TKR - DENL * TKL));
     const double TOL = fmax(tL[1], small), TOR = fmax(tR[1], small);
     const double dTO = -0.5 * (rulnormal * TOL + rurnormal * TOR - eigTKO * (DENR * TOR - DENL * TOL));
-    G[5 * tot + c0] = dTK;
-    G[6 * tot + c0] = dTO;
+    out[5] = dTK;
+    out[6] = dTO;
+}
+
+// One thread per face of direction DIR: G[DIR-1] receives conv_face_flux.
+template <int DIR, int SPATIAL, int ROE>
+__global__ void __launch_bounds__(256) k_conv_face(trans_params p, int if_turb, Fields f,
+                                                   const BlkInfo* __restrict__ blks,
+                                                   const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int n0 = DIR == 1 ? b.ni : b.ni - 1;
+    const int n1 = DIR == 2 ? b.nj : b.nj - 1;
+    const int n2 = DIR == 3 ? b.nk : b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    int i, j, k;
+    if (DIR == 1) {
+        if (lin >= n0 * n1 * n2) return;
+        decode3(lin, n0, n1, i, j, k);
+    } else if (!decode_face_tile<DIR>(lin, n0, n1, n2, i, j, k)) {
+        return;
+    }
+    double o[7];
+    conv_face_flux<DIR, SPATIAL, ROE>(p, if_turb != 0, f, b, i, j, k, o);
+    const long c0 = cidx(b, i, j, k), tot = f.tot;
+    double* G = f.G[DIR - 1];
+#pragma unroll
+    for (int L = 0; L < 5; L++) G[(long)L * tot + c0] = o[L];
+    if (if_turb) {
+        G[5 * tot + c0] = o[5];
+        G[6 * tot + c0] = o[6];
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -626,6 +656,199 @@ __global__ void __launch_bounds__(256) k_accumulate_visc(trans_params p, int if_
             x = x + G[c + st[dir]];
         }
         f.D[(long)(5 + q) * tot + c] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// LDS-tiled face fluxes and accumulation in two passes (an alternative to the face kernels
+// through G and GV plus k_accumulate_visc, selected with TRANS_HIP_FUSED=1; measured slower on
+// the MI355X, see the design notes).  A workgroup owns an RTI x RTJ (i, j) tile of cells of one
+// block and marches k through a chunk of planes.  At each plane the fluxes of the tile's i and
+// j faces (its cells' low and high faces) are formed once into LDS, flat over the workgroup's
+// threads; each thread forms the flux of its cell's high k face in registers and keeps it as
+// the next plane's low k face.  Each cell then runs its sums in the reference order (i, j, k;
+// own face subtracted, next face added).  k_resid_visc forms the viscous faces and leaves each
+// cell's viscous sums with the rotating-frame source in D (D(1) = 0); k_resid_conv forms the
+// convective faces and continues the sums from D.  Every flux is the face kernels' own
+// expression, every sum the accumulation kernel's order, and the partial sums pass through HBM
+// unrounded (FP64), so D is bitwise that of the face-kernel path.  A chunk starts by forming
+// the low k faces of its first plane.
+constexpr int RTI = 64, RTJ = 4;
+constexpr int RT_NI = (RTI + 1) * RTJ;          // i-face slots: (fi, fj) at fj * (RTI + 1) + fi
+constexpr int RT_NF = RT_NI + RTI * (RTJ + 1);  // then j-face slots: (fi, fj) at RT_NI + fj * RTI + fi
+
+__global__ void __launch_bounds__(256, 2) k_resid_visc(trans_params p, int if_turb, Fields f,
+                                                    const BlkInfo* __restrict__ blks,
+                                                    const Seg* __restrict__ seg, int kchunk)
+{
+    __shared__ double lds[6 * RT_NF];
+    const Seg sg = seg[blockIdx.x];
+    const BlkInfo b = blks[sg.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int ntx = (NI1 + RTI - 1) / RTI, nty = (NJ1 + RTJ - 1) / RTJ;
+    const int tx = sg.base % ntx, ty = (sg.base / ntx) % nty, tz = sg.base / (ntx * nty);
+    const int i0 = tx * RTI + 1, j0 = ty * RTJ + 1, k0 = tz * kchunk + 1;
+    const int k1 = min(k0 + kchunk - 1, NK1);
+    const int ni_t = min(RTI, NI1 - i0 + 1), nj_t = min(RTJ, NJ1 - j0 + 1);
+    const int nfi = (ni_t + 1) * nj_t, nf = nfi + ni_t * (nj_t + 1);
+    const int t = threadIdx.x;
+    const int ci = t % RTI, cj = t / RTI;
+    const bool own = ci < ni_t && cj < nj_t;
+    const int i = i0 + ci, j = j0 + cj;
+    const bool turb = if_turb != 0;
+    const long tot = f.tot;
+    const int si = cj * (RTI + 1) + ci, sj = RT_NI + cj * RTI + ci;
+    double gv_lo[6] = {0, 0, 0, 0, 0, 0};
+    if (own) visc_face_flux<3>(p, turb, f, b, i, j, k0, gv_lo);
+    for (int k = k0; k <= k1; k++) {
+        for (int q = t; q < nf; q += 256) {
+            double o[6];
+            int slot;
+            if (q < nfi) {
+                const int fi = q % (ni_t + 1), fj = q / (ni_t + 1);
+                visc_face_flux<1>(p, turb, f, b, i0 + fi, j0 + fj, k, o);
+                slot = fj * (RTI + 1) + fi;
+            } else {
+                const int fi = (q - nfi) % ni_t, fj = (q - nfi) / ni_t;
+                visc_face_flux<2>(p, turb, f, b, i0 + fi, j0 + fj, k, o);
+                slot = RT_NI + fj * RTI + fi;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) lds[c * RT_NF + slot] = o[c];
+            if (turb) {
+                lds[4 * RT_NF + slot] = o[4];
+                lds[5 * RT_NF + slot] = o[5];
+            }
+        }
+        double gv_hi[6] = {0, 0, 0, 0, 0, 0};
+        if (own) visc_face_flux<3>(p, turb, f, b, i, j, k + 1, gv_hi);
+        __syncthreads();
+        if (own) {
+            const long c = cidx(b, i, j, k);
+            double x[7];
+            x[0] = 0.0;
+#pragma unroll
+            for (int L = 1; L < 5; L++) {
+                const double* V = lds + (L - 1) * RT_NF;
+                double y = 0.0;
+                y = y - V[si];
+                y = y + V[si + 1];
+                y = y - V[sj];
+                y = y + V[sj + RTI];
+                y = y - gv_lo[L - 1];
+                y = y + gv_hi[L - 1];
+                x[L] = y;
+            }
+            const double RoW = p.omega1;
+            const double v1 = f.V[c];
+            const double ss3 = 2.0 * RoW * f.V[3 * tot + c] + RoW * f.Yc[c] * p.omega1 * v1;
+            const double ss4 = -2.0 * RoW * f.V[2 * tot + c] + RoW * f.Zc[c] * p.omega1 * v1;
+            x[2] = x[2] + ss3 * f.Vol[c];
+            x[3] = x[3] + ss4 * f.Vol[c];
+#pragma unroll
+            for (int L = 0; L < 5; L++) f.D[(long)L * tot + c] = x[L];
+            if (turb)
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const double* V = lds + (4 + q) * RT_NF;
+                    double y = 0.0;
+                    y = y - V[si];
+                    y = y + V[si + 1];
+                    y = y - V[sj];
+                    y = y + V[sj + RTI];
+                    y = y - gv_lo[4 + q];
+                    y = y + gv_hi[4 + q];
+                    f.D[(long)(5 + q) * tot + c] = y;
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 6; c++) gv_lo[c] = gv_hi[c];
+    }
+}
+
+template <int SPATIAL, int ROE>
+__global__ void __launch_bounds__(256, 2) k_resid_conv(trans_params p, int if_euler, int if_turb, Fields f,
+                                                    const BlkInfo* __restrict__ blks,
+                                                    const Seg* __restrict__ seg, int kchunk)
+{
+    __shared__ double lds[7 * RT_NF];
+    const Seg sg = seg[blockIdx.x];
+    const BlkInfo b = blks[sg.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int ntx = (NI1 + RTI - 1) / RTI, nty = (NJ1 + RTJ - 1) / RTJ;
+    const int tx = sg.base % ntx, ty = (sg.base / ntx) % nty, tz = sg.base / (ntx * nty);
+    const int i0 = tx * RTI + 1, j0 = ty * RTJ + 1, k0 = tz * kchunk + 1;
+    const int k1 = min(k0 + kchunk - 1, NK1);
+    const int ni_t = min(RTI, NI1 - i0 + 1), nj_t = min(RTJ, NJ1 - j0 + 1);
+    const int nfi = (ni_t + 1) * nj_t, nf = nfi + ni_t * (nj_t + 1);
+    const int t = threadIdx.x;
+    const int ci = t % RTI, cj = t / RTI;
+    const bool own = ci < ni_t && cj < nj_t;
+    const int i = i0 + ci, j = j0 + cj;
+    const bool turb = if_turb != 0, visc = if_euler == 0;
+    const long tot = f.tot;
+    const int si = cj * (RTI + 1) + ci, sj = RT_NI + cj * RTI + ci;
+    double g_lo[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (own) conv_face_flux<3, SPATIAL, ROE>(p, turb, f, b, i, j, k0, g_lo);
+    for (int k = k0; k <= k1; k++) {
+        for (int q = t; q < nf; q += 256) {
+            double o[7];
+            int slot;
+            if (q < nfi) {
+                const int fi = q % (ni_t + 1), fj = q / (ni_t + 1);
+                conv_face_flux<1, SPATIAL, ROE>(p, turb, f, b, i0 + fi, j0 + fj, k, o);
+                slot = fj * (RTI + 1) + fi;
+            } else {
+                const int fi = (q - nfi) % ni_t, fj = (q - nfi) / ni_t;
+                conv_face_flux<2, SPATIAL, ROE>(p, turb, f, b, i0 + fi, j0 + fj, k, o);
+                slot = RT_NI + fj * RTI + fi;
+            }
+#pragma unroll
+            for (int c = 0; c < 5; c++) lds[c * RT_NF + slot] = o[c];
+            if (turb) {
+                lds[5 * RT_NF + slot] = o[5];
+                lds[6 * RT_NF + slot] = o[6];
+            }
+        }
+        double g_hi[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (own) conv_face_flux<3, SPATIAL, ROE>(p, turb, f, b, i, j, k + 1, g_hi);
+        __syncthreads();
+        if (own) {
+            const long c = cidx(b, i, j, k);
+#pragma unroll
+            for (int L = 0; L < 5; L++) {
+                const double* G = lds + L * RT_NF;
+                double y = visc ? f.D[(long)L * tot + c] : 0.0;
+                y = y - G[si];
+                y = y + G[si + 1];
+                y = y - G[sj];
+                y = y + G[sj + RTI];
+                y = y - g_lo[L];
+                y = y + g_hi[L];
+                f.D[(long)L * tot + c] = y;
+            }
+            if (turb) {
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const double* G = lds + (5 + q) * RT_NF;
+                    double y = visc ? f.D[(long)(5 + q) * tot + c] : 0.0;
+                    y = y - G[si];
+                    y = y + G[si + 1];
+                    y = y - G[sj];
+                    y = y + G[sj + RTI];
+                    y = y - g_lo[5 + q];
+                    y = y + g_hi[5 + q];
+                    f.D[(long)(5 + q) * tot + c] = y;
+                }
+            } else {
+                f.D[5 * tot + c] = 0.0;
+                f.D[6 * tot + c] = 0.0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 7; c++) g_lo[c] = g_hi[c];
     }
 }
 

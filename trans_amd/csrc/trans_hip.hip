@@ -72,6 +72,12 @@ struct trans_ctx {
     int n_outl = 0;
     SegTable seg_cell, seg_face[3];
     SegTable seg_wtile;  // WT x WT (i, j) tiles at each k: the tiled writers of the sweep buffer
+    // face fluxes + accumulation: 0 the face kernels through G and GV, then k_accumulate_visc;
+    // 1 the LDS-tiled viscous and convective passes (k_resid_visc, k_resid_conv) on RTI x RTJ
+    // (i, j) tiles marching chunks of rt_kchunk k planes (TRANS_HIP_FUSED=1 at creation)
+    SegTable seg_rtile;
+    int rt_kchunk = 8;
+    int fused = 0;
     int* d_wrange = nullptr;
     NormPart* d_part = nullptr;
     trans_resid* d_resid = nullptr;
@@ -306,6 +312,25 @@ static SegTable make_wtile_segs(trans_ctx* c)
     for (int b = 0; b < (int)c->hb.size(); b++) {
         const HostBlock& h = c->hb[b];
         const int nt = ((h.ni - 1 + WT - 1) / WT) * ((h.nj - 1 + WT - 1) / WT) * (h.nk - 1);
+        for (int q = 0; q < nt; q++) segs.push_back(Seg{b, q});
+        t.wrange.push_back((int)segs.size());
+    }
+    t.n = (int)segs.size();
+    if (dalloc(c, &t.d, segs.size()) == 0)
+        (void)h2d(c, t.d, segs.data(), segs.size() * sizeof(Seg));
+    return t;
+}
+
+// Schedule of k_resid_visc and k_resid_conv: per block, ceil(NI1/RTI) x ceil(NJ1/RTJ) tiles times
+// ceil(NK1/kchunk) chunks of k planes, entry Seg{block, tile index}.
+static SegTable make_rtile_segs(trans_ctx* c, int kchunk)
+{
+    std::vector<Seg> segs;
+    SegTable t;
+    t.wrange.push_back(0);
+    for (int b = 0; b < (int)c->hb.size(); b++) {
+        const HostBlock& h = c->hb[b];
+        const int nt = ((h.ni - 1 + RTI - 1) / RTI) * ((h.nj - 1 + RTJ - 1) / RTJ) * ((h.nk - 1 + kchunk - 1) / kchunk);
         for (int q = 0; q < nt; q++) segs.push_back(Seg{b, q});
         t.wrange.push_back((int)segs.size());
     }
@@ -730,6 +755,13 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
     c->seg_wtile = make_wtile_segs(c);
     {
+        const char* env = std::getenv("TRANS_HIP_FUSED");
+        if (env && (env[0] == '0' || env[0] == '1')) c->fused = env[0] - '0';
+        const char* kc = std::getenv("TRANS_HIP_RESID_KCHUNK");
+        if (kc && std::atoi(kc) > 0) c->rt_kchunk = std::atoi(kc);
+        c->seg_rtile = make_rtile_segs(c, c->rt_kchunk);
+    }
+    {
         // per-plane sweep launches: lines (i, k) of every block, march axis j
         std::vector<PlaneBlk> pb;
         std::vector<Seg> ps;
@@ -1004,6 +1036,26 @@ static void launch_conv_roe(trans_ctx* c, int if_turb)
         launch_conv<SPATIAL, 3>(c, if_turb);
 }
 
+// The LDS-tiled passes for the schemes the workloads use (MUSCL or MDCD with Roe or AUSMPW+);
+// false when this scheme runs the face kernels.
+static bool launch_fused(trans_ctx* c, int if_turb)
+{
+    const int sp = c->p.kind_spatial, ro = c->p.kind_roe;
+    if (!((sp == 2 || sp == 3) && (ro == 1 || ro == 3))) return false;
+    if (c->p.if_euler == 0)
+        KLAUNCH(c, "k_resid_visc", k_resid_visc, dim3(c->seg_rtile.n), dim3(256), c->p, if_turb, c->f, c->d_blk,
+                c->seg_rtile.d, c->rt_kchunk);
+#define RC(S, R)                                                                                                   \
+    KLAUNCH(c, "k_resid_conv", (k_resid_conv<S, R>), dim3(c->seg_rtile.n), dim3(256), c->p, c->p.if_euler, if_turb, \
+            c->f, c->d_blk, c->seg_rtile.d, c->rt_kchunk)
+    if (sp == 2 && ro == 1) RC(2, 1);
+    else if (sp == 2 && ro == 3) RC(2, 3);
+    else if (sp == 3 && ro == 1) RC(3, 1);
+    else RC(3, 3);
+#undef RC
+    return true;
+}
+
 extern "C" {
 
 int trans_hip_space(trans_ctx* c)
@@ -1018,6 +1070,7 @@ int trans_hip_space(trans_ctx* c)
         if ((rc = launch_check(c, "k_gradients"))) return rc;
         if ((rc = trans_hip_bc(c, 0))) return rc;
     }
+    if (c->fused && launch_fused(c, if_turb)) return launch_check(c, "space");
     fork_aux(c, 3);
     if (visc) {
         KLAUNCH_ON(c, c->aux[0], "k_visc_face<1>", k_visc_face<1>, dim3(c->seg_face[0].n), dim3(WG), c->p, if_turb,
@@ -1292,10 +1345,25 @@ static int average_fields_async(trans_ctx* c);  // io.h
 
 // The plain drop-in of main.F90:65-197 (a baseline, selected by set_tuning with sweep -3): each
 // routine is one launch per block, as a straight port of the reference's per-block calls would
-// run, and the LU-SGS sweeps are one launch per hyperplane per block.  LU-SGS decks only.
+// run, and the LU-SGS sweeps (mean flow and k-omega, or the k-omega sweeps of an RK stage) are
+// one launch per hyperplane per block.
 template <typename R>
 static void plain_sweeps(trans_ctx* c, R* W, int neq)
 {
+    if (neq == 2) {
+        for (int b = 0; b < (int)c->hb.size(); b++) {
+            const HostBlock& h = c->hb[b];
+            const int np = (h.ni - 1) + (h.nj - 1) + (h.nk - 1) - 2;
+            const int g0 = c->seg_plane.wrange[b], ng = c->seg_plane.wrange[b + 1] - g0;
+            for (int s0 = 0; s0 < np; s0++)
+                KLAUNCH(c, "k_turb_wplane<fwd>", (k_lusgs_wplane<R, true, 2>), dim3(ng), dim3(WG), c->f, W, c->d_blk,
+                        c->seg_plane.d + g0, c->d_pblk, s0);
+            for (int s0 = 0; s0 < np; s0++)
+                KLAUNCH(c, "k_turb_wplane<bwd>", (k_lusgs_wplane<R, false, 2>), dim3(ng), dim3(WG), c->f, W,
+                        c->d_blk, c->seg_plane.d + g0, c->d_pblk, s0);
+        }
+        return;
+    }
     for (int b = 0; b < (int)c->hb.size(); b++) {
         const HostBlock& h = c->hb[b];
         const int np = (h.ni - 1) + (h.nj - 1) + (h.nk - 1) - 2;
@@ -1338,7 +1406,8 @@ static int plain_body(trans_ctx* c)
     const int if_turb = if_turb_now(c);
     const bool visc = c->p.if_euler == 0;
     const int neq = if_turb ? 7 : 5;
-    const int nsub = c->p.kind_dual != 0 ? c->p.kchld : 1;
+    const bool rk = c->p.if_rk == 1;
+    const int nsub = rk ? 3 : (c->p.kind_dual != 0 ? c->p.kchld : 1);
     auto cells = [&](const SegTable& t, int b) { return dim3(t.wrange[b + 1] - t.wrange[b]); };
     for (int s = 0; s < nsub; s++) {
         for (int b = 0; b < nblk; b++)
@@ -1380,6 +1449,33 @@ static int plain_body(trans_ctx* c)
         for (int b = 0; b < nblk; b++)
             KLAUNCH(c, "k_accumulate_visc", k_accumulate_visc, cells(c->seg_cell, b), dim3(WG), c->p, c->p.if_euler,
                     if_turb, c->f, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
+        if (rk) {  // main.F90:158-170: stage s + 1 of the three-stage Runge-Kutta update
+            for (int b = 0; b < nblk; b++)
+                KLAUNCH(c, "k_rk_stage", k_rk_stage, cells(c->seg_cell, b), dim3(WG), c->p, s + 1, if_turb ? 0 : 1,
+                        c->f, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b], c->d_part + c->seg_cell.wrange[b]);
+            if (if_turb) {
+                for (int b = 0; b < nblk; b++)
+                    KLAUNCH(c, "k_sst_rhs", k_sst_rhs, cells(c->seg_cell, b), dim3(WG), c->p, c->f, c->d_blk,
+                            c->seg_cell.d + c->seg_cell.wrange[b]);
+                if (c->p.implicit_fp32)
+                    plain_sweeps<float>(c, c->f.Wf, 2);
+                else
+                    plain_sweeps<double>(c, c->f.W, 2);
+                for (int b = 0; b < nblk; b++) {
+                    if (c->p.implicit_fp32)
+                        KLAUNCH(c, "k_unskew", k_unskew<float>, cells(c->seg_cell, b), dim3(WG), 5, 2, c->f,
+                                (const float*)c->f.Wf, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
+                    else
+                        KLAUNCH(c, "k_unskew", k_unskew<double>, cells(c->seg_cell, b), dim3(WG), 5, 2, c->f,
+                                (const double*)c->f.W, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
+                    KLAUNCH(c, "k_rk_update_v", k_rk_update_v, cells(c->seg_cell, b), dim3(WG), c->p, c->ki, if_turb,
+                            c->f, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b], c->d_part + c->seg_cell.wrange[b]);
+                }
+            }
+            KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk,
+                    c->d_resid);
+            continue;
+        }
         for (int b = 0; b < nblk; b++) {
             KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w, cells(c->seg_wtile, b), dim3(WG), c->p, c->f, c->d_blk,
                     c->seg_wtile.d + c->seg_wtile.wrange[b]);
@@ -1405,6 +1501,10 @@ static int plain_body(trans_ctx* c)
         }
         KLAUNCH(c, "k_norm_final", k_norm_final, dim3(nblk), dim3(256), c->d_part, c->d_wrange, c->d_blk, c->d_resid);
     }
+    if (rk && c->p.kind_dual != 0)  // main.F90:192-197 after the Runge-Kutta stages
+        for (int b = 0; b < nblk; b++)
+            KLAUNCH(c, "k_update_u", k_update_u, cells(c->seg_cell, b), dim3(WG), c->p.kind_dual, c->f, c->d_blk,
+                    c->seg_cell.d + c->seg_cell.wrange[b]);
     return launch_check(c, "plain iteration");
 }
 

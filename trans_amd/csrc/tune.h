@@ -149,8 +149,8 @@ int trans_hip_set_tuning(trans_ctx* c, const trans_tune_entry* t)
     if (!c || !t) return TRANS_EINVAL;
     if (t->sweep < -3 || t->sweep > 3) return fail(c, TRANS_EINVAL, "sweep must be -3, -2, -1, 0, 1, 2 or 3");
     if (t->sweep == -3) {  // the plain drop-in: per-block launches, per-plane sweeps, one stream, no graph
-        if (c->p.if_rk != 0 || c->p.if_precondition)
-            return fail(c, TRANS_EUNSUPPORTED, "the plain drop-in runs LU-SGS decks without preconditioning");
+        if (c->p.if_precondition)
+            return fail(c, TRANS_EUNSUPPORTED, "the plain drop-in runs decks without preconditioning");
         HIPCHK(c, hipStreamSynchronize(c->stream));
         int rc = set_sweep(c, -2, true);
         if (rc) return rc;
