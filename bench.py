@@ -83,7 +83,9 @@ def alg_bytes(name, neq, fp32, dual_shift=True):
     return table.get(name, ("cell", None))
 
 
-def build_workload(name, implicit_fp32=0, scale=1.0, back_pressure=1.2):
+def build_workload(name, implicit_fp32=0, scale=1.0, back_pressure=1.2, deck=()):
+    """(blocks, prepared params, description) of a named workload; `deck` holds KEY=VALUE
+    overrides of original.org values (OrgDeck field names) applied before preparation."""
     from trans_amd import cases, solver
     if name == "bump":
         blocks, org, glob = cases.bump_channel()
@@ -108,6 +110,13 @@ def build_workload(name, implicit_fp32=0, scale=1.0, back_pressure=1.2):
                 f"Re_c 1e6, SST-DES, AUSMPW+ + MDCD, RK3 at CFL 0.8 with a global time step")
     else:
         raise SystemExit(f"unknown workload {name}")
+    for kv in deck:
+        key, _, val = kv.partition("=")
+        if not hasattr(org, key):
+            raise SystemExit(f"unknown deck value {key}")
+        setattr(org, key, type(getattr(org, key))(float(val)) if isinstance(getattr(org, key), int)
+                else float(val))
+        desc += f", {key}={val}"
     prm = solver.prepare_case(blocks, org, glob, implicit_fp32=implicit_fp32,
                               device_geometry=name in ("plate8m", "compressor", "cascade"))
     if name == "cascade":
@@ -436,6 +445,8 @@ def main():
                     help="compressor workload: grid scale (1 = 8.13M cells, 0.63 = 2.0M, 0.397 = 0.49M)")
     ap.add_argument("--back-pressure", type=float, default=1.2,
                     help="compressor workload: outlet static pressure / inlet total pressure")
+    ap.add_argument("--deck", action="append", default=[], metavar="KEY=VALUE",
+                    help="override an original.org value of the workload's deck (repeatable)")
     ap.add_argument("--wake-stats", type=int, default=0, metavar="N",
                     help="instead of the timed run: N iterations with mean/RMS statistics over the second "
                          "half, batched path against the plain drop-in path")
@@ -463,7 +474,7 @@ def main():
     # the compressor sector (8M cells) is quoted with the FP32 implicit operator and the FP64
     # residual; the flat plate with both in FP64
     primary32 = args.implicit_fp32 if args.implicit_fp32 is not None else int(args.workload == "compressor")
-    blocks, prm, desc = build_workload(args.workload, primary32, args.scale, args.back_pressure)
+    blocks, prm, desc = build_workload(args.workload, primary32, args.scale, args.back_pressure, args.deck)
     cells = sum((b.ni - 1) * (b.nj - 1) * (b.nk - 1) for b in blocks)
     if args.converge:
         converge(blocks, prm, desc, args.converge)
@@ -492,7 +503,7 @@ def main():
     kern = kernel_table(stats, blocks, neq, bool(primary32), prof_steps)
     other = None
     if not args.no_fp32:
-        b2, p2, _ = build_workload(args.workload, 1 - primary32, args.scale, args.back_pressure)
+        b2, p2, _ = build_workload(args.workload, 1 - primary32, args.scale, args.back_pressure, args.deck)
         dt2, res2, _ = timed_run(b2, p2, args.steps, args.warmup, 0, args.tune)
         other = {"implicit": "fp32" if not primary32 else "fp64",
                  "value": round(cells * args.steps / dt2 / 1e6, 3),

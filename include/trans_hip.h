@@ -333,9 +333,10 @@ int trans_hip_get_tuning(const trans_ctx* ctx, trans_tune_entry* out);
  * sweep = -3 selects the plain drop-in schedule (streams, graph and the tile fields ignored):
  * every routine of trans_hip_iterate is launched once per block, as a straight port of the
  * reference's per-block calls would run, each LU-SGS sweep one launch per hyperplane per
- * block, on one stream, without graph replay; the same arithmetic as the batched schedule.
- * LU-SGS decks without preconditioning only (TRANS_EUNSUPPORTED otherwise).  A baseline for
- * the batched schedule; autotune leaves it for the batched candidates.
+ * block (with IF_RK = 1: the stage kernels once per block, the k-omega sweeps one launch per
+ * hyperplane per block), on one stream, without graph replay; the same arithmetic as the
+ * batched schedule.  Decks without preconditioning only (TRANS_EUNSUPPORTED otherwise).  A
+ * baseline for the batched schedule; autotune leaves it for the batched candidates.
  * A tile-wavefront sweep whose progress wait exceeds its spin limit (a schedule that cannot
  * advance) ends and makes the next synchronising call return TRANS_ETIMEOUT. */
 int trans_hip_set_tuning(trans_ctx* ctx, const trans_tune_entry* t);

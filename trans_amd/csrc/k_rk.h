@@ -11,9 +11,10 @@
 //   3: V = 1/3 Wt2 + 2/3 V + 2/3 Dtm*D/Vol
 // on rho..rho*E.  `fused` (no k-omega this stage) also does the rest of
 // Time/TIME_RK_V_Update_New.F90: p and T of the new state (:94-97) and the k and omega
-// norms of D(6:7) (:265-316).  Otherwise the new state and the step sizes the k-omega
-// line-march sweep reads go to the hyperplane-major buffer when there is one.
-__global__ void __launch_bounds__(256) k_rk_stage(trans_params p, int stage, int fused, Fields f,
+// norms of D(6:7) (:265-316).  Otherwise, with `wout`, the new state and the step sizes the
+// k-omega line-march sweep reads go to the hyperplane-major buffer when there is one (k_rk_w
+// writes them tile by tile instead).
+__global__ void __launch_bounds__(256) k_rk_stage(trans_params p, int stage, int fused, int wout, Fields f,
                                                   const BlkInfo* __restrict__ blks,
                                                   const Seg* __restrict__ seg, NormPart* part)
 {
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(256) k_rk_stage(trans_params p, int stage, int
             f.T[c] = p.rxm2 * pp / vn[0];
             dr[1] = fabs(f.D[5 * tot + c] / vn[0]);
             dr[2] = fabs(f.D[6 * tot + c] / vn[0]);
-        } else if (has_w(f)) {
+        } else if (wout && has_w(f)) {
             // step sizes as Time/TURBTWO_TIME_LUSGS_SST.F90:567-568 forms them
             const double dtl = p.dt_lusgs_main;
             const double stept = 2.0 * dtm / (3.0 * dtm / dtl + 2.0);
@@ -67,6 +68,48 @@ __global__ void __launch_bounds__(256) k_rk_stage(trans_params p, int stage, int
         }
     }
     norm_reduce(dr, arg, part, 0);
+}
+
+// The k-omega sweep's inputs of an explicit stage into the hyperplane-major buffer (march axis
+// j) tile by tile (see k_lusgs_rhs_w): the step sizes of Time/TURBTWO_TIME_LUSGS_SST.F90:567-568,
+// 1/rho and the velocities of the stage's new state, and Rds, the values k_rk_stage writes
+// one element per cache line otherwise.
+__global__ void __launch_bounds__(256) k_rk_w(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
+                                              const Seg* __restrict__ seg)
+{
+    __shared__ double sv[9][WT * WT];
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int ti = threadIdx.x % WT, tj = threadIdx.x / WT;
+    const TileCell tc = wtile_cell(s, b, ti, tj);
+    const long tot = f.tot;
+    if (tc.ok) {
+        const long c = cidx(b, tc.i, tc.j, tc.k);
+        const double dtm = f.Dtm[c], vol = f.Vol[c];
+        const double dtl = p.dt_lusgs_main;
+        const double stept = 2.0 * dtm / (3.0 * dtm / dtl + 2.0);
+        sv[0][threadIdx.x] = stept;
+        sv[1][threadIdx.x] = stept / vol;
+        const double v0 = f.V[c];
+        sv[2][threadIdx.x] = 1.0 / v0;
+#pragma unroll
+        for (int L = 1; L < 4; L++) sv[2 + L][threadIdx.x] = f.V[(long)L * tot + c] * (1.0 / v0);
+#pragma unroll
+        for (int d = 0; d < 3; d++) sv[6 + d][threadIdx.x] = f.Rds[(long)d * tot + c];
+    }
+    __syncthreads();
+    int di, dj;
+    wtile_diag(threadIdx.x, di, dj);
+    const TileCell oc = wtile_cell(s, b, di, dj);
+    if (!oc.ok) return;
+    const long w = widx(b, 1, oc.i, oc.j, oc.k);
+    const int q = dj * WT + di;
+    wput(f, W_STEPT, w, sv[0][q]);
+    wput(f, W_CVS, w, sv[1][q]);
+#pragma unroll
+    for (int L = 0; L < 4; L++) wput(f, W_V + L, w, sv[2 + L][q]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) wput(f, W_RDS + d, w, sv[6 + d][q]);
 }
 
 // Time/TIME_RK_V_Update_New.F90:75-260 after the k-omega LU-SGS: p and T of the new state

@@ -1258,19 +1258,35 @@ static int rk_update_async(trans_ctx* c, int stage)
 {
     const int nblk = (int)c->hb.size();
     const int if_turb = if_turb_now(c);
-    KLAUNCH(c, "k_rk_stage", k_rk_stage, dim3(c->seg_cell.n), dim3(WG), c->p, stage, if_turb ? 0 : 1, c->f,
-            c->d_blk, c->seg_cell.d, c->d_part);
+    // the sweep buffer along j moves through LDS tiles (whole cache lines per diagonal)
+    const bool wtiled = (c->f.W || c->f.Wf) && c->f.march == 1;
+    KLAUNCH(c, "k_rk_stage", k_rk_stage, dim3(c->seg_cell.n), dim3(WG), c->p, stage, if_turb ? 0 : 1, wtiled ? 0 : 1,
+            c->f, c->d_blk, c->seg_cell.d, c->d_part);
     if (if_turb) {
-        KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+        if (wtiled) {
+            KLAUNCH(c, "k_rk_w", k_rk_w, dim3(c->seg_wtile.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_wtile.d);
+            KLAUNCH(c, "k_sst_rhs_w", k_sst_rhs_w, dim3(c->seg_wtile.n), dim3(WG), c->p, c->f, c->d_blk,
+                    c->seg_wtile.d);
+        } else {
+            KLAUNCH(c, "k_sst_rhs", k_sst_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
+        }
         if (c->f.W || c->f.Wf) {
             if (c->p.implicit_fp32) {
                 launch_sweeps<float, 2>(c, c->f.Wf);
-                KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), 5, 2, c->f,
-                        (const float*)c->f.Wf, c->d_blk, c->seg_cell.d);
+                if (wtiled)
+                    KLAUNCH(c, "k_unskew_w", k_unskew_w<float>, dim3(c->seg_wtile.n), dim3(WG), 5, 2, c->f,
+                            (const float*)c->f.Wf, c->d_blk, c->seg_wtile.d);
+                else
+                    KLAUNCH(c, "k_unskew", k_unskew<float>, dim3(c->seg_cell.n), dim3(WG), 5, 2, c->f,
+                            (const float*)c->f.Wf, c->d_blk, c->seg_cell.d);
             } else {
                 launch_sweeps<double, 2>(c, c->f.W);
-                KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), 5, 2, c->f,
-                        (const double*)c->f.W, c->d_blk, c->seg_cell.d);
+                if (wtiled)
+                    KLAUNCH(c, "k_unskew_w", k_unskew_w<double>, dim3(c->seg_wtile.n), dim3(WG), 5, 2, c->f,
+                            (const double*)c->f.W, c->d_blk, c->seg_wtile.d);
+                else
+                    KLAUNCH(c, "k_unskew", k_unskew<double>, dim3(c->seg_cell.n), dim3(WG), 5, 2, c->f,
+                            (const double*)c->f.W, c->d_blk, c->seg_cell.d);
             }
         } else if (c->p.implicit_fp32) {
             KLAUNCH(c, "k_convert<double,float>", (k_convert<double, float>), dim3(c->seg_cell.n), dim3(WG), 2, 5,
@@ -1451,7 +1467,7 @@ static int plain_body(trans_ctx* c)
                     if_turb, c->f, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
         if (rk) {  // main.F90:158-170: stage s + 1 of the three-stage Runge-Kutta update
             for (int b = 0; b < nblk; b++)
-                KLAUNCH(c, "k_rk_stage", k_rk_stage, cells(c->seg_cell, b), dim3(WG), c->p, s + 1, if_turb ? 0 : 1,
+                KLAUNCH(c, "k_rk_stage", k_rk_stage, cells(c->seg_cell, b), dim3(WG), c->p, s + 1, if_turb ? 0 : 1, 1,
                         c->f, c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b], c->d_part + c->seg_cell.wrange[b]);
             if (if_turb) {
                 for (int b = 0; b < nblk; b++)
