@@ -209,7 +209,12 @@ def serving_leg(n_req=16, seed=20261202, target=0.5, max_iter=4000, mean_gap_s=1
             ids.append(srv.submit(built[q][0], params(q, bp), target, max_iter))
             nxt += 1
         due = nxt < n_req and reqs[nxt][0] - now < 0.01
-        unfinished = srv.progress(wait=not due)
+        left = srv.progress(wait=not due)
+        if left < unfinished or (nxt == n_req and left == 0):
+            # one line per completion: the solves submitted and the ones still queued or running
+            print(f"[serving] {nxt} submitted, {left} queued or running, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+        unfinished = left
         if not unfinished and nxt < n_req:
             time.sleep(max(0.0, min(0.01, reqs[nxt][0] - (time.perf_counter() - t0))))
     sts = [srv.poll(i) for i in ids]
