@@ -186,3 +186,13 @@ def test_tile_sweep_is_default_for_large_cross_sections():
     gres = sv.lusgs_update()
     _check_update(blocks, sv, oc, gres, ores)
     sv.close()
+
+
+def test_line_march_is_default_when_cross_sections_fit():
+    """Blocks whose (i, k) cross sections fit one line-march workgroup take the line march
+    along j by default (the tile wavefront only where no axis fits)."""
+    blocks, org, glob = cases.flat_plate(nblk=3, ni_total=49, nj=33, nk=6, lo=9, hi=25)
+    prm = solver.prepare_case(blocks, org, glob)
+    sv = Solver(blocks, prm, ki=1)
+    assert sv.tuning()["sweep"] == 1
+    sv.close()

@@ -1,7 +1,8 @@
 // Device-side layout and helpers shared by all kernels.
 //
 // Every field of every block lives in one device buffer per field, component-major:
-//   F[c * tot + blk.off + (i+2) + nx*((j+2) + ny*(k+2))],   i,j,k in -2..N+2 (N = points)
+//   F[c * tot + blk.off + (i+2) + nx*((j+2) + ny*(k+2))],   i,j,k in -2..N+2 (N = points),
+//   nx = N_i + 5 rounded up to a multiple of 16, blk.off = 13 mod 16, tot a multiple of 16
 // so that i is the contiguous (coalesced) index and all blocks can be addressed by one
 // kernel launch through a per-workgroup schedule table (Seg).
 #pragma once
@@ -12,7 +13,7 @@
 
 struct BlkInfo {
     int ni, nj, nk;  // points
-    int nx, ny, nz;  // padded extents (N+5)
+    int nx, ny, nz;  // padded extents: row length (see ROW_ALIGN below), N_j + 5, N_k + 5
     long off;        // element offset of the block inside each field buffer
     int npatch;
     int patch0;      // first entry of this block in the device patch table
@@ -99,7 +100,12 @@ __device__ __forceinline__ void wput(const Fields& f, int comp, long w, double v
         f.W[(long)comp * f.wtot + w] = v;
 }
 
-__device__ __forceinline__ long cidx(const BlkInfo& b, int i, int j, int k)
+// Rows are padded to a multiple of 16 elements and every block's offset is 13 modulo 16, so
+// interior cell i = 1 of every row sits on a multiple of 16 elements (128 bytes in FP64): the
+// 16- and 64-wide row segments the kernels read start on cache-line boundaries.
+constexpr int ROW_ALIGN = 16;
+
+__host__ __device__ __forceinline__ long cidx(const BlkInfo& b, int i, int j, int k)
 {
     return b.off + (long)(i + 2) + (long)b.nx * ((long)(j + 2) + (long)b.ny * (long)(k + 2));
 }

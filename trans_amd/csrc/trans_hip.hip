@@ -623,15 +623,16 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         if (d.ni < 4 || d.nj < 4 || d.nk < 3) return fail(c, TRANS_EINVAL, "block too small");
         HostBlock h;
         h.ni = d.ni; h.nj = d.nj; h.nk = d.nk;
-        h.nx = d.ni + 5; h.ny = d.nj + 5; h.nz = d.nk + 5;
-        h.off = off;
-        off += (long)h.nx * h.ny * h.nz;
+        h.nx = (d.ni + 5 + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN; h.ny = d.nj + 5; h.nz = d.nk + 5;
+        // interior i = 1 (element off + 3 of each row) on a multiple of ROW_ALIGN
+        h.off = (off + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN + (ROW_ALIGN - 3);
+        off = h.off + (long)h.nx * h.ny * h.nz;
         c->hb.push_back(h);
         c->patch0.push_back((int)c->patches.size());
         for (int q = 0; q < d.npatch; q++) c->patches.push_back(d.patches[q]);
     }
     c->patch0.push_back((int)c->patches.size());
-    c->tot = off;
+    c->tot = (off + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN;
     for (int b = 0; b < nblk; b++)
         for (int q = c->patch0[b]; q < c->patch0[b + 1]; q++) {
             const trans_patch& pt = c->patches[q];
@@ -659,8 +660,8 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     }
 
     Fields& f = c->f;
-    f.tot = off;
-    const long n = off;
+    f.tot = c->tot;
+    const long n = c->tot;
     if ((rc = dalloc(c, &f.V, 7 * n)) || (rc = dalloc(c, &f.PP, n)) || (rc = dalloc(c, &f.T, n)) ||
         (rc = dalloc(c, &f.Rmiu, n)) || (rc = dalloc(c, &f.SD, 9 * n)) || (rc = dalloc(c, &f.Grad, 3 * n)) ||
         (rc = dalloc(c, &f.Alagm, 3 * n)) || (rc = dalloc(c, &f.Vol, n)) || (rc = dalloc(c, &f.rad, n)) ||
@@ -744,9 +745,10 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         const char* env = std::getenv("TRANS_HIP_LINE_SWEEP");
         const bool lines_ok = !(env && env[0] == '0') && !c->p.if_precondition;
         const int order[3] = {1, 2, 0};
-        int m = lines_ok ? 3 : -1;  // blocks too large for a line-march workgroup: the tile wavefront
+        int m = -1;
         for (int q = 0; q < 3 && lines_ok && m < 0; q++)
             if (march_fits(c, order[q], nullptr)) m = order[q];
+        if (lines_ok && m < 0) m = 3;  // blocks too large for a line-march workgroup: the tile wavefront
         if ((rc = set_sweep(c, m, false))) return rc;
     }
     if ((rc = dalloc(c, &c->d_blk, nblk))) return rc;

@@ -63,7 +63,7 @@ int trans_hip_autotune(trans_ctx* c, int iters, trans_tune_entry* out, int max_o
     // tile wavefront candidates: tile shape (lines along i x along k) and progress chunk; they
     // are timed with helper streams and graph replay only
     const int tshape[4][2] = {{16, 16}, {32, 8}, {64, 4}, {128, 2}};
-    const int tchunk[3] = {4, 8, 16};
+    const int tchunk[4] = {2, 4, 8, 16};
     struct Cand {
         int sweep, ta, tb, chunk;
     };

@@ -130,10 +130,10 @@ class Solver:
     def autotune(self, iters=10):
         """Times every launch configuration and keeps the fastest; returns
         (chosen configuration, list of all timings)."""
-        out = (capi.TransTuneEntry * 32)()
+        out = (capi.TransTuneEntry * 64)()
         n = C.c_int()
-        self._check(self.lib.trans_hip_autotune(self.ctx, iters, out, 32, C.byref(n)))
-        return self.tuning(), [out[q].as_dict() for q in range(min(n.value, 32))]
+        self._check(self.lib.trans_hip_autotune(self.ctx, iters, out, 64, C.byref(n)))
+        return self.tuning(), [out[q].as_dict() for q in range(min(n.value, 64))]
 
     def set_tuning(self, sweep, streams, graph, tile_a=16, tile_b=16, chunk=8):
         t = capi.TransTuneEntry(sweep, streams, graph, 0.0, tile_a, tile_b, chunk)
