@@ -41,16 +41,18 @@ def alg_bytes(name, neq, fp32, dual_shift=True):
     table = {
         # V(1:4), PP, T, Rmiu, Vol, SD (9 face values per cell) in; Rds(3), Dtm out
         "k_spectral_radius": ("cell", d * (4 + 1 + 1 + 1 + 1 + 9 + 3 + 1) + (4 * 3 if fp32 else 0)),
-        # V(1:4[,6:7]), T, Alagm(3), SD(9), Vol in; DQ(12[,18]), shock out
-        "k_gradients": ("cell", d * ((6 if turb else 4) + 1 + 3 + 9 + 1 + (18 if turb else 12) + 1)),
-        # per face of direction d: cell V(1:4,6:7), T, Rmiu, Dst, Alagm, Vol, DQ(18) once; SD(3); GV(6) out
+        # V(1:4[,6:7]), T, Alagm(3), SD(9), Vol in; DQ(12[,18]), shock, Q(3[,5]) out
+        "k_gradients": ("cell", d * ((6 if turb else 4) + 1 + 3 + 9 + 1 + (18 if turb else 12) + 1 + (5 if turb else 3))),
+        # per face of direction d: cell Q(5), V(1), T, Rmiu, Dst, Alagm, Vol, DQ(18) once; SD(3); GV(6) out
         "k_visc_face<1>": ("face1", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),
         "k_visc_face<2>": ("face2", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),
         "k_visc_face<3>": ("face3", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),
-        # per face: cell V(1:4[,6:7]), PP, T, rad, Alagm, shock once; SD(3), Grad; G(5[,7]) out
+        # per face: cell Q(3[,5]), V(1), PP, T, rad, Alagm, shock once; SD(3), Grad; G(5[,7]) out
         "k_conv_face<1>": ("face1", d * ((6 if turb else 4) + 5 + 4 + (7 if turb else 5))),
         "k_conv_face<2>": ("face2", d * ((6 if turb else 4) + 5 + 4 + (7 if turb else 5))),
         "k_conv_face<3>": ("face3", d * ((6 if turb else 4) + 5 + 4 + (7 if turb else 5))),
+        # ghost cells of the three layers beyond each face: V(1:4[,6:7]) in, Q(3[,5]) out
+        "k_prim_ghost": ("ghost", d * ((6 if turb else 4) + (5 if turb else 3))),
         # per cell: G 3 x 7, GV 3 x 6 (viscous), V(1,3,4), Yc, Zc, Vol; D(7) out
         "k_accumulate_visc": ("cell", d * (21 + 18 + 6 + 7)),
         # D(5), V(5), Wt0(5), Wt1(5), Rds(3), Dtm, Vol in; D(5), F1 out
@@ -239,10 +241,11 @@ def serving_leg(n_req=16, seed=20261202, target=0.5, max_iter=4000, mean_gap_s=1
 
 
 def units(blocks):
-    u = {"cell": 0, "face1": 0, "face2": 0, "face3": 0}
+    u = {"cell": 0, "face1": 0, "face2": 0, "face3": 0, "ghost": 0}
     for b in blocks:
         n1, n2, n3 = b.ni - 1, b.nj - 1, b.nk - 1
         u["cell"] += n1 * n2 * n3
+        u["ghost"] += 6 * (n2 * n3 + n1 * n3 + n1 * n2)
         u["face1"] += (n1 + 1) * n2 * n3
         u["face2"] += n1 * (n2 + 1) * n3
         u["face3"] += n1 * n2 * (n3 + 1)

@@ -52,6 +52,7 @@ struct Fields {
     double* Wt1;      // 7
     double* Wt2;      // 7
     double* DQ;       // 18
+    double* Q;        // 5: u, v, w, k/rho, omega/rho (the divisions of the face stencils, formed once per residual)
     double* shock;
     double* Fscheme;  // DES blending of the rotated Roe dissipation (zero for inviscid runs)
     double* F1;       // LU-SGS diagonal

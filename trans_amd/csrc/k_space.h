@@ -284,14 +284,14 @@ struct Prim {
     double q[6];
 };
 
+// u, v, w from Q (formed once per cell per residual: the same divisions), p, T, rad
 __device__ __forceinline__ Prim load_prim(const Fields& f, long c)
 {
     Prim r;
     const long tot = f.tot;
-    const double den = f.V[c];
-    r.q[0] = f.V[tot + c] / den;
-    r.q[1] = f.V[2 * tot + c] / den;
-    r.q[2] = f.V[3 * tot + c] / den;
+    r.q[0] = f.Q[c];
+    r.q[1] = f.Q[tot + c];
+    r.q[2] = f.Q[2 * tot + c];
     r.q[3] = f.PP[c];
     r.q[4] = f.T[c];
     r.q[5] = f.rad[c];

@@ -31,6 +31,90 @@ __device__ __forceinline__ CellQ load_q(const Fields& f, long c, bool turb)
     return q;
 }
 
+// The same values from Q, where the residual formed them once per cell (k_gradients for the
+// interior cells, k_prim_ghost for the ghost layers, k_prim_cells without gradients): the
+// same divisions, so the same bits.
+__device__ __forceinline__ CellQ load_qc(const Fields& f, long c, bool turb)
+{
+    const long tot = f.tot;
+    CellQ q;
+    q.u = f.Q[c];
+    q.v = f.Q[tot + c];
+    q.w = f.Q[2 * tot + c];
+    q.t = f.T[c];
+    if (turb) {
+        q.k = f.Q[3 * tot + c];
+        q.o = f.Q[4 * tot + c];
+    } else {
+        q.k = 0.0;
+        q.o = 0.0;
+    }
+    return q;
+}
+
+__device__ __forceinline__ void store_q(const Fields& f, long c, const CellQ& q, bool turb)
+{
+    const long tot = f.tot;
+    f.Q[c] = q.u;
+    f.Q[tot + c] = q.v;
+    f.Q[2 * tot + c] = q.w;
+    if (turb) {
+        f.Q[3 * tot + c] = q.k;
+        f.Q[4 * tot + c] = q.o;
+    }
+}
+
+// Q of the interior cells without a gradient pass (inviscid decks).
+__global__ void __launch_bounds__(256) k_prim_cells(int if_turb, Fields f, const BlkInfo* __restrict__ blks,
+                                                    const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    const int lin = s.base + threadIdx.x;
+    if (lin >= NI1 * NJ1 * NK1) return;
+    int i, j, k;
+    decode3(lin, NI1, NJ1, i, j, k);
+    const long c = cidx(b, i, j, k);
+    store_q(f, c, load_q(f, c, if_turb != 0), if_turb != 0);
+}
+
+// Q of the three ghost layers beyond each face of a block (the cells the face stencils reach
+// outside the interior), after BC(-1) has filled them.  Index space per block: direction d,
+// side, layer, then the face's interior cells with the lower tangential index fastest.
+__device__ __forceinline__ long ghost_slab_cells(const BlkInfo& b)
+{
+    const long NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    return 6 * (NJ1 * NK1 + NI1 * NK1 + NI1 * NJ1);
+}
+
+__global__ void __launch_bounds__(256) k_prim_ghost(int if_turb, Fields f, const BlkInfo* __restrict__ blks,
+                                                    const Seg* __restrict__ seg)
+{
+    const Seg s = seg[blockIdx.x];
+    const BlkInfo b = blks[s.blk];
+    const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
+    long lin = s.base + threadIdx.x;
+    if (lin >= ghost_slab_cells(b)) return;
+    const long area[3] = {(long)NJ1 * NK1, (long)NI1 * NK1, (long)NI1 * NJ1};
+    int d = 0;
+    while (lin >= 6 * area[d]) {
+        lin -= 6 * area[d];
+        d++;
+    }
+    const int side = (int)(lin / (3 * area[d]));
+    const int layer = (int)((lin / area[d]) % 3);
+    const long t = lin % area[d];
+    const int n[3] = {NI1, NJ1, NK1};
+    const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;  // tangential axes, ta fastest
+    int ijk[3];
+    ijk[ta] = 1 + (int)(t % n[ta]);
+    ijk[tb] = 1 + (int)(t / n[ta]);
+    ijk[d] = side ? n[d] + 1 + layer : -layer;
+    const long c = cidx(b, ijk[0], ijk[1], ijk[2]);
+    store_q(f, c, load_q(f, c, if_turb != 0), if_turb != 0);
+}
+
 // face value between cell m (below) and p (above) in direction d (SPACEINTERP)
 __device__ __forceinline__ CellQ face_q(const Fields& f, int d, long cm, long cp, const CellQ& qm,
                                         const CellQ& qp, bool turb)
@@ -71,6 +155,7 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
     const bool turb = if_turb != 0;
     const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
     const CellQ q0 = load_q(f, c, turb);
+    store_q(f, c, q0, turb);
     // face values on the low (m) and high (p) face of each direction
     CellQ fm[3], fp[3];
 #pragma unroll
@@ -192,10 +277,10 @@ __device__ __forceinline__ void visc_face_flux(const trans_params& p, bool turb,
     const long tot = f.tot;
     const bool wall = wall_face(f.mw + b.mwoff, b, DIR, i, j, k);
     // cells f-2, f-1, f, f+1 and the face values at faces f-1, f, f+1
-    const CellQ qm2 = load_q(f, c0 - 2 * st, turb);
-    const CellQ qm1 = load_q(f, cm, turb);
-    const CellQ q0 = load_q(f, c0, turb);
-    const CellQ qp1 = load_q(f, c0 + st, turb);
+    const CellQ qm2 = load_qc(f, c0 - 2 * st, turb);
+    const CellQ qm1 = load_qc(f, cm, turb);
+    const CellQ q0 = load_qc(f, c0, turb);
+    const CellQ qp1 = load_qc(f, c0 + st, turb);
     const int d = DIR - 1;
     const CellQ fnn = face_q(f, d, c0 - 2 * st, cm, qm2, qm1, turb);
     const CellQ f0 = face_q(f, d, cm, c0, qm1, q0, turb);
@@ -258,9 +343,9 @@ __device__ __forceinline__ void visc_face_flux(const trans_params& p, bool turb,
     const double TTL = all * f.T[cm] + alr * f.T[c0];
     const double RmiuLt = (1.0 + p.csthlnd) / (TTL + p.csthlnd) * pow(TTL, 1.5);
     const double DEN0 = f.V[c0];
-    const double TKK0 = f.V[5 * tot + c0] / DEN0, TOO0 = f.V[6 * tot + c0] / DEN0;
+    const double TKK0 = q0.k, TOO0 = q0.o;
     const double DEN1 = f.V[cm];
-    const double TKK1 = f.V[5 * tot + cm] / DEN1, TOO1 = f.V[6 * tot + cm] / DEN1;
+    const double TKK1 = qm1.k, TOO1 = qm1.o;
     const double Den = all * DEN1 + alr * DEN0;
     const double TKK = all * TKK1 + alr * TKK0;
     const double TOO = all * TOO1 + alr * TOO0;
@@ -429,10 +514,9 @@ __device__ __forceinline__ void conv_face_flux(const trans_params& p, bool turb,
 #pragma unroll
         for (int n = 0; n < 6; n++) {
             const long c = c0 + (long)(n - 3) * st;
-            const double den = f.V[c];
-            sv[0][n] = f.V[tot + c] / den;
-            sv[1][n] = f.V[2 * tot + c] / den;
-            sv[2][n] = f.V[3 * tot + c] / den;
+            sv[0][n] = f.Q[c];
+            sv[1][n] = f.Q[tot + c];
+            sv[2][n] = f.Q[2 * tot + c];
             sv[3][n] = f.PP[c];
             sv[4][n] = f.T[c];
             sv[5][n] = f.rad[c];
@@ -524,11 +608,11 @@ __device__ __forceinline__ void conv_face_flux(const trans_params& p, bool turb,
         const double am2 = Al[cm2], am1 = Al[cm1], a0 = Al[c0], ap1 = Al[cp1];
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const long off = (long)(5 + q) * tot;
-            const double vm2 = f.V[off + cm2] / f.V[cm2];
-            const double vm1 = f.V[off + cm1] / f.V[cm1];
-            const double v0 = f.V[off + c0] / f.V[c0];
-            const double vp1 = f.V[off + cp1] / f.V[cp1];
+            const double* Qq = f.Q + (long)(3 + q) * tot;  // k/rho, omega/rho
+            const double vm2 = Qq[cm2];
+            const double vm1 = Qq[cm1];
+            const double v0 = Qq[c0];
+            const double vp1 = Qq[cp1];
             const double avm = face_avg(am2, am1, vm2, vm1);
             const double av0 = face_avg(am1, a0, vm1, v0);
             const double avp = face_avg(a0, ap1, v0, vp1);

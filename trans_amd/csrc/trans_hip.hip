@@ -72,6 +72,7 @@ struct trans_ctx {
     int n_outl = 0;
     SegTable seg_cell, seg_face[3];
     SegTable seg_wtile;  // WT x WT (i, j) tiles at each k: the tiled writers of the sweep buffer
+    SegTable seg_ghost;  // the three ghost layers beyond each block face (k_prim_ghost)
     // face fluxes + accumulation: 0 the face kernels through G and GV, then k_accumulate_visc;
     // 1 the LDS-tiled viscous and convective passes (k_resid_visc, k_resid_conv) on RTI x RTJ
     // (i, j) tiles marching chunks of rt_kchunk k planes (TRANS_HIP_FUSED=1 at creation)
@@ -313,6 +314,25 @@ static SegTable make_wtile_segs(trans_ctx* c)
         const HostBlock& h = c->hb[b];
         const int nt = ((h.ni - 1 + WT - 1) / WT) * ((h.nj - 1 + WT - 1) / WT) * (h.nk - 1);
         for (int q = 0; q < nt; q++) segs.push_back(Seg{b, q});
+        t.wrange.push_back((int)segs.size());
+    }
+    t.n = (int)segs.size();
+    if (dalloc(c, &t.d, segs.size()) == 0)
+        (void)h2d(c, t.d, segs.data(), segs.size() * sizeof(Seg));
+    return t;
+}
+
+// Schedule of k_prim_ghost: per block, ghost_slab_cells in 256-cell workgroups.
+static SegTable make_ghost_segs(trans_ctx* c)
+{
+    std::vector<Seg> segs;
+    SegTable t;
+    t.wrange.push_back(0);
+    for (int b = 0; b < (int)c->hb.size(); b++) {
+        const HostBlock& h = c->hb[b];
+        const long n1 = h.ni - 1, n2 = h.nj - 1, n3 = h.nk - 1;
+        const long cnt = 6 * (n2 * n3 + n1 * n3 + n1 * n2);
+        for (long base = 0; base < cnt; base += WG) segs.push_back(Seg{b, (int)base});
         t.wrange.push_back((int)segs.size());
     }
     t.n = (int)segs.size();
@@ -669,7 +689,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         (rc = dalloc(c, &f.Zc, n)) || (rc = dalloc(c, &f.thtc, n)) || (rc = dalloc(c, &f.D, 7 * n)) ||
         (rc = dalloc(c, &f.Rds, 3 * n)) || (rc = dalloc(c, &f.Dtm, n)) || (rc = dalloc(c, &f.Wt0, 7 * n)) ||
         (rc = dalloc(c, &f.Wt1, 7 * n)) || (rc = dalloc(c, &f.Wt2, 7 * n)) || (rc = dalloc(c, &f.DQ, 18 * n)) ||
-        (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.Fscheme, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.F1t, n)) ||
+        (rc = dalloc(c, &f.Q, 5 * n)) || (rc = dalloc(c, &f.shock, n)) || (rc = dalloc(c, &f.Fscheme, n)) || (rc = dalloc(c, &f.F1, n)) || (rc = dalloc(c, &f.F1t, n)) ||
         (rc = dalloc(c, &f.SDm, 9 * n)) || (rc = dalloc(c, &f.Stp, 2 * n)) ||
         (rc = dalloc(c, &f.G[0], 7 * n)) || (rc = dalloc(c, &f.G[1], 7 * n)) || (rc = dalloc(c, &f.G[2], 7 * n)))
         return rc;
@@ -756,6 +776,7 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
     c->seg_cell = make_segs(c, 0);
     for (int d = 0; d < 3; d++) c->seg_face[d] = make_segs(c, d + 1);
     c->seg_wtile = make_wtile_segs(c);
+    c->seg_ghost = make_ghost_segs(c);
     {
         const char* env = std::getenv("TRANS_HIP_FUSED");
         if (env && (env[0] == '0' || env[0] == '1')) c->fused = env[0] - '0';
@@ -1066,12 +1087,19 @@ int trans_hip_space(trans_ctx* c)
     const int if_turb = if_turb_now(c);
     const bool visc = c->p.if_euler == 0;
     int rc;
+    // Q (u, v, w, k/rho, omega/rho) of every cell the face stencils read: the gradient pass
+    // forms it for the interior cells (k_prim_cells without one), k_prim_ghost for the ghosts
     if (visc) {
         KLAUNCH(c, "k_gradients", k_gradients, dim3(c->seg_cell.n), dim3(WG), c->p, if_turb, c->f, c->d_blk,
                 c->seg_cell.d);
         if ((rc = launch_check(c, "k_gradients"))) return rc;
-        if ((rc = trans_hip_bc(c, 0))) return rc;
+    } else {
+        KLAUNCH(c, "k_prim_cells", k_prim_cells, dim3(c->seg_cell.n), dim3(WG), if_turb, c->f, c->d_blk,
+                c->seg_cell.d);
     }
+    KLAUNCH(c, "k_prim_ghost", k_prim_ghost, dim3(c->seg_ghost.n), dim3(WG), if_turb, c->f, c->d_blk,
+            c->seg_ghost.d);
+    if (visc && (rc = trans_hip_bc(c, 0))) return rc;
     if (c->fused && launch_fused(c, if_turb)) return launch_check(c, "space");
     fork_aux(c, 3);
     if (visc) {
@@ -1439,10 +1467,17 @@ static int plain_body(trans_ctx* c)
                 KLAUNCH(c, "k_bc_outlet", k_bc_outlet, dim3(c->outl_wrange[b + 1] - c->outl_wrange[b]), dim3(256), c->p,
                         if_turb, c->f, c->d_blk, c->d_patch, c->d_outl + c->outl_wrange[b]);
         }
-        if (visc) {
-            for (int b = 0; b < nblk; b++)
+        for (int b = 0; b < nblk; b++) {
+            if (visc)
                 KLAUNCH(c, "k_gradients", k_gradients, cells(c->seg_cell, b), dim3(WG), c->p, if_turb, c->f, c->d_blk,
                         c->seg_cell.d + c->seg_cell.wrange[b]);
+            else
+                KLAUNCH(c, "k_prim_cells", k_prim_cells, cells(c->seg_cell, b), dim3(WG), if_turb, c->f, c->d_blk,
+                        c->seg_cell.d + c->seg_cell.wrange[b]);
+            KLAUNCH(c, "k_prim_ghost", k_prim_ghost, cells(c->seg_ghost, b), dim3(WG), if_turb, c->f, c->d_blk,
+                    c->seg_ghost.d + c->seg_ghost.wrange[b]);
+        }
+        if (visc) {
             for (int b = 0; b < nblk; b++)
                 if (c->bc_wrange[b + 1] > c->bc_wrange[b])
                     KLAUNCH(c, "k_bc", k_bc, dim3(c->bc_wrange[b + 1] - c->bc_wrange[b]), dim3(128), c->p, if_turb, 0,
