@@ -101,6 +101,7 @@ class TransPlaneIntegrals(C.Structure):
 
 FIELD_D, FIELD_RDS, FIELD_DTM, FIELD_DQDXYZ, FIELD_SHOCK, FIELD_FSCHEME = 0, 1, 2, 3, 4, 5
 TRANS_EINVAL, TRANS_EHIP, TRANS_ENOMEM, TRANS_EUNSUPPORTED, TRANS_ESCRATCH, TRANS_ETIMEOUT = -1, -2, -3, -4, -5, -6
+TRANS_ENONFINITE = 1  # a residual norm came out NaN (the reference's STOP)
 TRANS_WD_STATIC_DOUBLES = 4 << 20
 
 

@@ -898,35 +898,47 @@ __device__ __forceinline__ void sst_kw_update(const trans_params& p, const Field
 // into part[blockIdx.x]; the entries q < q0 are left as they are.  All threads call it.
 __device__ __forceinline__ void norm_reduce(const double dr[3], int arg, NormPart* part, int q0)
 {
-    __shared__ double ssum[3][256];
-    __shared__ double smax[3][256];
-    __shared__ int sarg[3][256];
+    // within each 64-lane wave a butterfly of cross-lane exchanges, then the waves' results
+    // through LDS (one barrier); the max keeps the last index among equal values in any order
+    constexpr int NW = 256 / 64;
+    __shared__ double ssum[3][NW];
+    __shared__ double smax[3][NW];
+    __shared__ int sarg[3][NW];
+    double sm[3], mx[3];
+    int ag[3];
+#pragma unroll
     for (int q = 0; q < 3; q++) {
-        ssum[q][threadIdx.x] = dr[q] * dr[q];
-        smax[q][threadIdx.x] = arg >= 0 ? dr[q] : -1.0;
-        sarg[q][threadIdx.x] = arg;
+        sm[q] = dr[q] * dr[q];
+        mx[q] = arg >= 0 ? dr[q] : -1.0;
+        ag[q] = arg;
     }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
-            for (int q = 0; q < 3; q++) {
-                double sm = ssum[q][threadIdx.x], mx = smax[q][threadIdx.x];
-                int a = sarg[q][threadIdx.x];
-                norm_combine(sm, mx, a, ssum[q][threadIdx.x + w], smax[q][threadIdx.x + w],
-                             sarg[q][threadIdx.x + w]);
-                ssum[q][threadIdx.x] = sm;
-                smax[q][threadIdx.x] = mx;
-                sarg[q][threadIdx.x] = a;
-            }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double s2 = __shfl_xor(sm[q], off, 64);
+            const double m2 = __shfl_xor(mx[q], off, 64);
+            const int a2 = __shfl_xor(ag[q], off, 64);
+            norm_combine(sm[q], mx[q], ag[q], s2, m2, a2);
         }
-        __syncthreads();
-    }
+    const int w = threadIdx.x / 64;
+    if (threadIdx.x % 64 == 0)
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            ssum[q][w] = sm[q];
+            smax[q][w] = mx[q];
+            sarg[q][w] = ag[q];
+        }
+    __syncthreads();
     if (threadIdx.x == 0) {
         NormPart* o = part + blockIdx.x;
         for (int q = q0; q < 3; q++) {
-            o->sum[q] = ssum[q][0];
-            o->mx[q] = smax[q][0];
-            o->arg[q] = sarg[q][0];
+            double s = ssum[q][0], m = smax[q][0];
+            int a = sarg[q][0];
+            for (int v = 1; v < NW; v++) norm_combine(s, m, a, ssum[q][v], smax[q][v], sarg[q][v]);
+            o->sum[q] = s;
+            o->mx[q] = m;
+            o->arg[q] = a;
         }
         o->pad = 0;
     }
