@@ -41,8 +41,10 @@ def alg_bytes(name, neq, fp32, dual_shift=True):
     table = {
         # V(1:4), PP, T, Rmiu, Vol, SD (9 face values per cell) in; Rds(3), Dtm out
         "k_spectral_radius": ("cell", d * (4 + 1 + 1 + 1 + 1 + 9 + 3 + 1) + (4 * 3 if fp32 else 0)),
-        # V(1:4[,6:7]), T, Alagm(3), SD(9), Vol in; DQ(12[,18]), shock, Q(3[,5]) out
-        "k_gradients": ("cell", d * ((6 if turb else 4) + 1 + 3 + 9 + 1 + (18 if turb else 12) + 1 + (5 if turb else 3))),
+        # V(1:4[,6:7]) in; Q(3[,5]) out
+        "k_prim_cells": ("cell", d * ((6 if turb else 4) + (5 if turb else 3))),
+        # Q(3[,5]), T, Alagm(3), SD(9), Vol in; DQ(12[,18]), shock out
+        "k_gradients": ("cell", d * ((5 if turb else 3) + 1 + 3 + 9 + 1 + (18 if turb else 12) + 1)),
         # per face of direction d: cell Q(5), V(1), T, Rmiu, Dst, Alagm, Vol, DQ(18) once; SD(3); GV(6) out
         "k_visc_face<1>": ("face1", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),
         "k_visc_face<2>": ("face2", d * (6 + 1 + 1 + 1 + 1 + 1 + 18 + 3 + 6)),

@@ -57,15 +57,19 @@ __device__ __forceinline__ int ld_prog(const int* p)
     return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Edge terms: coherent across the device's L2 caches (see above).
+// Edge terms: coherent across the device's L2 caches (see above).  Element e of one tile's edge
+// buffer (NEQ x NJ1 x tile width elements) at a wave-uniform base, so the address is the
+// scalar-base + 32-bit vector-offset form, with no 64-bit multiply per term.
 template <typename R>
-__device__ __forceinline__ R ld_edge(const R* p)
+__device__ __forceinline__ R ld_edge(const R* base, unsigned e)
 {
-    return __hip_atomic_load(const_cast<R*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    TRANS_GLOBAL R* p = (TRANS_GLOBAL R*)((TRANS_GLOBAL char*)const_cast<R*>(base) + e * (unsigned)sizeof(R));
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename R>
-__device__ __forceinline__ void st_edge(R* p, R v)
+__device__ __forceinline__ void st_edge(R* base, unsigned e, R v)
 {
+    TRANS_GLOBAL R* p = (TRANS_GLOBAL R*)((TRANS_GLOBAL char*)base + e * (unsigned)sizeof(R));
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -114,11 +118,13 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
     const int up_tile_a = FORWARD ? tl.lo_a : tl.hi_a;
     const int up_tile_b = FORWARD ? tl.lo_b : tl.hi_b;
     // edge buffers: the tile's own outgoing ones, and those of its upstream tiles it reads
-    // (element (L, j, lane along the edge))
-    R* const out_a = edge + tl.ea;
-    R* const out_b = edge + tl.eb;
-    const R* const in_a = up_tile_a >= 0 ? edge + tiles[up_tile_a].ea : edge;
-    const R* const in_b = up_tile_b >= 0 ? edge + tiles[up_tile_b].eb : edge;
+    // (element (L, j, lane along the edge) at L * NJ1 * width + j * width + lane; tiles in one
+    // row along a share nb, in one column along b share na)
+    R* const out_a = sbase(edge + tl.ea);
+    R* const out_b = sbase(edge + tl.eb);
+    const R* const in_a = sbase(up_tile_a >= 0 ? edge + tiles[up_tile_a].ea : edge);
+    const R* const in_b = sbase(up_tile_b >= 0 ? edge + tiles[up_tile_b].eb : edge);
+    const unsigned estr_a = (unsigned)NJ1 * (unsigned)tl.nb, estr_b = (unsigned)NJ1 * (unsigned)tl.na;
     const int start = FORWARD ? (a - 1) + (bb - 1) : (Na - a) + (Nb - bb);
     // the tile's steps
     const int t0 = FORWARD ? tl.a0 + tl.b0 : (Na - (tl.a0 + tl.na)) + (Nb - (tl.b0 + tl.nb));
@@ -137,14 +143,16 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
     auto load_edges = [&](const int t, EdgeIn<R, NEQ>& e) {
         const int rel = t - start;
         const bool act = active && rel >= 0 && rel < NJ1;
-        const int jrow = act ? rel : 0;
+        const unsigned jrow = act ? (unsigned)rel : 0u;
         if (edge_in_a && act) {
+            const unsigned e0 = __umul24(jrow, (unsigned)tl.nb) + (unsigned)ib;
 #pragma unroll
-            for (int L = 0; L < NEQ; L++) e.a[L] = ld_edge(in_a + ((long)L * NJ1 + jrow) * tl.nb + ib);
+            for (int L = 0; L < NEQ; L++) e.a[L] = ld_edge(in_a, e0 + (unsigned)L * estr_a);
         }
         if (edge_in_b && act) {
+            const unsigned e0 = __umul24(jrow, (unsigned)tl.na) + (unsigned)ia;
 #pragma unroll
-            for (int L = 0; L < NEQ; L++) e.b[L] = ld_edge(in_b + ((long)L * NJ1 + jrow) * tl.na + ia);
+            for (int L = 0; L < NEQ; L++) e.b[L] = ld_edge(in_b, e0 + (unsigned)L * estr_b);
         }
     };
     auto step = [&](const int t, const LineRaw<R, NEQ>& cur, LineRaw<R, NEQ>& nxt, EdgeIn<R, NEQ>& ec,
@@ -245,12 +253,14 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
 #pragma unroll
                 for (int L = 0; L < NEQ; L++) buf[((npar * 2 + dir) * NEQ + L) * row + slot] = o[L];
                 if (d == 0 && edge_out_a && act) {
+                    const unsigned e0 = __umul24((unsigned)jrow, (unsigned)tl.nb) + (unsigned)ib;
 #pragma unroll
-                    for (int L = 0; L < NEQ; L++) st_edge(out_a + ((long)L * NJ1 + jrow) * tl.nb + ib, o[L]);
+                    for (int L = 0; L < NEQ; L++) st_edge(out_a, e0 + (unsigned)L * estr_a, o[L]);
                 }
                 if (d == 2 && edge_out_b && act) {
+                    const unsigned e0 = __umul24((unsigned)jrow, (unsigned)tl.na) + (unsigned)ia;
 #pragma unroll
-                    for (int L = 0; L < NEQ; L++) st_edge(out_b + ((long)L * NJ1 + jrow) * tl.na + ia, o[L]);
+                    for (int L = 0; L < NEQ; L++) st_edge(out_b, e0 + (unsigned)L * estr_b, o[L]);
                 }
             }
         }

@@ -31,9 +31,9 @@ __device__ __forceinline__ CellQ load_q(const Fields& f, long c, bool turb)
     return q;
 }
 
-// The same values from Q, where the residual formed them once per cell (k_gradients for the
-// interior cells, k_prim_ghost for the ghost layers, k_prim_cells without gradients): the
-// same divisions, so the same bits.
+// The same values from Q, where the residual formed them once per cell before the gradient
+// pass (k_prim_cells for the interior cells, k_prim_ghost for the ghost layers): the same
+// divisions, so the same bits.
 __device__ __forceinline__ CellQ load_qc(const Fields& f, long c, bool turb)
 {
     const long tot = f.tot;
@@ -64,7 +64,7 @@ __device__ __forceinline__ void store_q(const Fields& f, long c, const CellQ& q,
     }
 }
 
-// Q of the interior cells without a gradient pass (inviscid decks).
+// Q of the interior cells.
 __global__ void __launch_bounds__(256) k_prim_cells(int if_turb, Fields f, const BlkInfo* __restrict__ blks,
                                                     const Seg* __restrict__ seg)
 {
@@ -139,6 +139,7 @@ __device__ __forceinline__ CellQ face_q(const Fields& f, int d, long cm, long cp
 // ---------------------------------------------------------------------------------------
 // Space/DIFF_DqDxyz.F90:25-78, :232-265 (interior cells) and the shock sensor :300-384.
 // The ghost-layer gradients of the reference are overwritten by BC(0) and are not formed.
+template <bool FSCHEME>
 __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, Fields f,
                                                    const BlkInfo* __restrict__ blks,
                                                    const Seg* __restrict__ seg)
@@ -154,35 +155,37 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
     const long tot = f.tot;
     const bool turb = if_turb != 0;
     const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
-    const CellQ q0 = load_q(f, c, turb);
-    store_q(f, c, q0, turb);
-    // face values on the low (m) and high (p) face of each direction
-    CellQ fm[3], fp[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const CellQ qm = load_q(f, c - st[d], turb);
-        const CellQ qp = load_q(f, c + st[d], turb);
-        fm[d] = face_q(f, d, c - st[d], c, qm, q0, turb);
-        fp[d] = face_q(f, d, c, c + st[d], q0, qp, turb);
-    }
-    const double volinv = 1.0 / f.Vol[c];
+    const CellQ q0 = load_qc(f, c, turb);
+    // Green-Gauss sums fp_i*S_i+ - fm_i*S_i- over i = 1, 2, 3, added left to right as the
+    // reference writes them; one direction's face values are live at a time
     double g[18];
 #pragma unroll
-    for (int x = 0; x < 3; x++) {
-        const double s1p = f.SD[(long)(3 * x + 0) * tot + c + st[0]], s1m = f.SD[(long)(3 * x + 0) * tot + c];
-        const double s2p = f.SD[(long)(3 * x + 1) * tot + c + st[1]], s2m = f.SD[(long)(3 * x + 1) * tot + c];
-        const double s3p = f.SD[(long)(3 * x + 2) * tot + c + st[2]], s3m = f.SD[(long)(3 * x + 2) * tot + c];
-#define GG(F) (fp[0].F * s1p - fm[0].F * s1m + fp[1].F * s2p - fm[1].F * s2m + fp[2].F * s3p - fm[2].F * s3m)
-        g[0 + x] = volinv * GG(u);
-        g[3 + x] = volinv * GG(v);
-        g[6 + x] = volinv * GG(w);
-        g[9 + x] = volinv * GG(t);
-        if (turb) {
-            g[12 + x] = volinv * GG(k);
-            g[15 + x] = volinv * GG(o);
-        }
+    for (int d = 0; d < 3; d++) {
+        const CellQ qm = load_qc(f, c - st[d], turb);
+        const CellQ qp = load_qc(f, c + st[d], turb);
+        const CellQ fm = face_q(f, d, c - st[d], c, qm, q0, turb);
+        const CellQ fp = face_q(f, d, c, c + st[d], q0, qp, turb);
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            const double sp = f.SD[(long)(3 * x + d) * tot + c + st[d]], sm = f.SD[(long)(3 * x + d) * tot + c];
+#define GG(I, F) g[I + x] = d == 0 ? fp.F * sp - fm.F * sm : g[I + x] + fp.F * sp - fm.F * sm
+            GG(0, u);
+            GG(3, v);
+            GG(6, w);
+            GG(9, t);
+            if (turb) {
+                GG(12, k);
+                GG(15, o);
+            }
 #undef GG
+        }
     }
+    const double volinv = 1.0 / f.Vol[c];
+#pragma unroll
+    for (int q = 0; q < 12; q++) g[q] = volinv * g[q];
+    if (turb)
+#pragma unroll
+        for (int q = 12; q < 18; q++) g[q] = volinv * g[q];
 #pragma unroll
     for (int q = 0; q < 12; q++) f.DQ[(long)q * tot + c] = g[q];
     if (turb)
@@ -194,7 +197,7 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
     const double Vort = sqrt(2. * (2. * W12 * W12 + 2. * W13 * W13 + 2. * W23 * W23));
     const double DivU = sqrt(DuDx * DuDx + DvDy * DvDy + DwDz * DwDz);
     f.shock[c] = DivU * DivU / fmax(DivU * DivU + Vort * Vort, 1.0);
-    if (p.kind_roe == 2) {
+    if (FSCHEME) {
         // DES blending Fscheme (Space/DIFF_DqDxyz.F90:295-381; CH1 = 3, CH2 = 1, CH3 = 2,
         // Cdes_epsi = 0.61, Cdes_omeg = 0.78, cmu = 0.09), read by the rotated Roe flux only
         const double S11 = (DuDx + DuDx) / 2., S22 = (DvDy + DvDy) / 2., S33 = (DwDz + DwDz) / 2.;

@@ -1087,18 +1087,21 @@ int trans_hip_space(trans_ctx* c)
     const int if_turb = if_turb_now(c);
     const bool visc = c->p.if_euler == 0;
     int rc;
-    // Q (u, v, w, k/rho, omega/rho) of every cell the face stencils read: the gradient pass
-    // forms it for the interior cells (k_prim_cells without one), k_prim_ghost for the ghosts
-    if (visc) {
-        KLAUNCH(c, "k_gradients", k_gradients, dim3(c->seg_cell.n), dim3(WG), c->p, if_turb, c->f, c->d_blk,
-                c->seg_cell.d);
-        if ((rc = launch_check(c, "k_gradients"))) return rc;
-    } else {
-        KLAUNCH(c, "k_prim_cells", k_prim_cells, dim3(c->seg_cell.n), dim3(WG), if_turb, c->f, c->d_blk,
-                c->seg_cell.d);
-    }
+    // Q (u, v, w, k/rho, omega/rho) of every cell the gradient and face stencils read, formed
+    // once: k_prim_cells for the interior cells, k_prim_ghost for the ghost layers
+    KLAUNCH(c, "k_prim_cells", k_prim_cells, dim3(c->seg_cell.n), dim3(WG), if_turb, c->f, c->d_blk,
+            c->seg_cell.d);
     KLAUNCH(c, "k_prim_ghost", k_prim_ghost, dim3(c->seg_ghost.n), dim3(WG), if_turb, c->f, c->d_blk,
             c->seg_ghost.d);
+    if (visc) {
+        if (c->p.kind_roe == 2)
+            KLAUNCH(c, "k_gradients", k_gradients<true>, dim3(c->seg_cell.n), dim3(WG), c->p, if_turb, c->f,
+                    c->d_blk, c->seg_cell.d);
+        else
+            KLAUNCH(c, "k_gradients", k_gradients<false>, dim3(c->seg_cell.n), dim3(WG), c->p, if_turb, c->f,
+                    c->d_blk, c->seg_cell.d);
+        if ((rc = launch_check(c, "k_gradients"))) return rc;
+    }
     if (visc && (rc = trans_hip_bc(c, 0))) return rc;
     if (c->fused && launch_fused(c, if_turb)) return launch_check(c, "space");
     fork_aux(c, 3);
@@ -1468,14 +1471,16 @@ static int plain_body(trans_ctx* c)
                         if_turb, c->f, c->d_blk, c->d_patch, c->d_outl + c->outl_wrange[b]);
         }
         for (int b = 0; b < nblk; b++) {
-            if (visc)
-                KLAUNCH(c, "k_gradients", k_gradients, cells(c->seg_cell, b), dim3(WG), c->p, if_turb, c->f, c->d_blk,
-                        c->seg_cell.d + c->seg_cell.wrange[b]);
-            else
-                KLAUNCH(c, "k_prim_cells", k_prim_cells, cells(c->seg_cell, b), dim3(WG), if_turb, c->f, c->d_blk,
-                        c->seg_cell.d + c->seg_cell.wrange[b]);
+            KLAUNCH(c, "k_prim_cells", k_prim_cells, cells(c->seg_cell, b), dim3(WG), if_turb, c->f, c->d_blk,
+                    c->seg_cell.d + c->seg_cell.wrange[b]);
             KLAUNCH(c, "k_prim_ghost", k_prim_ghost, cells(c->seg_ghost, b), dim3(WG), if_turb, c->f, c->d_blk,
                     c->seg_ghost.d + c->seg_ghost.wrange[b]);
+            if (visc && c->p.kind_roe == 2)
+                KLAUNCH(c, "k_gradients", k_gradients<true>, cells(c->seg_cell, b), dim3(WG), c->p, if_turb, c->f,
+                        c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
+            else if (visc)
+                KLAUNCH(c, "k_gradients", k_gradients<false>, cells(c->seg_cell, b), dim3(WG), c->p, if_turb, c->f,
+                        c->d_blk, c->seg_cell.d + c->seg_cell.wrange[b]);
         }
         if (visc) {
             for (int b = 0; b < nblk; b++)
