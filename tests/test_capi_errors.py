@@ -37,6 +37,18 @@ def test_rejects_blocks_below_the_stencil(ni, nj, nk):
         serving.footprint(blocks, solver.prepare_case(blocks, org, glob))
 
 
+def test_rejects_a_block_beyond_32_bit_indexing():
+    """The kernels' schedule tables and per-block index arithmetic are 32-bit: a block of more
+    than 2^28 padded cells is refused at creation (here 1100^3 points, about 1.4e9 cells)."""
+    blocks, prm = _bump()
+    descs, keep = serving._descs(blocks)
+    descs[0].ni = descs[0].nj = descs[0].nk = 1100
+    rc, _ = _rc(blocks, prm, descs)
+    assert rc == capi.TRANS_EINVAL
+    descs[0].ni, descs[0].nj, descs[0].nk = blocks[0].ni, blocks[0].nj, blocks[0].nk
+    assert _rc(blocks, prm, descs)[0] == 0
+
+
 def test_rejects_an_empty_block_table():
     blocks, prm = _bump()
     rc, _ = _rc(blocks, prm, n=0)

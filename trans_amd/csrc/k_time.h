@@ -99,11 +99,14 @@ __device__ __forceinline__ void lusgs_rhs_wvals(const trans_params& p, const Fie
     for (int d = 0; d < 3; d++) v[14 + d] = f.Rds[(long)d * tot + c];
 }
 
-// The right-hand side into the hyperplane-major buffer (march axis j), tile by tile.
+// The right-hand side into the hyperplane-major buffer (march axis j), tile by tile.  R is the
+// buffer's element type (Wf for the FP32 operator, W otherwise): the values are rounded to it
+// before the LDS stage, which halves the stage for the FP32 operator (more workgroups per CU).
+template <typename R>
 __global__ void __launch_bounds__(256) k_lusgs_rhs_w(trans_params p, Fields f, const BlkInfo* __restrict__ blks,
                                                      const Seg* __restrict__ seg)
 {
-    __shared__ double sv[RHS_NW][WT * WT];
+    __shared__ R sv[RHS_NW][WT * WT];
     const Seg s = seg[blockIdx.x];
     const BlkInfo b = blks[s.blk];
     const int ti = threadIdx.x % WT, tj = threadIdx.x / WT;
@@ -112,7 +115,7 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs_w(trans_params p, Fields f, c
         double v[RHS_NW];
         lusgs_rhs_wvals(p, f, cidx(b, tc.i, tc.j, tc.k), v);
 #pragma unroll
-        for (int q = 0; q < RHS_NW; q++) sv[q][threadIdx.x] = v[q];
+        for (int q = 0; q < RHS_NW; q++) sv[q][threadIdx.x] = (R)v[q];
     }
     __syncthreads();
     int di, dj;
@@ -120,8 +123,9 @@ __global__ void __launch_bounds__(256) k_lusgs_rhs_w(trans_params p, Fields f, c
     const TileCell oc = wtile_cell(s, b, di, dj);
     if (!oc.ok) return;
     const long w = widx(b, 1, oc.i, oc.j, oc.k);
+    R* const Wd = std::is_same<R, float>::value ? (R*)f.Wf : (R*)f.W;
 #pragma unroll
-    for (int q = 0; q < RHS_NW; q++) wput(f, rhs_w_comp(q), w, sv[q][dj * WT + di]);
+    for (int q = 0; q < RHS_NW; q++) Wd[(long)rhs_w_comp(q) * f.wtot + w] = sv[q][dj * WT + di];
 }
 
 __global__ void __launch_bounds__(256) k_lusgs_rhs(trans_params p, Fields f,

@@ -94,22 +94,23 @@ __global__ void __launch_bounds__(256) k_prim_ghost(int if_turb, Fields f, const
     const Seg s = seg[blockIdx.x];
     const BlkInfo b = blks[s.blk];
     const int NI1 = b.ni - 1, NJ1 = b.nj - 1, NK1 = b.nk - 1;
-    long lin = s.base + threadIdx.x;
-    if (lin >= ghost_slab_cells(b)) return;
-    const long area[3] = {(long)NJ1 * NK1, (long)NI1 * NK1, (long)NI1 * NJ1};
+    // 32-bit index arithmetic: a block's ghost slab count fits (checked at creation)
+    int lin = s.base + (int)threadIdx.x;
+    if (lin >= (int)ghost_slab_cells(b)) return;
+    const int area[3] = {NJ1 * NK1, NI1 * NK1, NI1 * NJ1};
     int d = 0;
     while (lin >= 6 * area[d]) {
         lin -= 6 * area[d];
         d++;
     }
-    const int side = (int)(lin / (3 * area[d]));
-    const int layer = (int)((lin / area[d]) % 3);
-    const long t = lin % area[d];
+    const int side = lin / (3 * area[d]);
+    const int layer = (lin / area[d]) % 3;
+    const int t = lin % area[d];
     const int n[3] = {NI1, NJ1, NK1};
     const int ta = d == 0 ? 1 : 0, tb = d == 2 ? 1 : 2;  // tangential axes, ta fastest
     int ijk[3];
-    ijk[ta] = 1 + (int)(t % n[ta]);
-    ijk[tb] = 1 + (int)(t / n[ta]);
+    ijk[ta] = 1 + t % n[ta];
+    ijk[tb] = 1 + t / n[ta];
     ijk[d] = side ? n[d] + 1 + layer : -layer;
     const long c = cidx(b, ijk[0], ijk[1], ijk[2]);
     store_q(f, c, load_q(f, c, if_turb != 0), if_turb != 0);

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include <climits>
 #include "dev_common.h"
 #include "k_bc.h"
 #include "k_geom.h"
@@ -644,6 +645,9 @@ static int create_ctx(const trans_params* params, int nblk, const trans_block_de
         HostBlock h;
         h.ni = d.ni; h.nj = d.nj; h.nk = d.nk;
         h.nx = (d.ni + 5 + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN; h.ny = d.nj + 5; h.nz = d.nk + 5;
+        // schedule tables and the per-block index arithmetic of the kernels are 32-bit
+        if ((long)h.nx * h.ny * h.nz > (long)INT_MAX / 8)
+            return fail(c, TRANS_EINVAL, "block too large (more than 2^28 padded cells)");
         // interior i = 1 (element off + 3 of each row) on a multiple of ROW_ALIGN
         h.off = (off + ROW_ALIGN - 1) / ROW_ALIGN * ROW_ALIGN + (ROW_ALIGN - 3);
         off = h.off + (long)h.nx * h.ny * h.nz;
@@ -1226,8 +1230,11 @@ static int lusgs_update_async(trans_ctx* c, int shift = 0)
     const int if_turb = if_turb_now(c);
     // the sweep buffer along j moves through LDS tiles (whole cache lines per diagonal)
     const bool wtiled = (c->f.W || c->f.Wf) && c->f.march == 1;
-    if (wtiled)
-        KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w, dim3(c->seg_wtile.n), dim3(WG), c->p, c->f, c->d_blk,
+    if (wtiled && c->f.Wf)
+        KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w<float>, dim3(c->seg_wtile.n), dim3(WG), c->p, c->f, c->d_blk,
+                c->seg_wtile.d);
+    else if (wtiled)
+        KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w<double>, dim3(c->seg_wtile.n), dim3(WG), c->p, c->f, c->d_blk,
                 c->seg_wtile.d);
     else
         KLAUNCH(c, "k_lusgs_rhs", k_lusgs_rhs, dim3(c->seg_cell.n), dim3(WG), c->p, c->f, c->d_blk, c->seg_cell.d);
@@ -1535,8 +1542,12 @@ static int plain_body(trans_ctx* c)
             continue;
         }
         for (int b = 0; b < nblk; b++) {
-            KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w, cells(c->seg_wtile, b), dim3(WG), c->p, c->f, c->d_blk,
-                    c->seg_wtile.d + c->seg_wtile.wrange[b]);
+            if (c->f.Wf)
+                KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w<float>, cells(c->seg_wtile, b), dim3(WG), c->p, c->f,
+                        c->d_blk, c->seg_wtile.d + c->seg_wtile.wrange[b]);
+            else
+                KLAUNCH(c, "k_lusgs_rhs_w", k_lusgs_rhs_w<double>, cells(c->seg_wtile, b), dim3(WG), c->p, c->f,
+                        c->d_blk, c->seg_wtile.d + c->seg_wtile.wrange[b]);
             if (if_turb)
                 KLAUNCH(c, "k_sst_rhs_w", k_sst_rhs_w, cells(c->seg_wtile, b), dim3(WG), c->p, c->f, c->d_blk,
                         c->seg_wtile.d + c->seg_wtile.wrange[b]);
