@@ -192,10 +192,12 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
         const bool act = active && rel >= 0 && rel < NJ1;
         const int s = FORWARD ? t : nplanes - 1 - t;
         const int sn = FORWARD ? min(s + 1, nplanes - 1) : max(s - 1, 0);
-        // the next step's inputs, for lanes whose line has a cell on that step (a lane off its
-        // line's range computes on stale registers and stores to its sink slot)
-        const int reln = t + 1 - start;
-        if (active && reln >= 0 && reln < NJ1) line_load<R, NEQ>(src, lw + (unsigned)sn * (unsigned)nlines, nxt);
+        // the next step's inputs, loaded by every lane: the slot of the lane's line on hyperplane
+        // sn exists in the block's area whether or not the line has a cell there (a lane off its
+        // line's range computes on what it loaded and stores to its sink slot).  Loads that no
+        // branch skips keep the compiler's memory-counter waits at one step's count, so they
+        // stay in flight while the current cell is computed.
+        line_load<R, NEQ>(src, lw + (unsigned)sn * (unsigned)nlines, nxt);
         if (t + 1 <= t1 && (t + 1 - t0) % chunk != 0) load_edges(t + 1, en);
         const unsigned wst = act ? lw + (unsigned)s * (unsigned)nlines : my_sink;
         const int jrow = act ? rel : 0;  // j - 1 of the lane's cell
