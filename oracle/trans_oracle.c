@@ -4,6 +4,13 @@
  * Plain-C FP64 restatement of the reference's hot-path routines.  Each function names
  * the reference file and lines it follows.  Loop order, operation order and the
  * reference's quirks are kept on purpose (they are what parity is measured against).
+ *
+ * Parity unpinned: the density and energy rows of ora_precond_dw (the low-Mach
+ * preconditioning transform): its structure is fixed by tests (momentum unchanged, linear in
+ * the increment, affine in (U_r/c)^2) but no closed form fixes its coefficients, which do not
+ * reduce to the identity at (U_r/c)^2 = 1.  Every other function is checked against a closed
+ * form, a limit, an invariant or brute force in tests/test_oracle_pins.py and
+ * tests/test_oracle_rotation_pins.py.
  */
 #include "trans_oracle.h"
 

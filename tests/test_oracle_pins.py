@@ -1811,3 +1811,95 @@ def test_sst_cross_diffusion_far_from_walls():
     b, prm, V, k2, w2 = run(1e3 * cdif)
     clip = 0.5 * prm.gama2 * rho * 50.0 ** 2 / Re
     np.testing.assert_allclose((w2 - w0)[I, J, K], (clip * stept / f)[I, J, K], rtol=1e-9)
+
+
+def test_sst_blend_inside_the_transition_layer():
+    """Menter's SST blend away from both of its ends: F1 = tanh(arg1^4) with
+    arg1 = min(max(sqrt(k) / (beta* omega d), 500 nu / (d^2 omega)), 4 rho sigma_w2 k / (CD d^2)),
+    written in this nondimensionalisation (omega carries Re, nu 1/Re:
+    arg1 = max(sqrt(k) / (beta* omega d Re), 500 nu / (d^2 omega Re^2)), the third term being
+    unbounded when grad k . grad omega = 0).  The wall distance is chosen so that the first term
+    is 0.9 (F1 = tanh(0.6561) = 0.576): at zero strain omega is then destroyed with the blended
+    beta = F1 beta_1 + (1 - F1) beta_2 (Time/TURBTWO_TIME_LUSGS_SST.F90:139-415), which a swapped
+    blend, a wrong power of arg1 or a wrong Re scaling moves by far more than the bar."""
+    b, prm, edges = _box(ni=5, nj=5, nk=5, IF_Euler=0, IF_Turb_Org=1)
+    oc = OracleCase([b], prm)
+    V = oc[0]["v"]
+    V[1:4] = 0.0
+    oc[0]["wt0"][...] = V
+    oc[0]["wt1"][...] = V
+    oc[0]["Rds"][...] = 0.0
+    oc[0]["Dtm"][...] = 0.7
+    oc[0]["D"][...] = 0.0
+    oc[0]["DQdxyz"][...] = 0.0
+    V[5] = V[0] * 1.0  # k = 1: the first term of arg1 then sets its maximum (below)
+    oc[0]["wt0"][...] = V
+    oc[0]["wt1"][...] = V
+    rho, rk, rw = V[0], V[5], V[6]
+    k, om, Re = rk / rho, rw / rho, prm.ref
+    a_target = 0.9
+    d = np.sqrt(k) / (prm.beta_star * om * Re * a_target)
+    oc[0]["dst"][...] = d
+    # arg1 = max(first, viscous term): the viscous term stays below half the first one for any
+    # laminar viscosity up to 100 times the reference value, so arg1 is the first term exactly
+    assert np.all(500.0 * 100.0 / rho / (d * d * om * Re * Re) < 0.5 * a_target)
+    oc.call("ora_turb_lusgs_sst", 0)
+    F1 = np.tanh(a_target ** 4)
+    assert 0.5 < F1 < 0.65
+    beta = F1 * prm.beta1 + (1.0 - F1) * prm.beta2
+    stept = 2.0 * 0.7 / (3.0 * 0.7 / prm.dt_lusgs_main + 2.0)
+    f = 1.0 + (prm.beta_star * om * Re + 2.0 * beta * om * Re) * stept
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    np.testing.assert_allclose(oc[0]["D"][5][I, J, K], (-prm.beta_star * om * Re * rk * stept / f)[I, J, K], rtol=1e-12)
+    np.testing.assert_allclose(oc[0]["D"][6][I, J, K], (-beta * om * Re * rw * stept / f)[I, J, K], rtol=1e-12)
+
+
+def test_fscheme_blended_range_against_travin():
+    """Fscheme in its blended range against the hybrid central/upwind weight of Travin, Shur,
+    Strelets and Spalart (2002, "Physical and numerical upgrades in the detached-eddy
+    simulation of complex turbulent flows"): sigma = tanh(A^C_H1),
+    A = C_H2 max(C_DES Delta / (l_turb g) - c_A, 0), l_turb = sqrt((nu + nu_t) / (C_mu^1.5 K)),
+    K = max(sqrt((S^2 + Omega^2) / 2), 0.1 / tau), g = tanh(B^4),
+    B = C_H3 Omega max(Omega, S) / max((S^2 + Omega^2) / 2, floor), with C_H1 = 3, C_H2 = 1,
+    C_H3 = 2, C_mu = 0.09, tau = 1, S = sqrt(2 S:S), Omega = sqrt(2 W:W), and the SST-DES
+    constant C_DES = (1 - F1) 0.61 + F1 0.78 with Menter's F1 (CD floor 1e-10).  Written in
+    this nondimensionalisation (nu carries 1/Re, K and omega Re), with the reference's two
+    constants (Space/DIFF_DqDxyz.F90:295-381): the offset c_A = 0.1 (Travin et al.: 0.5) and
+    the floor alscale * 0.1 (Travin et al.: 1e-20), and its lower limit F_limiter1.  The inputs
+    are the oracle's gradients (fixed by the Green-Gauss exactness pins)."""
+    b, prm = _des_box(F_limiter1=0.05)
+    I, J, K = sl(1, b.ni - 1), sl(1, b.nj - 1), sl(1, b.nk - 1)
+    rng = np.random.Generator(np.random.MT19937(53))
+    oc = OracleCase([b], prm)
+    oc[0]["rmiu"][...] = prm.ref * 10.0 ** rng.uniform(-8, -2, b["rmiu"].shape)
+    oc.call("ora_spaceinterp", 0, 1)
+    oc.call("ora_diff_dqdxyz", 0, 1)
+    A = oc[0]
+    G = A["DQdxyz"][:, I, J, K]
+    ux, uy, uz, vx, vy, vz, wx, wy, wz = G[:9]
+    W = np.array([[0 * ux, (uy - vx) / 2, (uz - wx) / 2], [(vx - uy) / 2, 0 * ux, (vz - wy) / 2],
+                  [(wx - uz) / 2, (wy - vz) / 2, 0 * ux]])
+    S = np.array([[ux, (uy + vx) / 2, (uz + wx) / 2], [(vx + uy) / 2, vy, (vz + wy) / 2],
+                  [(wx + uz) / 2, (wy + vz) / 2, wz]])
+    Om = np.sqrt(2.0 * np.einsum("ij...,ij...->...", W, W))
+    Sn = np.sqrt(2.0 * np.einsum("ij...,ij...->...", S, S))
+    rho = A["v"][0][I, J, K]
+    k, om = A["v"][5][I, J, K] / rho, A["v"][6][I, J, K] / rho
+    T, mut, d, delta = A["t"][I, J, K], A["rmiu"][I, J, K], A["dst"][I, J, K], A["aleng"][I, J, K]
+    Re = prm.ref
+    mul = (1 + prm.csthlnd) / (T * prm.csth1 + prm.csthlnd) * (prm.csth1 * T) ** 1.5  # Sutherland
+    cd = np.maximum(2.0 * rho * prm.sigo2 / om * np.einsum("q...,q...->...", G[12:15], G[15:18]), 1e-10)
+    arg1 = np.maximum(np.sqrt(k) / (prm.beta_star * om * d * Re), 500.0 * mul / (rho * d * d * om * Re * Re))
+    arg1 = np.minimum(arg1, 4.0 * rho * prm.sigo2 * k / (cd * d * d))
+    F1 = np.tanh(arg1 ** 4)
+    cdes = (1.0 - F1) * 0.61 + F1 * 0.78
+    Kt = np.maximum(np.sqrt((Sn ** 2 + Om ** 2) / 2.0), 0.1 / 1.0)
+    lturb = np.sqrt((mut + mul) / rho / (Re * Kt * 0.09 ** 1.5))
+    B = 2.0 * Om * np.maximum(Om, Sn) / np.maximum((Om ** 2 + Sn ** 2) / 2.0, prm.alscale * 0.1)
+    g = np.tanh(B ** 4)
+    Aw = np.maximum(cdes * delta / (lturb * g) - 0.1, 0.0)
+    want = np.maximum(np.tanh(Aw ** 3), prm.f_limiter1)
+    got = A["Fscheme"][I, J, K]
+    blended = (want > prm.f_limiter1 + 1e-3) & (want < 1.0 - 1e-3)
+    assert np.sum(blended) >= 20  # of 168 cells
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15)
