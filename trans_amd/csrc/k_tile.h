@@ -288,15 +288,17 @@ __device__ __forceinline__ void tile_sweep(const Fields& f, R* __restrict__ Dq, 
 }
 
 // Persistent tile-wavefront sweep over all blocks.  ctr[0] is the item counter, prog[] the
-// per-tile progress of this sweep.
-template <typename R, bool FORWARD>
+// per-tile progress of this sweep.  KIND 0: the table holds mean-flow and k-omega items; 5 or
+// 2: only that kind (an instantiation without the other path, whose registers and LDS it
+// would otherwise reserve).
+template <typename R, bool FORWARD, int KIND>
 __global__ void __launch_bounds__(TILE_MAXT) k_lusgs_tile(Fields f, R* Dq, const BlkInfo* __restrict__ blks,
                                                           const SweepTile* __restrict__ tiles, int ntiles,
                                                           int* __restrict__ ctr, int* __restrict__ prog,
                                                           R* __restrict__ edge, int* __restrict__ err, int chunk,
                                                           int spin_limit)
 {
-    __shared__ R buf[2 * 2 * 5 * (TILE_MAXT + 2)];
+    __shared__ R buf[2 * 2 * (KIND == 2 ? 2 : 5) * (TILE_MAXT + 2)];
     __shared__ int s_item;
     const unsigned sink = (unsigned)(f.wsink + (long)blockIdx.x * TILE_MAXT);
     for (;;) {
@@ -308,9 +310,9 @@ __global__ void __launch_bounds__(TILE_MAXT) k_lusgs_tile(Fields f, R* Dq, const
         const int me = FORWARD ? q : ntiles - 1 - q;
         const SweepTile tl = tiles[me];
         const BlkInfo b = blks[tl.blk];
-        if (tl.neq == 5)
+        if (KIND != 2 && (KIND == 5 || tl.neq == 5))
             tile_sweep<R, FORWARD, 5>(f, Dq, b, tl, me, tiles, edge, prog, err, chunk, spin_limit, sink, buf);
-        else
+        else if (KIND != 5)
             tile_sweep<R, FORWARD, 2>(f, Dq, b, tl, me, tiles, edge, prog, err, chunk, spin_limit, sink, buf);
         __syncthreads();
     }

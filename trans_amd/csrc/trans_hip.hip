@@ -112,7 +112,8 @@ struct trans_ctx {
     // and 2 (mean flow and k-omega, mean flow alone, k-omega alone), the tiles' edge buffers,
     // the item and progress counters of both sweeps, and the error flag of the bounded waits
     bool tile_mode = false;
-    int tile_a = 16, tile_b = 16, tile_chunk = 8, tile_grid = 0;
+    int tile_a = 16, tile_b = 16, tile_chunk = 8;
+    int tile_grid_k[3] = {0, 0, 0};  // persistent grid of the tile kernel for tables 0, 1, 2
     SweepTile* d_tiles[3] = {nullptr, nullptr, nullptr};
     int ntiles[3] = {0, 0, 0};
     void* tile_edge = nullptr;
@@ -538,11 +539,17 @@ static int build_tiles(trans_ctx* c)
     int dev = 0, cus = 0, occ = 0;
     HIPCHK(c, hipGetDevice(&dev));
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (c->p.implicit_fp32)
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lusgs_tile<float, true>, TILE_MAXT, 0));
-    else
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_lusgs_tile<double, true>, TILE_MAXT, 0));
-    c->tile_grid = std::max(1, std::min(TILE_GRID_MAX, std::max(1, occ) * std::max(1, cus)));
+    for (int w = 0; w < 3; w++) {
+        const void* kern;
+        if (c->p.implicit_fp32)
+            kern = w == 0 ? (const void*)k_lusgs_tile<float, true, 0>
+                          : (w == 1 ? (const void*)k_lusgs_tile<float, true, 5> : (const void*)k_lusgs_tile<float, true, 2>);
+        else
+            kern = w == 0 ? (const void*)k_lusgs_tile<double, true, 0>
+                          : (w == 1 ? (const void*)k_lusgs_tile<double, true, 5> : (const void*)k_lusgs_tile<double, true, 2>);
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, TILE_MAXT, 0));
+        c->tile_grid_k[w] = std::max(1, std::min(TILE_GRID_MAX, std::max(1, occ) * std::max(1, cus)));
+    }
     return 0;
 }
 
@@ -1196,13 +1203,15 @@ static void launch_sweeps(trans_ctx* c, R* dq)
         const int nt = c->ntiles[w];
         const int ntmax = std::max(c->ntiles[0], std::max(c->ntiles[1], c->ntiles[2]));
         const int nctr = 2 + 2 * ntmax;
-        const int grid = std::max(1, std::min(c->tile_grid, nt));
+        const int grid = std::max(1, std::min(c->tile_grid_k[w], nt));
         R* edge = (R*)c->tile_edge;
         KLAUNCH(c, "k_tile_reset", k_tile_reset, dim3((nctr + 255) / 256), dim3(256), c->d_tctr, nctr);
-        KLAUNCH(c, "k_lusgs_tile<fwd>", (k_lusgs_tile<R, true>), dim3(grid), dim3(TILE_MAXT), c->f, dq, c->d_blk,
-                c->d_tiles[w], nt, c->d_tctr, c->d_tctr + 2, edge, c->d_err, c->tile_chunk, TILE_SPIN_LIMIT);
-        KLAUNCH(c, "k_lusgs_tile<bwd>", (k_lusgs_tile<R, false>), dim3(grid), dim3(TILE_MAXT), c->f, dq, c->d_blk,
-                c->d_tiles[w], nt, c->d_tctr + 1, c->d_tctr + 2 + ntmax, edge, c->d_err, c->tile_chunk,
+        constexpr int KIND = NEQ == 7 ? 0 : NEQ;
+        KLAUNCH(c, "k_lusgs_tile<fwd>", (k_lusgs_tile<R, true, KIND>), dim3(grid), dim3(TILE_MAXT), c->f, dq,
+                c->d_blk, c->d_tiles[w], nt, c->d_tctr, c->d_tctr + 2, edge, c->d_err, c->tile_chunk,
+                TILE_SPIN_LIMIT);
+        KLAUNCH(c, "k_lusgs_tile<bwd>", (k_lusgs_tile<R, false, KIND>), dim3(grid), dim3(TILE_MAXT), c->f, dq,
+                c->d_blk, c->d_tiles[w], nt, c->d_tctr + 1, c->d_tctr + 2 + ntmax, edge, c->d_err, c->tile_chunk,
                 TILE_SPIN_LIMIT);
     } else if (c->plane_launch) {
         const bool kw = NEQ == 2;
