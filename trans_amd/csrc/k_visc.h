@@ -156,55 +156,74 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
     const long tot = f.tot;
     const bool turb = if_turb != 0;
     const long st[3] = {1, (long)b.nx, (long)b.nx * b.ny};
-    const CellQ q0 = load_qc(f, c, turb);
-    // Green-Gauss sums fp_i*S_i+ - fm_i*S_i- over i = 1, 2, 3, added left to right as the
-    // reference writes them; one direction's face values are live at a time
-    double g[18];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const CellQ qm = load_qc(f, c - st[d], turb);
-        const CellQ qp = load_qc(f, c + st[d], turb);
-        const CellQ fm = face_q(f, d, c - st[d], c, qm, q0, turb);
-        const CellQ fp = face_q(f, d, c, c + st[d], q0, qp, turb);
-#pragma unroll
-        for (int x = 0; x < 3; x++) {
-            const double sp = f.SD[(long)(3 * x + d) * tot + c + st[d]], sm = f.SD[(long)(3 * x + d) * tot + c];
-#define GG(I, F) g[I + x] = d == 0 ? fp.F * sp - fm.F * sm : g[I + x] + fp.F * sp - fm.F * sm
-            GG(0, u);
-            GG(3, v);
-            GG(6, w);
-            GG(9, t);
-            if (turb) {
-                GG(12, k);
-                GG(15, o);
-            }
-#undef GG
-        }
-    }
     const double volinv = 1.0 / f.Vol[c];
+    // Green-Gauss sums fp_i*S_i+ - fm_i*S_i- over i = 1, 2, 3, added left to right as the
+    // reference writes them, in two passes of three variables (u, v, w, then T, k/rho,
+    // omega/rho) so that nine sums and one direction's face values are live at a time.
+    // Variable Q component qc of cell c+o: Q(qc), or T for qc < 0.
+    auto qv = [&](int qc, long cc) { return qc < 0 ? f.T[cc] : f.Q[(long)qc * tot + cc]; };
+    auto gg3 = [&](const int qc[3], int nv, double g[9]) {
 #pragma unroll
-    for (int q = 0; q < 12; q++) g[q] = volinv * g[q];
-    if (turb)
+        for (int d = 0; d < 3; d++) {
+            const double* Al = f.Alagm + (long)d * tot;
+            const double am = Al[c - st[d]], a0 = Al[c], ap = Al[c + st[d]];
+            double fm[3], fp[3];
 #pragma unroll
-        for (int q = 12; q < 18; q++) g[q] = volinv * g[q];
+            for (int v = 0; v < 3; v++) {
+                if (v >= nv) break;
+                const double q0 = qv(qc[v], c);
+                fm[v] = face_avg(am, a0, qv(qc[v], c - st[d]), q0);
+                fp[v] = face_avg(a0, ap, q0, qv(qc[v], c + st[d]));
+            }
 #pragma unroll
-    for (int q = 0; q < 12; q++) f.DQ[(long)q * tot + c] = g[q];
-    if (turb)
+            for (int x = 0; x < 3; x++) {
+                const double sp = f.SD[(long)(3 * x + d) * tot + c + st[d]], sm = f.SD[(long)(3 * x + d) * tot + c];
 #pragma unroll
-        for (int q = 12; q < 18; q++) f.DQ[(long)q * tot + c] = g[q];
-    const double DuDx = g[0], DuDy = g[1], DuDz = g[2], DvDx = g[3], DvDy = g[4], DvDz = g[5];
-    const double DwDx = g[6], DwDy = g[7], DwDz = g[8];
+                for (int v = 0; v < 3; v++) {
+                    if (v >= nv) break;
+                    g[3 * v + x] = d == 0 ? fp[v] * sp - fm[v] * sm : g[3 * v + x] + fp[v] * sp - fm[v] * sm;
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            if (q < 3 * nv) g[q] = volinv * g[q];
+    };
+    // u, v, w: their gradients, the shock sensor and the strain and vorticity magnitudes
+    double gu[9];
+    {
+        const int qc[3] = {0, 1, 2};
+        gg3(qc, 3, gu);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) f.DQ[(long)q * tot + c] = gu[q];
+    const double DuDx = gu[0], DuDy = gu[1], DuDz = gu[2], DvDx = gu[3], DvDy = gu[4], DvDz = gu[5];
+    const double DwDx = gu[6], DwDy = gu[7], DwDz = gu[8];
     const double W12 = (DuDy - DvDx) / 2., W13 = (DuDz - DwDx) / 2., W23 = (DvDz - DwDy) / 2.;
     const double Vort = sqrt(2. * (2. * W12 * W12 + 2. * W13 * W13 + 2. * W23 * W23));
     const double DivU = sqrt(DuDx * DuDx + DvDy * DvDy + DwDz * DwDz);
     f.shock[c] = DivU * DivU / fmax(DivU * DivU + Vort * Vort, 1.0);
+    double Skl = 0.0;
+    if (FSCHEME) {
+        const double S11 = (DuDx + DuDx) / 2., S22 = (DvDy + DvDy) / 2., S33 = (DwDz + DwDz) / 2.;
+        const double S12 = (DuDy + DvDx) / 2., S13 = (DuDz + DwDx) / 2., S23 = (DwDy + DvDz) / 2.;
+        Skl = sqrt(2. * (1. * S11 * S11 + 1. * S22 * S22 + 1. * S33 * S33 + 2. * S12 * S12 + 2. * S13 * S13 +
+                         2. * S23 * S23));
+    }
+    // T (and k/rho, omega/rho)
+    double gt[9];
+    {
+        const int qc[3] = {-1, 3, 4};
+        gg3(qc, turb ? 3 : 1, gt);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) f.DQ[(long)(9 + q) * tot + c] = gt[q];
+    if (turb)
+#pragma unroll
+        for (int q = 3; q < 9; q++) f.DQ[(long)(9 + q) * tot + c] = gt[q];
     if (FSCHEME) {
         // DES blending Fscheme (Space/DIFF_DqDxyz.F90:295-381; CH1 = 3, CH2 = 1, CH3 = 2,
         // Cdes_epsi = 0.61, Cdes_omeg = 0.78, cmu = 0.09), read by the rotated Roe flux only
-        const double S11 = (DuDx + DuDx) / 2., S22 = (DvDy + DvDy) / 2., S33 = (DwDz + DwDz) / 2.;
-        const double S12 = (DuDy + DvDx) / 2., S13 = (DuDz + DwDx) / 2., S23 = (DwDy + DvDz) / 2.;
-        const double Skl = sqrt(2. * (1. * S11 * S11 + 1. * S22 * S22 + 1. * S33 * S33 + 2. * S12 * S12 +
-                                      2. * S13 * S13 + 2. * S23 * S23));
         const double Den = f.V[c], RmiuT = f.Rmiu[c];
         const double RMIUL = sutherland(p, f.T[c]);
         const double vK = f.V[5 * tot + c] / Den, vOmega = f.V[6 * tot + c] / Den;
@@ -212,7 +231,7 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
         const double tau = 1.0;
         double gko[6];
 #pragma unroll
-        for (int q = 0; q < 6; q++) gko[q] = turb ? g[12 + q] : f.DQ[(long)(12 + q) * tot + c];
+        for (int q = 0; q < 6; q++) gko[q] = turb ? gt[3 + q] : f.DQ[(long)(12 + q) * tot + c];
         const double Dist = f.Dst[c];
         const double dTKO = gko[0] * gko[3] + gko[1] * gko[4] + gko[2] * gko[5];
         const double CDKO = fmax(2.0 * Den * p.sigo2 / vOmega * (dTKO), 1.0e-10);
