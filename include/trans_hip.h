@@ -195,7 +195,9 @@ typedef struct trans_ctx trans_ctx;
 
 /* Allocate device state for all blocks.  Replaces INPUT_Read_BlockDimensions'
  * ALLOCATEs (Input/Read_BlockDimensions.F90:306-401) and PLOT3D_Read_And_Init's
- * connectivity (Input/PLOT3DReadInit.F90:1-349), which the host has already built. */
+ * connectivity (Input/PLOT3DReadInit.F90:1-349), which the host has already built.
+ * TRANS_EINVAL for a block below the stencils (ni, nj < 4 or nk < 3 points) or beyond the
+ * kernels' 32-bit per-block indexing (more than 2^28 padded cells). */
 int trans_hip_create(const trans_params* params, int nblk, const trans_block_desc* blocks,
                      trans_ctx** out);
 /* (The LU-SGS sweeps run as a line march when every block's cross-section fits one workgroup,
