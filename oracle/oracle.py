@@ -37,6 +37,40 @@ def build():
     subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
 
 
+# Routines of the reference itself (preconditioning, MUSCL limiters, AUSMPW+ face flux, hybrid
+# RANS/LES functions), built from its sources by `make ref` (only where the reference's sources
+# and a Fortran compiler are present) into _ref/; ref_driver.f90 holds the C-callable wrappers.
+REF_SRC = "/root/reference/BaseVersion"
+REF_LIB = os.path.join(HERE, "_ref", "libtrans_ref.so")
+REF_FC = "/opt/rocm/lib/llvm/bin/flang"
+
+
+def build_ref() -> bool:
+    """Build _ref/libtrans_ref.so when the reference's sources and flang are present;
+    returns whether it was built."""
+    if not (os.path.isdir(REF_SRC) and os.path.exists(REF_FC)):
+        return False
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "ref"], check=True)
+    return True
+
+
+_ref_lib = None
+
+
+def ref_lib():
+    """The reference-built library, or None where it has not been built."""
+    global _ref_lib
+    if _ref_lib is None and os.path.exists(REF_LIB):
+        R = C.CDLL(REF_LIB)
+        R.ref_precond_dw.argtypes = [C.c_double, DP] + [C.c_double] * 9
+        R.ref_precond_para.argtypes = [C.c_double] * 10 + [DP]
+        R.ref_limiters.argtypes = [C.c_double] * 2 + [DP] * 2
+        R.ref_ausmpw_common.argtypes = [C.c_double] * 3 + [DP] + [C.c_double] * 2 + [DP]
+        R.ref_hybrid.argtypes = [C.c_int] + [C.c_double] * 9 + [DP] * 2
+        _ref_lib = R
+    return _ref_lib
+
+
 _lib = None
 
 

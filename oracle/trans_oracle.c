@@ -5,12 +5,12 @@
  * the reference file and lines it follows.  Loop order, operation order and the
  * reference's quirks are kept on purpose (they are what parity is measured against).
  *
- * Parity unpinned: the density and energy rows of ora_precond_dw (the low-Mach
- * preconditioning transform): its structure is fixed by tests (momentum unchanged, linear in
- * the increment, affine in (U_r/c)^2) but no closed form fixes its coefficients, which do not
- * reduce to the identity at (U_r/c)^2 = 1.  Every other function is checked against a closed
- * form, a limit, an invariant or brute force in tests/test_oracle_pins.py and
- * tests/test_oracle_rotation_pins.py.
+ * Every function is checked against a closed form, a limit, an invariant or brute force in
+ * tests/test_oracle_pins.py and tests/test_oracle_rotation_pins.py.  Those of the reference's
+ * routines that compile unmodified from its own sources are also checked bit for bit against
+ * that build (oracle/_ref, tests/test_oracle_ref.py): ora_precond_para and ora_precond_dw,
+ * whose coefficients no closed form fixes, ora_alimiter, ora_muscl2, ora_ausmpw_common,
+ * ora_fsst, ora_iddes_fd and ora_iddes_fe.
  */
 #include "trans_oracle.h"
 

@@ -11,13 +11,12 @@
  * N+5 entries, with i fastest after the component index.  Component arrays put the
  * component first: V(L,i,j,k) is V[(L-1) + 7*IX(i,j,k)].
  *
- * Every routine is pinned by tests/test_oracle_pins.py against closed forms, invariants or
- * brute force, except for these parts, whose parity is unpinned (checked GPU-vs-oracle
- * only): the crossed/reversed orientations of the block-interface copy (ora_bc, lcross = 1
- * or revMatrix != 0), the SST blend for 0 < F1 < 1 (both ends pinned), the length-scale
- * blend coefficients of Fscheme inside its blended range (its limits and monotonicity are
- * pinned), and the density and energy rows of ora_precond_dw (PrecondDW; only its
- * structure is pinned: momentum passes, linear in the increment, affine in (U_r/c)^2).
+ * Every routine is pinned against closed forms, invariants or brute force
+ * (tests/test_oracle_pins.py; the rotating frame and all 48 orientations of the
+ * block-interface copy in tests/test_oracle_rotation_pins.py).  The routines whose reference
+ * source compiles unmodified (preconditioning, MUSCL limiters, AUSMPW+ face flux, hybrid
+ * RANS/LES functions) are also compared bit for bit with that build of the reference
+ * (oracle/_ref, tests/test_oracle_ref.py).
  */
 #ifndef TRANS_ORACLE_H
 #define TRANS_ORACLE_H

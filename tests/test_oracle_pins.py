@@ -1503,8 +1503,10 @@ def test_restart_pressure_formula():
 # (Common/UTIL_Precond_Funcs.F90:6-30), the preconditioned Roe eigenvalues
 # (Space/CONVECT_Roe_Original.F90:319-332, Space/CONVECT_Roe_Rotated.F90:669-680), the
 # preconditioned spectral radius (Time/SpectralRadius.F90:91-120), the conservative update
-# (Time/TIME_Update_V.F90:522-606).  PrecondDW itself is pinned only structurally (momentum
-# passes, linear, affine in (U_r/c)^2); its placement in LU-SGS is pinned by the dense solve.
+# (Time/TIME_Update_V.F90:522-606).  PrecondDW is pinned here structurally (momentum passes,
+# linear, affine in (U_r/c)^2) and its placement in LU-SGS by the dense solve; its
+# coefficients, which no closed form fixes, are compared bit for bit with the reference's own
+# routine in test_oracle_ref.py.
 
 def _pc_prm(ref=None, **kw):
     prm = _prm(**{**dict(XM=0.1, IF_Precondition=1), **kw})
