@@ -37,9 +37,10 @@ def build():
     subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
 
 
-# Routines of the reference itself (preconditioning, MUSCL limiters, AUSMPW+ face flux, hybrid
-# RANS/LES functions), built from its sources by `make ref` (only where the reference's sources
-# and a Fortran compiler are present) into _ref/; ref_driver.f90 holds the C-callable wrappers.
+# Routines of the reference itself (preconditioning, limiters, hybrid RANS/LES functions, the
+# convective reconstructions and fluxes, the dual-time shift, the RK save), built from its
+# sources by `make ref` (only where the reference's sources and a Fortran compiler are
+# present) into _ref/; ref_driver.f90 holds the C-callable wrappers.
 REF_SRC = "/root/reference/BaseVersion"
 REF_LIB = os.path.join(HERE, "_ref", "libtrans_ref.so")
 REF_FC = "/opt/rocm/lib/llvm/bin/flang"
@@ -67,6 +68,10 @@ def ref_lib():
         R.ref_limiters.argtypes = [C.c_double] * 2 + [DP] * 2
         R.ref_ausmpw_common.argtypes = [C.c_double] * 3 + [DP] + [C.c_double] * 2 + [DP]
         R.ref_hybrid.argtypes = [C.c_int] + [C.c_double] * 9 + [DP] * 2
+        R.ref_block_dims.argtypes = [C.c_int] * 3
+        R.ref_bind.argtypes = [C.c_int, C.c_void_p]
+        R.ref_block_scalars.argtypes = [C.c_double] * 5 + [C.c_int] * 4
+        R.ref_block_call.argtypes = [C.c_int]
         _ref_lib = R
     return _ref_lib
 

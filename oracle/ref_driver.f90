@@ -2,14 +2,17 @@
 ! unmodified from its own sources: the low-Mach preconditioning (getPrecondPara, PrecondDW;
 ! Common/UTIL_Precond_Funcs.F90:7-31, 91-129), the MUSCL limiters (ALIMITER, MUSCL2;
 ! Space/UTIL_MUSCL.F90), the AUSMPW+ face flux (CONVECT_AUSMPW_Plus_Common;
-! Space/CONVECT_AUSMPW_Plus.F90:204-291) and the hybrid RANS/LES functions (FSST,
-! fd_function, fe_function; Time/Hybrid_function.F90).  oracle/Makefile (target ref) compiles
+! Space/CONVECT_AUSMPW_Plus.F90:204-291), the hybrid RANS/LES functions (FSST,
+! fd_function, fe_function; Time/Hybrid_function.F90), and whole-block routines run on a
+! block whose arrays are the oracle's (the convective reconstructions and fluxes of Space/,
+! TIME_Update_U, TIME_RK_Save).  oracle/Makefile (target ref) compiles
 ! those files where they lie, with default REAL promoted to 8 bytes as the FP64 build of the
 ! reference is, and links them with this driver into oracle/_ref/libtrans_ref.so.  Only tests/
 ! load that library, to check the oracle's restatements against the reference bit for bit.
 !
 ! The routines read module variables of Module_Global.F90 (PPf, Tiny, XM, Csthlnd, Csth1, Ref,
-! Kind_SubHY); each wrapper sets the ones its routines read before the calls.
+! Kind_SubHY, gama_max, gama_min, F_limiter2, KI, Itr_Chld, Kind_Dual, Kind_Model); each
+! wrapper sets the ones its routines read before the calls.
 
 subroutine ref_precond_para(xm_in, csthlnd_in, csth1_in, ref_in, vm2, am2, rr, tt, rmiut, alagmr, ubetac2) &
     bind(C, name="ref_precond_para")
@@ -112,4 +115,166 @@ subroutine ref_hybrid(kind_subhy_in, ref_in, den, vk, vomega, dist, rmiul, cdiff
     res(1) = FSST(d, k, w, ds, ml, cd)
     res(2) = fd_function(d, mt, ds, al, x(1), x(2), x(3), x(4), x(5), x(6), x(7), x(8), x(9))
     res(3) = fe_function(d, mt, ml, ds, al, x(1), x(2), x(3), x(4), x(5), x(6), x(7), x(8), x(9))
+end subroutine
+
+! Whole-block routines.  ref_block_dims sets the block's dimensions (points ni, nj, nk) in a
+! block of this driver's own and points the module's ThisBlock at it; ref_bind points one of
+! the module's block arrays (Module_Global.F90:190-269) at an oracle array, which holds the
+! reference's layout (component first, every direction -2:N+2); ref_block_scalars sets the
+! module variables the routines read; ref_block_call runs one routine on the bound block.
+module ref_block_state
+    use Block, only: BlockStruct
+    implicit none
+    type(BlockStruct), target, save :: blk
+end module
+
+subroutine ref_block_dims(ni, nj, nk) bind(C, name="ref_block_dims")
+    use iso_c_binding, only: c_int
+    use Global, only: ThisBlock
+    use ref_block_state, only: blk
+    implicit none
+    integer(c_int), value :: ni, nj, nk
+    blk%NI = ni
+    blk%NJ = nj
+    blk%NK = nk
+    blk%NDim = [ni, nj, nk]
+    blk%NI1 = ni - 1
+    blk%NJ1 = nj - 1
+    blk%NK1 = nk - 1
+    blk%NI2 = ni - 2
+    blk%NJ2 = nj - 2
+    blk%NK2 = nk - 2
+    blk%NI3 = ni - 3
+    blk%NJ3 = nj - 3
+    blk%NK3 = nk - 3
+    blk%NIp1 = ni + 1
+    blk%NJp1 = nj + 1
+    blk%NKp1 = nk + 1
+    blk%NIp2 = ni + 2
+    blk%NJp2 = nj + 2
+    blk%NKp2 = nk + 2
+    blk%MI = ni + 2
+    blk%MJ = nj + 2
+    blk%MK = nk + 2
+    ThisBlock => blk
+end subroutine
+
+subroutine ref_bind(which, p) bind(C, name="ref_bind")
+    use iso_c_binding, only: c_int, c_ptr, c_f_pointer
+    use Global, only: ThisBlock, V, PP, T, rad, rccl, rccr, shock, VL, VR, D, SD, Grad, Alagm, &
+                      Dtm, Vol, Wt0, Wt1, Wt2, DQdxyz
+    implicit none
+    integer(c_int), value :: which
+    type(c_ptr), value :: p
+    real(8), pointer :: f(:)
+    integer :: a, b, c, n
+    a = ThisBlock%NI + 2
+    b = ThisBlock%NJ + 2
+    c = ThisBlock%NK + 2
+    n = (a + 3) * (b + 3) * (c + 3)
+    select case (which)
+    case (1)
+        call c_f_pointer(p, f, [7 * n])
+        V(1:7, -2:a, -2:b, -2:c) => f
+    case (2)
+        call c_f_pointer(p, f, [n])
+        PP(-2:a, -2:b, -2:c) => f
+    case (3)
+        call c_f_pointer(p, f, [n])
+        T(-2:a, -2:b, -2:c) => f
+    case (4)
+        call c_f_pointer(p, f, [n])
+        rad(-2:a, -2:b, -2:c) => f
+    case (5)
+        call c_f_pointer(p, f, [3 * n])
+        rccl(1:3, -2:a, -2:b, -2:c) => f
+    case (6)
+        call c_f_pointer(p, f, [3 * n])
+        rccr(1:3, -2:a, -2:b, -2:c) => f
+    case (7)
+        call c_f_pointer(p, f, [n])
+        shock(-2:a, -2:b, -2:c) => f
+    case (8)
+        call c_f_pointer(p, f, [24 * n])
+        VL(1:3, 1:8, -2:a, -2:b, -2:c) => f
+    case (9)
+        call c_f_pointer(p, f, [24 * n])
+        VR(1:3, 1:8, -2:a, -2:b, -2:c) => f
+    case (10)
+        call c_f_pointer(p, f, [7 * n])
+        D(1:7, -2:a, -2:b, -2:c) => f
+    case (11)
+        call c_f_pointer(p, f, [9 * n])
+        SD(1:3, 1:3, -2:a, -2:b, -2:c) => f
+    case (12)
+        call c_f_pointer(p, f, [3 * n])
+        Grad(1:3, -2:a, -2:b, -2:c) => f
+    case (13)
+        call c_f_pointer(p, f, [3 * n])
+        Alagm(1:3, -2:a, -2:b, -2:c) => f
+    case (14)
+        call c_f_pointer(p, f, [n])
+        Dtm(-2:a, -2:b, -2:c) => f
+    case (15)
+        call c_f_pointer(p, f, [n])
+        Vol(-2:a, -2:b, -2:c) => f
+    case (16)
+        call c_f_pointer(p, f, [7 * n])
+        Wt0(1:7, -2:a, -2:b, -2:c) => f
+    case (17)
+        call c_f_pointer(p, f, [7 * n])
+        Wt1(1:7, -2:a, -2:b, -2:c) => f
+    case (18)
+        call c_f_pointer(p, f, [7 * n])
+        Wt2(1:7, -2:a, -2:b, -2:c) => f
+    case (19)
+        call c_f_pointer(p, f, [18 * n])
+        DQdxyz(1:18, -2:a, -2:b, -2:c) => f
+    end select
+end subroutine
+
+subroutine ref_block_scalars(ppf_in, tiny_in, gmax, gmin, flim2, ki_in, itr_in, kdual, kmodel) &
+    bind(C, name="ref_block_scalars")
+    use iso_c_binding, only: c_double, c_int
+    use Global, only: PPf, Tiny, gama_max, gama_min, F_limiter2, KI, itr_chld, Kind_Dual, Kind_Model
+    implicit none
+    real(c_double), value :: ppf_in, tiny_in, gmax, gmin, flim2
+    integer(c_int), value :: ki_in, itr_in, kdual, kmodel
+    PPf = ppf_in
+    Tiny = tiny_in
+    gama_max = gmax
+    gama_min = gmin
+    F_limiter2 = flim2
+    KI = ki_in
+    itr_chld = itr_in
+    Kind_Dual = kdual
+    Kind_Model = kmodel
+end subroutine
+
+! 1 CONVECT_MUSCL, 2 CONVECT_MUSCL2, 3 CONVECT_MDCD, 4 CONVECT_MDCDHYB, 5 TURBTWO_CONVECT_MUSCL,
+! 6 CONVECT_AUSMPW_Plus, 7 TURBTWO_CONVECT_Upwind, 8 TIME_Update_U, 9 TIME_RK_Save
+subroutine ref_block_call(which) bind(C, name="ref_block_call")
+    use iso_c_binding, only: c_int
+    implicit none
+    integer(c_int), value :: which
+    select case (which)
+    case (1)
+        call CONVECT_MUSCL
+    case (2)
+        call CONVECT_MUSCL2
+    case (3)
+        call CONVECT_MDCD
+    case (4)
+        call CONVECT_MDCDHYB
+    case (5)
+        call TURBTWO_CONVECT_MUSCL
+    case (6)
+        call CONVECT_AUSMPW_Plus
+    case (7)
+        call TURBTWO_CONVECT_Upwind
+    case (8)
+        call TIME_Update_U
+    case (9)
+        call TIME_RK_Save
+    end select
 end subroutine

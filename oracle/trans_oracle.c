@@ -10,7 +10,9 @@
  * routines that compile unmodified from its own sources are also checked bit for bit against
  * that build (oracle/_ref, tests/test_oracle_ref.py): ora_precond_para and ora_precond_dw,
  * whose coefficients no closed form fixes, ora_alimiter, ora_muscl2, ora_ausmpw_common,
- * ora_fsst, ora_iddes_fd and ora_iddes_fe.
+ * ora_fsst, ora_iddes_fd, ora_iddes_fe, and the whole-block ora_convect_muscl,
+ * ora_convect_muscl2, ora_convect_mdcd, ora_convect_mdcdhyb, ora_turb_convect_muscl,
+ * ora_convect_ausmpw, ora_turb_convect_upwind, ora_update_u and ora_rk_save.
  */
 #include "trans_oracle.h"
 

@@ -14,9 +14,10 @@
  * Every routine is pinned against closed forms, invariants or brute force
  * (tests/test_oracle_pins.py; the rotating frame and all 48 orientations of the
  * block-interface copy in tests/test_oracle_rotation_pins.py).  The routines whose reference
- * source compiles unmodified (preconditioning, MUSCL limiters, AUSMPW+ face flux, hybrid
- * RANS/LES functions) are also compared bit for bit with that build of the reference
- * (oracle/_ref, tests/test_oracle_ref.py).
+ * source compiles unmodified (preconditioning, limiters, hybrid RANS/LES functions, the
+ * convective reconstructions, the AUSMPW+ and k-omega upwind fluxes, the dual-time shift, the
+ * RK save) are also compared bit for bit with that build of the reference (oracle/_ref,
+ * tests/test_oracle_ref.py).
  */
 #ifndef TRANS_ORACLE_H
 #define TRANS_ORACLE_H
