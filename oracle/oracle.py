@@ -38,7 +38,7 @@ def build():
 
 
 # Routines of the reference itself (preconditioning, limiters, hybrid RANS/LES functions, the
-# convective reconstructions and fluxes, the dual-time shift, the RK save), built from its
+# convective reconstructions and fluxes, the dual-time shift, the Runge-Kutta stage), built from its
 # sources by `make ref` (only where the reference's sources and a Fortran compiler are
 # present) into _ref/; ref_driver.f90 holds the C-callable wrappers.
 REF_SRC = "/root/reference/BaseVersion"
@@ -72,6 +72,8 @@ def ref_lib():
         R.ref_bind.argtypes = [C.c_int, C.c_void_p]
         R.ref_block_scalars.argtypes = [C.c_double] * 5 + [C.c_int] * 4
         R.ref_block_call.argtypes = [C.c_int]
+        R.ref_rk_scalars.argtypes = [C.c_double] * 7 + [C.c_int] * 4
+        R.ref_block_resid.argtypes = [DP, C.POINTER(C.c_int)]
         _ref_lib = R
     return _ref_lib
 

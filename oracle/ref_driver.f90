@@ -5,13 +5,14 @@
 ! Space/CONVECT_AUSMPW_Plus.F90:204-291), the hybrid RANS/LES functions (FSST,
 ! fd_function, fe_function; Time/Hybrid_function.F90), and whole-block routines run on a
 ! block whose arrays are the oracle's (the convective reconstructions and fluxes of Space/,
-! TIME_Update_U, TIME_RK_Save).  oracle/Makefile (target ref) compiles
+! TIME_Update_U and the Runge-Kutta stage routines).  oracle/Makefile (target ref) compiles
 ! those files where they lie, with default REAL promoted to 8 bytes as the FP64 build of the
 ! reference is, and links them with this driver into oracle/_ref/libtrans_ref.so.  Only tests/
 ! load that library, to check the oracle's restatements against the reference bit for bit.
 !
 ! The routines read module variables of Module_Global.F90 (PPf, Tiny, XM, Csthlnd, Csth1, Ref,
-! Kind_SubHY, gama_max, gama_min, F_limiter2, KI, Itr_Chld, Kind_Dual, Kind_Model); each
+! Kind_SubHY, gama_max, gama_min, F_limiter2, KI, Itr_Chld, Kind_Dual, Kind_Model, A1,
+! Beta_Star, Prod_Lmt, RXM2, If_Euler, IF_Turb, KItm); each
 ! wrapper sets the ones its routines read before the calls.
 
 subroutine ref_precond_para(xm_in, csthlnd_in, csth1_in, ref_in, vm2, am2, rr, tt, rmiut, alagmr, ubetac2) &
@@ -162,7 +163,7 @@ end subroutine
 subroutine ref_bind(which, p) bind(C, name="ref_bind")
     use iso_c_binding, only: c_int, c_ptr, c_f_pointer
     use Global, only: ThisBlock, V, PP, T, rad, rccl, rccr, shock, VL, VR, D, SD, Grad, Alagm, &
-                      Dtm, Vol, Wt0, Wt1, Wt2, DQdxyz
+                      Dtm, Vol, Wt0, Wt1, Wt2, DQdxyz, Rmiu, Dst
     implicit none
     integer(c_int), value :: which
     type(c_ptr), value :: p
@@ -230,6 +231,12 @@ subroutine ref_bind(which, p) bind(C, name="ref_bind")
     case (19)
         call c_f_pointer(p, f, [18 * n])
         DQdxyz(1:18, -2:a, -2:b, -2:c) => f
+    case (20)
+        call c_f_pointer(p, f, [n])
+        Rmiu(-2:a, -2:b, -2:c) => f
+    case (21)
+        call c_f_pointer(p, f, [n])
+        Dst(-2:a, -2:b, -2:c) => f
     end select
 end subroutine
 
@@ -251,8 +258,42 @@ subroutine ref_block_scalars(ppf_in, tiny_in, gmax, gmin, flim2, ki_in, itr_in, 
     Kind_Model = kmodel
 end subroutine
 
+! The module variables TIME_RK_V_Update_New reads besides those of ref_block_scalars.
+subroutine ref_rk_scalars(a1_in, beta_star_in, prod_lmt_in, ref_in, rxm2_in, csthlnd_in, csth1_in, &
+                          if_euler_in, if_turb_in, kind_model_in, kitm_in) bind(C, name="ref_rk_scalars")
+    use iso_c_binding, only: c_double, c_int
+    use Global, only: A1, Beta_Star, Prod_Lmt, Ref, RXM2, Csthlnd, Csth1, If_Euler, IF_Turb, Kind_Model, KItm
+    implicit none
+    real(c_double), value :: a1_in, beta_star_in, prod_lmt_in, ref_in, rxm2_in, csthlnd_in, csth1_in
+    integer(c_int), value :: if_euler_in, if_turb_in, kind_model_in, kitm_in
+    A1 = a1_in
+    Beta_Star = beta_star_in
+    Prod_Lmt = prod_lmt_in
+    Ref = ref_in
+    RXM2 = rxm2_in
+    Csthlnd = csthlnd_in
+    Csth1 = csth1_in
+    If_Euler = if_euler_in
+    IF_Turb = if_turb_in
+    Kind_Model = kind_model_in
+    KItm = kitm_in
+end subroutine
+
+! The block's residual norms and their locations as TIME_RK and TIME_RK_V_Update_New leave them:
+! r = DRSa, DUa, DRSk, DUk, DRSo, DUo; m = Ima, Jma, Kma, Imk, Jmk, Kmk, Imo, Jmo, Kmo.
+subroutine ref_block_resid(r, m) bind(C, name="ref_block_resid")
+    use iso_c_binding, only: c_double, c_int
+    use ref_block_state, only: blk
+    implicit none
+    real(c_double), intent(out) :: r(6)
+    integer(c_int), intent(out) :: m(9)
+    r = [blk%DRSa, blk%DUa, blk%DRSk, blk%DUk, blk%DRSo, blk%DUo]
+    m = [blk%Ima, blk%Jma, blk%Kma, blk%Imk, blk%Jmk, blk%Kmk, blk%Imo, blk%Jmo, blk%Kmo]
+end subroutine
+
 ! 1 CONVECT_MUSCL, 2 CONVECT_MUSCL2, 3 CONVECT_MDCD, 4 CONVECT_MDCDHYB, 5 TURBTWO_CONVECT_MUSCL,
-! 6 CONVECT_AUSMPW_Plus, 7 TURBTWO_CONVECT_Upwind, 8 TIME_Update_U, 9 TIME_RK_Save
+! 6 CONVECT_AUSMPW_Plus, 7 TURBTWO_CONVECT_Upwind, 8 TIME_Update_U, 9 TIME_RK_Save, 10 TIME_RK,
+! 11 TIME_RK_V_Update_New
 subroutine ref_block_call(which) bind(C, name="ref_block_call")
     use iso_c_binding, only: c_int
     implicit none
@@ -276,5 +317,9 @@ subroutine ref_block_call(which) bind(C, name="ref_block_call")
         call TIME_Update_U
     case (9)
         call TIME_RK_Save
+    case (10)
+        call TIME_RK
+    case (11)
+        call TIME_RK_V_Update_New
     end select
 end subroutine

@@ -1168,7 +1168,8 @@ static void wdp_limit(const trans_params* p, ora_block* b, int I, int J, int K, 
     double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
     double Skl = sqrt(1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk + 2.0 * Sij * Sij + 2.0 * Sik * Sik +
                       2.0 * Sjk * Sjk);
-    ARTO = fmax(ARTO, pow(pRo * fabs(prod) / (p->prod_lmt * p->beta_star), 0.5) / p->ref);
+    /* X**0.5 is the square root (compilers evaluate the reference's **0.5 so) */
+    ARTO = fmax(ARTO, sqrt(pRo * fabs(prod) / (p->prod_lmt * p->beta_star)) / p->ref);
     ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo);
     V(6, I, J, K) = ARTk;
     V(7, I, J, K) = ARTO;
@@ -1309,7 +1310,8 @@ void ora_update_v(const trans_params* p, int ki, int if_turb, ora_block* b)
                     double Skl = sqrt(2.0 * (1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk +
                                              2.0 * Sij * Sij + 2.0 * Sik * Sik + 2.0 * Sjk * Sjk));
                     double Cmua = fmin(1.0, p->a1 * ARTO / pRo / ((Skl)*arg2 + 1.e-10) * p->ref);
-                    ARTO = fmax(ARTO, pow(pRo * Cmua * fabs(prod) / (p->prod_lmt * p->beta_star), 0.5) / p->ref);
+                    /* X**0.5 is the square root (compilers evaluate the reference's **0.5 so) */
+                    ARTO = fmax(ARTO, sqrt(pRo * Cmua * fabs(prod) / (p->prod_lmt * p->beta_star)) / p->ref);
                     ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo / Cmua);
                     V(6, I, J, K) = ARTk;
                     V(7, I, J, K) = ARTO;
@@ -1482,7 +1484,8 @@ void ora_rk_update_v(const trans_params* p, int ki, int if_turb, ora_block* b)
                     double Skl = sqrt(2.0 * (1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk +
                                              2.0 * Sij * Sij + 2.0 * Sik * Sik + 2.0 * Sjk * Sjk));
                     double Cmua = fmin(1.0, p->a1 * ARTO / pRo / ((Skl)*arg2 + 1.e-10) * p->ref);
-                    ARTO = fmax(ARTO, pow(pRo * Cmua * fabs(prod) / (p->prod_lmt * p->beta_star), 0.5) / p->ref);
+                    /* X**0.5 is the square root (compilers evaluate the reference's **0.5 so) */
+                    ARTO = fmax(ARTO, sqrt(pRo * Cmua * fabs(prod) / (p->prod_lmt * p->beta_star)) / p->ref);
                     ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo / Cmua);
                     V(6, I, J, K) = ARTk;
                     V(7, I, J, K) = ARTO;

@@ -112,8 +112,9 @@ void ora_diff_dqdxyz(const trans_params* p, int if_turb, ora_block* b)
                 double aLgrd = ((1. - FunC1) * 0.61 + FunC1 * 0.78) * DLTxyz;
                 double B0 = 2. * Vort * fmax(Vort, Skl) / fmax((Vort * Vort + Skl * Skl) / 2., p->alscale * 0.1);
                 double g0 = tanh(B0 * B0 * B0 * B0);
-                double sok = fmax(pow((Vort * Vort + Skl * Skl) / 2., 0.5), 0.1 / tau);
-                double alturb = pow((RmiuT + RMIUL) / Den / (p->ref * sok * pow(0.09, 1.5)), 0.5);
+                /* X**0.5 is the square root (compilers evaluate the reference's **0.5 so) */
+                double sok = fmax(sqrt((Vort * Vort + Skl * Skl) / 2.), 0.1 / tau);
+                double alturb = sqrt((RmiuT + RMIUL) / Den / (p->ref * sok * pow(0.09, 1.5)));
                 double A0 = 1. * fmax(((aLgrd / alturb / g0) - 0.1), 0.0);
                 FSCHEME(I, J, K) = fmax(1. * tanh(pow(A0, 3.)), p->f_limiter1);
                 SHOCK(I, J, K) = DivU * DivU / fmax(DivU * DivU + Vort * Vort, 1.0);

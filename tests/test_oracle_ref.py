@@ -202,7 +202,7 @@ def test_hybrid_functions_equal_reference(ref):
 
 _BIND = {"v": 1, "pp": 2, "t": 3, "rad": 4, "rccl": 5, "rccr": 6, "shock": 7, "VL": 8, "VR": 9, "D": 10,
          "sd": 11, "grad": 12, "alagm": 13, "Dtm": 14, "vol": 15, "wt0": 16, "wt1": 17, "wt2": 18,
-         "DQdxyz": 19}
+         "DQdxyz": 19, "rmiu": 20, "dst": 21}
 _KI = 5
 
 
@@ -240,9 +240,10 @@ def sector():
     return oc, prm
 
 
-def _run_both(ref, oc, prm, which, ora, pre=None):
-    """Per block: the oracle's routine, then the reference's on the same inputs; every array
-    bitwise equal.  Returns the names of the arrays the routines changed."""
+def _run_both(ref, oc, prm, which, ora, pre=None, itr=1, after=None):
+    """Per block: the oracle's routine, then the reference's on the same inputs (stage itr of
+    a Runge-Kutta step); every array bitwise equal, and after(n) checks further outputs.
+    Returns the names of the arrays the routines changed."""
     changed = set()
     for n in range(oc.nblk):
         A, b = oc[n], oc.ob[n]
@@ -257,9 +258,13 @@ def _run_both(ref, oc, prm, which, ora, pre=None):
         ref.ref_block_dims(b.ni, b.nj, b.nk)
         for k, w in _BIND.items():
             ref.ref_bind(w, A[k].ctypes.data)
-        ref.ref_block_scalars(prm.ppf, prm.tiny, prm.gama_max, prm.gama_min, prm.f_limiter2, _KI, 1,
+        ref.ref_block_scalars(prm.ppf, prm.tiny, prm.gama_max, prm.gama_min, prm.f_limiter2, _KI, itr,
                               prm.kind_dual, prm.kind_model)
+        ref.ref_rk_scalars(prm.a1, prm.beta_star, prm.prod_lmt, prm.ref, prm.rxm2, prm.csthlnd, prm.csth1,
+                           prm.if_euler, orc.lib().ora_if_turb(C.byref(prm), _KI), prm.kind_model, prm.kitm)
         ref.ref_block_call(which)
+        if after is not None:
+            after(n)
         for k in A:
             assert np.array_equal(A[k], got[k], equal_nan=True), (n, k, np.nanmax(np.abs(A[k] - got[k])))
             if not np.array_equal(A[k], snap[k], equal_nan=True):
@@ -342,3 +347,54 @@ def test_rk_save_equals_reference(ref, sector):
     oc, prm = sector
     L = orc.lib()
     assert _run_both(ref, oc, prm, 9, lambda b: L.ora_rk_save(1, C.byref(b))) == {"wt2"}
+
+
+_RESID = ("drsa", "dua", "drsk", "duk", "drso", "duo", "ima", "jma", "kma", "imk", "jmk", "kmk", "imo", "jmo", "kmo")
+
+
+def _ref_resid(ref):
+    r, m = (C.c_double * 6)(), (C.c_int * 9)()
+    ref.ref_block_resid(r, m)
+    return dict(zip(_RESID, list(r) + list(m)))
+
+
+@pytest.mark.parametrize("itr", [1, 2, 3])
+def test_rk_stage_equals_reference(ref, sector, itr):
+    """TIME_RK at each of the three stages: V(1:5) and the density-residual norms with the
+    cell of the maximum (DRSa, DUa, Ima, Jma, Kma)."""
+    oc, prm = sector
+    L = orc.lib()
+    for n in range(oc.nblk):
+        assert (oc[n]["Dtm"][3:-3, 3:-3, 3:-3] > 0).all()
+    norms = {}
+
+    def ora(b):
+        L.ora_rk(itr, C.byref(b))
+        norms["ora"] = b.res.as_dict()
+
+    def after(n):
+        got = _ref_resid(ref)
+        for k in ("drsa", "dua", "ima", "jma", "kma"):
+            assert got[k] == norms["ora"][k], (n, k, got[k], norms["ora"][k])
+
+    assert _run_both(ref, oc, prm, 10, ora, itr=itr, after=after) == {"v"}
+
+
+def test_rk_update_v_equals_reference(ref, sector):
+    """TIME_RK_V_Update_New: p and T from the new state, the positivity-damped k and omega
+    update with the SST limiter, and the k and omega norms with their cells."""
+    oc, prm = sector
+    L = orc.lib()
+    it = L.ora_if_turb(C.byref(prm), _KI)
+    norms = {}
+
+    def ora(b):
+        L.ora_rk_update_v(C.byref(prm), _KI, it, C.byref(b))
+        norms["ora"] = b.res.as_dict()
+
+    def after(n):
+        got = _ref_resid(ref)
+        for k in ("drsk", "duk", "imk", "jmk", "kmk", "drso", "duo", "imo", "jmo", "kmo"):
+            assert got[k] == norms["ora"][k], (n, k, got[k], norms["ora"][k])
+
+    assert _run_both(ref, oc, prm, 11, ora, after=after) >= {"v", "pp", "t"}

@@ -877,7 +877,8 @@ __device__ __forceinline__ void sst_kw_update(const trans_params& p, const Field
         const double Skl = sqrt(2.0 * (1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk +
                                        2.0 * Sij * Sij + 2.0 * Sik * Sik + 2.0 * Sjk * Sjk));
         const double Cmua = fmin(1.0, p.a1 * ARTO / pRo / ((Skl)*arg2 + 1.e-10) * p.ref);
-        ARTO = fmax(ARTO, pow(pRo * Cmua * fabs(prod) / (p.prod_lmt * p.beta_star), 0.5) / p.ref);
+        // X**0.5 of the reference is the (correctly rounded) square root, as compilers evaluate it
+        ARTO = fmax(ARTO, sqrt(pRo * Cmua * fabs(prod) / (p.prod_lmt * p.beta_star)) / p.ref);
         ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo / Cmua);
         f.V[5 * tot + c] = ARTk;
         f.V[6 * tot + c] = ARTO;
@@ -889,7 +890,8 @@ __device__ __forceinline__ void sst_kw_update(const trans_params& p, const Field
         const double Sij = (dudy + dvdx) / 2.0, Sik = (dudz + dwdx) / 2.0, Sjk = (dvdz + dwdy) / 2.0;
         const double Skl = sqrt(1.0 * Sii * Sii + 1.0 * Sjj * Sjj + 1.0 * Skk * Skk + 2.0 * Sij * Sij +
                                 2.0 * Sik * Sik + 2.0 * Sjk * Sjk);
-        ARTO = fmax(ARTO, pow(pRo * fabs(prod) / (p.prod_lmt * p.beta_star), 0.5) / p.ref);
+        // X**0.5 of the reference is the (correctly rounded) square root, as compilers evaluate it
+        ARTO = fmax(ARTO, sqrt(pRo * fabs(prod) / (p.prod_lmt * p.beta_star)) / p.ref);
         ARTk = fmin(ARTk, 100000.0 * RmiuL * ARTO / pRo);
         f.V[5 * tot + c] = ARTk;
         f.V[6 * tot + c] = ARTO;

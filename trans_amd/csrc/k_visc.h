@@ -243,8 +243,9 @@ __global__ void __launch_bounds__(256) k_gradients(trans_params p, int if_turb, 
         const double aLgrd = ((1. - FunC1) * 0.61 + FunC1 * 0.78) * DLTxyz;
         const double B0 = 2. * Vort * fmax(Vort, Skl) / fmax((Vort * Vort + Skl * Skl) / 2., p.alscale * 0.1);
         const double g0 = tanh(B0 * B0 * B0 * B0);
-        const double sok = fmax(pow((Vort * Vort + Skl * Skl) / 2., 0.5), 0.1 / tau);
-        const double alturb = pow((RmiuT + RMIUL) / Den / (p.ref * sok * pow(0.09, 1.5)), 0.5);
+        // X**0.5 of the reference is the (correctly rounded) square root, as compilers evaluate it
+        const double sok = fmax(sqrt((Vort * Vort + Skl * Skl) / 2.), 0.1 / tau);
+        const double alturb = sqrt((RmiuT + RMIUL) / Den / (p.ref * sok * pow(0.09, 1.5)));
         const double A0 = 1. * fmax(((aLgrd / alturb / g0) - 0.1), 0.0);
         f.Fscheme[c] = fmax(1. * tanh(pow(A0, 3.)), p.f_limiter1);
     }
